@@ -1,0 +1,177 @@
+"""KB steps/s of the FP64 2PI Kadanoff-Baym step at 32^3, N_mem = 400 on one
+MI355X, with the memory-integral kernel's HBM roofline fraction and the
+oracle (CPU restatement of the reference algorithm) timed on the host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Set-up follows the reference's benchmark harness (TwoPIScalar_Bechmark.jl:
+62-104): the memory window is filled first (here with the free propagators
+of the thermal T = 2 state, so every value is finite and realistic), then W
+untimed steps, then K timed steps.  Every timed step is a complete evolve!:
+self-energy FFT stage, Hartree reduction, memory integrals, leapfrog of the
+new row and its diagonal.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "qftdynamicsm.jl_amd"))
+
+import numpy as np  # noqa: E402
+
+WORKLOAD = dict(Nx=32, sdim=3, NstepsinMemory=400, dt=0.1, Lambda=1.0, ONgroup=1, Mass=1.0, T=2.0)
+HBM_PEAK_GBS = 8000.0
+
+
+def thermal_n(kL2, Mass, T):
+    return 1.0 / (np.exp(np.sqrt(kL2 + Mass ** 2) / (T * Mass)) - 1.0)
+
+
+def bytes_memint(N, V):
+    """Algorithmic bytes of one memory-integral + leapfrog launch: the stored
+    window of N-1 live times read once ((N-1)N F and rho values per site),
+    SigF/Sigr of N-1 times, the new row of N F and rho values written."""
+    return ((N - 1) * N + 2 * (N - 1) + 2 * N) * V * 8
+
+
+def load_pmc(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(w, budget_s=30.0):
+    """Oracle (C + OpenMP, reference double-read order) on a bounded sample:
+    the same 32^3 window filled the same way, then 1-2 timed KB steps."""
+    from oracle.oracle import Oracle
+
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    N = w["NstepsinMemory"]
+    V = w["Nx"] ** w["sdim"]
+    need = 2 * N * (N + 1) // 2 * V * 8
+    avail = None
+    try:
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemAvailable"):
+                    avail = int(line.split()[1]) * 1024
+    except OSError:
+        pass
+    note = ""
+    if avail is not None and avail < 1.25 * need:
+        N = 200
+        note = f"; host RAM {avail / 2**30:.0f} GiB < 1.25x the N_mem=400 window, timed at N_mem=200"
+    o = Oracle(w["Nx"], w["sdim"], N, w["dt"], w["Lambda"], w["ONgroup"], w["Mass"], w["Mass"] ** 2,
+               nthreads=nthreads)
+    n = thermal_n(o.kL2(), w["Mass"], w["T"])
+    o.fill_free_window(N, n, w["Mass"])
+    times = []
+    t = N + 1
+    t0 = time.perf_counter()
+    while True:
+        a = time.perf_counter()
+        o.evolve(t)
+        times.append(time.perf_counter() - a)
+        t += 1
+        if len(times) >= 2 or (time.perf_counter() - t0) + times[-1] > budget_s:
+            break
+    o.close()
+    per = sum(times) / len(times)
+    return {"value": 1.0 / per, "unit": "KB steps/s", "cores": nthreads, "kind": "port",
+            "sample": f"oracle (CPU restatement of the reference CPUfull algorithm, double-read order), "
+                      f"32^3 N_mem={N} window, {len(times)} timed step(s) after fill{note}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
+    ap.add_argument("--nmem", type=int, default=WORKLOAD["NstepsinMemory"])
+    ap.add_argument("--nx", type=int, default=WORKLOAD["Nx"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not supported "
+                                   "(multi-GPU sharding is design-only)"}))
+        sys.exit(2)
+
+    from qftkb import KBContext
+
+    w = dict(WORKLOAD)
+    w["NstepsinMemory"] = args.nmem
+    w["Nx"] = args.nx
+    N = w["NstepsinMemory"]
+    V = w["Nx"] ** w["sdim"]
+    ctx = KBContext(w["Nx"], w["sdim"], N, w["dt"], w["Lambda"], w["ONgroup"], w["Mass"], w["Mass"] ** 2)
+    n = thermal_n(ctx.kL2(), w["Mass"], w["T"])
+    ctx.fill_free_window(N, n, w["Mass"])
+    print(f"[bench] window filled: {w['Nx']}^3 N_mem={N}, device bytes {ctx.device_bytes() / 1e9:.2f} GB",
+          file=sys.stderr, flush=True)
+    t = N + 1
+    for _ in range(args.warmup):
+        ctx.step(t)
+        t += 1
+    ctx.synchronize()
+    ctx.enable_timing(True)
+    ctx.reset_timing()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.step(t)
+        t += 1
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_mi, n_mi = ctx.timing(0)
+    ms_fft, n_fft = ctx.timing(1)
+    ms_h, _ = ctx.timing(2)
+    ctx.enable_timing(False)
+    ctx.check_finite()
+    print(f"[bench] {args.steps} steps in {elapsed:.4f} s; memint {ms_mi / max(n_mi, 1):.3f} ms, "
+          f"sigma {ms_fft / max(n_fft, 1):.3f} ms per step", file=sys.stderr, flush=True)
+    ctx.close()
+
+    steps_per_s = args.steps / elapsed
+    avg_mi_s = ms_mi / max(n_mi, 1) / 1e3
+    B = bytes_memint(N, V)
+    achieved = B / avg_mi_s / 1e9
+    pmc = load_pmc(os.path.join(ROOT, "profiles", "memint_pmc_32c_n400.json"))
+    traffic = None
+    if pmc and pmc.get("Nx") == w["Nx"] and pmc.get("N") == N:
+        traffic = pmc.get("traffic_bytes")
+    line = {
+        "metric": "KB steps/s at 32^3 N_mem=400 FP64 (% HBM peak); adjoint/forward cost ratio",
+        "value": steps_per_s,
+        "unit": "KB steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (window filled with free thermal propagators, as the reference benchmark fills "
+                "its window without physics)",
+        "config": {"workload": f"2PI KB step, LambdaNLO sunset, phi^4 O(1), {w['Nx']}^3 lattice, "
+                               f"N_mem={N}, thermal T=2, lambda=1, m=1, dt=0.1",
+                   "Nx": w["Nx"], "sdim": 3, "NstepsinMemory": N, "modes": V},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "memint_step", "algorithmic_bytes": B,
+                     "avg_kernel_ms": avg_mi_s * 1e3, "sigma_stage_ms": ms_fft / max(n_fft, 1),
+                     "hartree_ms": ms_h / max(n_mi, 1)},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(w)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

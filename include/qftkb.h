@@ -1,0 +1,156 @@
+/*
+ * qftkb.h -- C ABI of libqftkb.so: the MI355X (gfx950) back-end for the
+ * 2PI Kadanoff-Baym time step of the reference's src/TwoPIScalar model
+ * (CPUfull storage semantics, LambdaLO / LambdaNLO quantum expansions).
+ *
+ * The reference selects its numerics back-end by Julia dispatch on the
+ * "Num" parameter (TwoPIScalar_Modelfile.jl:104-108) and then calls
+ * initSimData!, evolve!, getHartreeMass2 and createMeasurement on the
+ * back-end's SimData type.  Each entry point below replaces one of those
+ * methods for a new numerics type; the Julia binding a maintainer would add
+ * is shown in INTEGRATION.md.
+ *
+ * Conventions
+ *   - Every call returns an int status: KB_OK (0) or a KB_ERR_* code; the
+ *     message is available from kb_last_error().  No C++ exception crosses
+ *     the ABI.
+ *   - Times are 1-based as in the reference.  The memory window is
+ *     [tmin, tmax] (simdata.indices, src/DefSimDataElements.jl:141-165);
+ *     an index outside it returns KB_ERR_WINDOW (the reference throws
+ *     BoundsError, DefSimDataElements.jl:187-188).
+ *   - Host buffers are caller-owned, contiguous float64 lattices of
+ *     V = Nx^sdim sites in column-major site order (x fastest), i.e. the
+ *     memory of a Julia Array{Float64,sdim}.
+ *   - rho is always exchanged as the physical (antisymmetric) spectral
+ *     function rho(i,j); the reference's sign-folded storage
+ *     (r[i,j] * thesign(i,j), DefSimDataElements.jl:224-230) is internal.
+ *   - The library owns all device memory.  A context is not re-entrant: one
+ *     HIP stream, one caller thread.  kb_step enqueues asynchronously;
+ *     calls that return data to the host synchronise the stream.
+ */
+#ifndef QFTKB_H
+#define QFTKB_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KB_OK 0
+#define KB_ERR_ARG 1       /* invalid argument / unsupported size          */
+#define KB_ERR_WINDOW 2    /* time index outside the memory window         */
+#define KB_ERR_OOM 3       /* device or host allocation failed             */
+#define KB_ERR_HIP 4       /* HIP runtime error                            */
+#define KB_ERR_STATE 5     /* call out of order (e.g. step before init)    */
+#define KB_ERR_NONFINITE 6 /* NaN/Inf detected by kb_check_finite          */
+
+#define KB_PEXP_LAMBDA_LO 0  /* TwoPIScalarLambdaLO (Problemdefinition.jl:34) */
+#define KB_PEXP_LAMBDA_NLO 1 /* TwoPIScalarLambdaNLOquantum (:36)            */
+
+typedef struct kb_ctx kb_ctx;
+
+/* Problem description.  Mirrors TwoPIScalarDiscretizationLattice
+ * (TwoPIScalar_Discretization.jl:4-17) and Phi4 (ModelExpansion-
+ * Renormalization.jl:5-9).  Mass2 is the model's (possibly renormalised)
+ * Mass2; Mass is the discretisation mass used for omega^2 at set-up and for
+ * the measurement time. */
+typedef struct kb_params {
+  int Nx;             /* sites per dimension, 1 .. 128                   */
+  int sdim;           /* 1, 2 or 3                                       */
+  int NstepsinMemory; /* memory window length N_mem, 3 .. 640            */
+  int ONgroup;        /* N of O(N)                                       */
+  double dt;
+  double Lambda;
+  double Mass;
+  double Mass2;
+  int pexp;   /* KB_PEXP_*                                                */
+  int device; /* HIP device ordinal                                       */
+} kb_params;
+
+/* Scalar part of MeasurementTwoPIScalar (TwoPIScalar_Measurement.jl:2-23). */
+typedef struct kb_meas {
+  double time, E, E_pot, E_kin, E_mem, M2;
+  int n_unstable; /* sites where 1 - dt^2 omega^2/4 < 0 (the reference
+                     prints "E" and clamps c = 1, Measurement.jl:48-53) */
+} kb_meas;
+
+/* Replaces getTwoPISimData / getTwoPIScalarTmpData for the new numerics
+ * type (TwoPIScalar_SimDataStruct.jl:669-721): allocates the device-resident
+ * two-time window of F and rho, the self-energy ring and scratch. */
+int kb_create(const kb_params *p, kb_ctx **out);
+int kb_destroy(kb_ctx *ctx);
+const char *kb_last_error(const kb_ctx *ctx);
+
+/* Number of lattice sites V and bytes of device memory held. */
+int kb_info(const kb_ctx *ctx, long long *V, long long *device_bytes);
+
+/* Per-site lattice momentum squared kL2 (SimDataStruct.jl:673-679).  The
+ * library fills it at creation with sum_i 4 sin^2(pi n_i / Nx); a caller
+ * holding FFTWhelper levels may overwrite it. */
+int kb_get_kL2(kb_ctx *ctx, double *kL2);
+int kb_set_kL2(kb_ctx *ctx, const double *kL2);
+
+/* initSimData! (Initialization.jl:8-29): from per-site occupations n[V] and
+ * the initial mass, write F(1,1), F(2,1), F(2,2), rho(2,1) = dt, then the
+ * Hartree mass at (2,2).  The window becomes [1,2]. */
+int kb_init_particles(kb_ctx *ctx, const double *n, double initmass);
+/* initSimData! with the three F lattices supplied (Gauss init,
+ * Initialization.jl:95-112, or any raw Gaussian state). */
+int kb_init_rows(kb_ctx *ctx, const double *F11, const double *F21,
+                 const double *F22);
+
+/* evolve! (TwoPIScalar_Evolution.jl:924-981): one KB step producing row t
+ * of F and rho.  t must equal tmax + 1 and be >= 3. */
+int kb_step(kb_ctx *ctx, int t);
+
+/* simdata.indices after the last call. */
+int kb_window(const kb_ctx *ctx, int *tmin, int *tmax);
+
+/* simdata.hartreemass2[1]: the Hartree mass^2 used by the last step (or set
+ * by init).  Synchronises. */
+int kb_hartree(kb_ctx *ctx, double *m2);
+
+/* createMeasurement (TwoPIScalar_Measurement.jl:31-99) at time t (t must be
+ * tmax).  Energies are exact sums over all V sites.  Per-site outputs may be
+ * NULL; otherwise each receives V doubles:
+ *   F = F(t-1,t-1), ddF, r = rho(t-1,t-1), n, omega, c.  Synchronises. */
+int kb_measure(kb_ctx *ctx, int t, kb_meas *out, double *F, double *ddF,
+               double *r, double *n, double *omega, double *c);
+/* The history part of the same record (Measurement.jl:90-96), at site 1:
+ * SigF, Sigr for tp = t-1 down to tmin, Fonet = F(tmin,tp) and
+ * ronet = rho(tmin,tp) for tp = tmin .. t-1.  Each buffer holds
+ * t - tmin doubles; any may be NULL. */
+int kb_measure_history(kb_ctx *ctx, int t, double *SigF, double *Sigr,
+                       double *Fonet, double *ronet);
+
+/* Test / checkpoint access to the window (which: 0 = F, 1 = rho). */
+int kb_get_entry(kb_ctx *ctx, int which, int i, int j, double *out);
+int kb_set_entry(kb_ctx *ctx, int which, int i, int j, const double *in);
+/* Self-energy lattices of the last step: which 0 = SigF, 1 = Sigr, at tp. */
+int kb_get_sigma(kb_ctx *ctx, int which, int tp, double *out);
+/* Scan the window for NaN/Inf; returns KB_ERR_NONFINITE if any. */
+int kb_check_finite(kb_ctx *ctx);
+
+/* Benchmark set-up in the spirit of the reference's benchmark harness
+ * (TwoPIScalar_Bechmark.jl:62-69, which fills the window without physics):
+ * writes a full, deterministic window [tmax-N_mem+1, tmax] holding the free
+ * propagators F = (n+1/2)/w cos(w(a-b)dt), rho = sin(w(a-b)dt)/w with
+ * w = sqrt(kL2 + m^2), and sets the window indices.  n[V] may be NULL (0). */
+int kb_fill_free_window(kb_ctx *ctx, int tmax, const double *n, double mass);
+
+/* Stream and timing. */
+int kb_synchronize(kb_ctx *ctx);
+/* Launch on a caller-provided hipStream_t (NULL = library's own stream). */
+int kb_set_stream(kb_ctx *ctx, void *hip_stream);
+/* HIP-event timing of the kernels of kb_step, recorded on the launch
+ * stream.  stage: 0 = memory integral + leapfrog kernel, 1 = self-energy
+ * FFT stage (all its passes), 2 = Hartree reduction.  kb_timing
+ * synchronises and returns the accumulated milliseconds and launch count
+ * since the last reset. */
+int kb_enable_timing(kb_ctx *ctx, int on);
+int kb_timing(kb_ctx *ctx, int stage, double *total_ms, long long *count);
+int kb_reset_timing(kb_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QFTKB_H */

@@ -28,6 +28,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/mman.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -35,6 +36,16 @@
 typedef struct {
   double re, im;
 } cplx;
+
+/* zeroed allocation for the large history arrays: anonymous mapping with
+ * transparent huge pages requested (first-touch faults dominate otherwise) */
+static void *big_alloc(size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  void *p = mmap(NULL, bytes, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+  if (p == MAP_FAILED) return NULL;
+  madvise(p, bytes, MADV_HUGEPAGE);
+  return p;
+}
 
 /* ------------------------------------------------------------------ */
 /* 1D mixed-radix Stockham FFT                                         */
@@ -123,6 +134,7 @@ struct or_ctx {
   int ON, pexp;
   int tmin, tmax;
   double *F, *r;       /* N(N+1)/2 lattices each, slot triangle */
+  size_t bigbytes;
   double *SigF, *Sigr; /* N lattices each */
   double *kL2, *omega2;
   double hartreemass2;
@@ -209,11 +221,13 @@ or_ctx *or_create(int Nx, int sdim, int Nmem, double dt, double lambda,
   c->pexp = pexp;
   c->nthreads = nthreads > 0 ? nthreads : 1;
   size_t np = (size_t)Nmem * (Nmem + 1) / 2;
-  /* SymMatrix lattices are created as ones(...) (DefSimDataElements.jl:
-     176); CyclVector as zeros (:107).  Ones are kept so the initial
-     content matches the reference. */
-  c->F = (double *)malloc(sizeof(double) * np * c->V);
-  c->r = (double *)malloc(sizeof(double) * np * c->V);
+  /* The reference creates SymMatrix lattices as ones(...) (DefSimDataElements.jl:
+     176).  No entry is read before it is written (init writes rows 1-2, each
+     step writes its row before any later step reads it), so that initial
+     content is not observable; zero pages are used instead. */
+  c->bigbytes = np * c->V * sizeof(double);
+  c->F = (double *)big_alloc(c->bigbytes);
+  c->r = (double *)big_alloc(c->bigbytes);
   c->SigF = (double *)calloc((size_t)Nmem * c->V, sizeof(double));
   c->Sigr = (double *)calloc((size_t)Nmem * c->V, sizeof(double));
   c->kL2 = (double *)malloc(sizeof(double) * c->V);
@@ -221,11 +235,6 @@ or_ctx *or_create(int Nx, int sdim, int Nmem, double dt, double lambda,
   if (!c->F || !c->r || !c->SigF || !c->Sigr || !c->kL2 || !c->omega2) {
     or_destroy(c);
     return NULL;
-  }
-#pragma omp parallel for num_threads(c->nthreads) schedule(static)
-  for (long long i = 0; i < (long long)(np * c->V); ++i) {
-    c->F[i] = 1.0;
-    c->r[i] = 1.0;
   }
   compute_kL2(c);
   reset_omega2(c);
@@ -237,8 +246,8 @@ or_ctx *or_create(int Nx, int sdim, int Nmem, double dt, double lambda,
 
 void or_destroy(or_ctx *c) {
   if (!c) return;
-  free(c->F);
-  free(c->r);
+  if (c->F) munmap(c->F, c->bigbytes);
+  if (c->r) munmap(c->r, c->bigbytes);
   free(c->SigF);
   free(c->Sigr);
   free(c->kL2);
@@ -732,5 +741,47 @@ int or_measure(or_ctx *c, int t, double *scal, double *Fo, double *ddFo,
     scal[5] = c->hartreemass2;
   }
   if (nstab) *nstab = bad;
+  return 0;
+}
+
+/* Benchmark set-up, as the reference's benchmark harness fills the window
+ * without physics (TwoPIScalar_Bechmark.jl:62-69): a full window
+ * [tmax-N+1, tmax] of free propagators F = (n+1/2)/w cos(w(a-b)dt),
+ * rho(a,b) = sin(w(a-b)dt)/w, w = sqrt(kL2 + m^2).  n may be NULL. */
+int or_fill_free_window(or_ctx *c, int tmax, const double *n, double mass) {
+  if (tmax < c->N) return fail(c, "fill: tmax %d < N %d", tmax, c->N);
+  c->tmin = tmax - c->N + 1;
+  c->tmax = tmax;
+  const int V = c->V, N = c->N;
+  const int tmin = c->tmin;
+  /* the free propagators depend on the lag a-b only: tabulate N lags */
+  double *Ft = (double *)malloc(sizeof(double) * (size_t)N * V);
+  double *rt = (double *)malloc(sizeof(double) * (size_t)N * V);
+  if (!Ft || !rt) {
+    free(Ft);
+    free(rt);
+    return fail(c, "fill: out of memory");
+  }
+#pragma omp parallel for num_threads(c->nthreads) schedule(static)
+  for (int d = 0; d < N; ++d)
+    for (int i = 0; i < V; ++i) {
+      const double w = sqrt(c->kL2[i] + mass * mass);
+      const double nn = n ? n[i] : 0.0;
+      const double x = (double)d * c->dt;
+      Ft[(size_t)d * V + i] = (nn + 0.5) / w * cos(w * x);
+      rt[(size_t)d * V + i] = sin(w * x) / w;
+    }
+#pragma omp parallel for num_threads(c->nthreads) schedule(dynamic, 1)
+  for (int a = tmin; a <= tmax; ++a)
+    for (int b = tmin; b <= a; ++b) {
+      double *Fl = symptr(c, c->F, a, b), *rl = symptr(c, c->r, a, b);
+      const double s = thesign(a, b, N);
+      const double *Fs = Ft + (size_t)(a - b) * V, *rs = rt + (size_t)(a - b) * V;
+      memcpy(Fl, Fs, sizeof(double) * V);
+      for (int i = 0; i < V; ++i) rl[i] = rs[i] * s;
+    }
+  free(Ft);
+  free(rt);
+  c->hartreemass2 = or_hartree(c, tmax, tmax);
   return 0;
 }

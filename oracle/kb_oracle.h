@@ -51,6 +51,9 @@ int or_init_particles(or_ctx *c, const double *n, double initmass);
 int or_init_rows(or_ctx *c, const double *F11, const double *F21,
                  const double *F22);
 
+/* Benchmark set-up: full window of free propagators (see kb_oracle.c). */
+int or_fill_free_window(or_ctx *c, int tmax, const double *n, double mass);
+
 /* One KB step, evolve! (Evolution.jl:924-981).  t must be tmax+1. */
 int or_evolve(or_ctx *c, int t);
 

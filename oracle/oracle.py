@@ -68,6 +68,7 @@ def lib():
         L.or_measure.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                  ctypes.POINTER(ctypes.c_int)]
         L.or_dft.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int]
+        L.or_fill_free_window.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_double]
         _lib = L
     return _lib
 
@@ -124,6 +125,11 @@ class Oracle:
     def init_rows(self, F11, F21, F22):
         a = [np.ascontiguousarray(x, dtype=np.float64) for x in (F11, F21, F22)]
         self._chk(self._L.or_init_rows(self._c, *[_ptr(x) for x in a]))
+
+    def fill_free_window(self, tmax, n=None, mass=1.0):
+        nn = None if n is None else np.ascontiguousarray(n, dtype=np.float64)
+        self._chk(self._L.or_fill_free_window(self._c, int(tmax), None if nn is None else _ptr(nn),
+                                              float(mass)))
 
     def evolve(self, t):
         self._chk(self._L.or_evolve(self._c, int(t)))

@@ -1,0 +1,660 @@
+// kb_api.hip -- host side of libqftkb: the extern "C" boundary declared in
+// include/qftkb.h.  Owns device memory, window bookkeeping (the reference's
+// simdata.indices, src/DefSimDataElements.jl:141-165) and kernel launches.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/qftkb.h"
+#include "kb_kernels.h"
+
+using namespace qftkb;
+
+namespace {
+
+struct EventPair {
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct kb_ctx {
+  kb_params p{};
+  int V = 0, Vpad = 0, NB = 0, N = 0;
+  long NP = 0;
+  int nx = 1, ny = 1, nz = 1;
+  int tmin = 0, tmax = 0;
+  bool initialised = false;
+  // device memory
+  double* H = nullptr;     // history
+  double* S = nullptr;     // self-energy ring
+  double2* W = nullptr;    // FFT workspace
+  double* kL2 = nullptr;
+  double* m2 = nullptr;    // [0] current Hartree mass^2
+  double* partial = nullptr;
+  double* modes = nullptr; // 6 x V measurement outputs
+  double* vbuf = nullptr;  // 4 x V staging
+  double2* tw = nullptr;
+  int* flag = nullptr;
+  size_t bytes = 0;
+  FFTPlan plan{};
+  int fftP = 1;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  bool timing = false;
+  std::vector<EventPair> pending[3];
+  std::vector<hipEvent_t> freeEv;
+  double tsum[3] = {0, 0, 0};
+  long long tcount[3] = {0, 0, 0};
+  std::string err;
+};
+
+namespace {
+
+int seterr(kb_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                                     \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return seterr((ctx), KB_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                    __FILE__, __LINE__);                                                      \
+  } while (0)
+
+#define LAUNCHCHK(ctx)                                                                          \
+  do {                                                                                          \
+    hipError_t e_ = hipGetLastError();                                                          \
+    if (e_ != hipSuccess)                                                                       \
+      return seterr((ctx), KB_ERR_HIP, "kernel launch failed: %s (%s:%d)", hipGetErrorString(e_), \
+                    __FILE__, __LINE__);                                                        \
+  } while (0)
+
+inline int slot_of(const kb_ctx* c, int t) { return (t - 1) % c->N; }
+inline bool inwin(const kb_ctx* c, int t) { return c->tmin <= t && t <= c->tmax; }
+
+void plan_fft(FFTPlan& pl, int n) {
+  pl.n = n;
+  pl.nf = 0;
+  int m = n;
+  while (m % 4 == 0 && pl.nf < FFT_MAXF) { pl.fac[pl.nf++] = 4; m /= 4; }
+  while (m % 2 == 0 && pl.nf < FFT_MAXF) { pl.fac[pl.nf++] = 2; m /= 2; }
+  for (int f = 3; m > 1 && pl.nf < FFT_MAXF; f += 2)
+    while (m % f == 0) { pl.fac[pl.nf++] = f; m /= f; }
+}
+
+hipEvent_t take_event(kb_ctx* c) {
+  if (!c->freeEv.empty()) {
+    hipEvent_t e = c->freeEv.back();
+    c->freeEv.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+struct StageTimer {
+  kb_ctx* c;
+  int stage;
+  EventPair ev{nullptr, nullptr};
+  StageTimer(kb_ctx* c_, int s) : c(c_), stage(s) {
+    if (c->timing) {
+      ev.a = take_event(c);
+      ev.b = take_event(c);
+      if (ev.a) (void)hipEventRecord(ev.a, c->stream);
+    }
+  }
+  ~StageTimer() {
+    if (c->timing && ev.a && ev.b) {
+      (void)hipEventRecord(ev.b, c->stream);
+      c->pending[stage].push_back(ev);
+    }
+  }
+};
+
+int collect_timing(kb_ctx* c) {
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int s = 0; s < 3; ++s) {
+    for (auto& ev : c->pending[s]) {
+      float ms = 0.f;
+      HIPCHK(c, hipEventElapsedTime(&ms, ev.a, ev.b));
+      c->tsum[s] += ms;
+      c->tcount[s] += 1;
+      c->freeEv.push_back(ev.a);
+      c->freeEv.push_back(ev.b);
+    }
+    c->pending[s].clear();
+  }
+  return KB_OK;
+}
+
+void expand(kb_ctx* c) {
+  // expandSimData!, SimDataStruct.jl:262-278
+  if (c->tmax + 1 > c->N) {
+    c->tmin += 1;
+    c->tmax += 1;
+  } else {
+    c->tmin = 1;
+    c->tmax += 1;
+  }
+}
+
+int launch_hartree(kb_ctx* c, int i, int j) {
+  const double H = c->p.Lambda * (double)(c->p.ONgroup + 2) / (6.0 * (double)c->p.ONgroup);
+  const double hfac = H * (1.0 / (double)c->V);
+  const long pr = pair_index(slot_of(c, i), slot_of(c, j));
+  hipLaunchKernelGGL(hartree_kernel, dim3(1), dim3(1024), 0, c->stream, c->H, c->NP, c->V, pr, hfac, c->p.Mass2,
+                     c->m2);
+  LAUNCHCHK(c);
+  return KB_OK;
+}
+
+int launch_sigma(kb_ctx* c, int t) {
+  const int L = t - c->tmin;  // rows t' = tmin .. t-1
+  if (L <= 0) return KB_OK;
+  const double ivol = 1.0 / (double)c->V;
+  const double lam = c->p.Lambda, ON = (double)c->p.ONgroup;
+  SigmaArgs a{};
+  a.H = c->H;
+  a.S = c->S;
+  a.W = c->W;
+  a.tw = c->tw;
+  a.plan = c->plan;
+  a.NP = c->NP;
+  a.N = c->N;
+  a.V = c->V;
+  a.n = c->p.Nx;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.nz = c->nz;
+  a.P = c->fftP;
+  a.npencils = c->V / c->p.Nx;
+  a.tmin = c->tmin;
+  a.s_tm1 = slot_of(c, t - 1);
+  a.preF = ivol * ivol * ivol * (-(lam * lam) * (ON + 2.0) / (18.0 * ON * ON));
+  a.preR = ivol * ivol * ivol * (-(lam * lam) * (ON + 2.0) / (6.0 * ON * ON));
+  const size_t lds = sizeof(double2) * 2 * (size_t)a.P * a.n;
+  const dim3 grid((a.npencils + a.P - 1) / a.P, L);
+  std::vector<std::pair<int, int>> passes;  // (axis, flags)
+  const int sd = c->p.sdim;
+  if (sd == 1) {
+    passes.push_back({0, SIGMA_GATHER | SIGMA_SUNSET | SIGMA_SCATTER});
+  } else if (sd == 2) {
+    passes.push_back({0, SIGMA_GATHER});
+    passes.push_back({1, SIGMA_SUNSET});
+    passes.push_back({0, SIGMA_INVERSE | SIGMA_SCATTER});
+  } else {
+    passes.push_back({0, SIGMA_GATHER});
+    passes.push_back({1, 0});
+    passes.push_back({2, SIGMA_SUNSET});
+    passes.push_back({1, SIGMA_INVERSE});
+    passes.push_back({0, SIGMA_INVERSE | SIGMA_SCATTER});
+  }
+  for (auto& ps : passes) {
+    a.axis = ps.first;
+    a.flags = ps.second;
+    hipLaunchKernelGGL(sigma_pass, grid, dim3(256), lds, c->stream, a);
+    LAUNCHCHK(c);
+  }
+  return KB_OK;
+}
+
+int launch_memint(kb_ctx* c, int t) {
+  MemintArgs a{};
+  a.H = c->H;
+  a.S = c->S;
+  a.kL2 = c->kL2;
+  a.m2 = c->m2;
+  a.NP = c->NP;
+  a.N = c->N;
+  a.V = c->V;
+  const bool full = t > c->N;
+  a.Rmax = full ? c->N : t - 1;
+  a.s_t = slot_of(c, t);
+  a.dead = full ? a.s_t : -1;
+  a.s_tm1 = slot_of(c, t - 1);
+  a.s_tm2 = slot_of(c, t - 2);
+  a.s_min = slot_of(c, c->tmin);
+  a.L = t - c->tmin;
+  a.tmin = c->tmin;
+  a.t = t;
+  a.dt = c->p.dt;
+  a.integral = c->p.pexp == KB_PEXP_LAMBDA_NLO;
+  const size_t lds = sizeof(double) * 4 * (size_t)c->N * MB;
+  hipLaunchKernelGGL(memint_step, dim3(c->NB), dim3(MEMINT_THREADS), lds, c->stream, a);
+  LAUNCHCHK(c);
+  return KB_OK;
+}
+
+int upload(kb_ctx* c, double* dst, const double* src, size_t n) {
+  HIPCHK(c, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  return KB_OK;
+}
+
+int download(kb_ctx* c, double* dst, const double* src, size_t n) {
+  HIPCHK(c, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return KB_OK;
+}
+
+void free_all(kb_ctx* c) {
+  for (void* ptr : {(void*)c->H, (void*)c->S, (void*)c->W, (void*)c->kL2, (void*)c->m2, (void*)c->partial,
+                    (void*)c->modes, (void*)c->vbuf, (void*)c->tw, (void*)c->flag})
+    if (ptr) (void)hipFree(ptr);
+  for (int s = 0; s < 3; ++s)
+    for (auto& ev : c->pending[s]) {
+      (void)hipEventDestroy(ev.a);
+      (void)hipEventDestroy(ev.b);
+    }
+  for (auto e : c->freeEv) (void)hipEventDestroy(e);
+  if (c->own) (void)hipStreamDestroy(c->own);
+}
+
+int dmalloc(kb_ctx* c, void** ptr, size_t bytes) {
+  hipError_t e = hipMalloc(ptr, bytes ? bytes : 8);
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    return seterr(c, KB_ERR_OOM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+  }
+  c->bytes += bytes;
+  return KB_OK;
+}
+
+int meas_grid(const kb_ctx* c) { return (c->V + MEAS_THREADS - 1) / MEAS_THREADS; }
+
+}  // namespace
+
+extern "C" {
+
+const char* kb_last_error(const kb_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int kb_create(const kb_params* p, kb_ctx** out) {
+  if (!p || !out) return KB_ERR_ARG;
+  *out = nullptr;
+  if (p->sdim < 1 || p->sdim > 3 || p->Nx < 1 || p->Nx > 128 || p->NstepsinMemory < 3 ||
+      p->NstepsinMemory > 640 || p->ONgroup < 1 || !(p->dt > 0))
+    return KB_ERR_ARG;
+  kb_ctx* c = new (std::nothrow) kb_ctx();
+  if (!c) return KB_ERR_OOM;
+  c->p = *p;
+  c->N = p->NstepsinMemory;
+  c->NP = (long)c->N * (c->N + 1) / 2;
+  c->V = 1;
+  for (int d = 0; d < p->sdim; ++d) c->V *= p->Nx;
+  c->nx = p->Nx;
+  c->ny = p->sdim >= 2 ? p->Nx : 1;
+  c->nz = p->sdim >= 3 ? p->Nx : 1;
+  c->NB = (c->V + MB - 1) / MB;
+  c->Vpad = c->NB * MB;
+  plan_fft(c->plan, p->Nx);
+  // pencils per workgroup: 16 complex elements per pencil row at least
+  c->fftP = p->Nx >= 64 ? 4 : (p->Nx >= 16 ? 16 : 32);
+  if (c->fftP > c->V / p->Nx) c->fftP = c->V / p->Nx;
+  int rc;
+  auto fail = [&](int code) {
+    free_all(c);
+    delete c;
+    return code;
+  };
+  if (hipSetDevice(p->device) != hipSuccess) return fail(KB_ERR_HIP);
+  if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) return fail(KB_ERR_HIP);
+  c->stream = c->own;
+  const size_t hcount = (size_t)c->NB * (size_t)c->NP * MB * 2;
+  if ((rc = dmalloc(c, (void**)&c->H, hcount * sizeof(double)))) return fail(rc);
+  if ((rc = dmalloc(c, (void**)&c->S, (size_t)c->NB * c->N * MB * 2 * sizeof(double)))) return fail(rc);
+  if ((rc = dmalloc(c, (void**)&c->W, (size_t)c->N * c->V * sizeof(double2)))) return fail(rc);
+  if ((rc = dmalloc(c, (void**)&c->kL2, (size_t)c->Vpad * sizeof(double)))) return fail(rc);
+  if ((rc = dmalloc(c, (void**)&c->m2, 8 * sizeof(double)))) return fail(rc);
+  if ((rc = dmalloc(c, (void**)&c->partial, (size_t)6 * meas_grid(c) * sizeof(double)))) return fail(rc);
+  if ((rc = dmalloc(c, (void**)&c->modes, (size_t)6 * c->V * sizeof(double)))) return fail(rc);
+  // staging: 4 lattices, or the 4 site-1 history vectors of kb_measure_history
+  const size_t vlen = (size_t)(c->V > c->N ? c->V : c->N);
+  if ((rc = dmalloc(c, (void**)&c->vbuf, 4 * vlen * sizeof(double)))) return fail(rc);
+  if ((rc = dmalloc(c, (void**)&c->tw, (size_t)p->Nx * sizeof(double2)))) return fail(rc);
+  if ((rc = dmalloc(c, (void**)&c->flag, sizeof(int) * 4))) return fail(rc);
+  {
+    const size_t lds = sizeof(double) * 4 * (size_t)c->N * MB;
+    hipError_t e = hipFuncSetAttribute((const void*)memint_step, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) {
+      seterr(c, KB_ERR_ARG, "memory window N=%d needs %zu bytes of LDS: %s", c->N, lds, hipGetErrorString(e));
+      return fail(KB_ERR_ARG);
+    }
+  }
+  if (hipMemsetAsync(c->H, 0, hcount * sizeof(double), c->stream) != hipSuccess) return fail(KB_ERR_HIP);
+  if (hipMemsetAsync(c->S, 0, (size_t)c->NB * c->N * MB * 2 * sizeof(double), c->stream) != hipSuccess)
+    return fail(KB_ERR_HIP);
+  if (hipMemsetAsync(c->m2, 0, 8 * sizeof(double), c->stream) != hipSuccess) return fail(KB_ERR_HIP);
+  // twiddles exp(-2 pi i k / Nx) and the default k^2 table
+  std::vector<double2> tw(p->Nx);
+  for (int k = 0; k < p->Nx; ++k) {
+    long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)p->Nx;
+    tw[k] = make_double2((double)cosl(a), (double)sinl(a));
+  }
+  std::vector<double> k2(c->Vpad, 0.0);
+  for (int i = 0; i < c->V; ++i) {
+    int rem = i;
+    double s2 = 0.0;
+    for (int d = 0; d < p->sdim; ++d) {
+      const int nd = rem % p->Nx;
+      rem /= p->Nx;
+      const double s = std::sin(M_PI * (double)nd / (double)p->Nx);
+      s2 += 4.0 * s * s;
+    }
+    k2[i] = s2;
+  }
+  if (hipMemcpy(c->tw, tw.data(), sizeof(double2) * p->Nx, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(KB_ERR_HIP);
+  if (hipMemcpy(c->kL2, k2.data(), sizeof(double) * c->Vpad, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(KB_ERR_HIP);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(KB_ERR_HIP);
+  *out = c;
+  return KB_OK;
+}
+
+int kb_destroy(kb_ctx* ctx) {
+  if (!ctx) return KB_OK;
+  (void)hipSetDevice(ctx->p.device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  free_all(ctx);
+  delete ctx;
+  return KB_OK;
+}
+
+int kb_info(const kb_ctx* ctx, long long* V, long long* device_bytes) {
+  if (!ctx) return KB_ERR_ARG;
+  if (V) *V = ctx->V;
+  if (device_bytes) *device_bytes = (long long)ctx->bytes;
+  return KB_OK;
+}
+
+int kb_get_kL2(kb_ctx* c, double* out) {
+  if (!c || !out) return KB_ERR_ARG;
+  return download(c, out, c->kL2, c->V);
+}
+
+int kb_set_kL2(kb_ctx* c, const double* in) {
+  if (!c || !in) return KB_ERR_ARG;
+  int rc = upload(c, c->kL2, in, c->V);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return KB_OK;
+}
+
+int kb_init_rows(kb_ctx* c, const double* F11, const double* F21, const double* F22) {
+  if (!c || !F11 || !F21 || !F22) return KB_ERR_ARG;
+  if (c->tmax != 0) return seterr(c, KB_ERR_STATE, "init: context already initialised");
+  int rc;
+  double* d = c->vbuf;
+  if ((rc = upload(c, d, F11, c->V)) || (rc = upload(c, d + c->V, F21, c->V)) ||
+      (rc = upload(c, d + 2 * c->V, F22, c->V)))
+    return rc;
+  expand(c);
+  expand(c);
+  const int blocks = (c->V + 255) / 256;
+  hipLaunchKernelGGL(init_rows_kernel, dim3(blocks), dim3(256), 0, c->stream, c->H, c->NP, c->V, c->N, d,
+                     d + c->V, d + 2 * c->V, c->p.dt);
+  LAUNCHCHK(c);
+  if ((rc = launch_hartree(c, 2, 2))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->initialised = true;
+  return KB_OK;
+}
+
+int kb_init_particles(kb_ctx* c, const double* n, double initmass) {
+  if (!c || !n) return KB_ERR_ARG;
+  if (c->tmax != 0) return seterr(c, KB_ERR_STATE, "init: context already initialised");
+  int rc;
+  double* d = c->vbuf;
+  if ((rc = upload(c, d + 3 * c->V, n, c->V))) return rc;
+  const int blocks = (c->V + 255) / 256;
+  hipLaunchKernelGGL(init_particles_kernel, dim3(blocks), dim3(256), 0, c->stream, c->kL2, d + 3 * c->V,
+                     initmass, c->p.dt, c->V, d, d + c->V, d + 2 * c->V);
+  LAUNCHCHK(c);
+  expand(c);
+  expand(c);
+  hipLaunchKernelGGL(init_rows_kernel, dim3(blocks), dim3(256), 0, c->stream, c->H, c->NP, c->V, c->N, d,
+                     d + c->V, d + 2 * c->V, c->p.dt);
+  LAUNCHCHK(c);
+  if ((rc = launch_hartree(c, 2, 2))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->initialised = true;
+  return KB_OK;
+}
+
+int kb_step(kb_ctx* c, int t) {
+  if (!c) return KB_ERR_ARG;
+  if (!c->initialised) return seterr(c, KB_ERR_STATE, "step before init");
+  if (t != c->tmax + 1 || t < 3)
+    return seterr(c, KB_ERR_WINDOW, "evolve: t=%d but window is [%d,%d]", t, c->tmin, c->tmax);
+  expand(c);
+  int rc;
+  if (c->p.pexp == KB_PEXP_LAMBDA_NLO) {
+    StageTimer st(c, 1);
+    if ((rc = launch_sigma(c, t))) return rc;
+  }
+  {
+    StageTimer st(c, 2);
+    if ((rc = launch_hartree(c, t - 1, t - 1))) return rc;
+  }
+  {
+    StageTimer st(c, 0);
+    if ((rc = launch_memint(c, t))) return rc;
+  }
+  return KB_OK;
+}
+
+int kb_window(const kb_ctx* c, int* tmin, int* tmax) {
+  if (!c) return KB_ERR_ARG;
+  if (tmin) *tmin = c->tmin;
+  if (tmax) *tmax = c->tmax;
+  return KB_OK;
+}
+
+int kb_hartree(kb_ctx* c, double* m2) {
+  if (!c || !m2) return KB_ERR_ARG;
+  return download(c, m2, c->m2, 1);
+}
+
+int kb_measure(kb_ctx* c, int t, kb_meas* out, double* F, double* ddF, double* r, double* n, double* omega,
+               double* cc) {
+  if (!c || !out) return KB_ERR_ARG;
+  if (!c->initialised) return seterr(c, KB_ERR_STATE, "measure before init");
+  if (t != c->tmax || !inwin(c, t - 1)) return seterr(c, KB_ERR_WINDOW, "measure: t=%d, window [%d,%d]", t, c->tmin, c->tmax);
+  MeasArgs a{};
+  a.H = c->H;
+  a.S = c->S;
+  a.kL2 = c->kL2;
+  a.NP = c->NP;
+  a.N = c->N;
+  a.V = c->V;
+  a.s_t = slot_of(c, t);
+  a.s_tm1 = slot_of(c, t - 1);
+  a.tmin = c->tmin;
+  a.t = t;
+  a.dt = c->p.dt;
+  a.integral = c->p.pexp == KB_PEXP_LAMBDA_NLO;
+  const bool want = F || ddF || r || n || omega || cc;
+  if (want) {
+    a.outF = c->modes;
+    a.outddF = c->modes + c->V;
+    a.outr = c->modes + 2 * c->V;
+    a.outn = c->modes + 3 * c->V;
+    a.outom = c->modes + 4 * c->V;
+    a.outc = c->modes + 5 * c->V;
+  }
+  a.partial = c->partial;
+  const int g = meas_grid(c);
+  hipLaunchKernelGGL(measure_kernel, dim3(g), dim3(MEAS_THREADS), 0, c->stream, a);
+  LAUNCHCHK(c);
+  std::vector<double> part((size_t)6 * g);
+  int rc;
+  if ((rc = download(c, part.data(), c->partial, part.size()))) return rc;
+  double m2 = 0;
+  if ((rc = download(c, &m2, c->m2, 1))) return rc;
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < g; ++b)
+    for (int k = 0; k < 6; ++k) s[k] += part[(size_t)b * 6 + k];
+  const double dt = c->p.dt;
+  const double H = c->p.Lambda * (double)(c->p.ONgroup + 2) / (6.0 * (double)c->p.ONgroup);
+  const double ivol = 1.0 / (double)c->V;
+  out->time = (double)(t - 1) * dt * c->p.Mass;
+  out->E_kin = s[0];
+  // E_pot = sum_p 1/2 (Mass2 + kL2 + 1/2 ivol H sum_q F) F  (Measurement.jl:135-149)
+  out->E_pot = 0.5 * (c->p.Mass2 + 0.5 * ivol * H * s[1]) * s[1] + 0.5 * s[2];
+  double Emem = 0.0;
+  if (t != 2 && a.integral) {
+    Emem = 0.5 * 0.25 * dt * s[3];
+    if (t > 3) Emem += 0.25 * dt * s[4];
+  }
+  out->E_mem = Emem;
+  out->E = out->E_kin + out->E_pot + out->E_mem;
+  out->M2 = m2;
+  out->n_unstable = (int)s[5];
+  double* dsts[6] = {F, ddF, r, n, omega, cc};
+  for (int k = 0; k < 6; ++k)
+    if (dsts[k] && (rc = download(c, dsts[k], c->modes + (size_t)k * c->V, c->V))) return rc;
+  return KB_OK;
+}
+
+int kb_measure_history(kb_ctx* c, int t, double* SigF, double* Sigr, double* Fonet, double* ronet) {
+  if (!c) return KB_ERR_ARG;
+  if (t != c->tmax) return seterr(c, KB_ERR_WINDOW, "history: t=%d, window [%d,%d]", t, c->tmin, c->tmax);
+  const int L = t - c->tmin;
+  if (L <= 0) return KB_OK;
+  hipLaunchKernelGGL(history_kernel, dim3((L + 255) / 256), dim3(256), 0, c->stream, c->H, c->S, c->NP, c->N,
+                     c->tmin, t, c->vbuf);
+  LAUNCHCHK(c);
+  std::vector<double> h((size_t)4 * L);
+  int rc;
+  if ((rc = download(c, h.data(), c->vbuf, h.size()))) return rc;
+  double* dsts[4] = {SigF, Sigr, Fonet, ronet};
+  for (int k = 0; k < 4; ++k)
+    if (dsts[k]) memcpy(dsts[k], h.data() + (size_t)k * L, sizeof(double) * L);
+  return KB_OK;
+}
+
+int kb_get_entry(kb_ctx* c, int which, int i, int j, double* out) {
+  if (!c || !out || (which != 0 && which != 1)) return KB_ERR_ARG;
+  if (!inwin(c, i) || !inwin(c, j))
+    return seterr(c, KB_ERR_WINDOW, "get: (%d,%d) outside window [%d,%d]", i, j, c->tmin, c->tmax);
+  const int si = slot_of(c, i), sj = slot_of(c, j);
+  const double sg = which ? (si >= sj ? 1.0 : -1.0) : 1.0;
+  hipLaunchKernelGGL(gather_pair, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->H, c->NP, c->V,
+                     pair_index(si, sj), which, sg, c->vbuf);
+  LAUNCHCHK(c);
+  return download(c, out, c->vbuf, c->V);
+}
+
+int kb_set_entry(kb_ctx* c, int which, int i, int j, const double* in) {
+  if (!c || !in || (which != 0 && which != 1)) return KB_ERR_ARG;
+  if (!inwin(c, i) || !inwin(c, j))
+    return seterr(c, KB_ERR_WINDOW, "set: (%d,%d) outside window [%d,%d]", i, j, c->tmin, c->tmax);
+  const int si = slot_of(c, i), sj = slot_of(c, j);
+  const double sg = which ? (si >= sj ? 1.0 : -1.0) : 1.0;
+  int rc;
+  if ((rc = upload(c, c->vbuf, in, c->V))) return rc;
+  hipLaunchKernelGGL(scatter_pair, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->H, c->NP, c->V,
+                     pair_index(si, sj), which, sg, c->vbuf);
+  LAUNCHCHK(c);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return KB_OK;
+}
+
+int kb_get_sigma(kb_ctx* c, int which, int tp, double* out) {
+  if (!c || !out || (which != 0 && which != 1)) return KB_ERR_ARG;
+  if (!inwin(c, tp)) return seterr(c, KB_ERR_WINDOW, "sigma: %d outside window [%d,%d]", tp, c->tmin, c->tmax);
+  hipLaunchKernelGGL(gather_sigma, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->S, c->N, c->V,
+                     slot_of(c, tp), which, c->vbuf);
+  LAUNCHCHK(c);
+  return download(c, out, c->vbuf, c->V);
+}
+
+int kb_check_finite(kb_ctx* c) {
+  if (!c) return KB_ERR_ARG;
+  HIPCHK(c, hipMemsetAsync(c->flag, 0, sizeof(int), c->stream));
+  const size_t count = (size_t)c->NB * (size_t)c->NP * MB * 2;
+  hipLaunchKernelGGL(finite_kernel, dim3(4096), dim3(256), 0, c->stream, c->H, count, c->flag);
+  LAUNCHCHK(c);
+  int f = 0;
+  HIPCHK(c, hipMemcpyAsync(&f, c->flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (f) return seterr(c, KB_ERR_NONFINITE, "non-finite value in the two-time window");
+  return KB_OK;
+}
+
+int kb_fill_free_window(kb_ctx* c, int tmax, const double* n, double mass) {
+  if (!c) return KB_ERR_ARG;
+  if (tmax < c->N) return seterr(c, KB_ERR_ARG, "fill: tmax=%d < NstepsinMemory=%d", tmax, c->N);
+  int rc;
+  const double* dn = nullptr;
+  if (n) {
+    if ((rc = upload(c, c->vbuf, n, c->V))) return rc;
+    dn = c->vbuf;
+  }
+  const long total = c->NP * (long)c->NB * MB;
+  const long blocks = (total + 255) / 256;
+  hipLaunchKernelGGL(fill_free_kernel, dim3((unsigned)blocks), dim3(256), 0, c->stream, c->H, c->NP, c->V, c->N,
+                     tmax, c->kL2, dn, mass, c->p.dt, c->NP);
+  LAUNCHCHK(c);
+  c->tmin = tmax - c->N + 1;
+  c->tmax = tmax;
+  c->initialised = true;
+  if ((rc = launch_hartree(c, tmax, tmax))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return KB_OK;
+}
+
+int kb_synchronize(kb_ctx* c) {
+  if (!c) return KB_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return KB_OK;
+}
+
+int kb_set_stream(kb_ctx* c, void* s) {
+  if (!c) return KB_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->stream = s ? (hipStream_t)s : c->own;
+  return KB_OK;
+}
+
+int kb_enable_timing(kb_ctx* c, int on) {
+  if (!c) return KB_ERR_ARG;
+  c->timing = on != 0;
+  return KB_OK;
+}
+
+int kb_timing(kb_ctx* c, int stage, double* total_ms, long long* count) {
+  if (!c || stage < 0 || stage > 2) return KB_ERR_ARG;
+  int rc = collect_timing(c);
+  if (rc) return rc;
+  if (total_ms) *total_ms = c->tsum[stage];
+  if (count) *count = c->tcount[stage];
+  return KB_OK;
+}
+
+int kb_reset_timing(kb_ctx* c) {
+  if (!c) return KB_ERR_ARG;
+  int rc = collect_timing(c);
+  if (rc) return rc;
+  for (int s = 0; s < 3; ++s) {
+    c->tsum[s] = 0;
+    c->tcount[s] = 0;
+  }
+  return KB_OK;
+}
+
+}  // extern "C"

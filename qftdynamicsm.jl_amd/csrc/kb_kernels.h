@@ -1,0 +1,86 @@
+// kb_kernels.h -- kernel argument blocks and declarations (libqftkb).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kb_device.h"
+
+namespace qftkb {
+
+constexpr int FFT_MAXF = 24;
+
+struct FFTPlan {
+  int n;
+  int nf;
+  int fac[FFT_MAXF];
+};
+
+enum SigmaFlags {
+  SIGMA_GATHER = 1,   // read F(t-1,t'), rho(t-1,t') from the history (axis 0, forward)
+  SIGMA_SUNSET = 2,   // after the forward transform: pointwise sunset, inverse transform
+  SIGMA_SCATTER = 4,  // write SigF/Sigr to the self-energy ring (axis 0, inverse)
+  SIGMA_INVERSE = 8,  // transform direction of a plain pass
+};
+
+struct SigmaArgs {
+  const double* H;
+  double* S;
+  double2* W;  // workspace rows [L][V] complex
+  const double2* tw;
+  FFTPlan plan;
+  long NP;
+  int N, V, n, nx, ny, nz;
+  int axis, flags, P, npencils;
+  int tmin, s_tm1;
+  double preF, preR;
+};
+
+struct MemintArgs {
+  double* H;
+  const double* S;
+  const double* kL2;
+  const double* m2;
+  long NP;
+  int N, V;
+  int Rmax;  // slots [0, Rmax) are in use
+  int dead;  // slot being overwritten by row t (full window), else -1
+  int s_t, s_tm1, s_tm2, s_min;
+  int L;     // number of live times t - tmin
+  int tmin, t;
+  double dt;
+  int integral;  // 0 for LambdaLO
+};
+
+struct MeasArgs {
+  const double* H;
+  const double* S;
+  const double* kL2;
+  long NP;
+  int N, V;
+  int s_t, s_tm1;
+  int tmin, t;
+  double dt;
+  int integral;
+  double *outF, *outddF, *outr, *outn, *outom, *outc;
+  double* partial;
+};
+
+__global__ void sigma_pass(SigmaArgs a);
+__global__ void hartree_kernel(const double* H, long NP, int V, long pair, double hfac, double Mass2, double* out);
+__global__ void memint_step(MemintArgs a);
+__global__ void measure_kernel(MeasArgs a);
+__global__ void history_kernel(const double* H, const double* S, long NP, int N, int tmin, int t, double* out);
+__global__ void gather_pair(const double* H, long NP, int V, long pair, int comp, double sign, double* out);
+__global__ void scatter_pair(double* H, long NP, int V, long pair, int comp, double sign, const double* in);
+__global__ void gather_sigma(const double* S, int N, int V, int slot, int comp, double* out);
+__global__ void init_rows_kernel(double* H, long NP, int V, int N, const double* F11, const double* F21,
+                                 const double* F22, double dt);
+__global__ void init_particles_kernel(const double* kL2, const double* n, double m, double dt, int V, double* F11,
+                                      double* F21, double* F22);
+__global__ void fill_free_kernel(double* H, long NP, int V, int N, int tmax, const double* kL2, const double* n,
+                                 double mass, double dt, long npairs_total);
+__global__ void finite_kernel(const double* H, size_t count, int* flag);
+
+constexpr int MEMINT_THREADS = 512;
+constexpr int MEAS_THREADS = 256;
+
+}  // namespace qftkb

@@ -1,0 +1,642 @@
+// kb_kernels.hip -- gfx950 kernels of the 2PI Kadanoff-Baym step.
+//
+//   sigma_pass     self-energy: batched pencil FFTs staged in LDS with the
+//                  sunset products fused into the last forward pass
+//                  (TwoPIScalar_Evolution.jl:16-31)
+//   hartree        Hartree mass^2 reduction (TwoPIScalar_Initialization.jl:
+//                  207-218)
+//   memint_step    memory integrals + leapfrog + diagonal of one KB step,
+//                  one workgroup per mode block, history read once
+//                  (TwoPIScalar_Evolution.jl:766-782, 846-854, 890-915,
+//                  951-974)
+//   measure        equal-time observables and energies
+//                  (TwoPIScalar_Measurement.jl:31-99, 135-174)
+// plus small gather/scatter kernels for the host API.
+#include "kb_kernels.h"
+
+#include <math.h>
+
+namespace qftkb {
+
+// ---------------------------------------------------------------------------
+// LDS FFT (Stockham, mixed radix)
+// ---------------------------------------------------------------------------
+__device__ inline double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+// exp(-+ 2 pi i k / n) from the table, conjugated for the inverse
+__device__ inline double2 twid(const double2* __restrict__ tw, int k, bool inv) {
+  double2 w = tw[k];
+  if (inv) w.y = -w.y;
+  return w;
+}
+
+template <int R>
+__device__ inline void small_dft(double2 (&v)[R], const double2* __restrict__ tw, int n, bool inv) {
+  if constexpr (R == 2) {
+    double2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    // forward: multiply by -i for the odd quarter; inverse: +i
+    double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    double2 b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
+    double2 jb1 = inv ? make_double2(-b1.y, b1.x) : make_double2(b1.y, -b1.x);
+    v[0] = cadd(a0, b0);
+    v[2] = csub(a0, b0);
+    v[1] = cadd(a1, jb1);
+    v[3] = csub(a1, jb1);
+  } else {
+    double2 o[R];
+    const int m = n / R;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      double2 s = v[0];
+#pragma unroll
+      for (int r = 1; r < R; ++r) s = cadd(s, cmul(v[r], twid(tw, ((q * r) % R) * m, inv)));
+      o[q] = s;
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = o[q];
+  }
+}
+
+template <int R>
+__device__ inline void stockham_stage(const double2* __restrict__ src, double2* __restrict__ dst, int n,
+                                      int P, int Ns, const double2* __restrict__ tw, bool inv) {
+  const int m = n / R;
+  const int step = n / (Ns * R);
+  for (int idx = threadIdx.x; idx < P * m; idx += blockDim.x) {
+    const int k = idx / m;
+    const int j = idx - k * m;
+    const double2* s = src + k * n;
+    double2* d = dst + k * n;
+    const int kk = j % Ns;
+    double2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      double2 x = s[j + r * m];
+      v[r] = r == 0 ? x : cmul(x, twid(tw, (r * kk * step) % n, inv));
+    }
+    small_dft<R>(v, tw, n, inv);
+    const int base = (j / Ns) * Ns * R + kk;
+#pragma unroll
+    for (int q = 0; q < R; ++q) d[base + q * Ns] = v[q];
+  }
+}
+
+// radix > 8: direct sum per output element
+__device__ inline void direct_stage(const double2* __restrict__ src, double2* __restrict__ dst, int n, int P,
+                                    int Ns, int R, const double2* __restrict__ tw, bool inv) {
+  const int m = n / R;
+  const int step = n / (Ns * R);
+  for (int idx = threadIdx.x; idx < P * n; idx += blockDim.x) {
+    const int k = idx / n;
+    const int rem = idx - k * n;
+    const int j = rem / R;
+    const int q = rem - j * R;
+    const double2* s = src + k * n;
+    const int kk = j % Ns;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int r = 0; r < R; ++r) {
+      double2 x = s[j + r * m];
+      if (r) x = cmul(x, twid(tw, (int)(((long)r * kk * step) % n), inv));
+      acc = cadd(acc, cmul(x, twid(tw, ((q * r) % R) * m, inv)));
+    }
+    dst[k * n + (j / Ns) * Ns * R + kk + q * Ns] = acc;
+  }
+}
+
+// Transforms P pencils of length n held in a (ping-pong with b).  Returns
+// the buffer holding the result.  Ends with a barrier.
+__device__ double2* lds_fft(double2* a, double2* b, int n, int P, const FFTPlan& pl,
+                            const double2* __restrict__ tw, bool inv) {
+  int Ns = 1;
+  for (int s = 0; s < pl.nf; ++s) {
+    const int R = pl.fac[s];
+    switch (R) {
+      case 2: stockham_stage<2>(a, b, n, P, Ns, tw, inv); break;
+      case 3: stockham_stage<3>(a, b, n, P, Ns, tw, inv); break;
+      case 4: stockham_stage<4>(a, b, n, P, Ns, tw, inv); break;
+      case 5: stockham_stage<5>(a, b, n, P, Ns, tw, inv); break;
+      case 7: stockham_stage<7>(a, b, n, P, Ns, tw, inv); break;
+      default: direct_stage(a, b, n, P, Ns, R, tw, inv); break;
+    }
+    __syncthreads();
+    double2* t = a;
+    a = b;
+    b = t;
+    Ns *= R;
+  }
+  return a;
+}
+
+// ---------------------------------------------------------------------------
+// Self-energy pass.  One workgroup: row q (t' = tmin + q) x P pencils along
+// `axis`.  See SigmaArgs for the flags.
+// ---------------------------------------------------------------------------
+__device__ inline int site_of(const SigmaArgs& a, int pencil, int j) {
+  // dims (nx, ny, nz); pencil enumerates the two non-transform coordinates
+  if (a.axis == 0) {
+    const int iy = pencil % a.ny, iz = pencil / a.ny;
+    return j + a.nx * (iy + a.ny * iz);
+  } else if (a.axis == 1) {
+    const int ix = pencil % a.nx, iz = pencil / a.nx;
+    return ix + a.nx * (j + a.ny * iz);
+  } else {
+    const int ix = pencil % a.nx, iy = pencil / a.nx;
+    return ix + a.nx * (iy + a.ny * j);
+  }
+}
+
+__device__ inline int neg_site(const SigmaArgs& a, int s) {
+  const int ix = s % a.nx;
+  const int iy = (s / a.nx) % a.ny;
+  const int iz = s / (a.nx * a.ny);
+  const int jx = (a.nx - ix) % a.nx, jy = (a.ny - iy) % a.ny, jz = (a.nz - iz) % a.nz;
+  return jx + a.nx * (jy + a.ny * jz);
+}
+
+__global__ __launch_bounds__(256) void sigma_pass(SigmaArgs a) {
+  extern __shared__ double2 lds[];
+  const int n = a.n;
+  const int P = a.P;
+  double2* A = lds;
+  double2* B = lds + P * n;
+  const int q = blockIdx.y;
+  const int pen0 = blockIdx.x * P;
+  const int tp = a.tmin + q;
+  const int slot = (tp - 1) % a.N;
+  const size_t rowoff = (size_t)q * a.V;
+  const bool contiguous = a.axis == 0;
+
+  // ---- load ----
+  for (int e = threadIdx.x; e < P * n; e += blockDim.x) {
+    int k, j;
+    if (contiguous) {
+      k = e / n;
+      j = e - k * n;
+    } else {
+      j = e / P;
+      k = e - j * P;
+    }
+    const int pencil = pen0 + k;
+    double2 z = make_double2(0.0, 0.0);
+    if (pencil < a.npencils) {
+      const int s = site_of(a, pencil, j);
+      if (a.flags & SIGMA_GATHER) {
+        // physical F(t-1,t') and rho(t-1,t'); rho sign from the slot order
+        const long pr = pair_index(a.s_tm1, slot);
+        const double sg = a.s_tm1 >= slot ? 1.0 : -1.0;
+        const int s2 = neg_site(a, s);
+        const double* h1 = a.H + hoff(s / MB, pr, s % MB, a.NP);
+        const double* h2 = a.H + hoff(s2 / MB, pr, s2 % MB, a.NP);
+        // real(rfft(X)) only sees the inversion-even part of X
+        z.x = 0.5 * (h1[0] + h2[0]);
+        z.y = 0.5 * (h1[1] + h2[1]) * sg;
+      } else {
+        z = a.W[rowoff + s];
+      }
+    }
+    A[k * n + j] = z;
+  }
+  __syncthreads();
+
+  // ---- transform(s) ----
+  double2* res = lds_fft(A, B, n, P, a.plan, a.tw, (a.flags & SIGMA_INVERSE) != 0);
+  if (a.flags & SIGMA_SUNSET) {
+    double2* other = res == A ? B : A;
+    for (int e = threadIdx.x; e < P * n; e += blockDim.x) {
+      const double2 z = res[e];
+      const double Fx = z.x, rx = z.y;
+      const double FF = Fx * Fx, rr = rx * rx;
+      res[e] = make_double2(Fx * (FF - 0.75 * rr), rx * (FF - (1.0 / 12.0) * rr));
+    }
+    __syncthreads();
+    res = lds_fft(res, other, n, P, a.plan, a.tw, true);
+  }
+
+  // ---- store ----
+  for (int e = threadIdx.x; e < P * n; e += blockDim.x) {
+    int k, j;
+    if (contiguous) {
+      k = e / n;
+      j = e - k * n;
+    } else {
+      j = e / P;
+      k = e - j * P;
+    }
+    const int pencil = pen0 + k;
+    if (pencil >= a.npencils) continue;
+    const int s = site_of(a, pencil, j);
+    const double2 z = res[k * n + j];
+    if (a.flags & SIGMA_SCATTER) {
+      double* o = a.S + soff(s / MB, slot, s % MB, a.N);
+      o[0] = a.preF * z.x;
+      o[1] = a.preR * z.y;
+    } else {
+      a.W[rowoff + s] = z;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Hartree mass: m2 = Mass2 + H ivol sum_p F(t-1,t-1;p)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void hartree_kernel(const double* __restrict__ H, long NP, int V, long pair,
+                                                       double hfac, double Mass2, double* __restrict__ out) {
+  __shared__ double red[1024];
+  double acc = 0.0;
+  for (int p = threadIdx.x; p < V; p += blockDim.x) acc += H[hoff(p / MB, pair, p % MB, NP)];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double h = red[0];
+    h *= hfac;
+    h += Mass2;
+    out[0] = h;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Memory integrals + leapfrog + diagonal, one workgroup per mode block.
+//
+// With rho antisymmetric and F symmetric, the reference's
+//   R_r(t') = -sum_{z=t'+1}^{t-2} Sr(z) r(z,t')                    (Evolution.jl:846-854)
+//   R_F(t') = -sum_{z=tmin}^{t-2} w_z Sr(z) F(z,t')
+//             +sum_{z=tmin}^{t'-1} w_z SF(z) r(z,t')                (Evolution.jl:766-782)
+// (w = 1/2 at tmin, 1 otherwise; S(z) = Sigma(t-1,z)) make every stored
+// entry (a,b), a>b, feed two output rows: with u = m w Sr, v = w SF,
+// ms = m Sr (m = 0 at z = t-1):
+//   R_F(a) += -u(b) F - v(b) r,  R_F(b) += -u(a) F,  R_r(b) += -ms(a) r,
+// and the diagonal (a,a) feeds R_F(a) += -u(a) F.  The kernel walks the
+// slot triangle once in 8x8 tiles and accumulates both rows into LDS.
+// In slot space, pair (hi, lo) has time(hi) < time(lo) exactly when
+// lo < s_min <= hi; those pairs swap the roles of a and b.
+// ---------------------------------------------------------------------------
+constexpr int MI_THREADS = 512;
+constexpr int MI_RS = 8;  // rows per work item
+constexpr int MI_CB = 8;  // columns per work item
+
+__global__ __launch_bounds__(MI_THREADS) void memint_step(MemintArgs a) {
+  extern __shared__ double smem[];
+  const int N = a.N;
+  double* sF = smem;            // SigF per slot, lane
+  double* sR = sF + N * MB;     // Sigr
+  double* accF = sR + N * MB;   // R_F
+  double* accR = accF + N * MB; // R_r
+  __shared__ double red[MI_THREADS];
+  __shared__ double diagF[2][MB];
+
+  const int mb = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int lane = tid & (MB - 1);
+  const int worker = tid >> 3;
+  const int nworkers = blockDim.x >> 3;
+  const int Rmax = a.Rmax;
+  const int dead = a.dead;
+  const int smin = a.s_min;
+  const int L = a.L;
+
+  {
+    const double* Sb = a.S + soff(mb, 0, 0, N);
+    for (int i = tid; i < N * MB; i += blockDim.x) {
+      sF[i] = a.integral ? Sb[2 * i] : 0.0;
+      sR[i] = a.integral ? Sb[2 * i + 1] : 0.0;
+      accF[i] = 0.0;
+      accR[i] = 0.0;
+    }
+  }
+  __syncthreads();
+
+  const double* __restrict__ Hb = a.H + hoff(mb, 0, 0, a.NP);
+
+  if (a.integral) {
+    int it = 0;
+    const int nJ = (Rmax + MI_CB - 1) / MI_CB;
+    for (int J = 0; J < nJ; ++J) {
+      const int c0 = J * MI_CB;
+      const int nstrips = (Rmax - c0 + MI_RS - 1) / MI_RS;
+      for (int k = 0; k < nstrips; ++k, ++it) {
+        if (it % nworkers != worker) continue;
+        // ---- column coefficients ----
+        double uC[MI_CB], vC[MI_CB], msC[MI_CB], cF[MI_CB], cR[MI_CB];
+        bool okC[MI_CB], phiC[MI_CB];
+#pragma unroll
+        for (int jc = 0; jc < MI_CB; ++jc) {
+          const int c = c0 + jc;
+          const bool ok = c < Rmax && c != dead;
+          okC[jc] = ok;
+          phiC[jc] = c < smin;
+          uC[jc] = vC[jc] = msC[jc] = 0.0;
+          if (ok) {
+            const int rel = (c - smin + N) % N;
+            const double w = rel == 0 ? 0.5 : 1.0;
+            const double m = rel == L - 1 ? 0.0 : 1.0;
+            const double s = sR[c * MB + lane], f = sF[c * MB + lane];
+            uC[jc] = m * w * s;
+            vC[jc] = w * f;
+            msC[jc] = m * s;
+          }
+          cF[jc] = cR[jc] = 0.0;
+        }
+        const int r0 = c0 + k * MI_RS;
+        const int r1 = min(r0 + MI_RS, Rmax);
+        for (int r = r0; r < r1; ++r) {
+          if (r == dead) continue;
+          const int rel = (r - smin + N) % N;
+          const double w = rel == 0 ? 0.5 : 1.0;
+          const double m = rel == L - 1 ? 0.0 : 1.0;
+          const double s = sR[r * MB + lane], f = sF[r * MB + lane];
+          const double ur = m * w * s, vr = w * f, msr = m * s;
+          const bool eps = r >= smin;
+          const long pb = (long)r * (r + 1) / 2 + c0;
+          double rowF = 0.0, rowR = 0.0;
+          if (r >= c0 + MI_CB) {
+            double2 e[MI_CB];
+#pragma unroll
+            for (int jc = 0; jc < MI_CB; ++jc)
+              e[jc] = okC[jc] ? *reinterpret_cast<const double2*>(Hb + ((size_t)(pb + jc) * MB + lane) * 2)
+                              : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int jc = 0; jc < MI_CB; ++jc) {
+              const double Fv = e[jc].x, Pv = e[jc].y;
+              const bool neg = eps && phiC[jc];
+              rowF += -uC[jc] * Fv - (neg ? 0.0 : vC[jc]) * Pv;
+              rowR += (neg ? msC[jc] : 0.0) * Pv;
+              cF[jc] += -ur * Fv + (neg ? vr : 0.0) * Pv;
+              cR[jc] += -(neg ? 0.0 : msr) * Pv;
+            }
+          } else {
+            // tile crossing the diagonal: columns c0 .. r
+            for (int jc = 0; jc < MI_CB; ++jc) {
+              const int c = c0 + jc;
+              if (c > r) break;
+              if (!okC[jc]) continue;
+              const double2 e = *reinterpret_cast<const double2*>(Hb + ((size_t)(pb + jc) * MB + lane) * 2);
+              const double Fv = e.x, Pv = e.y;
+              if (c == r) {
+                rowF += -ur * Fv;
+              } else {
+                const bool neg = eps && phiC[jc];
+                rowF += -uC[jc] * Fv - (neg ? 0.0 : vC[jc]) * Pv;
+                rowR += (neg ? msC[jc] : 0.0) * Pv;
+                cF[jc] += -ur * Fv + (neg ? vr : 0.0) * Pv;
+                cR[jc] += -(neg ? 0.0 : msr) * Pv;
+              }
+            }
+          }
+          atomicAdd(&accF[r * MB + lane], rowF);
+          atomicAdd(&accR[r * MB + lane], rowR);
+        }
+#pragma unroll
+        for (int jc = 0; jc < MI_CB; ++jc)
+          if (okC[jc]) {
+            atomicAdd(&accF[(c0 + jc) * MB + lane], cF[jc]);
+            atomicAdd(&accR[(c0 + jc) * MB + lane], cR[jc]);
+          }
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- leapfrog of row t (evolve_r!, evolve_F!: Evolution.jl:890-915) ----
+  const int p = mb * MB + lane;
+  const bool real_site = p < a.V;
+  const double m2 = a.m2[0];
+  const double dt = a.dt;
+  const double dt2 = dt * dt, dt3 = dt * dt * dt;
+  const double Om = (a.kL2[p] + m2) * dt2;  // setomega2values!, Evolution.jl:918-921
+  double dsum = 0.0;
+  double* __restrict__ Hw = a.H + hoff(mb, 0, 0, a.NP);
+  for (int x = worker; x < Rmax; x += nworkers) {
+    if (x == dead) continue;
+    const int rel = (x - smin + N) % N;
+    const int tx = a.tmin + rel;
+    const double2 e1 = *reinterpret_cast<const double2*>(Hw + ((size_t)pair_index(a.s_tm1, x) * MB + lane) * 2);
+    const double2 e2 = *reinterpret_cast<const double2*>(Hw + ((size_t)pair_index(a.s_tm2, x) * MB + lane) * 2);
+    const double r1 = e1.y * (a.s_tm1 >= x ? 1.0 : -1.0);
+    const double r2 = e2.y * (a.s_tm2 >= x ? 1.0 : -1.0);
+    const double RHSF = accF[x * MB + lane] * dt3;
+    const double Fn = (2.0 - Om) * e1.x - e2.x + RHSF;
+    double rn;
+    if (tx == a.t - 1) {
+      rn = dt;
+    } else {
+      const double RHSr = accR[x * MB + lane] * dt3;
+      rn = (2.0 - Om) * r1 - r2 + RHSr;
+    }
+    if (real_site) {
+      double2 o = make_double2(Fn, rn * (a.s_t >= x ? 1.0 : -1.0));
+      *reinterpret_cast<double2*>(Hw + ((size_t)pair_index(a.s_t, x) * MB + lane) * 2) = o;
+    }
+    // terms of getRHS_F!(t,t) (Evolution.jl:973) over the fresh row
+    const double w = rel == 0 ? 0.5 : 1.0;
+    const double m = rel == L - 1 ? 0.0 : 1.0;
+    dsum += -(m * w * sR[x * MB + lane]) * Fn - (w * sF[x * MB + lane]) * rn;
+    if (tx == a.t - 1) diagF[0][lane] = Fn;
+    if (tx == a.t - 2) diagF[1][lane] = Fn;
+  }
+  red[tid] = dsum;
+  __syncthreads();
+  if (tid < MB) {
+    double RF = 0.0;
+    for (int wk = 0; wk < nworkers; ++wk) RF += red[wk * MB + tid];
+    const double Ftt = (2.0 - Om) * diagF[0][tid] - diagF[1][tid] + RF * dt3;
+    if (real_site) {
+      *reinterpret_cast<double2*>(Hw + ((size_t)pair_index(a.s_t, a.s_t) * MB + tid) * 2) =
+          make_double2(Ftt, 0.0);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Measurement (createMeasurement / getE_kin / getE_pot / getE_mem)
+// partial sums per workgroup: [0] sum ddF/2, [1] sum F, [2] sum kL2 F,
+// [3] E_mem boundary integrand, [4] E_mem interior integrand, [5] n_unstable
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void measure_kernel(MeasArgs a) {
+  __shared__ double red[6][256];
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  if (p < a.V) {
+    const int mb = p / MB, lane = p % MB;
+    const double dt = a.dt;
+    const double Ftt = a.H[hoff(mb, pair_index(a.s_t, a.s_t), lane, a.NP)];
+    const double Fmt = a.H[hoff(mb, pair_index(a.s_tm1, a.s_t), lane, a.NP)];
+    const double* emm = a.H + hoff(mb, pair_index(a.s_tm1, a.s_tm1), lane, a.NP);
+    const double Fmm = emm[0];
+    const double ddF = (Ftt - 2.0 * Fmt + Fmm) / (dt * dt);
+    const double om = (ddF / Fmm) > 0 ? sqrt(ddF / Fmm) : 0.0;
+    double c;
+    int bad = 0;
+    if (0.25 * (dt * dt) * (om * om) < 1.0) {
+      c = sqrt(1.0 - 0.25 * (dt * dt) * (om * om));
+    } else {
+      c = 1.0;
+      bad = 1;
+    }
+    const double nn = (ddF * Fmm > 0) ? (c * sqrt(ddF * Fmm)) - 0.5 : 0.0;
+    if (a.outF) {
+      a.outF[p] = Fmm;
+      a.outddF[p] = ddF;
+      a.outr[p] = emm[1];
+      a.outn[p] = nn;
+      a.outom[p] = om;
+      a.outc[p] = c;
+    }
+    v[0] = 0.5 * ddF;
+    v[1] = Fmm;
+    v[2] = a.kL2[p] * Fmm;
+    v[5] = bad;
+    if (a.t != 2 && a.integral) {
+      // E_mem integrand over t' in [tmin, t-2] against F(t',t-1), rho(t',t-1)
+      for (int tp = a.tmin; tp <= a.t - 2; ++tp) {
+        const int x = (tp - 1) % a.N;
+        const double* e = a.H + hoff(mb, pair_index(x, a.s_tm1), lane, a.NP);
+        const double rr = e[1] * (x >= a.s_tm1 ? 1.0 : -1.0);
+        const double* S = a.S + soff(mb, x, lane, a.N);
+        const double term = S[1] * e[0] - S[0] * rr;
+        if (tp == a.tmin)
+          v[3] += term;
+        else
+          v[4] += term;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int k = 0; k < 6; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) a.partial[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// site-1 history (Measurement.jl:90-96); q = 0 .. t-tmin-1
+__global__ void history_kernel(const double* __restrict__ H, const double* __restrict__ S, long NP, int N,
+                               int tmin, int t, double* __restrict__ out) {
+  const int L = t - tmin;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= L) return;
+  // SigF / Sigr for tp = t-1 down to tmin
+  const int tpS = t - 1 - q;
+  const int xS = (tpS - 1) % N;
+  out[q] = S[soff(0, xS, 0, N)];
+  out[L + q] = S[soff(0, xS, 0, N) + 1];
+  // F(tmin,tp), rho(tmin,tp) for tp = tmin + q
+  const int xm = (tmin - 1) % N;
+  const int x = (tmin + q - 1) % N;
+  const double* e = H + hoff(0, pair_index(xm, x), 0, NP);
+  out[2 * L + q] = e[0];
+  out[3 * L + q] = e[1] * (xm >= x ? 1.0 : -1.0);
+}
+
+// ---------------------------------------------------------------------------
+// host-API helpers
+// ---------------------------------------------------------------------------
+// out[p] = comp(pair)[p] * sign for p < V
+__global__ void gather_pair(const double* __restrict__ H, long NP, int V, long pair, int comp, double sign,
+                            double* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < V) out[p] = H[hoff(p / MB, pair, p % MB, NP) + comp] * sign;
+}
+
+__global__ void scatter_pair(double* __restrict__ H, long NP, int V, long pair, int comp, double sign,
+                             const double* __restrict__ in) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < V) H[hoff(p / MB, pair, p % MB, NP) + comp] = in[p] * sign;
+}
+
+__global__ void gather_sigma(const double* __restrict__ S, int N, int V, int slot, int comp,
+                             double* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < V) out[p] = S[soff(p / MB, slot, p % MB, N) + comp];
+}
+
+// initSimData! rows (Initialization.jl:8-29): F(1,1)=F(2,1)=(n+1/2)/w,
+// F(2,2)=w(n+1/2)dt^2+(n+1/2)/w, rho(2,1)=dt, diagonal rho = 0
+__global__ void init_rows_kernel(double* __restrict__ H, long NP, int V, int N, const double* __restrict__ F11,
+                                 const double* __restrict__ F21, const double* __restrict__ F22, double dt) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= V) return;
+  const int mb = p / MB, lane = p % MB;
+  const int s1 = 0, s2 = 1 % N;
+  double* e11 = H + hoff(mb, pair_index(s1, s1), lane, NP);
+  double* e21 = H + hoff(mb, pair_index(s2, s1), lane, NP);
+  double* e22 = H + hoff(mb, pair_index(s2, s2), lane, NP);
+  e11[0] = F11[p];
+  e11[1] = 0.0;
+  e21[0] = F21[p];
+  e21[1] = dt * (s2 >= s1 ? 1.0 : -1.0);  // r[2,1] = dt * thesign(2,1)
+  e22[0] = F22[p];
+  e22[1] = 0.0;
+}
+
+__global__ void init_particles_kernel(const double* __restrict__ kL2, const double* __restrict__ n, double m,
+                                      double dt, int V, double* __restrict__ F11, double* __restrict__ F21,
+                                      double* __restrict__ F22) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= V) return;
+  const double omega = sqrt(kL2[p] + m * m);
+  const double np = n[p];
+  F11[p] = (1.0 / omega) * (np + 0.5);
+  F21[p] = (1.0 / omega) * (np + 0.5);
+  F22[p] = (omega * (np + 0.5) * (dt * dt) + (1.0 / omega) * (np + 0.5));
+}
+
+// full deterministic window of free propagators (benchmark set-up)
+__global__ void fill_free_kernel(double* __restrict__ H, long NP, int V, int N, int tmax,
+                                 const double* __restrict__ kL2, const double* __restrict__ n, double mass,
+                                 double dt, long npairs_total) {
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  // gid enumerates (pair, site) with site fastest inside a block of MB
+  const int nb = (V + MB - 1) / MB;
+  if (gid >= npairs_total * nb * MB) return;
+  const int lane = gid % MB;
+  const long rest = gid / MB;
+  const long pair = rest % npairs_total;
+  const int mb = (int)(rest / npairs_total);
+  const int p = mb * MB + lane;
+  // decode pair -> (hi, lo)
+  int hi = (int)floor((sqrt(8.0 * (double)pair + 1.0) - 1.0) * 0.5);
+  while ((long)hi * (hi + 1) / 2 > pair) --hi;
+  while ((long)(hi + 1) * (hi + 2) / 2 <= pair) ++hi;
+  const int lo = (int)(pair - (long)hi * (hi + 1) / 2);
+  double F = 0.0, r = 0.0;
+  if (p < V) {
+    // times of the slots in the window [tmax-N+1, tmax]
+    const int tmin = tmax - N + 1;
+    const int smin = (tmin - 1) % N;
+    const int th = tmin + (hi - smin + N) % N;
+    const int tl = tmin + (lo - smin + N) % N;
+    const double w = sqrt(kL2[p] + mass * mass);
+    const double nn = n ? n[p] : 0.0;
+    const double d = (double)(th - tl) * dt;
+    F = (nn + 0.5) / w * cos(w * d);
+    r = sin(w * d) / w;  // rho(th, tl) = stored value for (hi, lo)
+  }
+  double* e = H + hoff(mb, pair, lane, NP);
+  e[0] = F;
+  e[1] = r;
+}
+
+__global__ void finite_kernel(const double* __restrict__ H, size_t count, int* __restrict__ flag) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  int bad = 0;
+  for (; i < count; i += stride) bad |= !isfinite(H[i]);
+  if (bad) atomicOr(flag, 1);
+}
+
+}  // namespace qftkb

@@ -1,0 +1,240 @@
+"""ctypes binding of libqftkb.so (include/qftkb.h).
+
+The shared library is built in-tree next to this file.  There is no CPU
+fallback: if the library is missing, or no HIP device is present, the calls
+fail loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+LIB_NAME = "libqftkb.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+KB_OK = 0
+KB_ERR_ARG = 1
+KB_ERR_WINDOW = 2
+KB_ERR_OOM = 3
+KB_ERR_HIP = 4
+KB_ERR_STATE = 5
+KB_ERR_NONFINITE = 6
+
+KB_PEXP_LAMBDA_LO = 0
+KB_PEXP_LAMBDA_NLO = 1
+
+# every symbol include/qftkb.h declares
+EXPORTED = (
+    "kb_create", "kb_destroy", "kb_last_error", "kb_info", "kb_get_kL2", "kb_set_kL2",
+    "kb_init_particles", "kb_init_rows", "kb_step", "kb_window", "kb_hartree", "kb_measure",
+    "kb_measure_history", "kb_get_entry", "kb_set_entry", "kb_get_sigma", "kb_check_finite",
+    "kb_fill_free_window", "kb_synchronize", "kb_set_stream", "kb_enable_timing", "kb_timing",
+    "kb_reset_timing",
+)
+
+
+class KBError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libqftkb error {code}: {msg}")
+        self.code = code
+
+
+class kb_params(ctypes.Structure):
+    _fields_ = [
+        ("Nx", ctypes.c_int), ("sdim", ctypes.c_int), ("NstepsinMemory", ctypes.c_int),
+        ("ONgroup", ctypes.c_int), ("dt", ctypes.c_double), ("Lambda", ctypes.c_double),
+        ("Mass", ctypes.c_double), ("Mass2", ctypes.c_double), ("pexp", ctypes.c_int),
+        ("device", ctypes.c_int),
+    ]
+
+
+class kb_meas(ctypes.Structure):
+    _fields_ = [
+        ("time", ctypes.c_double), ("E", ctypes.c_double), ("E_pot", ctypes.c_double),
+        ("E_kin", ctypes.c_double), ("E_mem", ctypes.c_double), ("M2", ctypes.c_double),
+        ("n_unstable", ctypes.c_int),
+    ]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libqftkb.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with __graft_entry__.build() "
+                          "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+    L = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    L.kb_create.argtypes = [ctypes.POINTER(kb_params), ctypes.POINTER(vp)]
+    L.kb_destroy.argtypes = [vp]
+    L.kb_last_error.restype = ctypes.c_char_p
+    L.kb_last_error.argtypes = [vp]
+    L.kb_info.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
+    L.kb_get_kL2.argtypes = [vp, _dp]
+    L.kb_set_kL2.argtypes = [vp, _dp]
+    L.kb_init_particles.argtypes = [vp, _dp, ctypes.c_double]
+    L.kb_init_rows.argtypes = [vp, _dp, _dp, _dp]
+    L.kb_step.argtypes = [vp, ctypes.c_int]
+    L.kb_window.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.kb_hartree.argtypes = [vp, _dp]
+    L.kb_measure.argtypes = [vp, ctypes.c_int, ctypes.POINTER(kb_meas), _dp, _dp, _dp, _dp, _dp, _dp]
+    L.kb_measure_history.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp, _dp]
+    L.kb_get_entry.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+    L.kb_set_entry.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+    L.kb_get_sigma.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp]
+    L.kb_check_finite.argtypes = [vp]
+    L.kb_fill_free_window.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_double]
+    L.kb_synchronize.argtypes = [vp]
+    L.kb_set_stream.argtypes = [vp, vp]
+    L.kb_enable_timing.argtypes = [vp, ctypes.c_int]
+    L.kb_timing.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                            ctypes.POINTER(ctypes.c_longlong)]
+    L.kb_reset_timing.argtypes = [vp]
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a, V):
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+    if a.size != V:
+        raise ValueError(f"expected {V} lattice values, got {a.size}")
+    return a
+
+
+class KBContext:
+    """One device-resident KB simulation (a kb_ctx)."""
+
+    def __init__(self, Nx, sdim, NstepsinMemory, dt, Lambda, ONgroup, Mass, Mass2,
+                 pexp=KB_PEXP_LAMBDA_NLO, device=0):
+        self._L = load_library()
+        p = kb_params(int(Nx), int(sdim), int(NstepsinMemory), int(ONgroup), float(dt),
+                      float(Lambda), float(Mass), float(Mass2), int(pexp), int(device))
+        h = ctypes.c_void_p()
+        rc = self._L.kb_create(ctypes.byref(p), ctypes.byref(h))
+        if rc != KB_OK:
+            raise KBError(rc, "kb_create failed (invalid parameters, no HIP device, or out of memory)")
+        self._h = h
+        self.params = p
+        self.V = int(Nx) ** int(sdim)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.kb_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != KB_OK:
+            raise KBError(rc, self._L.kb_last_error(self._h).decode())
+
+    # ---- set-up -------------------------------------------------------
+    def device_bytes(self):
+        V, b = ctypes.c_longlong(), ctypes.c_longlong()
+        self._chk(self._L.kb_info(self._h, ctypes.byref(V), ctypes.byref(b)))
+        return b.value
+
+    def kL2(self):
+        out = np.empty(self.V)
+        self._chk(self._L.kb_get_kL2(self._h, _ptr(out)))
+        return out
+
+    def set_kL2(self, k2):
+        self._chk(self._L.kb_set_kL2(self._h, _ptr(_f64(k2, self.V))))
+
+    def init_particles(self, n, initmass):
+        self._chk(self._L.kb_init_particles(self._h, _ptr(_f64(n, self.V)), float(initmass)))
+
+    def init_rows(self, F11, F21, F22):
+        a = [_f64(x, self.V) for x in (F11, F21, F22)]
+        self._chk(self._L.kb_init_rows(self._h, *[_ptr(x) for x in a]))
+
+    def fill_free_window(self, tmax, n=None, mass=1.0):
+        nn = None if n is None else _f64(n, self.V)
+        self._chk(self._L.kb_fill_free_window(self._h, int(tmax), _ptr(nn), float(mass)))
+
+    # ---- stepping -----------------------------------------------------
+    def step(self, t):
+        self._chk(self._L.kb_step(self._h, int(t)))
+
+    def window(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._chk(self._L.kb_window(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def hartreemass2(self):
+        m = ctypes.c_double()
+        self._chk(self._L.kb_hartree(self._h, ctypes.byref(m)))
+        return m.value
+
+    def synchronize(self):
+        self._chk(self._L.kb_synchronize(self._h))
+
+    def set_stream(self, handle):
+        self._chk(self._L.kb_set_stream(self._h, ctypes.c_void_p(handle) if handle else None))
+
+    # ---- read-out -----------------------------------------------------
+    def measure(self, t, modes=True):
+        m = kb_meas()
+        arrs = {k: np.empty(self.V) for k in ("F", "ddF", "r", "n", "omega", "c")} if modes else {}
+        g = arrs.get
+        self._chk(self._L.kb_measure(self._h, int(t), ctypes.byref(m), _ptr(g("F")), _ptr(g("ddF")),
+                                     _ptr(g("r")), _ptr(g("n")), _ptr(g("omega")), _ptr(g("c"))))
+        out = dict(time=m.time, E=m.E, E_pot=m.E_pot, E_kin=m.E_kin, E_mem=m.E_mem, M2=m.M2,
+                   n_unstable=m.n_unstable)
+        out.update(arrs)
+        return out
+
+    def measure_history(self, t):
+        tmin, tmax = self.window()
+        L = t - tmin
+        bufs = [np.empty(max(L, 0)) for _ in range(4)]
+        self._chk(self._L.kb_measure_history(self._h, int(t), *[_ptr(b) for b in bufs]))
+        return dict(SigF=bufs[0], Sigr=bufs[1], Fonet=bufs[2], ronet=bufs[3])
+
+    def get(self, which, i, j):
+        out = np.empty(self.V)
+        self._chk(self._L.kb_get_entry(self._h, 1 if which == "r" else 0, int(i), int(j), _ptr(out)))
+        return out
+
+    def set(self, which, i, j, arr):
+        self._chk(self._L.kb_set_entry(self._h, 1 if which == "r" else 0, int(i), int(j),
+                                       _ptr(_f64(arr, self.V))))
+
+    def sigma(self, which, tp):
+        out = np.empty(self.V)
+        self._chk(self._L.kb_get_sigma(self._h, 1 if which == "r" else 0, int(tp), _ptr(out)))
+        return out
+
+    def check_finite(self):
+        self._chk(self._L.kb_check_finite(self._h))
+
+    # ---- timing -------------------------------------------------------
+    def enable_timing(self, on=True):
+        self._chk(self._L.kb_enable_timing(self._h, 1 if on else 0))
+
+    def timing(self, stage):
+        ms, n = ctypes.c_double(), ctypes.c_longlong()
+        self._chk(self._L.kb_timing(self._h, int(stage), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def reset_timing(self):
+        self._chk(self._L.kb_reset_timing(self._h))

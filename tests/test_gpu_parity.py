@@ -1,0 +1,169 @@
+"""Parity of the HIP path (through the C ABI) with the oracle.
+
+Tolerance (north-star): F, rho and n(p,t) within 1e-10 relative error.  For
+lattice-valued quantities the relative error is norm-wise per lattice,
+max_p |gpu - oracle| / max_p |oracle|; scalars use |gpu - oracle| / |oracle|
+with the energy E as the scale for E_mem (a small difference of large
+terms).
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+def rel(a, b):
+    d = np.max(np.abs(np.asarray(a) - np.asarray(b)))
+    s = np.max(np.abs(np.asarray(b)))
+    return d / s if s > 0 else d
+
+
+def make_pair(Nx, sdim, Nmem, dt=0.1, lam=1.0, ON=1, Mass=1.0, Mass2=None, pexp=1):
+    from qftkb import KBContext
+    Mass2 = Mass * Mass if Mass2 is None else Mass2
+    o = Oracle(Nx, sdim, Nmem, dt, lam, ON, Mass, Mass2, pexp=pexp)
+    g = KBContext(Nx, sdim, Nmem, dt, lam, ON, Mass, Mass2, pexp=pexp)
+    return o, g
+
+
+def thermal(kL2, T=2.0, m=1.0):
+    return 1.0 / (np.exp(np.sqrt(kL2 + m * m) / (T * m)) - 1.0)
+
+
+def compare_window(o, g, tol=RTOL):
+    assert o.window() == g.window()
+    tmin, tmax = o.window()
+    worst = 0.0
+    for a in range(tmin, tmax + 1):
+        for b in range(tmin, a + 1):
+            for w in "Fr":
+                e = rel(g.get(w, a, b), o.get(w, a, b))
+                worst = max(worst, e)
+                assert e <= tol, f"{w}({a},{b}) rel err {e:.3e}"
+    return worst
+
+
+def compare_meas(o, g, t, tol=RTOL):
+    mo = o.measure(t)
+    mg = g.measure(t)
+    for k in ("E", "E_kin", "E_pot", "M2"):
+        assert abs(mg[k] - mo[k]) <= tol * abs(mo[k]) + 1e-300, f"{k}: {mg[k]} vs {mo[k]}"
+    assert abs(mg["E_mem"] - mo["E_mem"]) <= tol * max(abs(mo["E"]), abs(mo["E_mem"])), "E_mem"
+    assert mg["time"] == mo["time"]
+    assert mg["n_unstable"] == mo["nstab"]
+    for k in ("F", "ddF", "n", "omega", "c"):
+        assert rel(mg[k], mo[k]) <= tol, f"per-site {k}"
+    return mo, mg
+
+
+CASES = [
+    # Nx, sdim, Nmem, steps: full memory, sliding windows, odd sizes, 1D/2D
+    (4, 3, 5, 18),
+    (6, 3, 40, 14),
+    (5, 3, 4, 16),
+    (8, 3, 7, 20),
+    (6, 2, 6, 20),
+    (7, 2, 30, 12),
+    (12, 1, 5, 25),
+    (1, 3, 6, 20),
+]
+
+
+@pytest.mark.parametrize("Nx,sdim,Nmem,steps", CASES)
+def test_trajectory_thermal(Nx, sdim, Nmem, steps):
+    print(f"\n[case] trajectory Nx={Nx} sdim={sdim} Nmem={Nmem} steps={steps}", flush=True)
+    o, g = make_pair(Nx, sdim, Nmem)
+    k2 = o.kL2()
+    np.testing.assert_array_equal(g.kL2(), k2)
+    n = thermal(k2)
+    o.init_particles(n, 1.0)
+    g.init_particles(n, 1.0)
+    compare_window(o, g)
+    compare_meas(o, g, 2)
+    for t in range(3, steps + 1):
+        o.evolve(t)
+        g.step(t)
+        compare_window(o, g)
+        for tp in range(o.window()[0], t):
+            for w in "Fr":
+                assert rel(g.sigma(w, tp), o.sigma(w, tp)) <= RTOL, f"Sigma{w}({t - 1},{tp})"
+        compare_meas(o, g, t)
+    h = g.measure_history(steps)
+    tmin, tmax = o.window()
+    np.testing.assert_allclose(h["Fonet"], [o.get("F", tmin, tp)[0] for tp in range(tmin, tmax)], rtol=1e-10)
+    np.testing.assert_allclose(h["ronet"], [o.get("r", tmin, tp)[0] for tp in range(tmin, tmax)], rtol=1e-10,
+                               atol=1e-14)
+    np.testing.assert_allclose(h["SigF"], [o.sigma("F", tp)[0] for tp in range(tmax - 1, tmin - 1, -1)],
+                               rtol=1e-10)
+
+
+def test_lambda_lo():
+    """LambdaLO: no self-energy, leapfrog + Hartree only."""
+    o, g = make_pair(6, 3, 5, lam=2.0, pexp=0)
+    n = thermal(o.kL2())
+    o.init_particles(n, 1.0)
+    g.init_particles(n, 1.0)
+    for t in range(3, 15):
+        o.evolve(t)
+        g.step(t)
+        compare_meas(o, g, t)
+    compare_window(o, g)
+
+
+def test_gauss_rows_on_ON4():
+    """Gauss initial rows, O(4), dt = 0.05, lambda = 3."""
+    from qftkb import lattice
+    o, g = make_pair(4, 3, 9, dt=0.05, lam=3.0, ON=4)
+    rows = lattice.gauss_rows(o.V, 0.05, 0.4, 1.5, 1.0)
+    o.init_rows(*rows)
+    g.init_rows(*rows)
+    for t in range(3, 20):
+        o.evolve(t)
+        g.step(t)
+    compare_window(o, g)
+    compare_meas(o, g, 19)
+
+
+def test_random_window_one_step():
+    """One step from an identical, non-symmetric random state: the oracle's
+    real(rfft) semantics (inversion-even part only) must hold on the GPU."""
+    rng = np.random.default_rng(1234)
+    Nx, sdim, N = 6, 3, 7
+    o, g = make_pair(Nx, sdim, N, lam=1.5)
+    n = thermal(o.kL2())
+    o.init_particles(n, 1.0)
+    g.init_particles(n, 1.0)
+    for t in range(3, 12):  # slide the window past one wrap
+        o.evolve(t)
+        g.step(t)
+    tmin, tmax = o.window()
+    for a in range(tmin, tmax + 1):
+        for b in range(tmin, a + 1):
+            F = rng.standard_normal(o.V)
+            r = np.zeros(o.V) if a == b else rng.standard_normal(o.V)
+            for ctx in (o, g):
+                ctx.set("F", a, b, F)
+                ctx.set("r", a, b, r)
+    t = tmax + 1
+    o.evolve(t)
+    g.step(t)
+    compare_window(o, g, tol=1e-11)
+
+
+def test_errors_and_window_bounds():
+    from qftkb import KBError
+    o, g = make_pair(4, 3, 5)
+    with pytest.raises(KBError):
+        g.step(3)  # before init
+    g.init_particles(np.zeros(g.V), 1.0)
+    with pytest.raises(KBError) as e:
+        g.step(4)  # skipping t = 3
+    assert e.value.code == 2
+    with pytest.raises(KBError):
+        g.get("F", 3, 1)  # outside [1, 2]
+    g.step(3)
+    g.check_finite()
