@@ -42,6 +42,8 @@ struct kb_ctx {
   double* vbuf = nullptr;  // 4 x V staging
   double2* tw = nullptr;
   int* flag = nullptr;
+  int2* items = nullptr;  // memory-integral work items
+  int nitems = 0;
   size_t bytes = 0;
   FFTPlan plan{};
   int fftP = 1;
@@ -156,7 +158,7 @@ void expand(kb_ctx* c) {
 int launch_hartree(kb_ctx* c, int i, int j) {
   const double H = c->p.Lambda * (double)(c->p.ONgroup + 2) / (6.0 * (double)c->p.ONgroup);
   const double hfac = H * (1.0 / (double)c->V);
-  const long pr = pair_index(slot_of(c, i), slot_of(c, j));
+  const long pr = pair_off(slot_of(c, i), slot_of(c, j), c->N);
   hipLaunchKernelGGL(hartree_kernel, dim3(1), dim3(1024), 0, c->stream, c->H, c->NP, c->V, pr, hfac, c->p.Mass2,
                      c->m2);
   LAUNCHCHK(c);
@@ -234,7 +236,9 @@ int launch_memint(kb_ctx* c, int t) {
   a.t = t;
   a.dt = c->p.dt;
   a.integral = c->p.pexp == KB_PEXP_LAMBDA_NLO;
-  const size_t lds = sizeof(double) * 4 * (size_t)c->N * MB;
+  a.items = c->items;
+  a.nitems = c->nitems;
+  const size_t lds = sizeof(double) * 2 * (size_t)c->N * MB;
   hipLaunchKernelGGL(memint_step, dim3(c->NB), dim3(MEMINT_THREADS), lds, c->stream, a);
   LAUNCHCHK(c);
   return KB_OK;
@@ -253,7 +257,7 @@ int download(kb_ctx* c, double* dst, const double* src, size_t n) {
 
 void free_all(kb_ctx* c) {
   for (void* ptr : {(void*)c->H, (void*)c->S, (void*)c->W, (void*)c->kL2, (void*)c->m2, (void*)c->partial,
-                    (void*)c->modes, (void*)c->vbuf, (void*)c->tw, (void*)c->flag})
+                    (void*)c->modes, (void*)c->vbuf, (void*)c->tw, (void*)c->flag, (void*)c->items})
     if (ptr) (void)hipFree(ptr);
   for (int s = 0; s < 3; ++s)
     for (auto& ev : c->pending[s]) {
@@ -292,7 +296,7 @@ int kb_create(const kb_params* p, kb_ctx** out) {
   if (!c) return KB_ERR_OOM;
   c->p = *p;
   c->N = p->NstepsinMemory;
-  c->NP = (long)c->N * (c->N + 1) / 2;
+  c->NP = slot_pairs(c->N);
   c->V = 1;
   for (int d = 0; d < p->sdim; ++d) c->V *= p->Nx;
   c->nx = p->Nx;
@@ -326,8 +330,17 @@ int kb_create(const kb_params* p, kb_ctx** out) {
   if ((rc = dmalloc(c, (void**)&c->vbuf, 4 * vlen * sizeof(double)))) return fail(rc);
   if ((rc = dmalloc(c, (void**)&c->tw, (size_t)p->Nx * sizeof(double2)))) return fail(rc);
   if ((rc = dmalloc(c, (void**)&c->flag, sizeof(int) * 4))) return fail(rc);
+  // memory-integral work items: chunks of MEMINT_ROWS rows of each column
+  // block, in storage order
+  std::vector<int2> items;
+  for (int J = 0; J * CB < c->N; ++J)
+    for (int k = 0; J * CB + k * MEMINT_ROWS < c->N; ++k) items.push_back(make_int2(J, k));
+  c->nitems = (int)items.size();
+  if ((rc = dmalloc(c, (void**)&c->items, sizeof(int2) * items.size()))) return fail(rc);
+  if (hipMemcpy(c->items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(KB_ERR_HIP);
   {
-    const size_t lds = sizeof(double) * 4 * (size_t)c->N * MB;
+    const size_t lds = sizeof(double) * 2 * (size_t)c->N * MB;
     hipError_t e = hipFuncSetAttribute((const void*)memint_step, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) {
       seterr(c, KB_ERR_ARG, "memory window N=%d needs %zu bytes of LDS: %s", c->N, lds, hipGetErrorString(e));
@@ -554,7 +567,7 @@ int kb_get_entry(kb_ctx* c, int which, int i, int j, double* out) {
   const int si = slot_of(c, i), sj = slot_of(c, j);
   const double sg = which ? (si >= sj ? 1.0 : -1.0) : 1.0;
   hipLaunchKernelGGL(gather_pair, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->H, c->NP, c->V,
-                     pair_index(si, sj), which, sg, c->vbuf);
+                     pair_off(si, sj, c->N), which, sg, c->vbuf);
   LAUNCHCHK(c);
   return download(c, out, c->vbuf, c->V);
 }
@@ -568,7 +581,7 @@ int kb_set_entry(kb_ctx* c, int which, int i, int j, const double* in) {
   int rc;
   if ((rc = upload(c, c->vbuf, in, c->V))) return rc;
   hipLaunchKernelGGL(scatter_pair, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->H, c->NP, c->V,
-                     pair_index(si, sj), which, sg, c->vbuf);
+                     pair_off(si, sj, c->N), which, sg, c->vbuf);
   LAUNCHCHK(c);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return KB_OK;
@@ -605,10 +618,10 @@ int kb_fill_free_window(kb_ctx* c, int tmax, const double* n, double mass) {
     if ((rc = upload(c, c->vbuf, n, c->V))) return rc;
     dn = c->vbuf;
   }
-  const long total = c->NP * (long)c->NB * MB;
-  const long blocks = (total + 255) / 256;
-  hipLaunchKernelGGL(fill_free_kernel, dim3((unsigned)blocks), dim3(256), 0, c->stream, c->H, c->NP, c->V, c->N,
-                     tmax, c->kL2, dn, mass, c->p.dt, c->NP);
+  const long total = (long)c->NB * MB * c->N;
+  const dim3 grid((unsigned)((total + 255) / 256), (unsigned)c->N);
+  hipLaunchKernelGGL(fill_free_kernel, grid, dim3(256), 0, c->stream, c->H, c->NP, c->V, c->N, tmax, c->kL2, dn,
+                     mass, c->p.dt, c->NB);
   LAUNCHCHK(c);
   c->tmin = tmax - c->N + 1;
   c->tmax = tmax;

@@ -1,23 +1,26 @@
 // kb_device.h -- device-side layout shared by the kernels of libqftkb.
 //
-// Two-time history layout in HBM ("mode-block" layout)
+// Two-time history layout in HBM ("mode-block, column-block" layout)
 //   The reference stores F and r as a lower triangle of N = NstepsinMemory
 //   ring slots, every entry a full lattice (src/DefSimDataElements.jl:
-//   169-221).  Here the same slot triangle is kept, but the lattice sites are
-//   grouped in blocks of MB = 8 consecutive sites and the block index is the
-//   slowest dimension:
+//   169-221).  Here the same slot triangle is kept, but
+//     * lattice sites are grouped in blocks of MB = 8 consecutive sites and
+//       the site block is the slowest dimension, and
+//     * inside one site block the triangle is stored in column blocks of
+//       CB = 4 slots: column block J holds rows r = CB*J .. N-1 of columns
+//       CB*J .. CB*J+3, row by row (the CB*(CB-1)/2 slots above the
+//       diagonal of each block are padding):
 //
-//       H[mb][pair(slot_i, slot_j)][lane][comp]      (double)
+//       H[mb][pair_off(slot_i, slot_j)][lane][comp]      (double)
 //
-//   mb    = site / 8, lane = site % 8,
-//   pair  = hi*(hi+1)/2 + lo for slots hi >= lo (packed lower triangle),
-//   comp  = 0: F,  1: r (rho in the reference's sign-folded convention:
-//           the value stored for slots (hi, lo) is rho(time(hi), time(lo))).
+//   mb = site / 8, lane = site % 8, comp 0: F, 1: r (rho in the reference's
+//   sign-folded convention: the value stored for slots (hi, lo), hi >= lo,
+//   is rho(time(hi), time(lo))).
 //
-//   One pair of one block is 8 sites x (F, r) = 128 bytes, i.e. one L2
-//   cache line, and the whole two-time history of one block is a single
-//   contiguous stream of N(N+1)/2 lines.  The memory-integral kernel gives
-//   one workgroup one block and reads that stream exactly once per step.
+//   One slot of one site block is 8 sites x (F, r) = 128 bytes, one L2
+//   cache line.  A 16-row x 4-column tile of the triangle is one
+//   contiguous 8 KiB run, and the whole two-time history of a site block is
+//   one contiguous region that the memory-integral kernel sweeps in order.
 //
 // Self-energy ring: S[mb][slot][lane][comp], comp 0 = SigF, 1 = Sigr.
 #pragma once
@@ -26,11 +29,22 @@
 namespace qftkb {
 
 constexpr int MB = 8;  // lattice sites per mode block
+constexpr int CB = 4;  // slot columns per storage column block
 
-__host__ __device__ inline long pair_index(int a, int b) {
+// first slot of column block J: CB * sum_{J' < J} (N - CB J')
+__host__ __device__ inline long colblock_offset(int J, int N) {
+  return (long)CB * ((long)N * J - (long)CB * J * (J - 1) / 2);
+}
+
+// slots per site block (including the diagonal padding)
+__host__ __device__ inline long slot_pairs(int N) { return colblock_offset((N + CB - 1) / CB, N); }
+
+// storage slot of the unordered slot pair {a, b}
+__host__ __device__ inline long pair_off(int a, int b, int N) {
   const int hi = a > b ? a : b;
   const int lo = a > b ? b : a;
-  return (long)hi * (hi + 1) / 2 + lo;
+  const int J = lo / CB;
+  return colblock_offset(J, N) + (long)(hi - J * CB) * CB + (lo - J * CB);
 }
 
 // double offset of (block, pair, lane), component F; component r is +1

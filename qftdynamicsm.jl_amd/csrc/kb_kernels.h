@@ -48,6 +48,8 @@ struct MemintArgs {
   int tmin, t;
   double dt;
   int integral;  // 0 for LambdaLO
+  const int2* items;  // (column block J, row chunk k) in storage order
+  int nitems;
 };
 
 struct MeasArgs {
@@ -77,10 +79,11 @@ __global__ void init_rows_kernel(double* H, long NP, int V, int N, const double*
 __global__ void init_particles_kernel(const double* kL2, const double* n, double m, double dt, int V, double* F11,
                                       double* F21, double* F22);
 __global__ void fill_free_kernel(double* H, long NP, int V, int N, int tmax, const double* kL2, const double* n,
-                                 double mass, double dt, long npairs_total);
+                                 double mass, double dt, int nb);
 __global__ void finite_kernel(const double* H, size_t count, int* flag);
 
-constexpr int MEMINT_THREADS = 512;
+constexpr int MEMINT_THREADS = 256;
+constexpr int MEMINT_ROWS = 64;  // rows per work item
 constexpr int MEAS_THREADS = 256;
 
 }  // namespace qftkb

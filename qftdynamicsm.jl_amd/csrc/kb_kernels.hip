@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) void sigma_pass(SigmaArgs a) {
       const int s = site_of(a, pencil, j);
       if (a.flags & SIGMA_GATHER) {
         // physical F(t-1,t') and rho(t-1,t'); rho sign from the slot order
-        const long pr = pair_index(a.s_tm1, slot);
+        const long pr = pair_off(a.s_tm1, slot, a.N);
         const double sg = a.s_tm1 >= slot ? 1.0 : -1.0;
         const int s2 = neg_site(a, s);
         const double* h1 = a.H + hoff(s / MB, pr, s % MB, a.NP);
@@ -277,23 +277,89 @@ __global__ __launch_bounds__(1024) void hartree_kernel(const double* __restrict_
 // ms = m Sr (m = 0 at z = t-1):
 //   R_F(a) += -u(b) F - v(b) r,  R_F(b) += -u(a) F,  R_r(b) += -ms(a) r,
 // and the diagonal (a,a) feeds R_F(a) += -u(a) F.  The kernel walks the
-// slot triangle once in 8x8 tiles and accumulates both rows into LDS.
+// slot triangle once: every 8-lane worker streams chunks of 64 rows of one
+// 4-column block (contiguous in HBM) with two 4-row batches in flight,
+// keeps the 4 column sums in registers and adds row and column sums into
+// per-slot LDS accumulators with ds_add_f64.
 // In slot space, pair (hi, lo) has time(hi) < time(lo) exactly when
 // lo < s_min <= hi; those pairs swap the roles of a and b.
 // ---------------------------------------------------------------------------
-constexpr int MI_THREADS = 512;
-constexpr int MI_RS = 8;  // rows per work item
-constexpr int MI_CB = 8;  // columns per work item
+constexpr int MI_ROWS = MEMINT_ROWS;  // slot rows per work item (one column block chunk)
+constexpr int MI_B = 4;      // rows per batch; two batches are in flight
 
-__global__ __launch_bounds__(MI_THREADS) void memint_step(MemintArgs a) {
+__device__ __forceinline__ void slot_weights(int x, int smin, int N, int L, double& w, double& m) {
+  const int rel = (x - smin + N) % N;
+  w = rel == 0 ? 0.5 : 1.0;
+  m = rel == L - 1 ? 0.0 : 1.0;
+}
+
+struct RowBatch {
+  double2 e[MI_B][CB];  // (F, r) of slots (row, c0 + jc)
+  double2 fs[MI_B];     // (SigF, Sigr) of the row slot
+};
+
+// rows r .. r+MI_B-1 of the column block starting at storage slot cbase;
+// rows past rlast are clamped to rlast (re-read, not used)
+__device__ __forceinline__ void load_batch(RowBatch& b, const double* __restrict__ Hb,
+                                           const double2* __restrict__ Sb, long cbase, int c0, int r, int rlast) {
+#pragma unroll
+  for (int h = 0; h < MI_B; ++h) {
+    const int rr = min(r + h, rlast);
+    const long pb = cbase + (long)(rr - c0) * CB;
+#pragma unroll
+    for (int jc = 0; jc < CB; ++jc) b.e[h][jc] = *reinterpret_cast<const double2*>(Hb + (size_t)(pb + jc) * MB * 2);
+    b.fs[h] = Sb[(size_t)rr * MB];
+  }
+}
+
+struct ColState {
+  double u[CB], v[CB], ms[CB], cF[CB], cR[CB];
+  unsigned phi;  // bit jc: column slot lies before s_min (its time is after the wrap)
+};
+
+// Pair (r, c), r > c in slot space.  Time order is reversed (time(r) <
+// time(c)) exactly when r >= s_min > c; `neg` carries that per column.
+__device__ __forceinline__ void pair_update(double Fv, double Pv, bool n, double uc, double vc, double msc,
+                                            double ur, double vr, double msr, double& rowF, double& rowR,
+                                            double& cF, double& cR) {
+  rowF -= uc * Fv + (n ? 0.0 : vc) * Pv;
+  rowR += (n ? msc : 0.0) * Pv;
+  cF += (n ? vr : 0.0) * Pv - ur * Fv;
+  cR -= (n ? 0.0 : msr) * Pv;
+}
+
+__device__ __forceinline__ void process_batch(const RowBatch& b, int r, int r1, int dead, int smin, int N, int L,
+                                              ColState& cs, double* accF, double* accR, int lane) {
+#pragma unroll
+  for (int h = 0; h < MI_B; ++h) {
+    const int rr = r + h;
+    if (rr < r1) {
+      const bool okr = rr != dead;
+      double w, m;
+      slot_weights(rr, smin, N, L, w, m);
+      const double ur = okr ? m * w * b.fs[h].y : 0.0;
+      const double vr = okr ? w * b.fs[h].x : 0.0;
+      const double msr = okr ? m * b.fs[h].y : 0.0;
+      const unsigned neg = rr >= smin ? cs.phi : 0u;
+      double rowF = 0.0, rowR = 0.0;
+#pragma unroll
+      for (int jc = 0; jc < CB; ++jc)
+        pair_update(b.e[h][jc].x, b.e[h][jc].y, (neg >> jc) & 1u, cs.u[jc], cs.v[jc], cs.ms[jc], ur, vr, msr, rowF,
+                    rowR, cs.cF[jc], cs.cR[jc]);
+      atomicAdd(&accF[rr * MB + lane], rowF);
+      atomicAdd(&accR[rr * MB + lane], rowR);
+    }
+  }
+}
+
+__global__ __launch_bounds__(MEMINT_THREADS, 2) void memint_step(MemintArgs a) {
   extern __shared__ double smem[];
   const int N = a.N;
-  double* sF = smem;            // SigF per slot, lane
-  double* sR = sF + N * MB;     // Sigr
-  double* accF = sR + N * MB;   // R_F
-  double* accR = accF + N * MB; // R_r
-  __shared__ double red[MI_THREADS];
+  double* accF = smem;           // R_F per slot, lane
+  double* accR = accF + N * MB;  // R_r
+  __shared__ double red[MEMINT_THREADS / 64][MB];
   __shared__ double diagF[2][MB];
+  __shared__ int next_item;
 
   const int mb = blockIdx.x;
   const int tid = threadIdx.x;
@@ -305,104 +371,100 @@ __global__ __launch_bounds__(MI_THREADS) void memint_step(MemintArgs a) {
   const int smin = a.s_min;
   const int L = a.L;
 
-  {
-    const double* Sb = a.S + soff(mb, 0, 0, N);
-    for (int i = tid; i < N * MB; i += blockDim.x) {
-      sF[i] = a.integral ? Sb[2 * i] : 0.0;
-      sR[i] = a.integral ? Sb[2 * i + 1] : 0.0;
-      accF[i] = 0.0;
-      accR[i] = 0.0;
-    }
+  for (int i = tid; i < N * MB; i += blockDim.x) {
+    accF[i] = 0.0;
+    accR[i] = 0.0;
   }
+  if (tid == 0) next_item = nworkers;
   __syncthreads();
 
-  const double* __restrict__ Hb = a.H + hoff(mb, 0, 0, a.NP);
+  const double* __restrict__ Hb = a.H + hoff(mb, 0, 0, a.NP) + lane * 2;
+  // self-energies of this block: (SigF, Sigr) per slot for this lane
+  const double2* __restrict__ Sb = reinterpret_cast<const double2*>(a.S + soff(mb, 0, lane, N));
 
   if (a.integral) {
-    int it = 0;
-    const int nJ = (Rmax + MI_CB - 1) / MI_CB;
-    for (int J = 0; J < nJ; ++J) {
-      const int c0 = J * MI_CB;
-      const int nstrips = (Rmax - c0 + MI_RS - 1) / MI_RS;
-      for (int k = 0; k < nstrips; ++k, ++it) {
-        if (it % nworkers != worker) continue;
-        // ---- column coefficients ----
-        double uC[MI_CB], vC[MI_CB], msC[MI_CB], cF[MI_CB], cR[MI_CB];
-        bool okC[MI_CB], phiC[MI_CB];
+    // work items: (column block J, chunk of MI_ROWS rows), in storage order;
+    // the first item of each worker is static, later ones are taken from an
+    // LDS counter, so workers that finish early take more
+    int item = worker;
+    while (item < a.nitems) {
+      const int2 jk = a.items[item];
+      const int J = jk.x;
+      const int c0 = J * CB;
+      const int r0 = c0 + jk.y * MI_ROWS;
+      const int r1 = min(r0 + MI_ROWS, Rmax);
+      if (c0 < Rmax && r0 < r1) {
+        const long cbase = colblock_offset(J, N);
+        ColState cs;
+        cs.phi = 0;
 #pragma unroll
-        for (int jc = 0; jc < MI_CB; ++jc) {
+        for (int jc = 0; jc < CB; ++jc) {
           const int c = c0 + jc;
           const bool ok = c < Rmax && c != dead;
-          okC[jc] = ok;
-          phiC[jc] = c < smin;
-          uC[jc] = vC[jc] = msC[jc] = 0.0;
-          if (ok) {
-            const int rel = (c - smin + N) % N;
-            const double w = rel == 0 ? 0.5 : 1.0;
-            const double m = rel == L - 1 ? 0.0 : 1.0;
-            const double s = sR[c * MB + lane], f = sF[c * MB + lane];
-            uC[jc] = m * w * s;
-            vC[jc] = w * f;
-            msC[jc] = m * s;
-          }
-          cF[jc] = cR[jc] = 0.0;
+          const double2 fs = Sb[(size_t)min(c, N - 1) * MB];
+          double w, m;
+          slot_weights(c, smin, N, L, w, m);
+          cs.u[jc] = ok ? m * w * fs.y : 0.0;
+          cs.v[jc] = ok ? w * fs.x : 0.0;
+          cs.ms[jc] = ok ? m * fs.y : 0.0;
+          cs.cF[jc] = cs.cR[jc] = 0.0;
+          cs.phi |= (c < smin ? 1u : 0u) << jc;
         }
-        const int r0 = c0 + k * MI_RS;
-        const int r1 = min(r0 + MI_RS, Rmax);
-        for (int r = r0; r < r1; ++r) {
-          if (r == dead) continue;
-          const int rel = (r - smin + N) % N;
-          const double w = rel == 0 ? 0.5 : 1.0;
-          const double m = rel == L - 1 ? 0.0 : 1.0;
-          const double s = sR[r * MB + lane], f = sF[r * MB + lane];
-          const double ur = m * w * s, vr = w * f, msr = m * s;
-          const bool eps = r >= smin;
-          const long pb = (long)r * (r + 1) / 2 + c0;
+        int r = r0;
+        // rows crossing the diagonal of the block: columns c0 .. r only
+        for (; r < r1 && r < c0 + CB; ++r) {
+          const bool okr = r != dead;
+          const double2 fs = Sb[(size_t)r * MB];
+          double w, m;
+          slot_weights(r, smin, N, L, w, m);
+          const double ur = okr ? m * w * fs.y : 0.0;
+          const double vr = okr ? w * fs.x : 0.0;
+          const double msr = okr ? m * fs.y : 0.0;
+          const unsigned neg = r >= smin ? cs.phi : 0u;
+          const long pb = cbase + (long)(r - c0) * CB;
           double rowF = 0.0, rowR = 0.0;
-          if (r >= c0 + MI_CB) {
-            double2 e[MI_CB];
-#pragma unroll
-            for (int jc = 0; jc < MI_CB; ++jc)
-              e[jc] = okC[jc] ? *reinterpret_cast<const double2*>(Hb + ((size_t)(pb + jc) * MB + lane) * 2)
-                              : make_double2(0.0, 0.0);
-#pragma unroll
-            for (int jc = 0; jc < MI_CB; ++jc) {
-              const double Fv = e[jc].x, Pv = e[jc].y;
-              const bool neg = eps && phiC[jc];
-              rowF += -uC[jc] * Fv - (neg ? 0.0 : vC[jc]) * Pv;
-              rowR += (neg ? msC[jc] : 0.0) * Pv;
-              cF[jc] += -ur * Fv + (neg ? vr : 0.0) * Pv;
-              cR[jc] += -(neg ? 0.0 : msr) * Pv;
-            }
-          } else {
-            // tile crossing the diagonal: columns c0 .. r
-            for (int jc = 0; jc < MI_CB; ++jc) {
-              const int c = c0 + jc;
-              if (c > r) break;
-              if (!okC[jc]) continue;
-              const double2 e = *reinterpret_cast<const double2*>(Hb + ((size_t)(pb + jc) * MB + lane) * 2);
-              const double Fv = e.x, Pv = e.y;
-              if (c == r) {
-                rowF += -ur * Fv;
-              } else {
-                const bool neg = eps && phiC[jc];
-                rowF += -uC[jc] * Fv - (neg ? 0.0 : vC[jc]) * Pv;
-                rowR += (neg ? msC[jc] : 0.0) * Pv;
-                cF[jc] += -ur * Fv + (neg ? vr : 0.0) * Pv;
-                cR[jc] += -(neg ? 0.0 : msr) * Pv;
-              }
-            }
+          for (int jc = 0; jc < CB; ++jc) {
+            const int c = c0 + jc;
+            if (c > r) break;
+            const double2 e = *reinterpret_cast<const double2*>(Hb + (size_t)(pb + jc) * MB * 2);
+            if (c == r)
+              rowF -= ur * e.x;
+            else
+              pair_update(e.x, e.y, (neg >> jc) & 1u, cs.u[jc], cs.v[jc], cs.ms[jc], ur, vr, msr, rowF, rowR,
+                          cs.cF[jc], cs.cR[jc]);
           }
           atomicAdd(&accF[r * MB + lane], rowF);
           atomicAdd(&accR[r * MB + lane], rowR);
         }
-#pragma unroll
-        for (int jc = 0; jc < MI_CB; ++jc)
-          if (okC[jc]) {
-            atomicAdd(&accF[(c0 + jc) * MB + lane], cF[jc]);
-            atomicAdd(&accR[(c0 + jc) * MB + lane], cR[jc]);
+        // full rows: two batches of MI_B rows in flight
+        if (r < r1) {
+          const int rlast = r1 - 1;
+          RowBatch A, B;
+          load_batch(A, Hb, Sb, cbase, c0, r, rlast);
+          while (true) {
+            load_batch(B, Hb, Sb, cbase, c0, r + MI_B, rlast);
+            process_batch(A, r, r1, dead, smin, N, L, cs, accF, accR, lane);
+            r += MI_B;
+            if (r >= r1) break;
+            load_batch(A, Hb, Sb, cbase, c0, r + MI_B, rlast);
+            process_batch(B, r, r1, dead, smin, N, L, cs, accF, accR, lane);
+            r += MI_B;
+            if (r >= r1) break;
           }
+        }
+#pragma unroll
+        for (int jc = 0; jc < CB; ++jc) {
+          const int c = c0 + jc;
+          if (c < Rmax && c != dead) {
+            atomicAdd(&accF[c * MB + lane], cs.cF[jc]);
+            atomicAdd(&accR[c * MB + lane], cs.cR[jc]);
+          }
+        }
       }
+      // next item: one LDS atomic per worker
+      int nxt = 0;
+      if (lane == 0) nxt = atomicAdd(&next_item, 1);
+      item = __shfl(nxt, tid & ~(MB - 1) & 63);
     }
   }
   __syncthreads();
@@ -416,42 +478,59 @@ __global__ __launch_bounds__(MI_THREADS) void memint_step(MemintArgs a) {
   const double Om = (a.kL2[p] + m2) * dt2;  // setomega2values!, Evolution.jl:918-921
   double dsum = 0.0;
   double* __restrict__ Hw = a.H + hoff(mb, 0, 0, a.NP);
-  for (int x = worker; x < Rmax; x += nworkers) {
-    if (x == dead) continue;
-    const int rel = (x - smin + N) % N;
-    const int tx = a.tmin + rel;
-    const double2 e1 = *reinterpret_cast<const double2*>(Hw + ((size_t)pair_index(a.s_tm1, x) * MB + lane) * 2);
-    const double2 e2 = *reinterpret_cast<const double2*>(Hw + ((size_t)pair_index(a.s_tm2, x) * MB + lane) * 2);
-    const double r1 = e1.y * (a.s_tm1 >= x ? 1.0 : -1.0);
-    const double r2 = e2.y * (a.s_tm2 >= x ? 1.0 : -1.0);
-    const double RHSF = accF[x * MB + lane] * dt3;
-    const double Fn = (2.0 - Om) * e1.x - e2.x + RHSF;
-    double rn;
-    if (tx == a.t - 1) {
-      rn = dt;
-    } else {
-      const double RHSr = accR[x * MB + lane] * dt3;
-      rn = (2.0 - Om) * r1 - r2 + RHSr;
+  constexpr int LU = 4;  // slots per worker in flight
+  for (int x0 = worker * LU; x0 < Rmax; x0 += nworkers * LU) {
+    double2 e1[LU], e2[LU], fsx[LU];
+#pragma unroll
+    for (int u = 0; u < LU; ++u) {
+      const int x = min(x0 + u, Rmax - 1);
+      e1[u] = *reinterpret_cast<const double2*>(Hw + ((size_t)pair_off(a.s_tm1, x, N) * MB + lane) * 2);
+      e2[u] = *reinterpret_cast<const double2*>(Hw + ((size_t)pair_off(a.s_tm2, x, N) * MB + lane) * 2);
+      fsx[u] = Sb[(size_t)x * MB];
     }
-    if (real_site) {
-      double2 o = make_double2(Fn, rn * (a.s_t >= x ? 1.0 : -1.0));
-      *reinterpret_cast<double2*>(Hw + ((size_t)pair_index(a.s_t, x) * MB + lane) * 2) = o;
+#pragma unroll
+    for (int u = 0; u < LU; ++u) {
+      const int x = x0 + u;
+      if (x >= Rmax || x == dead) continue;
+      const int rel = (x - smin + N) % N;
+      const int tx = a.tmin + rel;
+      const double r1 = e1[u].y * (a.s_tm1 >= x ? 1.0 : -1.0);
+      const double r2 = e2[u].y * (a.s_tm2 >= x ? 1.0 : -1.0);
+      const double RHSF = accF[x * MB + lane] * dt3;
+      const double Fn = (2.0 - Om) * e1[u].x - e2[u].x + RHSF;
+      double rn;
+      if (tx == a.t - 1) {
+        rn = dt;
+      } else {
+        const double RHSr = accR[x * MB + lane] * dt3;
+        rn = (2.0 - Om) * r1 - r2 + RHSr;
+      }
+      if (real_site) {
+        double2 o = make_double2(Fn, rn * (a.s_t >= x ? 1.0 : -1.0));
+        *reinterpret_cast<double2*>(Hw + ((size_t)pair_off(a.s_t, x, N) * MB + lane) * 2) = o;
+      }
+      // terms of getRHS_F!(t,t) (Evolution.jl:973) over the fresh row
+      if (a.integral) {
+        const double w = rel == 0 ? 0.5 : 1.0;
+        const double m = rel == L - 1 ? 0.0 : 1.0;
+        dsum += -(m * w * fsx[u].y) * Fn - (w * fsx[u].x) * rn;
+      }
+      if (tx == a.t - 1) diagF[0][lane] = Fn;
+      if (tx == a.t - 2) diagF[1][lane] = Fn;
     }
-    // terms of getRHS_F!(t,t) (Evolution.jl:973) over the fresh row
-    const double w = rel == 0 ? 0.5 : 1.0;
-    const double m = rel == L - 1 ? 0.0 : 1.0;
-    dsum += -(m * w * sR[x * MB + lane]) * Fn - (w * sF[x * MB + lane]) * rn;
-    if (tx == a.t - 1) diagF[0][lane] = Fn;
-    if (tx == a.t - 2) diagF[1][lane] = Fn;
   }
-  red[tid] = dsum;
+  // reduce over the 8 workers of the wave (lane bits 3..5), then over waves
+  dsum += __shfl_xor(dsum, 8);
+  dsum += __shfl_xor(dsum, 16);
+  dsum += __shfl_xor(dsum, 32);
+  if ((tid & 63) < MB) red[tid >> 6][lane] = dsum;
   __syncthreads();
   if (tid < MB) {
     double RF = 0.0;
-    for (int wk = 0; wk < nworkers; ++wk) RF += red[wk * MB + tid];
+    for (int wv = 0; wv < (int)(blockDim.x >> 6); ++wv) RF += red[wv][tid];
     const double Ftt = (2.0 - Om) * diagF[0][tid] - diagF[1][tid] + RF * dt3;
     if (real_site) {
-      *reinterpret_cast<double2*>(Hw + ((size_t)pair_index(a.s_t, a.s_t) * MB + tid) * 2) =
+      *reinterpret_cast<double2*>(Hw + ((size_t)pair_off(a.s_t, a.s_t, N) * MB + tid) * 2) =
           make_double2(Ftt, 0.0);
     }
   }
@@ -469,9 +548,9 @@ __global__ __launch_bounds__(256) void measure_kernel(MeasArgs a) {
   if (p < a.V) {
     const int mb = p / MB, lane = p % MB;
     const double dt = a.dt;
-    const double Ftt = a.H[hoff(mb, pair_index(a.s_t, a.s_t), lane, a.NP)];
-    const double Fmt = a.H[hoff(mb, pair_index(a.s_tm1, a.s_t), lane, a.NP)];
-    const double* emm = a.H + hoff(mb, pair_index(a.s_tm1, a.s_tm1), lane, a.NP);
+    const double Ftt = a.H[hoff(mb, pair_off(a.s_t, a.s_t, a.N), lane, a.NP)];
+    const double Fmt = a.H[hoff(mb, pair_off(a.s_tm1, a.s_t, a.N), lane, a.NP)];
+    const double* emm = a.H + hoff(mb, pair_off(a.s_tm1, a.s_tm1, a.N), lane, a.NP);
     const double Fmm = emm[0];
     const double ddF = (Ftt - 2.0 * Fmt + Fmm) / (dt * dt);
     const double om = (ddF / Fmm) > 0 ? sqrt(ddF / Fmm) : 0.0;
@@ -500,7 +579,7 @@ __global__ __launch_bounds__(256) void measure_kernel(MeasArgs a) {
       // E_mem integrand over t' in [tmin, t-2] against F(t',t-1), rho(t',t-1)
       for (int tp = a.tmin; tp <= a.t - 2; ++tp) {
         const int x = (tp - 1) % a.N;
-        const double* e = a.H + hoff(mb, pair_index(x, a.s_tm1), lane, a.NP);
+        const double* e = a.H + hoff(mb, pair_off(x, a.s_tm1, a.N), lane, a.NP);
         const double rr = e[1] * (x >= a.s_tm1 ? 1.0 : -1.0);
         const double* S = a.S + soff(mb, x, lane, a.N);
         const double term = S[1] * e[0] - S[0] * rr;
@@ -537,7 +616,7 @@ __global__ void history_kernel(const double* __restrict__ H, const double* __res
   // F(tmin,tp), rho(tmin,tp) for tp = tmin + q
   const int xm = (tmin - 1) % N;
   const int x = (tmin + q - 1) % N;
-  const double* e = H + hoff(0, pair_index(xm, x), 0, NP);
+  const double* e = H + hoff(0, pair_off(xm, x, N), 0, NP);
   out[2 * L + q] = e[0];
   out[3 * L + q] = e[1] * (xm >= x ? 1.0 : -1.0);
 }
@@ -572,9 +651,9 @@ __global__ void init_rows_kernel(double* __restrict__ H, long NP, int V, int N, 
   if (p >= V) return;
   const int mb = p / MB, lane = p % MB;
   const int s1 = 0, s2 = 1 % N;
-  double* e11 = H + hoff(mb, pair_index(s1, s1), lane, NP);
-  double* e21 = H + hoff(mb, pair_index(s2, s1), lane, NP);
-  double* e22 = H + hoff(mb, pair_index(s2, s2), lane, NP);
+  double* e11 = H + hoff(mb, pair_off(s1, s1, N), lane, NP);
+  double* e21 = H + hoff(mb, pair_off(s2, s1, N), lane, NP);
+  double* e22 = H + hoff(mb, pair_off(s2, s2, N), lane, NP);
   e11[0] = F11[p];
   e11[1] = 0.0;
   e21[0] = F21[p];
@@ -595,40 +674,34 @@ __global__ void init_particles_kernel(const double* __restrict__ kL2, const doub
   F22[p] = (omega * (np + 0.5) * (dt * dt) + (1.0 / omega) * (np + 0.5));
 }
 
-// full deterministic window of free propagators (benchmark set-up)
+// full deterministic window of free propagators (benchmark set-up);
+// blockIdx.y = slot column c, threads over (site, slot row r >= c)
 __global__ void fill_free_kernel(double* __restrict__ H, long NP, int V, int N, int tmax,
                                  const double* __restrict__ kL2, const double* __restrict__ n, double mass,
-                                 double dt, long npairs_total) {
+                                 double dt, int nb) {
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  // gid enumerates (pair, site) with site fastest inside a block of MB
-  const int nb = (V + MB - 1) / MB;
-  if (gid >= npairs_total * nb * MB) return;
-  const int lane = gid % MB;
-  const long rest = gid / MB;
-  const long pair = rest % npairs_total;
-  const int mb = (int)(rest / npairs_total);
-  const int p = mb * MB + lane;
-  // decode pair -> (hi, lo)
-  int hi = (int)floor((sqrt(8.0 * (double)pair + 1.0) - 1.0) * 0.5);
-  while ((long)hi * (hi + 1) / 2 > pair) --hi;
-  while ((long)(hi + 1) * (hi + 2) / 2 <= pair) ++hi;
-  const int lo = (int)(pair - (long)hi * (hi + 1) / 2);
-  double F = 0.0, r = 0.0;
+  const int c = blockIdx.y;
+  if (gid >= (long)nb * MB * N) return;
+  const int r = (int)(gid % N);
+  const int p = (int)(gid / N);
+  if (r < c) return;
+  const int mb = p / MB, lane = p % MB;
+  double F = 0.0, rho = 0.0;
   if (p < V) {
     // times of the slots in the window [tmax-N+1, tmax]
     const int tmin = tmax - N + 1;
     const int smin = (tmin - 1) % N;
-    const int th = tmin + (hi - smin + N) % N;
-    const int tl = tmin + (lo - smin + N) % N;
+    const int th = tmin + (r - smin + N) % N;
+    const int tl = tmin + (c - smin + N) % N;
     const double w = sqrt(kL2[p] + mass * mass);
     const double nn = n ? n[p] : 0.0;
     const double d = (double)(th - tl) * dt;
     F = (nn + 0.5) / w * cos(w * d);
-    r = sin(w * d) / w;  // rho(th, tl) = stored value for (hi, lo)
+    rho = sin(w * d) / w;  // rho(th, tl): the stored value for slots (r, c)
   }
-  double* e = H + hoff(mb, pair, lane, NP);
+  double* e = H + hoff(mb, pair_off(r, c, N), lane, NP);
   e[0] = F;
-  e[1] = r;
+  e[1] = rho;
 }
 
 __global__ void finite_kernel(const double* __restrict__ H, size_t count, int* __restrict__ flag) {
