@@ -96,7 +96,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--nmem", type=int, default=WORKLOAD["NstepsinMemory"])
     ap.add_argument("--nx", type=int, default=WORKLOAD["Nx"])
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain 2 GiB device copy (torch) to scale rocprofv3 FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        x = torch.full((1 << 28,), 1.0, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read": x.numel() * 8, "bytes_written": y.numel() * 8}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not supported "
                                    "(multi-GPU sharding is design-only)"}))

@@ -167,3 +167,89 @@ def test_errors_and_window_bounds():
         g.get("F", 3, 1)  # outside [1, 2]
     g.step(3)
     g.check_finite()
+
+
+def test_free_theory_golden():
+    """lambda = 0 against the 50-digit closed form (tests/golden/free_theory.npz)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "free_theory.npz"))
+    from qftkb import KBContext
+    ctx = KBContext(int(g["Nx"]), int(g["sdim"]), int(g["Nmem"]), float(g["dt"]), 0.0, 1, float(g["Mass"]),
+                    float(g["Mass"]) ** 2)
+    ctx.init_particles(g["occ"], float(g["Mass"]))
+    for t in range(3, int(g["tend"]) + 1):
+        ctx.step(t)
+    for i, (a, b) in enumerate(g["pairs"]):
+        assert rel(ctx.get("F", a, b), g["F"][i]) <= 1e-12
+        if a == b:
+            assert np.all(ctx.get("r", a, b) == 0.0)
+        else:
+            assert rel(ctx.get("r", a, b), g["rho"][i]) <= 1e-12
+
+
+def test_config1_box_16cubed_full_memory():
+    """16^3, lambda = 1, m = 1, box n(p) = 2 Theta(1-|p|),
+    dt = 0.1, 200 steps, full memory (N_mem = Nsteps + 2): n(p,t) and E at
+    every step against the oracle."""
+    from qftkb import lattice
+    print("\n[case] config 1: 16^3 box, N_mem=202, 200 steps", flush=True)
+    Nx, steps = 16, 200
+    o, g = make_pair(Nx, 3, steps + 2)
+    n = lattice.n_box(o.kL2(), 2.0, 1.0)
+    o.init_particles(n, 1.0)
+    g.init_particles(n, 1.0)
+    worst = 0.0
+    for t in range(3, steps + 3):
+        o.evolve(t)
+        g.step(t)
+        mo, mg = compare_meas(o, g, t)
+        worst = max(worst, rel(mg["n"], mo["n"]))
+    compare_window(o, g)
+    print(f"[case] config 1 worst per-site n rel err {worst:.2e}", flush=True)
+
+
+def test_32cubed_identical_window_steps():
+    """32^3 lattice (the benchmark lattice) with N_mem = 24: the GPU-filled
+    window is copied into the oracle, then both take 4 steps."""
+    print("\n[case] 32^3 N_mem=24 from identical window", flush=True)
+    o, g = make_pair(32, 3, 24)
+    n = thermal(o.kL2())
+    g.fill_free_window(24, n, 1.0)
+    o.fill_free_window(24, n, 1.0)  # sets the window indices; values overwritten below
+    for a in range(1, 25):
+        for b in range(1, a + 1):
+            o.set("F", a, b, g.get("F", a, b))
+            o.set("r", a, b, g.get("r", a, b))
+    o.hartreemass2 = g.hartreemass2()
+    for t in range(25, 29):
+        o.evolve(t)
+        g.step(t)
+        compare_meas(o, g, t)
+    compare_window(o, g)
+
+
+def test_full_size_invariants_32cubed_nmem400():
+    """The benchmark workload itself (32^3, N_mem = 400): after a full
+    window, rho(t,t) = 0 and rho(t,t-1) = dt exactly, the new row keeps the
+    cubic symmetry of the thermal state, and nothing is non-finite."""
+    from qftkb import KBContext
+    print("\n[case] 32^3 N_mem=400 invariants", flush=True)
+    N, Nx = 400, 32
+    ctx = KBContext(Nx, 3, N, 0.1, 1.0, 1, 1.0, 1.0)
+    n = thermal(ctx.kL2())
+    ctx.fill_free_window(N, n, 1.0)
+    for t in range(N + 1, N + 4):
+        ctx.step(t)
+    t = N + 3
+    assert np.all(ctx.get("r", t, t) == 0.0)
+    assert np.all(ctx.get("r", t, t - 1) == 0.1)
+    for b in (t, t - 1, t - 200, t - N + 1):
+        for w in "Fr":
+            X = ctx.get(w, t, b).reshape(Nx, Nx, Nx)
+            s = np.abs(X).max()
+            for perm in [(1, 0, 2), (2, 1, 0), (0, 2, 1)]:
+                assert np.max(np.abs(X - X.transpose(perm))) <= 1e-12 * s
+            assert np.max(np.abs(X - X[:, :, (-np.arange(Nx)) % Nx])) <= 1e-12 * s
+    ctx.check_finite()
+    m = ctx.measure(t)
+    assert np.isfinite(m["E"]) and m["n_unstable"] == 0

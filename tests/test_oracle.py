@@ -1,0 +1,189 @@
+"""Pins of the oracle (CPU restatement of the reference KB step).
+
+The reference ships no known-answer tests and cannot run here (Julia), so
+the oracle is checked against: numpy's FFT, a direct O(V^3) double
+convolution for the sunset, an independent numpy restatement with different
+storage and transforms, the lambda = 0 closed form (golden fixture), the
+Gauss-state energy closed form (golden fixture) and symmetry invariants.
+Each check is chosen so that a dropped term, a wrong sign or index, or a
+transposed operand fails it.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle.kb_numpy import NumpyKB, kL2_table
+from oracle.oracle import Oracle
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def thermal(k2, T=2.0, m=1.0):
+    return 1.0 / (np.exp(np.sqrt(k2 + m * m) / (T * m)) - 1.0)
+
+
+@pytest.mark.parametrize("Nx,sdim", [(8, 3), (6, 3), (5, 3), (12, 2), (7, 2), (30, 1), (1, 3), (10, 3)])
+def test_dft_matches_numpy(Nx, sdim):
+    o = Oracle(Nx, sdim, 4, 0.1, 1.0, 1, 1.0, 1.0, nthreads=2)
+    rng = np.random.default_rng(Nx * 10 + sdim)
+    z = rng.standard_normal(o.V) + 1j * rng.standard_normal(o.V)
+    shape = (Nx,) * sdim
+    ref = np.fft.fftn(z.reshape(shape[::-1])).reshape(-1)
+    assert np.max(np.abs(o.dft(z, -1) - ref)) <= 1e-13 * np.max(np.abs(ref))
+    ref_b = np.fft.ifftn(z.reshape(shape[::-1])).reshape(-1) * o.V
+    assert np.max(np.abs(o.dft(z, +1) - ref_b)) <= 1e-13 * np.max(np.abs(ref_b))
+
+
+@pytest.mark.parametrize("Nx,sdim", [(4, 3), (3, 3), (6, 2), (5, 2), (8, 1), (2, 3)])
+def test_sunset_fft_equals_direct_convolution(Nx, sdim):
+    """Evolution.jl:25-31 (FFT) against the momentum-space double
+    convolution on random, non-symmetric two-time data (the role of the
+    reference's fast-vs-simple test, Tests/..._LambdaNLO_checkfastimplementation.jl)."""
+    N = 5
+    rng = np.random.default_rng(7 + Nx)
+    o_fft = Oracle(Nx, sdim, N, 0.1, 1.7, 3, 1.0, 1.0, nthreads=2)
+    o_dir = Oracle(Nx, sdim, N, 0.1, 1.7, 3, 1.0, 1.0, nthreads=2)
+    for o in (o_fft, o_dir):
+        o.init_particles(np.zeros(o.V), 1.0)
+        o.expand()
+        o.expand()  # window [1, 4]
+    tmin, tmax = o_fft.window()
+    for a in range(tmin, tmax + 1):
+        for b in range(tmin, a + 1):
+            F = rng.standard_normal(o_fft.V)
+            r = rng.standard_normal(o_fft.V)
+            for o in (o_fft, o_dir):
+                o.set("F", a, b, F)
+                o.set("r", a, b, r)
+    o_fft.selfenergies(tmax - 1)
+    o_dir.selfenergies(tmax - 1, direct=True)
+    for tp in range(tmin, tmax):
+        for w in "Fr":
+            x, y = o_fft.sigma(w, tp), o_dir.sigma(w, tp)
+            assert np.max(np.abs(x - y)) <= 1e-12 * np.max(np.abs(y)), (w, tp)
+
+
+def test_sunset_prefactors_against_numpy_rfft():
+    """The prefactors ivol^3 (-lambda^2 (N+2)/(6N^2)) and /(18N^2) and the
+    rfft/brfft normalisation, against numpy's rfftn/irfftn path."""
+    Nx, sdim = 6, 3
+    rng = np.random.default_rng(3)
+    o = Oracle(Nx, sdim, 4, 0.1, 2.5, 4, 1.0, 1.0, nthreads=2)
+    p = NumpyKB(Nx, sdim, 4, 0.1, 2.5, 4, 1.0, 1.0)
+    o.init_particles(np.zeros(o.V), 1.0)
+    o.expand()
+    F = rng.standard_normal(o.V)
+    r = rng.standard_normal(o.V)
+    o.set("F", 2, 1, F)
+    o.set("r", 2, 1, r)
+    o.selfenergies(2)
+    SF, Sr = p._sunset(F, r)
+    assert np.max(np.abs(o.sigma("F", 1) - SF)) <= 1e-12 * np.max(np.abs(SF))
+    assert np.max(np.abs(o.sigma("r", 1) - Sr)) <= 1e-12 * np.max(np.abs(Sr))
+
+
+@pytest.mark.parametrize("Nx,sdim,Nmem,steps", [(4, 3, 5, 22), (6, 2, 40, 20), (5, 3, 4, 15), (8, 1, 7, 30),
+                                                (1, 3, 3, 25)])
+def test_oracle_matches_numpy_twin(Nx, sdim, Nmem, steps):
+    args = (Nx, sdim, Nmem, 0.1, 1.3, 2, 1.0, 1.0)
+    o = Oracle(*args, nthreads=2)
+    p = NumpyKB(*args)
+    np.testing.assert_allclose(o.kL2(), kL2_table(Nx, sdim), rtol=4e-16, atol=4e-16)
+    n = thermal(p.kL2)
+    o.init_particles(n, 1.0)
+    p.init_particles(n, 1.0)
+    for t in range(3, steps + 1):
+        o.evolve(t)
+        p.evolve(t)
+        mo, mp = o.measure(t), p.measure(t)
+        for k in ("E", "E_kin", "E_pot", "M2"):
+            assert abs(mo[k] - mp[k]) <= 1e-12 * abs(mp[k])
+        assert abs(mo["E_mem"] - mp["E_mem"]) <= 1e-12 * abs(mp["E"])
+        assert np.max(np.abs(mo["n"] - mp["n"])) <= 1e-11 * np.max(np.abs(mp["n"]))
+    tmin, tmax = o.window()
+    for a in range(tmin, tmax + 1):
+        for b in range(tmin, a + 1):
+            assert np.max(np.abs(o.get("F", a, b) - p.getF(a, b))) <= 1e-12 * np.max(np.abs(p.getF(a, b)))
+            y = p.getr(a, b)
+            assert np.max(np.abs(o.get("r", a, b) - y)) <= 1e-12 * max(np.max(np.abs(y)), 1e-300)
+
+
+def test_free_theory_closed_form():
+    g = np.load(os.path.join(GOLDEN, "free_theory.npz"))
+    o = Oracle(int(g["Nx"]), int(g["sdim"]), int(g["Nmem"]), float(g["dt"]), 0.0, 1, float(g["Mass"]),
+               float(g["Mass"]) ** 2, nthreads=2)
+    o.init_particles(g["occ"], float(g["Mass"]))
+    for t in range(3, int(g["tend"]) + 1):
+        o.evolve(t)
+    for i, (a, b) in enumerate(g["pairs"]):
+        assert np.max(np.abs(o.get("F", a, b) - g["F"][i])) <= 1e-12 * np.max(np.abs(g["F"][i]))
+        if a == b:
+            assert np.all(o.get("r", a, b) == 0.0)
+        else:
+            assert np.max(np.abs(o.get("r", a, b) - g["rho"][i])) <= 1e-12 * np.max(np.abs(g["rho"][i]))
+
+
+def test_gauss_energy_closed_form():
+    """Nx = 1 (zero dimensions, Anders_NNLOpaper.jl:12) Gauss state: the
+    measured energy at t = 2 is the printed conserved energy."""
+    from qftkb import lattice
+
+    for case in json.load(open(os.path.join(GOLDEN, "gauss_energy.json"))):
+        dt = 0.01
+        o = Oracle(1, 3, 6, dt, case["Lambda"], case["ONgroup"], np.sqrt(case["Mass2"]), case["Mass2"],
+                   nthreads=1)
+        o.init_rows(*lattice.gauss_rows(1, dt, case["xi"], case["eta"], case["sig"]))
+        m = o.measure(2)
+        assert abs(m["E"] - case["E"]) <= 1e-12 * abs(case["E"])
+        assert m["E_mem"] == 0.0
+
+
+def test_energy_drift_small_lo():
+    """LambdaLO (Hartree only): the measured energy stays bounded.  The
+    measured E uses E_kin = ddF/2 (Measurement.jl:151-159), an estimator with
+    O((omega dt)^2) error, (omega dt)^2 ~ 0.03 at the lattice edge, so it
+    oscillates at the percent level without secular drift."""
+    o = Oracle(6, 3, 6, 0.05, 2.0, 1, 1.0, 1.0, pexp=0, nthreads=2)
+    o.init_particles(thermal(o.kL2()), 1.0)
+    E = [o.measure(2)["E"]]
+    for t in range(3, 80):
+        o.evolve(t)
+        E.append(o.measure(t)["E"])
+    E = np.array(E)
+    assert np.max(np.abs(E - E[0])) <= 2e-2 * abs(E[0])
+    assert abs(E[-1] - E[len(E) // 2]) <= 1e-2 * abs(E[0])
+
+
+def test_invariants_and_cubic_symmetry():
+    Nx = 6
+    o = Oracle(Nx, 3, 5, 0.1, 1.5, 1, 1.0, 1.0, nthreads=2)
+    o.init_particles(thermal(o.kL2()), 1.0)
+    for t in range(3, 16):
+        o.evolve(t)
+        assert np.all(o.get("r", t, t) == 0.0)
+        assert np.all(o.get("r", t, t - 1) == 0.1)
+        assert np.all(o.get("r", t - 1, t) == -0.1)
+    tmin, tmax = o.window()
+    for a, b in [(tmax, tmax), (tmax, tmin), (tmax - 1, tmin + 1)]:
+        for w in "Fr":
+            X = o.get(w, a, b).reshape(Nx, Nx, Nx)
+            for perm in [(1, 0, 2), (2, 1, 0), (0, 2, 1)]:
+                assert np.allclose(X, X.transpose(perm), rtol=1e-12, atol=1e-14 * np.abs(X).max())
+            refl = X[:, :, (-np.arange(Nx)) % Nx]
+            assert np.allclose(X, refl, rtol=1e-12, atol=1e-14 * np.abs(X).max())
+
+
+def test_window_bounds_errors():
+    o = Oracle(4, 3, 5, 0.1, 1.0, 1, 1.0, 1.0, nthreads=1)
+    o.init_particles(np.zeros(o.V), 1.0)
+    with pytest.raises(RuntimeError):
+        o.evolve(4)
+    with pytest.raises(RuntimeError):
+        o.get("F", 3, 1)
+    for t in range(3, 9):
+        o.evolve(t)
+    assert o.window() == (4, 8)
+    with pytest.raises(RuntimeError):
+        o.get("F", 8, 3)
