@@ -189,6 +189,11 @@ int launch_sigma(kb_ctx* c, int t) {
   a.s_tm1 = slot_of(c, t - 1);
   a.preF = ivol * ivol * ivol * (-(lam * lam) * (ON + 2.0) / (18.0 * ON * ON));
   a.preR = ivol * ivol * ivol * (-(lam * lam) * (ON + 2.0) / (6.0 * ON * ON));
+  if (sigma_planes_supported(c->p.Nx)) {
+    hipError_t e = launch_sigma_planes(a, L, c->stream);
+    if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "self-energy launch failed: %s", hipGetErrorString(e));
+    return KB_OK;
+  }
   const size_t lds = sizeof(double2) * 2 * (size_t)a.P * a.n;
   const dim3 grid((a.npencils + a.P - 1) / a.P, L);
   std::vector<std::pair<int, int>> passes;  // (axis, flags)
@@ -339,6 +344,10 @@ int kb_create(const kb_params* p, kb_ctx** out) {
   if ((rc = dmalloc(c, (void**)&c->items, sizeof(int2) * items.size()))) return fail(rc);
   if (hipMemcpy(c->items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice) != hipSuccess)
     return fail(KB_ERR_HIP);
+  if (sigma_planes_supported(p->Nx) && sigma_planes_prepare(p->Nx) != hipSuccess) {
+    seterr(c, KB_ERR_ARG, "self-energy plane kernels: LDS attribute rejected");
+    return fail(KB_ERR_ARG);
+  }
   {
     const size_t lds = sizeof(double) * 2 * (size_t)c->N * MB;
     hipError_t e = hipFuncSetAttribute((const void*)memint_step, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

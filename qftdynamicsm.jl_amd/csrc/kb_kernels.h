@@ -82,6 +82,11 @@ __global__ void fill_free_kernel(double* H, long NP, int V, int N, int tmax, con
                                  double mass, double dt, int nb);
 __global__ void finite_kernel(const double* H, size_t count, int* flag);
 
+// self-energy stage on whole planes (kb_sigma.hip), Nx <= 64
+bool sigma_planes_supported(int Nx);
+hipError_t sigma_planes_prepare(int n);
+hipError_t launch_sigma_planes(const SigmaArgs& a, int L, hipStream_t st);
+
 constexpr int MEMINT_THREADS = 256;
 constexpr int MEMINT_ROWS = 64;  // rows per work item
 constexpr int MEAS_THREADS = 256;

@@ -1,0 +1,389 @@
+// kb_sigma.hip -- self-energy stage of the KB step on whole lattice planes
+// (TwoPIScalar_Evolution.jl:16-31), for Nx <= 64.
+//
+// For every live t' the sunset needs Fx = real(rfft(F(t-1,t'))) and
+// rx = real(rfft(rho(t-1,t'))), the pointwise products rx (Fx^2 - rx^2/12)
+// and Fx (Fx^2 - 3/4 rx^2), and their inverse transforms.  real(rfft(X))
+// only sees the inversion-even part X_e(p) = (X(p) + X(-p))/2, whose
+// transform is real, so F_e + i rho_e is transformed as one complex lattice
+// and both transforms come out as its real and imaginary parts.
+//
+//   3D: plane_fwd  gathers the z-planes iz and -iz of F, rho, symmetrises,
+//                  2D-transforms plane iz in LDS and writes plane iz and its
+//                  point reflection as plane -iz (the 2D transform of the
+//                  even lattice's plane -iz is the reflection of plane iz's);
+//       zpass      z transform, sunset products, inverse z transform;
+//       plane_inv  inverse 2D transform, scale, write SigF/Sigr.
+//   2D / 1D: plane_fwd does everything for its row in one workgroup.
+//
+// All transforms are unnormalised Stockham FFTs in LDS; lines of a plane are
+// padded to Nx+1 elements so that column access does not hit one bank.
+// The FFT length is a template parameter for the common sizes (0 = runtime).
+#include "kb_kernels.h"
+
+namespace qftkb {
+
+namespace {
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 twid(const double2* __restrict__ tw, int k, bool inv) {
+  double2 w = tw[k];
+  if (inv) w.y = -w.y;
+  return w;
+}
+
+struct Fac {
+  int nf;
+  int f[24];
+};
+
+constexpr Fac factorize(int n) {
+  Fac F{0, {}};
+  int m = n;
+  while (m % 4 == 0) { F.f[F.nf++] = 4; m /= 4; }
+  while (m % 2 == 0) { F.f[F.nf++] = 2; m /= 2; }
+  for (int p = 3; m > 1; p += 2)
+    while (m % p == 0) { F.f[F.nf++] = p; m /= p; }
+  return F;
+}
+
+template <int R>
+__device__ __forceinline__ void small_dft(double2 (&v)[R], const double2* __restrict__ tw, int n, bool inv) {
+  if constexpr (R == 2) {
+    const double2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    const double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    const double2 b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
+    const double2 jb1 = inv ? make_double2(-b1.y, b1.x) : make_double2(b1.y, -b1.x);
+    v[0] = cadd(a0, b0);
+    v[2] = csub(a0, b0);
+    v[1] = cadd(a1, jb1);
+    v[3] = csub(a1, jb1);
+  } else {
+    double2 o[R];
+    const int m = n / R;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      double2 s = v[0];
+#pragma unroll
+      for (int r = 1; r < R; ++r) s = cadd(s, cmul(v[r], twid(tw, ((q * r) % R) * m, inv)));
+      o[q] = s;
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = o[q];
+  }
+}
+
+// One Stockham stage on `nlines` lines; element (line, j) at line*ls + j*es.
+template <int R>
+__device__ __forceinline__ void stage(const double2* __restrict__ src, double2* __restrict__ dst, int n, int Ns,
+                                      int nlines, int ls, int es, const double2* __restrict__ tw, bool inv) {
+  const int m = n / R;
+  const int step = n / (Ns * R);
+  for (int idx = threadIdx.x; idx < nlines * m; idx += blockDim.x) {
+    const int line = idx / m;
+    const int j = idx - line * m;
+    const int kk = j % Ns;
+    const double2* s = src + line * ls;
+    double2* d = dst + line * ls;
+    double2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double2 x = s[(j + r * m) * es];
+      v[r] = r == 0 ? x : cmul(x, twid(tw, r * kk * step, inv));  // r*kk*step < n
+    }
+    small_dft<R>(v, tw, n, inv);
+    const int base = (j / Ns) * Ns * R + kk;
+#pragma unroll
+    for (int q = 0; q < R; ++q) d[(base + q * Ns) * es] = v[q];
+  }
+}
+
+// prime radix > 7: one output element per thread
+__device__ void stage_direct(const double2* __restrict__ src, double2* __restrict__ dst, int n, int Ns, int R,
+                             int nlines, int ls, int es, const double2* __restrict__ tw, bool inv) {
+  const int m = n / R;
+  const int step = n / (Ns * R);
+  for (int idx = threadIdx.x; idx < nlines * n; idx += blockDim.x) {
+    const int line = idx / n;
+    const int rem = idx - line * n;
+    const int j = rem / R;
+    const int q = rem - j * R;
+    const int kk = j % Ns;
+    const double2* s = src + line * ls;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int r = 0; r < R; ++r) {
+      double2 x = s[(j + r * m) * es];
+      if (r) x = cmul(x, twid(tw, r * kk * step, inv));
+      acc = cadd(acc, cmul(x, twid(tw, ((q * r) % R) * m, inv)));
+    }
+    dst[line * ls + ((j / Ns) * Ns * R + kk + q * Ns) * es] = acc;
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void stage_any(const double2* src, double2* dst, int n, int Ns, int nlines, int ls, int es,
+                                          const double2* tw, bool inv) {
+  if constexpr (R == 2 || R == 3 || R == 4 || R == 5 || R == 7)
+    stage<R>(src, dst, n, Ns, nlines, ls, es, tw, inv);
+  else
+    stage_direct(src, dst, n, Ns, R, nlines, ls, es, tw, inv);
+}
+
+// compile-time length: every stage's radix, span and stride are constants
+template <int N, int S, int NS>
+__device__ __forceinline__ double2* stages_ct(double2* a, double2* b, int nlines, int ls, int es,
+                                              const double2* tw, bool inv) {
+  constexpr Fac F = factorize(N);
+  if constexpr (S >= F.nf) {
+    return a;
+  } else {
+    constexpr int R = F.f[S];
+    stage_any<R>(a, b, N, NS, nlines, ls, es, tw, inv);
+    __syncthreads();
+    return stages_ct<N, S + 1, NS * R>(b, a, nlines, ls, es, tw, inv);
+  }
+}
+
+__device__ double2* stages_rt(double2* a, double2* b, const FFTPlan& pl, int nlines, int ls, int es,
+                              const double2* tw, bool inv) {
+  int Ns = 1;
+  const int n = pl.n;
+  for (int s = 0; s < pl.nf; ++s) {
+    const int R = pl.fac[s];
+    switch (R) {
+      case 2: stage<2>(a, b, n, Ns, nlines, ls, es, tw, inv); break;
+      case 3: stage<3>(a, b, n, Ns, nlines, ls, es, tw, inv); break;
+      case 4: stage<4>(a, b, n, Ns, nlines, ls, es, tw, inv); break;
+      case 5: stage<5>(a, b, n, Ns, nlines, ls, es, tw, inv); break;
+      case 7: stage<7>(a, b, n, Ns, nlines, ls, es, tw, inv); break;
+      default: stage_direct(a, b, n, Ns, R, nlines, ls, es, tw, inv); break;
+    }
+    __syncthreads();
+    double2* t = a;
+    a = b;
+    b = t;
+    Ns *= R;
+  }
+  return a;
+}
+
+// Transform lines held in a (ping-pong with b); returns the result buffer.
+template <int NT>
+__device__ __forceinline__ double2* fft_lines(double2* a, double2* b, const FFTPlan& pl, int nlines, int ls, int es,
+                                              const double2* tw, bool inv) {
+  if constexpr (NT > 0)
+    return stages_ct<NT, 0, 1>(a, b, nlines, ls, es, tw, inv);
+  else
+    return stages_rt(a, b, pl, nlines, ls, es, tw, inv);
+}
+
+__device__ __forceinline__ double2 sunset(double2 z) {
+  const double Fx = z.x, rx = z.y;
+  const double FF = Fx * Fx, rr = rx * rx;
+  return make_double2(Fx * (FF - 0.75 * rr), rx * (FF - (1.0 / 12.0) * rr));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// blockIdx.x: plane iz (3D: 0 .. nz/2), blockIdx.y: row q (t' = tmin + q)
+template <int NT>
+__global__ __launch_bounds__(256) void sigma_plane_fwd(SigmaArgs a) {
+  extern __shared__ double2 lds[];
+  const int n = NT > 0 ? NT : a.n;
+  const int ny = a.ny, nz = a.nz;
+  const int ls = n + 1;  // padded line
+  double2* A = lds;
+  double2* B = lds + ny * ls;
+  const int q = blockIdx.y;
+  const int iz = blockIdx.x;
+  const int izn = (nz - iz) % nz;
+  const int tp = a.tmin + q;
+  const int slot = (tp - 1) % a.N;
+  const long pr = pair_off(a.s_tm1, slot, a.N);
+  const double sg = a.s_tm1 >= slot ? 1.0 : -1.0;
+  const size_t rowoff = (size_t)q * a.V;
+  const int nplane = n * ny;
+
+  // physical F(t-1,t'), rho(t-1,t') of plane iz and the reflected plane -iz,
+  // inversion-symmetrised
+  for (int e = threadIdx.x; e < nplane; e += blockDim.x) {
+    const int iy = e / n, ix = e - iy * n;
+    const int s1 = ix + n * (iy + ny * iz);
+    const int s2 = (n - ix) % n + n * ((ny - iy) % ny + ny * izn);
+    const double* h1 = a.H + hoff(s1 / MB, pr, s1 % MB, a.NP);
+    const double* h2 = a.H + hoff(s2 / MB, pr, s2 % MB, a.NP);
+    A[iy * ls + ix] = make_double2(0.5 * (h1[0] + h2[0]), 0.5 * (h1[1] + h2[1]) * sg);
+  }
+  __syncthreads();
+  double2* res = fft_lines<NT>(A, B, a.plan, ny, ls, 1, a.tw, false);
+  double2* oth = res == A ? B : A;
+  if (ny > 1) {
+    res = fft_lines<NT>(res, oth, a.plan, n, 1, ls, a.tw, false);
+    oth = res == A ? B : A;
+  }
+  if (nz == 1) {
+    // 1D / 2D: the whole lattice is in LDS -- sunset, inverse, write Sigma
+    for (int e = threadIdx.x; e < nplane; e += blockDim.x) {
+      const int iy = e / n, ix = e - iy * n;
+      res[iy * ls + ix] = sunset(res[iy * ls + ix]);
+    }
+    __syncthreads();
+    if (ny > 1) {
+      res = fft_lines<NT>(res, oth, a.plan, n, 1, ls, a.tw, true);
+      oth = res == A ? B : A;
+    }
+    res = fft_lines<NT>(res, oth, a.plan, ny, ls, 1, a.tw, true);
+    for (int e = threadIdx.x; e < nplane; e += blockDim.x) {
+      const int iy = e / n, ix = e - iy * n;
+      const int s = ix + n * iy;
+      const double2 z = res[iy * ls + ix];
+      double* o = a.S + soff(s / MB, slot, s % MB, a.N);
+      o[0] = a.preF * z.x;
+      o[1] = a.preR * z.y;
+    }
+    return;
+  }
+  for (int e = threadIdx.x; e < nplane; e += blockDim.x) {
+    const int iy = e / n, ix = e - iy * n;
+    a.W[rowoff + ix + n * (iy + ny * iz)] = res[iy * ls + ix];
+    if (izn != iz) {
+      const int jx = (n - ix) % n, jy = (ny - iy) % ny;
+      a.W[rowoff + ix + n * (iy + ny * izn)] = res[jy * ls + jx];
+    }
+  }
+}
+
+// blockIdx.x: y row iy, blockIdx.y: row q.  Lines along z for every x.
+template <int NT>
+__global__ __launch_bounds__(256) void sigma_zpass(SigmaArgs a) {
+  extern __shared__ double2 lds[];
+  const int n = NT > 0 ? NT : a.n;
+  const int ny = a.ny, nz = a.nz;
+  const int ls = nz + 1;
+  double2* A = lds;
+  double2* B = lds + n * ls;
+  const int q = blockIdx.y;
+  const int iy = blockIdx.x;
+  const size_t rowoff = (size_t)q * a.V;
+  for (int e = threadIdx.x; e < n * nz; e += blockDim.x) {
+    const int iz = e / n, ix = e - iz * n;
+    A[ix * ls + iz] = a.W[rowoff + ix + n * (iy + ny * iz)];
+  }
+  __syncthreads();
+  double2* res = fft_lines<NT>(A, B, a.plan, n, ls, 1, a.tw, false);
+  double2* oth = res == A ? B : A;
+  for (int e = threadIdx.x; e < n * nz; e += blockDim.x) {
+    const int ix = e / nz, iz = e - ix * nz;
+    res[ix * ls + iz] = sunset(res[ix * ls + iz]);
+  }
+  __syncthreads();
+  res = fft_lines<NT>(res, oth, a.plan, n, ls, 1, a.tw, true);
+  for (int e = threadIdx.x; e < n * nz; e += blockDim.x) {
+    const int iz = e / n, ix = e - iz * n;
+    a.W[rowoff + ix + n * (iy + ny * iz)] = res[ix * ls + iz];
+  }
+}
+
+// blockIdx.x: plane iz, blockIdx.y: row q.  Inverse 2D transform + scatter.
+template <int NT>
+__global__ __launch_bounds__(256) void sigma_plane_inv(SigmaArgs a) {
+  extern __shared__ double2 lds[];
+  const int n = NT > 0 ? NT : a.n;
+  const int ny = a.ny;
+  const int ls = n + 1;
+  double2* A = lds;
+  double2* B = lds + ny * ls;
+  const int q = blockIdx.y;
+  const int iz = blockIdx.x;
+  const int tp = a.tmin + q;
+  const int slot = (tp - 1) % a.N;
+  const size_t rowoff = (size_t)q * a.V;
+  const int nplane = n * ny;
+  for (int e = threadIdx.x; e < nplane; e += blockDim.x) {
+    const int iy = e / n, ix = e - iy * n;
+    A[iy * ls + ix] = a.W[rowoff + ix + n * (iy + ny * iz)];
+  }
+  __syncthreads();
+  double2* res = fft_lines<NT>(A, B, a.plan, n, 1, ls, a.tw, true);
+  double2* oth = res == A ? B : A;
+  res = fft_lines<NT>(res, oth, a.plan, ny, ls, 1, a.tw, true);
+  for (int e = threadIdx.x; e < nplane; e += blockDim.x) {
+    const int iy = e / n, ix = e - iy * n;
+    const int s = ix + n * (iy + ny * iz);
+    const double2 z = res[iy * ls + ix];
+    double* o = a.S + soff(s / MB, slot, s % MB, a.N);
+    o[0] = a.preF * z.x;
+    o[1] = a.preR * z.y;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host-side launch of the whole stage
+// ---------------------------------------------------------------------------
+template <int NT>
+static hipError_t launch_planes_t(const SigmaArgs& a, int L, hipStream_t st) {
+  const int n = a.n;
+  const size_t lds_plane = sizeof(double2) * 2 * (size_t)a.ny * (n + 1);
+  if (a.nz == 1) {
+    hipLaunchKernelGGL(sigma_plane_fwd<NT>, dim3(1, L), dim3(256), lds_plane, st, a);
+    return hipGetLastError();
+  }
+  const size_t lds_z = sizeof(double2) * 2 * (size_t)n * (a.nz + 1);
+  hipLaunchKernelGGL(sigma_plane_fwd<NT>, dim3(a.nz / 2 + 1, L), dim3(256), lds_plane, st, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(sigma_zpass<NT>, dim3(a.ny, L), dim3(256), lds_z, st, a);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(sigma_plane_inv<NT>, dim3(a.nz, L), dim3(256), lds_plane, st, a);
+  return hipGetLastError();
+}
+
+bool sigma_planes_supported(int Nx) { return Nx >= 1 && Nx <= 64; }
+
+template <int NT>
+static hipError_t prepare_t(size_t lds) {
+  hipError_t e = hipFuncSetAttribute((const void*)sigma_plane_fwd<NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)sigma_zpass<NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void*)sigma_plane_inv<NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+// allow the dynamic LDS of the plane kernels for lattice size n
+hipError_t sigma_planes_prepare(int n) {
+  const size_t lds = sizeof(double2) * 2 * (size_t)n * (n + 1);
+  switch (n) {
+    case 8: return prepare_t<8>(lds);
+    case 16: return prepare_t<16>(lds);
+    case 24: return prepare_t<24>(lds);
+    case 32: return prepare_t<32>(lds);
+    case 40: return prepare_t<40>(lds);
+    case 48: return prepare_t<48>(lds);
+    case 64: return prepare_t<64>(lds);
+    default: return prepare_t<0>(lds);
+  }
+}
+
+hipError_t launch_sigma_planes(const SigmaArgs& a, int L, hipStream_t st) {
+  switch (a.n) {
+    case 8: return launch_planes_t<8>(a, L, st);
+    case 16: return launch_planes_t<16>(a, L, st);
+    case 24: return launch_planes_t<24>(a, L, st);
+    case 32: return launch_planes_t<32>(a, L, st);
+    case 40: return launch_planes_t<40>(a, L, st);
+    case 48: return launch_planes_t<48>(a, L, st);
+    case 64: return launch_planes_t<64>(a, L, st);
+    default: return launch_planes_t<0>(a, L, st);
+  }
+}
+
+}  // namespace qftkb
