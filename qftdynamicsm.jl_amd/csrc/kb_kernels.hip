@@ -293,6 +293,21 @@ __device__ __forceinline__ void slot_weights(int x, int smin, int N, int L, doub
   m = rel == L - 1 ? 0.0 : 1.0;
 }
 
+// History lines are streamed once per step: load them non-temporally so
+// they do not evict the self-energy lines that are re-read from L2.
+#ifndef KB_NT_LOADS
+#define KB_NT_LOADS 1
+#endif
+typedef double kb_d2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ld_stream(const double* p) {
+#if KB_NT_LOADS
+  const kb_d2v v = __builtin_nontemporal_load(reinterpret_cast<const kb_d2v*>(p));
+  return make_double2(v.x, v.y);
+#else
+  return *reinterpret_cast<const double2*>(p);
+#endif
+}
+
 struct RowBatch {
   double2 e[MI_B][CB];  // (F, r) of slots (row, c0 + jc)
   double2 fs[MI_B];     // (SigF, Sigr) of the row slot
@@ -307,7 +322,7 @@ __device__ __forceinline__ void load_batch(RowBatch& b, const double* __restrict
     const int rr = min(r + h, rlast);
     const long pb = cbase + (long)(rr - c0) * CB;
 #pragma unroll
-    for (int jc = 0; jc < CB; ++jc) b.e[h][jc] = *reinterpret_cast<const double2*>(Hb + (size_t)(pb + jc) * MB * 2);
+    for (int jc = 0; jc < CB; ++jc) b.e[h][jc] = ld_stream(Hb + (size_t)(pb + jc) * MB * 2);
     b.fs[h] = Sb[(size_t)rr * MB];
   }
 }

@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 LIB_NAME = "libqftkb.so"
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+# QFTKB_LIBRARY may point at another build of the same library (A/B timing)
+LIB_PATH = os.environ.get("QFTKB_LIBRARY") or os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
 KB_OK = 0
 KB_ERR_ARG = 1
