@@ -87,7 +87,10 @@ bool sigma_planes_supported(int Nx);
 hipError_t sigma_planes_prepare(int n);
 hipError_t launch_sigma_planes(const SigmaArgs& a, int L, hipStream_t st);
 
-constexpr int MEMINT_THREADS = 256;
+#ifndef KB_MEMINT_THREADS
+#define KB_MEMINT_THREADS 256
+#endif
+constexpr int MEMINT_THREADS = KB_MEMINT_THREADS;
 constexpr int MEMINT_ROWS = 64;  // rows per work item
 constexpr int MEAS_THREADS = 256;
 
