@@ -102,6 +102,12 @@ int kb_init_rows(kb_ctx *ctx, const double *F11, const double *F21,
  * of F and rho.  t must equal tmax + 1 and be >= 3. */
 int kb_step(kb_ctx *ctx, int t);
 
+/* Replay each step from a captured hipGraph (on = 1): the first kb_step
+ * after enabling captures the step's kernels (advance, self-energy,
+ * Hartree, memory integral), later steps launch the graph.  Ignored while
+ * timing is enabled.  No counterpart in the reference (launch mechanics). */
+int kb_use_graph(kb_ctx *ctx, int on);
+
 /* simdata.indices after the last call. */
 int kb_window(const kb_ctx *ctx, int *tmin, int *tmax);
 
@@ -149,6 +155,32 @@ int kb_set_stream(kb_ctx *ctx, void *hip_stream);
 int kb_enable_timing(kb_ctx *ctx, int on);
 int kb_timing(kb_ctx *ctx, int stage, double *total_ms, long long *count);
 int kb_reset_timing(kb_ctx *ctx);
+
+/* ---- ensembles ------------------------------------------------------------
+ * Many independent KB runs (uneven lattices, couplings, memory windows)
+ * advanced together: every member keeps its own device state; one ensemble
+ * step launches each member's self-energy and Hartree kernels and ONE
+ * memory-integral launch over the site blocks of all members.  The
+ * reference runs such ensembles as separate jobs (G_SimulationArgs.jl,
+ * submit*.sh); there is no ensemble type to mirror. */
+typedef struct kb_ens kb_ens;
+
+/* nruns parameter sets; all members live on params[0].device. */
+int kb_ens_create(const kb_params *params, int nruns, kb_ens **out);
+int kb_ens_destroy(kb_ens *ens);
+const char *kb_ens_last_error(const kb_ens *ens);
+/* Member i as a kb_ctx for initialisation, measurement and read-out
+ * (kb_init_*, kb_measure, kb_get_entry, ...).  kb_step on a member is
+ * rejected; members advance only through kb_ens_step. */
+int kb_ens_member(kb_ens *ens, int i, kb_ctx **member);
+/* Every member takes step t (all members must be at tmax = t - 1). */
+int kb_ens_step(kb_ens *ens, int t);
+/* Capture one ensemble step as a hipGraph and replay it (on = 1). */
+int kb_ens_use_graph(kb_ens *ens, int on);
+int kb_ens_synchronize(kb_ens *ens);
+/* HIP-event time of whole ensemble steps since the last reset. */
+int kb_ens_enable_timing(kb_ens *ens, int on);
+int kb_ens_timing(kb_ens *ens, double *total_ms, long long *count);
 
 #ifdef __cplusplus
 }

@@ -29,8 +29,9 @@ struct kb_ctx {
   int V = 0, Vpad = 0, NB = 0, N = 0;
   long NP = 0;
   int nx = 1, ny = 1, nz = 1;
-  int tmin = 0, tmax = 0;
+  StepState st{};          // host mirror of the window (simdata.indices)
   bool initialised = false;
+  bool in_ensemble = false;
   // device memory
   double* H = nullptr;     // history
   double* S = nullptr;     // self-energy ring
@@ -44,6 +45,15 @@ struct kb_ctx {
   int* flag = nullptr;
   int2* items = nullptr;  // memory-integral work items
   int nitems = 0;
+  // device-resident step state and launch tables (graph-capturable steps)
+  StepState* st_dev = nullptr;
+  StepState** stptr_dev = nullptr;  // {st_dev}
+  int* N_dev = nullptr;
+  MemintArgs* margs_dev = nullptr;
+  int* wg0_dev = nullptr;  // {0}
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
   size_t bytes = 0;
   FFTPlan plan{};
   int fftP = 1;
@@ -86,7 +96,7 @@ int seterr(kb_ctx* c, int code, const char* fmt, ...) {
   } while (0)
 
 inline int slot_of(const kb_ctx* c, int t) { return (t - 1) % c->N; }
-inline bool inwin(const kb_ctx* c, int t) { return c->tmin <= t && t <= c->tmax; }
+inline bool inwin(const kb_ctx* c, int t) { return c->st.tmin <= t && t <= c->st.tmax; }
 
 void plan_fft(FFTPlan& pl, int n) {
   pl.n = n;
@@ -144,15 +154,19 @@ int collect_timing(kb_ctx* c) {
   return KB_OK;
 }
 
-void expand(kb_ctx* c) {
-  // expandSimData!, SimDataStruct.jl:262-278
-  if (c->tmax + 1 > c->N) {
-    c->tmin += 1;
-    c->tmax += 1;
-  } else {
-    c->tmin = 1;
-    c->tmax += 1;
-  }
+// expandSimData! (SimDataStruct.jl:262-278) on the host mirror
+void expand(kb_ctx* c) { advance_state(c->st, c->N); }
+
+int upload_state(kb_ctx* c) {
+  HIPCHK(c, hipMemcpyAsync(c->st_dev, &c->st, sizeof(StepState), hipMemcpyHostToDevice, c->stream));
+  return KB_OK;
+}
+
+void drop_graph(kb_ctx* c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
 }
 
 int launch_hartree(kb_ctx* c, int i, int j) {
@@ -165,9 +179,8 @@ int launch_hartree(kb_ctx* c, int i, int j) {
   return KB_OK;
 }
 
-int launch_sigma(kb_ctx* c, int t) {
-  const int L = t - c->tmin;  // rows t' = tmin .. t-1
-  if (L <= 0) return KB_OK;
+int launch_sigma(kb_ctx* c) {
+  const int rows = c->N - 1;  // grid rows; rows q >= L exit at once
   const double ivol = 1.0 / (double)c->V;
   const double lam = c->p.Lambda, ON = (double)c->p.ONgroup;
   SigmaArgs a{};
@@ -185,10 +198,10 @@ int launch_sigma(kb_ctx* c, int t) {
   a.nz = c->nz;
   a.P = c->fftP;
   a.npencils = c->V / c->p.Nx;
-  a.tmin = c->tmin;
-  a.s_tm1 = slot_of(c, t - 1);
+  a.st = c->st_dev;
   a.preF = ivol * ivol * ivol * (-(lam * lam) * (ON + 2.0) / (18.0 * ON * ON));
   a.preR = ivol * ivol * ivol * (-(lam * lam) * (ON + 2.0) / (6.0 * ON * ON));
+  const int L = rows;
   if (sigma_planes_supported(c->p.Nx)) {
     hipError_t e = launch_sigma_planes(a, L, c->stream);
     if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "self-energy launch failed: %s", hipGetErrorString(e));
@@ -220,7 +233,7 @@ int launch_sigma(kb_ctx* c, int t) {
   return KB_OK;
 }
 
-int launch_memint(kb_ctx* c, int t) {
+MemintArgs memint_args(const kb_ctx* c) {
   MemintArgs a{};
   a.H = c->H;
   a.S = c->S;
@@ -229,23 +242,45 @@ int launch_memint(kb_ctx* c, int t) {
   a.NP = c->NP;
   a.N = c->N;
   a.V = c->V;
-  const bool full = t > c->N;
-  a.Rmax = full ? c->N : t - 1;
-  a.s_t = slot_of(c, t);
-  a.dead = full ? a.s_t : -1;
-  a.s_tm1 = slot_of(c, t - 1);
-  a.s_tm2 = slot_of(c, t - 2);
-  a.s_min = slot_of(c, c->tmin);
-  a.L = t - c->tmin;
-  a.tmin = c->tmin;
-  a.t = t;
   a.dt = c->p.dt;
   a.integral = c->p.pexp == KB_PEXP_LAMBDA_NLO;
   a.items = c->items;
   a.nitems = c->nitems;
-  const size_t lds = sizeof(double) * 2 * (size_t)c->N * MB;
-  hipLaunchKernelGGL(memint_step, dim3(c->NB), dim3(MEMINT_THREADS), lds, c->stream, a);
+  a.st = c->st_dev;
+  return a;
+}
+
+size_t memint_lds(int N) { return sizeof(double) * 2 * (size_t)N * MB; }
+
+int launch_hartree_step(kb_ctx* c) {
+  const double H = c->p.Lambda * (double)(c->p.ONgroup + 2) / (6.0 * (double)c->p.ONgroup);
+  const double hfac = H * (1.0 / (double)c->V);
+  hipLaunchKernelGGL(hartree_step_kernel, dim3(1), dim3(1024), 0, c->stream, c->H, c->NP, c->V, c->N, c->st_dev,
+                     hfac, c->p.Mass2, c->m2);
   LAUNCHCHK(c);
+  return KB_OK;
+}
+
+// The kernels of one KB step (evolve!, Evolution.jl:924-981); every
+// argument is step-invariant, the window comes from st_dev.
+int enqueue_step(kb_ctx* c) {
+  int rc;
+  hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, c->stream, c->stptr_dev, c->N_dev, 1);
+  LAUNCHCHK(c);
+  if (c->p.pexp == KB_PEXP_LAMBDA_NLO) {
+    StageTimer st(c, 1);
+    if ((rc = launch_sigma(c))) return rc;
+  }
+  {
+    StageTimer st(c, 2);
+    if ((rc = launch_hartree_step(c))) return rc;
+  }
+  {
+    StageTimer st(c, 0);
+    hipLaunchKernelGGL(memint_step, dim3(c->NB), dim3(MEMINT_THREADS), memint_lds(c->N), c->stream, c->margs_dev,
+                       c->wg0_dev, 1);
+    LAUNCHCHK(c);
+  }
   return KB_OK;
 }
 
@@ -260,9 +295,13 @@ int download(kb_ctx* c, double* dst, const double* src, size_t n) {
   return KB_OK;
 }
 
+void drop_graph(kb_ctx* c);
+
 void free_all(kb_ctx* c) {
+  drop_graph(c);
   for (void* ptr : {(void*)c->H, (void*)c->S, (void*)c->W, (void*)c->kL2, (void*)c->m2, (void*)c->partial,
-                    (void*)c->modes, (void*)c->vbuf, (void*)c->tw, (void*)c->flag, (void*)c->items})
+                    (void*)c->modes, (void*)c->vbuf, (void*)c->tw, (void*)c->flag, (void*)c->items,
+                    (void*)c->st_dev, (void*)c->stptr_dev, (void*)c->N_dev, (void*)c->margs_dev, (void*)c->wg0_dev})
     if (ptr) (void)hipFree(ptr);
   for (int s = 0; s < 3; ++s)
     for (auto& ev : c->pending[s]) {
@@ -344,15 +383,32 @@ int kb_create(const kb_params* p, kb_ctx** out) {
   if ((rc = dmalloc(c, (void**)&c->items, sizeof(int2) * items.size()))) return fail(rc);
   if (hipMemcpy(c->items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice) != hipSuccess)
     return fail(KB_ERR_HIP);
+  if ((rc = dmalloc(c, (void**)&c->st_dev, sizeof(StepState)))) return fail(rc);
+  if ((rc = dmalloc(c, (void**)&c->stptr_dev, sizeof(StepState*)))) return fail(rc);
+  if ((rc = dmalloc(c, (void**)&c->N_dev, sizeof(int)))) return fail(rc);
+  if ((rc = dmalloc(c, (void**)&c->margs_dev, sizeof(MemintArgs)))) return fail(rc);
+  if ((rc = dmalloc(c, (void**)&c->wg0_dev, sizeof(int)))) return fail(rc);
+  {
+    const int zero = 0;
+    const MemintArgs ma = memint_args(c);
+    if (hipMemcpy(c->st_dev, &c->st, sizeof(StepState), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->stptr_dev, &c->st_dev, sizeof(StepState*), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->N_dev, &c->N, sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->margs_dev, &ma, sizeof(MemintArgs), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->wg0_dev, &zero, sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(KB_ERR_HIP);
+  }
   if (sigma_planes_supported(p->Nx) && sigma_planes_prepare(p->Nx) != hipSuccess) {
     seterr(c, KB_ERR_ARG, "self-energy plane kernels: LDS attribute rejected");
     return fail(KB_ERR_ARG);
   }
   {
-    const size_t lds = sizeof(double) * 2 * (size_t)c->N * MB;
+    // the attribute is per function, shared by every context: allow the
+    // largest supported window, not this context's
+    const size_t lds = memint_lds(640);
     hipError_t e = hipFuncSetAttribute((const void*)memint_step, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) {
-      seterr(c, KB_ERR_ARG, "memory window N=%d needs %zu bytes of LDS: %s", c->N, lds, hipGetErrorString(e));
+      seterr(c, KB_ERR_ARG, "memory-integral kernel: %zu bytes of LDS rejected: %s", lds, hipGetErrorString(e));
       return fail(KB_ERR_ARG);
     }
   }
@@ -418,7 +474,7 @@ int kb_set_kL2(kb_ctx* c, const double* in) {
 
 int kb_init_rows(kb_ctx* c, const double* F11, const double* F21, const double* F22) {
   if (!c || !F11 || !F21 || !F22) return KB_ERR_ARG;
-  if (c->tmax != 0) return seterr(c, KB_ERR_STATE, "init: context already initialised");
+  if (c->st.tmax != 0) return seterr(c, KB_ERR_STATE, "init: context already initialised");
   int rc;
   double* d = c->vbuf;
   if ((rc = upload(c, d, F11, c->V)) || (rc = upload(c, d + c->V, F21, c->V)) ||
@@ -426,6 +482,7 @@ int kb_init_rows(kb_ctx* c, const double* F11, const double* F21, const double* 
     return rc;
   expand(c);
   expand(c);
+  if ((rc = upload_state(c))) return rc;
   const int blocks = (c->V + 255) / 256;
   hipLaunchKernelGGL(init_rows_kernel, dim3(blocks), dim3(256), 0, c->stream, c->H, c->NP, c->V, c->N, d,
                      d + c->V, d + 2 * c->V, c->p.dt);
@@ -438,7 +495,7 @@ int kb_init_rows(kb_ctx* c, const double* F11, const double* F21, const double* 
 
 int kb_init_particles(kb_ctx* c, const double* n, double initmass) {
   if (!c || !n) return KB_ERR_ARG;
-  if (c->tmax != 0) return seterr(c, KB_ERR_STATE, "init: context already initialised");
+  if (c->st.tmax != 0) return seterr(c, KB_ERR_STATE, "init: context already initialised");
   int rc;
   double* d = c->vbuf;
   if ((rc = upload(c, d + 3 * c->V, n, c->V))) return rc;
@@ -448,6 +505,7 @@ int kb_init_particles(kb_ctx* c, const double* n, double initmass) {
   LAUNCHCHK(c);
   expand(c);
   expand(c);
+  if ((rc = upload_state(c))) return rc;
   hipLaunchKernelGGL(init_rows_kernel, dim3(blocks), dim3(256), 0, c->stream, c->H, c->NP, c->V, c->N, d,
                      d + c->V, d + 2 * c->V, c->p.dt);
   LAUNCHCHK(c);
@@ -460,29 +518,41 @@ int kb_init_particles(kb_ctx* c, const double* n, double initmass) {
 int kb_step(kb_ctx* c, int t) {
   if (!c) return KB_ERR_ARG;
   if (!c->initialised) return seterr(c, KB_ERR_STATE, "step before init");
-  if (t != c->tmax + 1 || t < 3)
-    return seterr(c, KB_ERR_WINDOW, "evolve: t=%d but window is [%d,%d]", t, c->tmin, c->tmax);
-  expand(c);
-  int rc;
-  if (c->p.pexp == KB_PEXP_LAMBDA_NLO) {
-    StageTimer st(c, 1);
-    if ((rc = launch_sigma(c, t))) return rc;
+  if (c->in_ensemble) return seterr(c, KB_ERR_STATE, "context is an ensemble member: use kb_ens_step");
+  if (t != c->st.tmax + 1 || t < 3)
+    return seterr(c, KB_ERR_WINDOW, "evolve: t=%d but window is [%d,%d]", t, c->st.tmin, c->st.tmax);
+  expand(c);  // the device state is advanced by the step's first kernel
+  if (c->use_graph && !c->timing) {
+    if (!c->gexec) {
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int rc = enqueue_step(c);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "step capture failed: %s", hipGetErrorString(e));
+      c->graph = g;
+      HIPCHK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    return KB_OK;
   }
-  {
-    StageTimer st(c, 2);
-    if ((rc = launch_hartree(c, t - 1, t - 1))) return rc;
-  }
-  {
-    StageTimer st(c, 0);
-    if ((rc = launch_memint(c, t))) return rc;
-  }
+  return enqueue_step(c);
+}
+
+int kb_use_graph(kb_ctx* c, int on) {
+  if (!c) return KB_ERR_ARG;
+  if (!on) drop_graph(c);
+  c->use_graph = on != 0;
   return KB_OK;
 }
 
 int kb_window(const kb_ctx* c, int* tmin, int* tmax) {
   if (!c) return KB_ERR_ARG;
-  if (tmin) *tmin = c->tmin;
-  if (tmax) *tmax = c->tmax;
+  if (tmin) *tmin = c->st.tmin;
+  if (tmax) *tmax = c->st.tmax;
   return KB_OK;
 }
 
@@ -495,7 +565,7 @@ int kb_measure(kb_ctx* c, int t, kb_meas* out, double* F, double* ddF, double* r
                double* cc) {
   if (!c || !out) return KB_ERR_ARG;
   if (!c->initialised) return seterr(c, KB_ERR_STATE, "measure before init");
-  if (t != c->tmax || !inwin(c, t - 1)) return seterr(c, KB_ERR_WINDOW, "measure: t=%d, window [%d,%d]", t, c->tmin, c->tmax);
+  if (t != c->st.tmax || !inwin(c, t - 1)) return seterr(c, KB_ERR_WINDOW, "measure: t=%d, window [%d,%d]", t, c->st.tmin, c->st.tmax);
   MeasArgs a{};
   a.H = c->H;
   a.S = c->S;
@@ -505,7 +575,7 @@ int kb_measure(kb_ctx* c, int t, kb_meas* out, double* F, double* ddF, double* r
   a.V = c->V;
   a.s_t = slot_of(c, t);
   a.s_tm1 = slot_of(c, t - 1);
-  a.tmin = c->tmin;
+  a.tmin = c->st.tmin;
   a.t = t;
   a.dt = c->p.dt;
   a.integral = c->p.pexp == KB_PEXP_LAMBDA_NLO;
@@ -554,11 +624,11 @@ int kb_measure(kb_ctx* c, int t, kb_meas* out, double* F, double* ddF, double* r
 
 int kb_measure_history(kb_ctx* c, int t, double* SigF, double* Sigr, double* Fonet, double* ronet) {
   if (!c) return KB_ERR_ARG;
-  if (t != c->tmax) return seterr(c, KB_ERR_WINDOW, "history: t=%d, window [%d,%d]", t, c->tmin, c->tmax);
-  const int L = t - c->tmin;
+  if (t != c->st.tmax) return seterr(c, KB_ERR_WINDOW, "history: t=%d, window [%d,%d]", t, c->st.tmin, c->st.tmax);
+  const int L = t - c->st.tmin;
   if (L <= 0) return KB_OK;
   hipLaunchKernelGGL(history_kernel, dim3((L + 255) / 256), dim3(256), 0, c->stream, c->H, c->S, c->NP, c->N,
-                     c->tmin, t, c->vbuf);
+                     c->st.tmin, t, c->vbuf);
   LAUNCHCHK(c);
   std::vector<double> h((size_t)4 * L);
   int rc;
@@ -572,7 +642,7 @@ int kb_measure_history(kb_ctx* c, int t, double* SigF, double* Sigr, double* Fon
 int kb_get_entry(kb_ctx* c, int which, int i, int j, double* out) {
   if (!c || !out || (which != 0 && which != 1)) return KB_ERR_ARG;
   if (!inwin(c, i) || !inwin(c, j))
-    return seterr(c, KB_ERR_WINDOW, "get: (%d,%d) outside window [%d,%d]", i, j, c->tmin, c->tmax);
+    return seterr(c, KB_ERR_WINDOW, "get: (%d,%d) outside window [%d,%d]", i, j, c->st.tmin, c->st.tmax);
   const int si = slot_of(c, i), sj = slot_of(c, j);
   const double sg = which ? (si >= sj ? 1.0 : -1.0) : 1.0;
   hipLaunchKernelGGL(gather_pair, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->H, c->NP, c->V,
@@ -584,7 +654,7 @@ int kb_get_entry(kb_ctx* c, int which, int i, int j, double* out) {
 int kb_set_entry(kb_ctx* c, int which, int i, int j, const double* in) {
   if (!c || !in || (which != 0 && which != 1)) return KB_ERR_ARG;
   if (!inwin(c, i) || !inwin(c, j))
-    return seterr(c, KB_ERR_WINDOW, "set: (%d,%d) outside window [%d,%d]", i, j, c->tmin, c->tmax);
+    return seterr(c, KB_ERR_WINDOW, "set: (%d,%d) outside window [%d,%d]", i, j, c->st.tmin, c->st.tmax);
   const int si = slot_of(c, i), sj = slot_of(c, j);
   const double sg = which ? (si >= sj ? 1.0 : -1.0) : 1.0;
   int rc;
@@ -598,7 +668,7 @@ int kb_set_entry(kb_ctx* c, int which, int i, int j, const double* in) {
 
 int kb_get_sigma(kb_ctx* c, int which, int tp, double* out) {
   if (!c || !out || (which != 0 && which != 1)) return KB_ERR_ARG;
-  if (!inwin(c, tp)) return seterr(c, KB_ERR_WINDOW, "sigma: %d outside window [%d,%d]", tp, c->tmin, c->tmax);
+  if (!inwin(c, tp)) return seterr(c, KB_ERR_WINDOW, "sigma: %d outside window [%d,%d]", tp, c->st.tmin, c->st.tmax);
   hipLaunchKernelGGL(gather_sigma, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->S, c->N, c->V,
                      slot_of(c, tp), which, c->vbuf);
   LAUNCHCHK(c);
@@ -632,8 +702,10 @@ int kb_fill_free_window(kb_ctx* c, int tmax, const double* n, double mass) {
   hipLaunchKernelGGL(fill_free_kernel, grid, dim3(256), 0, c->stream, c->H, c->NP, c->V, c->N, tmax, c->kL2, dn,
                      mass, c->p.dt, c->NB);
   LAUNCHCHK(c);
-  c->tmin = tmax - c->N + 1;
-  c->tmax = tmax;
+  c->st = StepState{};
+  c->st.tmin = tmax - c->N + 1;
+  c->st.tmax = tmax;
+  if ((rc = upload_state(c))) return rc;
   c->initialised = true;
   if ((rc = launch_hartree(c, tmax, tmax))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -650,6 +722,7 @@ int kb_set_stream(kb_ctx* c, void* s) {
   if (!c) return KB_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->stream = s ? (hipStream_t)s : c->own;
+  drop_graph(c);
   return KB_OK;
 }
 
@@ -676,6 +749,241 @@ int kb_reset_timing(kb_ctx* c) {
     c->tsum[s] = 0;
     c->tcount[s] = 0;
   }
+  return KB_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// ensembles
+// ===========================================================================
+struct kb_ens {
+  std::vector<kb_ctx*> m;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  StepState** stptr_dev = nullptr;
+  int* N_dev = nullptr;
+  MemintArgs* margs_dev = nullptr;
+  int* wg_dev = nullptr;
+  int total_wg = 0;
+  int maxN = 0;
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool timing = false;
+  std::vector<EventPair> pending;
+  double tsum = 0.0;
+  long long tcount = 0;
+  std::string err;
+};
+
+namespace {
+
+int ens_err(kb_ens* e, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (e) e->err = buf;
+  return code;
+}
+
+void ens_drop_graph(kb_ens* e) {
+  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->gexec = nullptr;
+  e->graph = nullptr;
+}
+
+void ens_free(kb_ens* e) {
+  ens_drop_graph(e);
+  for (auto* c : e->m) {
+    if (c) {
+      c->stream = c->own;
+      kb_destroy(c);
+    }
+  }
+  for (void* p : {(void*)e->stptr_dev, (void*)e->N_dev, (void*)e->margs_dev, (void*)e->wg_dev})
+    if (p) (void)hipFree(p);
+  for (auto& ev : e->pending) {
+    (void)hipEventDestroy(ev.a);
+    (void)hipEventDestroy(ev.b);
+  }
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+}
+
+// one ensemble step: advance all windows, each member's self-energy and
+// Hartree kernels, then one memory-integral launch over all members
+int ens_enqueue(kb_ens* e) {
+  const int nr = (int)e->m.size();
+  hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64 * ((nr + 63) / 64)), 0, e->stream, e->stptr_dev, e->N_dev, nr);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return ens_err(e, KB_ERR_HIP, "advance launch failed: %s", hipGetErrorString(err));
+  for (auto* c : e->m) {
+    int rc;
+    if (c->p.pexp == KB_PEXP_LAMBDA_NLO && (rc = launch_sigma(c)))
+      return ens_err(e, rc, "member self-energy: %s", c->err.c_str());
+    if ((rc = launch_hartree_step(c))) return ens_err(e, rc, "member Hartree: %s", c->err.c_str());
+  }
+  hipLaunchKernelGGL(memint_step, dim3(e->total_wg), dim3(MEMINT_THREADS), memint_lds(e->maxN), e->stream,
+                     e->margs_dev, e->wg_dev, nr);
+  err = hipGetLastError();
+  if (err != hipSuccess) return ens_err(e, KB_ERR_HIP, "memint launch failed: %s", hipGetErrorString(err));
+  return KB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* kb_ens_last_error(const kb_ens* e) { return e ? e->err.c_str() : "null ensemble"; }
+
+int kb_ens_create(const kb_params* params, int nruns, kb_ens** out) {
+  if (!params || nruns < 1 || nruns > 64 || !out) return KB_ERR_ARG;
+  *out = nullptr;
+  kb_ens* e = new (std::nothrow) kb_ens();
+  if (!e) return KB_ERR_OOM;
+  e->device = params[0].device;
+  auto fail = [&](int code) {
+    ens_free(e);
+    delete e;
+    return code;
+  };
+  for (int i = 0; i < nruns; ++i) {
+    if (params[i].device != e->device) return fail(KB_ERR_ARG);
+    kb_ctx* c = nullptr;
+    int rc = kb_create(&params[i], &c);
+    if (rc) return fail(rc);
+    e->m.push_back(c);
+  }
+  if (hipSetDevice(e->device) != hipSuccess) return fail(KB_ERR_HIP);
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) return fail(KB_ERR_HIP);
+  std::vector<StepState*> sp;
+  std::vector<int> Ns, wg;
+  std::vector<MemintArgs> ma;
+  int total = 0;
+  for (auto* c : e->m) {
+    (void)hipStreamSynchronize(c->stream);
+    c->stream = e->stream;
+    c->in_ensemble = true;
+    sp.push_back(c->st_dev);
+    Ns.push_back(c->N);
+    wg.push_back(total);
+    total += c->NB;
+    ma.push_back(memint_args(c));
+    if (c->N > e->maxN) e->maxN = c->N;
+  }
+  e->total_wg = total;
+  if (hipMalloc((void**)&e->stptr_dev, sizeof(StepState*) * nruns) != hipSuccess ||
+      hipMalloc((void**)&e->N_dev, sizeof(int) * nruns) != hipSuccess ||
+      hipMalloc((void**)&e->margs_dev, sizeof(MemintArgs) * nruns) != hipSuccess ||
+      hipMalloc((void**)&e->wg_dev, sizeof(int) * nruns) != hipSuccess)
+    return fail(KB_ERR_OOM);
+  if (hipMemcpy(e->stptr_dev, sp.data(), sizeof(StepState*) * nruns, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->N_dev, Ns.data(), sizeof(int) * nruns, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->margs_dev, ma.data(), sizeof(MemintArgs) * nruns, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->wg_dev, wg.data(), sizeof(int) * nruns, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(KB_ERR_HIP);
+  *out = e;
+  return KB_OK;
+}
+
+int kb_ens_destroy(kb_ens* e) {
+  if (!e) return KB_OK;
+  (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  ens_free(e);
+  delete e;
+  return KB_OK;
+}
+
+int kb_ens_member(kb_ens* e, int i, kb_ctx** member) {
+  if (!e || !member || i < 0 || i >= (int)e->m.size()) return KB_ERR_ARG;
+  *member = e->m[i];
+  return KB_OK;
+}
+
+int kb_ens_step(kb_ens* e, int t) {
+  if (!e) return KB_ERR_ARG;
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    const kb_ctx* c = e->m[i];
+    if (!c->initialised) return ens_err(e, KB_ERR_STATE, "member %zu not initialised", i);
+    if (t != c->st.tmax + 1 || t < 3)
+      return ens_err(e, KB_ERR_WINDOW, "member %zu: t=%d but window is [%d,%d]", i, t, c->st.tmin, c->st.tmax);
+  }
+  for (auto* c : e->m) expand(c);
+  EventPair ev{nullptr, nullptr};
+  if (e->timing) {
+    if (hipEventCreate(&ev.a) != hipSuccess || hipEventCreate(&ev.b) != hipSuccess)
+      return ens_err(e, KB_ERR_HIP, "event creation failed");
+    (void)hipEventRecord(ev.a, e->stream);
+  }
+  int rc = KB_OK;
+  if (e->use_graph) {
+    if (!e->gexec) {
+      hipError_t err = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
+      if (err != hipSuccess) return ens_err(e, KB_ERR_HIP, "capture: %s", hipGetErrorString(err));
+      rc = ens_enqueue(e);
+      hipGraph_t g = nullptr;
+      err = hipStreamEndCapture(e->stream, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (err != hipSuccess) return ens_err(e, KB_ERR_HIP, "capture: %s", hipGetErrorString(err));
+      e->graph = g;
+      err = hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0);
+      if (err != hipSuccess) return ens_err(e, KB_ERR_HIP, "instantiate: %s", hipGetErrorString(err));
+    }
+    hipError_t err = hipGraphLaunch(e->gexec, e->stream);
+    if (err != hipSuccess) return ens_err(e, KB_ERR_HIP, "graph launch: %s", hipGetErrorString(err));
+  } else {
+    rc = ens_enqueue(e);
+    if (rc) return rc;
+  }
+  if (e->timing) {
+    (void)hipEventRecord(ev.b, e->stream);
+    e->pending.push_back(ev);
+  }
+  return KB_OK;
+}
+
+int kb_ens_use_graph(kb_ens* e, int on) {
+  if (!e) return KB_ERR_ARG;
+  if (!on) ens_drop_graph(e);
+  e->use_graph = on != 0;
+  return KB_OK;
+}
+
+int kb_ens_synchronize(kb_ens* e) {
+  if (!e) return KB_ERR_ARG;
+  hipError_t err = hipStreamSynchronize(e->stream);
+  if (err != hipSuccess) return ens_err(e, KB_ERR_HIP, "synchronize: %s", hipGetErrorString(err));
+  return KB_OK;
+}
+
+int kb_ens_enable_timing(kb_ens* e, int on) {
+  if (!e) return KB_ERR_ARG;
+  e->timing = on != 0;
+  return KB_OK;
+}
+
+int kb_ens_timing(kb_ens* e, double* total_ms, long long* count) {
+  if (!e) return KB_ERR_ARG;
+  hipError_t err = hipStreamSynchronize(e->stream);
+  if (err != hipSuccess) return ens_err(e, KB_ERR_HIP, "synchronize: %s", hipGetErrorString(err));
+  for (auto& ev : e->pending) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ev.a, ev.b);
+    e->tsum += ms;
+    e->tcount += 1;
+    (void)hipEventDestroy(ev.a);
+    (void)hipEventDestroy(ev.b);
+  }
+  e->pending.clear();
+  if (total_ms) *total_ms = e->tsum;
+  if (count) *count = e->tcount;
   return KB_OK;
 }
 

@@ -21,19 +21,53 @@ enum SigmaFlags {
   SIGMA_INVERSE = 8,  // transform direction of a plain pass
 };
 
+// Window of the step being taken, after expandSimData! (SimDataStruct.jl:
+// 262-278).  Kept in device memory and advanced by advance_kernel, so a
+// step's kernels have step-invariant arguments and can be replayed from a
+// captured hipGraph.
+struct StepState {
+  int t, tmin, tmax;
+  int L;     // live times t' in [tmin, t-1]: t - tmin
+  int Rmax;  // slots [0, Rmax) are in use
+  int dead;  // slot overwritten by row t when the window is full, else -1
+  int s_t, s_tm1, s_tm2, s_min;
+};
+
+// expand the window by one step and derive the slots of step t = tmax + 1
+__host__ __device__ inline void advance_state(StepState& s, int N) {
+  if (s.tmax + 1 > N) {
+    s.tmin += 1;
+    s.tmax += 1;
+  } else {
+    s.tmin = 1;
+    s.tmax += 1;
+  }
+  const int t = s.tmax;
+  const bool full = t > N;
+  s.t = t;
+  s.L = t - s.tmin;
+  s.Rmax = full ? N : t - 1;
+  s.s_t = (t - 1) % N;
+  s.dead = full ? s.s_t : -1;
+  s.s_tm1 = (t - 2) % N;
+  s.s_tm2 = (t - 3) % N;
+  s.s_min = (s.tmin - 1) % N;
+}
+
 struct SigmaArgs {
   const double* H;
   double* S;
-  double2* W;  // workspace rows [L][V] complex
+  double2* W;  // workspace rows [N-1][V] complex
   const double2* tw;
   FFTPlan plan;
   long NP;
   int N, V, n, nx, ny, nz;
   int axis, flags, P, npencils;
-  int tmin, s_tm1;
+  const StepState* st;  // rows q < st->L (t' = tmin + q) are live
   double preF, preR;
 };
 
+// Per-run arguments of memint_step; step-dependent values come from *st.
 struct MemintArgs {
   double* H;
   const double* S;
@@ -41,15 +75,11 @@ struct MemintArgs {
   const double* m2;
   long NP;
   int N, V;
-  int Rmax;  // slots [0, Rmax) are in use
-  int dead;  // slot being overwritten by row t (full window), else -1
-  int s_t, s_tm1, s_tm2, s_min;
-  int L;     // number of live times t - tmin
-  int tmin, t;
   double dt;
-  int integral;  // 0 for LambdaLO
+  int integral;       // 0 for LambdaLO
   const int2* items;  // (column block J, row chunk k) in storage order
   int nitems;
+  const StepState* st;
 };
 
 struct MeasArgs {
@@ -68,7 +98,12 @@ struct MeasArgs {
 
 __global__ void sigma_pass(SigmaArgs a);
 __global__ void hartree_kernel(const double* H, long NP, int V, long pair, double hfac, double Mass2, double* out);
-__global__ void memint_step(MemintArgs a);
+// Hartree mass of the step being taken: F(t-1,t-1) from *st
+__global__ void hartree_step_kernel(const double* H, long NP, int V, int N, const StepState* st, double hfac,
+                                    double Mass2, double* out);
+// one launch over all runs: workgroups [wg_start[r], wg_start[r+1]) belong to run r
+__global__ void memint_step(const MemintArgs* runs, const int* wg_start, int nruns);
+__global__ void advance_kernel(StepState* const* states, const int* Ns, int nruns);
 __global__ void measure_kernel(MeasArgs a);
 __global__ void history_kernel(const double* H, const double* S, long NP, int N, int tmin, int t, double* out);
 __global__ void gather_pair(const double* H, long NP, int V, long pair, int comp, double sign, double* out);
@@ -85,7 +120,7 @@ __global__ void finite_kernel(const double* H, size_t count, int* flag);
 // self-energy stage on whole planes (kb_sigma.hip), Nx <= 64
 bool sigma_planes_supported(int Nx);
 hipError_t sigma_planes_prepare(int n);
-hipError_t launch_sigma_planes(const SigmaArgs& a, int L, hipStream_t st);
+hipError_t launch_sigma_planes(const SigmaArgs& a, int rows, hipStream_t st);
 
 #ifndef KB_MEMINT_THREADS
 #define KB_MEMINT_THREADS 256

@@ -167,8 +167,10 @@ __global__ __launch_bounds__(256) void sigma_pass(SigmaArgs a) {
   double2* A = lds;
   double2* B = lds + P * n;
   const int q = blockIdx.y;
+  if (q >= a.st->L) return;  // rows t' = tmin + q, q < L, are live
   const int pen0 = blockIdx.x * P;
-  const int tp = a.tmin + q;
+  const int tp = a.st->tmin + q;
+  const int s_tm1 = a.st->s_tm1;
   const int slot = (tp - 1) % a.N;
   const size_t rowoff = (size_t)q * a.V;
   const bool contiguous = a.axis == 0;
@@ -189,8 +191,8 @@ __global__ __launch_bounds__(256) void sigma_pass(SigmaArgs a) {
       const int s = site_of(a, pencil, j);
       if (a.flags & SIGMA_GATHER) {
         // physical F(t-1,t') and rho(t-1,t'); rho sign from the slot order
-        const long pr = pair_off(a.s_tm1, slot, a.N);
-        const double sg = a.s_tm1 >= slot ? 1.0 : -1.0;
+        const long pr = pair_off(s_tm1, slot, a.N);
+        const double sg = s_tm1 >= slot ? 1.0 : -1.0;
         const int s2 = neg_site(a, s);
         const double* h1 = a.H + hoff(s / MB, pr, s % MB, a.NP);
         const double* h2 = a.H + hoff(s2 / MB, pr, s2 % MB, a.NP);
@@ -262,6 +264,36 @@ __global__ __launch_bounds__(1024) void hartree_kernel(const double* __restrict_
     h *= hfac;
     h += Mass2;
     out[0] = h;
+  }
+}
+
+__global__ __launch_bounds__(1024) void hartree_step_kernel(const double* __restrict__ H, long NP, int V, int N,
+                                                            const StepState* __restrict__ st, double hfac,
+                                                            double Mass2, double* __restrict__ out) {
+  __shared__ double red[1024];
+  const long pair = pair_off(st->s_tm1, st->s_tm1, N);  // F(t-1,t-1), Evolution.jl:945
+  double acc = 0.0;
+  for (int p = threadIdx.x; p < V; p += blockDim.x) acc += H[hoff(p / MB, pair, p % MB, NP)];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double h = red[0];
+    h *= hfac;
+    h += Mass2;
+    out[0] = h;
+  }
+}
+
+__global__ void advance_kernel(StepState* const* states, const int* Ns, int nruns) {
+  const int r = threadIdx.x;
+  if (r < nruns) {
+    StepState s = *states[r];
+    advance_state(s, Ns[r]);
+    *states[r] = s;
   }
 }
 
@@ -367,8 +399,14 @@ __device__ __forceinline__ void process_batch(const RowBatch& b, int r, int r1, 
   }
 }
 
-__global__ __launch_bounds__(MEMINT_THREADS, 2) void memint_step(MemintArgs a) {
+__global__ __launch_bounds__(MEMINT_THREADS, 2) void memint_step(const MemintArgs* __restrict__ runs,
+                                                                 const int* __restrict__ wg_start, int nruns) {
   extern __shared__ double smem[];
+  // which run this workgroup belongs to (runs are few; wg_start is uniform)
+  int run = 0;
+  while (run + 1 < nruns && (int)blockIdx.x >= wg_start[run + 1]) ++run;
+  const MemintArgs a = runs[run];
+  const StepState ss = *a.st;
   const int N = a.N;
   double* accF = smem;           // R_F per slot, lane
   double* accR = accF + N * MB;  // R_r
@@ -376,15 +414,15 @@ __global__ __launch_bounds__(MEMINT_THREADS, 2) void memint_step(MemintArgs a) {
   __shared__ double diagF[2][MB];
   __shared__ int next_item;
 
-  const int mb = blockIdx.x;
+  const int mb = blockIdx.x - wg_start[run];
   const int tid = threadIdx.x;
   const int lane = tid & (MB - 1);
   const int worker = tid >> 3;
   const int nworkers = blockDim.x >> 3;
-  const int Rmax = a.Rmax;
-  const int dead = a.dead;
-  const int smin = a.s_min;
-  const int L = a.L;
+  const int Rmax = ss.Rmax;
+  const int dead = ss.dead;
+  const int smin = ss.s_min;
+  const int L = ss.L;
 
   for (int i = tid; i < N * MB; i += blockDim.x) {
     accF[i] = 0.0;
@@ -499,8 +537,8 @@ __global__ __launch_bounds__(MEMINT_THREADS, 2) void memint_step(MemintArgs a) {
 #pragma unroll
     for (int u = 0; u < LU; ++u) {
       const int x = min(x0 + u, Rmax - 1);
-      e1[u] = *reinterpret_cast<const double2*>(Hw + ((size_t)pair_off(a.s_tm1, x, N) * MB + lane) * 2);
-      e2[u] = *reinterpret_cast<const double2*>(Hw + ((size_t)pair_off(a.s_tm2, x, N) * MB + lane) * 2);
+      e1[u] = *reinterpret_cast<const double2*>(Hw + ((size_t)pair_off(ss.s_tm1, x, N) * MB + lane) * 2);
+      e2[u] = *reinterpret_cast<const double2*>(Hw + ((size_t)pair_off(ss.s_tm2, x, N) * MB + lane) * 2);
       fsx[u] = Sb[(size_t)x * MB];
     }
 #pragma unroll
@@ -508,21 +546,21 @@ __global__ __launch_bounds__(MEMINT_THREADS, 2) void memint_step(MemintArgs a) {
       const int x = x0 + u;
       if (x >= Rmax || x == dead) continue;
       const int rel = (x - smin + N) % N;
-      const int tx = a.tmin + rel;
-      const double r1 = e1[u].y * (a.s_tm1 >= x ? 1.0 : -1.0);
-      const double r2 = e2[u].y * (a.s_tm2 >= x ? 1.0 : -1.0);
+      const int tx = ss.tmin + rel;
+      const double r1 = e1[u].y * (ss.s_tm1 >= x ? 1.0 : -1.0);
+      const double r2 = e2[u].y * (ss.s_tm2 >= x ? 1.0 : -1.0);
       const double RHSF = accF[x * MB + lane] * dt3;
       const double Fn = (2.0 - Om) * e1[u].x - e2[u].x + RHSF;
       double rn;
-      if (tx == a.t - 1) {
+      if (tx == ss.t - 1) {
         rn = dt;
       } else {
         const double RHSr = accR[x * MB + lane] * dt3;
         rn = (2.0 - Om) * r1 - r2 + RHSr;
       }
       if (real_site) {
-        double2 o = make_double2(Fn, rn * (a.s_t >= x ? 1.0 : -1.0));
-        *reinterpret_cast<double2*>(Hw + ((size_t)pair_off(a.s_t, x, N) * MB + lane) * 2) = o;
+        double2 o = make_double2(Fn, rn * (ss.s_t >= x ? 1.0 : -1.0));
+        *reinterpret_cast<double2*>(Hw + ((size_t)pair_off(ss.s_t, x, N) * MB + lane) * 2) = o;
       }
       // terms of getRHS_F!(t,t) (Evolution.jl:973) over the fresh row
       if (a.integral) {
@@ -530,8 +568,8 @@ __global__ __launch_bounds__(MEMINT_THREADS, 2) void memint_step(MemintArgs a) {
         const double m = rel == L - 1 ? 0.0 : 1.0;
         dsum += -(m * w * fsx[u].y) * Fn - (w * fsx[u].x) * rn;
       }
-      if (tx == a.t - 1) diagF[0][lane] = Fn;
-      if (tx == a.t - 2) diagF[1][lane] = Fn;
+      if (tx == ss.t - 1) diagF[0][lane] = Fn;
+      if (tx == ss.t - 2) diagF[1][lane] = Fn;
     }
   }
   // reduce over the 8 workers of the wave (lane bits 3..5), then over waves
@@ -545,7 +583,7 @@ __global__ __launch_bounds__(MEMINT_THREADS, 2) void memint_step(MemintArgs a) {
     for (int wv = 0; wv < (int)(blockDim.x >> 6); ++wv) RF += red[wv][tid];
     const double Ftt = (2.0 - Om) * diagF[0][tid] - diagF[1][tid] + RF * dt3;
     if (real_site) {
-      *reinterpret_cast<double2*>(Hw + ((size_t)pair_off(a.s_t, a.s_t, N) * MB + tid) * 2) =
+      *reinterpret_cast<double2*>(Hw + ((size_t)pair_off(ss.s_t, ss.s_t, N) * MB + tid) * 2) =
           make_double2(Ftt, 0.0);
     }
   }

@@ -203,12 +203,14 @@ __global__ __launch_bounds__(256) void sigma_plane_fwd(SigmaArgs a) {
   double2* A = lds;
   double2* B = lds + ny * ls;
   const int q = blockIdx.y;
+  if (q >= a.st->L) return;  // rows t' = tmin + q, q < L, are live
   const int iz = blockIdx.x;
   const int izn = (nz - iz) % nz;
-  const int tp = a.tmin + q;
+  const int tp = a.st->tmin + q;
+  const int s_tm1 = a.st->s_tm1;
   const int slot = (tp - 1) % a.N;
-  const long pr = pair_off(a.s_tm1, slot, a.N);
-  const double sg = a.s_tm1 >= slot ? 1.0 : -1.0;
+  const long pr = pair_off(s_tm1, slot, a.N);
+  const double sg = s_tm1 >= slot ? 1.0 : -1.0;
   const size_t rowoff = (size_t)q * a.V;
   const int nplane = n * ny;
 
@@ -271,6 +273,7 @@ __global__ __launch_bounds__(256) void sigma_zpass(SigmaArgs a) {
   double2* A = lds;
   double2* B = lds + n * ls;
   const int q = blockIdx.y;
+  if (q >= a.st->L) return;
   const int iy = blockIdx.x;
   const size_t rowoff = (size_t)q * a.V;
   for (int e = threadIdx.x; e < n * nz; e += blockDim.x) {
@@ -302,8 +305,9 @@ __global__ __launch_bounds__(256) void sigma_plane_inv(SigmaArgs a) {
   double2* A = lds;
   double2* B = lds + ny * ls;
   const int q = blockIdx.y;
+  if (q >= a.st->L) return;
   const int iz = blockIdx.x;
-  const int tp = a.tmin + q;
+  const int tp = a.st->tmin + q;
   const int slot = (tp - 1) % a.N;
   const size_t rowoff = (size_t)q * a.V;
   const int nplane = n * ny;
@@ -329,7 +333,7 @@ __global__ __launch_bounds__(256) void sigma_plane_inv(SigmaArgs a) {
 // host-side launch of the whole stage
 // ---------------------------------------------------------------------------
 template <int NT>
-static hipError_t launch_planes_t(const SigmaArgs& a, int L, hipStream_t st) {
+static hipError_t launch_planes_t(const SigmaArgs& a, int L, hipStream_t st) {  // L: grid rows
   const int n = a.n;
   const size_t lds_plane = sizeof(double2) * 2 * (size_t)a.ny * (n + 1);
   if (a.nz == 1) {
@@ -358,9 +362,11 @@ static hipError_t prepare_t(size_t lds) {
   return hipFuncSetAttribute((const void*)sigma_plane_inv<NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
-// allow the dynamic LDS of the plane kernels for lattice size n
+// allow the dynamic LDS of the plane kernels for lattice size n (the
+// attribute is per function and shared by all contexts: use the bound for
+// the largest supported plane, Nx = 64)
 hipError_t sigma_planes_prepare(int n) {
-  const size_t lds = sizeof(double2) * 2 * (size_t)n * (n + 1);
+  const size_t lds = sizeof(double2) * 2 * (size_t)64 * (64 + 1);
   switch (n) {
     case 8: return prepare_t<8>(lds);
     case 16: return prepare_t<16>(lds);
@@ -373,7 +379,7 @@ hipError_t sigma_planes_prepare(int n) {
   }
 }
 
-hipError_t launch_sigma_planes(const SigmaArgs& a, int L, hipStream_t st) {
+hipError_t launch_sigma_planes(const SigmaArgs& a, int L, hipStream_t st) {  // L: grid rows
   switch (a.n) {
     case 8: return launch_planes_t<8>(a, L, st);
     case 16: return launch_planes_t<16>(a, L, st);

@@ -32,7 +32,8 @@ EXPORTED = (
     "kb_init_particles", "kb_init_rows", "kb_step", "kb_window", "kb_hartree", "kb_measure",
     "kb_measure_history", "kb_get_entry", "kb_set_entry", "kb_get_sigma", "kb_check_finite",
     "kb_fill_free_window", "kb_synchronize", "kb_set_stream", "kb_enable_timing", "kb_timing",
-    "kb_reset_timing",
+    "kb_reset_timing", "kb_use_graph", "kb_ens_create", "kb_ens_destroy", "kb_ens_last_error", "kb_ens_member",
+    "kb_ens_step", "kb_ens_use_graph", "kb_ens_synchronize", "kb_ens_enable_timing", "kb_ens_timing",
 )
 
 
@@ -98,6 +99,17 @@ def load_library(path: str = LIB_PATH):
     L.kb_timing.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                             ctypes.POINTER(ctypes.c_longlong)]
     L.kb_reset_timing.argtypes = [vp]
+    L.kb_use_graph.argtypes = [vp, ctypes.c_int]
+    L.kb_ens_create.argtypes = [ctypes.POINTER(kb_params), ctypes.c_int, ctypes.POINTER(vp)]
+    L.kb_ens_destroy.argtypes = [vp]
+    L.kb_ens_last_error.restype = ctypes.c_char_p
+    L.kb_ens_last_error.argtypes = [vp]
+    L.kb_ens_member.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
+    L.kb_ens_step.argtypes = [vp, ctypes.c_int]
+    L.kb_ens_use_graph.argtypes = [vp, ctypes.c_int]
+    L.kb_ens_synchronize.argtypes = [vp]
+    L.kb_ens_enable_timing.argtypes = [vp, ctypes.c_int]
+    L.kb_ens_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)]
     _lib = L
     return L
 
@@ -116,25 +128,34 @@ def _f64(a, V):
     return a
 
 
+def make_params(Nx, sdim, NstepsinMemory, dt, Lambda, ONgroup, Mass, Mass2, pexp=KB_PEXP_LAMBDA_NLO, device=0):
+    return kb_params(int(Nx), int(sdim), int(NstepsinMemory), int(ONgroup), float(dt), float(Lambda),
+                     float(Mass), float(Mass2), int(pexp), int(device))
+
+
 class KBContext:
     """One device-resident KB simulation (a kb_ctx)."""
 
     def __init__(self, Nx, sdim, NstepsinMemory, dt, Lambda, ONgroup, Mass, Mass2,
-                 pexp=KB_PEXP_LAMBDA_NLO, device=0):
+                 pexp=KB_PEXP_LAMBDA_NLO, device=0, _handle=None, _owner=None):
         self._L = load_library()
-        p = kb_params(int(Nx), int(sdim), int(NstepsinMemory), int(ONgroup), float(dt),
-                      float(Lambda), float(Mass), float(Mass2), int(pexp), int(device))
-        h = ctypes.c_void_p()
-        rc = self._L.kb_create(ctypes.byref(p), ctypes.byref(h))
-        if rc != KB_OK:
-            raise KBError(rc, "kb_create failed (invalid parameters, no HIP device, or out of memory)")
+        p = make_params(Nx, sdim, NstepsinMemory, dt, Lambda, ONgroup, Mass, Mass2, pexp, device)
+        if _handle is None:
+            h = ctypes.c_void_p()
+            rc = self._L.kb_create(ctypes.byref(p), ctypes.byref(h))
+            if rc != KB_OK:
+                raise KBError(rc, "kb_create failed (invalid parameters, no HIP device, or out of memory)")
+        else:
+            h = _handle
         self._h = h
+        self._owner = _owner  # ensemble members are destroyed with their ensemble
         self.params = p
         self.V = int(Nx) ** int(sdim)
 
     def close(self):
         if getattr(self, "_h", None):
-            self._L.kb_destroy(self._h)
+            if self._owner is None:
+                self._L.kb_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -189,6 +210,9 @@ class KBContext:
     def synchronize(self):
         self._chk(self._L.kb_synchronize(self._h))
 
+    def use_graph(self, on=True):
+        self._chk(self._L.kb_use_graph(self._h, 1 if on else 0))
+
     def set_stream(self, handle):
         self._chk(self._L.kb_set_stream(self._h, ctypes.c_void_p(handle) if handle else None))
 
@@ -239,3 +263,56 @@ class KBContext:
 
     def reset_timing(self):
         self._chk(self._L.kb_reset_timing(self._h))
+
+
+class KBEnsemble:
+    """Several KB runs advanced together (a kb_ens).  members[i] is a
+    KBContext view of run i for initialisation and read-out."""
+
+    def __init__(self, runs, device=0):
+        self._L = load_library()
+        arr = (kb_params * len(runs))(*[make_params(**dict(r, device=device)) for r in runs])
+        h = ctypes.c_void_p()
+        rc = self._L.kb_ens_create(arr, len(runs), ctypes.byref(h))
+        if rc != KB_OK:
+            raise KBError(rc, "kb_ens_create failed")
+        self._h = h
+        self.members = []
+        for i, r in enumerate(runs):
+            m = ctypes.c_void_p()
+            self._chk(self._L.kb_ens_member(self._h, i, ctypes.byref(m)))
+            self.members.append(KBContext(_handle=m, _owner=self, device=device, **r))
+
+    def _chk(self, rc):
+        if rc != KB_OK:
+            raise KBError(rc, self._L.kb_ens_last_error(self._h).decode())
+
+    def step(self, t):
+        self._chk(self._L.kb_ens_step(self._h, int(t)))
+
+    def use_graph(self, on=True):
+        self._chk(self._L.kb_ens_use_graph(self._h, 1 if on else 0))
+
+    def synchronize(self):
+        self._chk(self._L.kb_ens_synchronize(self._h))
+
+    def enable_timing(self, on=True):
+        self._chk(self._L.kb_ens_enable_timing(self._h, 1 if on else 0))
+
+    def timing(self):
+        ms, n = ctypes.c_double(), ctypes.c_longlong()
+        self._chk(self._L.kb_ens_timing(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for m in self.members:
+                m._h = None
+            self._L.kb_ens_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

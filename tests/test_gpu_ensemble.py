@@ -1,0 +1,111 @@
+"""hipGraph-replayed steps and batched ensembles, through the C ABI, against
+the oracle (tolerance 1e-10, norm-wise relative per lattice)."""
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+def rel(a, b):
+    d = np.max(np.abs(np.asarray(a) - np.asarray(b)))
+    s = np.max(np.abs(np.asarray(b)))
+    return d / s if s > 0 else d
+
+
+def thermal(k2, T=2.0, m=1.0):
+    return 1.0 / (np.exp(np.sqrt(k2 + m * m) / (T * m)) - 1.0)
+
+
+def check_window(o, g, tol=RTOL):
+    assert o.window() == g.window()
+    tmin, tmax = o.window()
+    for a in range(tmin, tmax + 1):
+        for b in range(tmin, a + 1):
+            for w in "Fr":
+                assert rel(g.get(w, a, b), o.get(w, a, b)) <= tol, (w, a, b)
+
+
+def test_graph_replay_matches_oracle():
+    """Steps replayed from one captured graph, with eager measurement and
+    synchronisation between replays, follow the oracle step by step."""
+    from qftkb import KBContext
+    print("\n[case] graph replay 6^3 N_mem=5", flush=True)
+    args = (6, 3, 5, 0.1, 1.2, 1, 1.0, 1.0)
+    o = Oracle(*args)
+    g = KBContext(*args)
+    n = thermal(o.kL2())
+    o.init_particles(n, 1.0)
+    g.init_particles(n, 1.0)
+    g.use_graph(True)
+    for t in range(3, 30):
+        o.evolve(t)
+        g.step(t)
+        mo, mg = o.measure(t), g.measure(t)
+        assert abs(mg["E"] - mo["E"]) <= RTOL * abs(mo["E"])
+        assert rel(mg["n"], mo["n"]) <= RTOL
+        g.synchronize()
+    check_window(o, g)
+
+
+def test_graph_and_eager_agree_bitwise_at_32cubed():
+    """At the benchmark lattice: the same 6 steps eager and from a graph."""
+    from qftkb import KBContext
+    print("\n[case] graph vs eager 32^3 N_mem=16", flush=True)
+    ctxs = []
+    for graph in (False, True):
+        c = KBContext(32, 3, 16, 0.1, 1.0, 1, 1.0, 1.0)
+        c.fill_free_window(16, thermal(c.kL2()), 1.0)
+        c.use_graph(graph)
+        for t in range(17, 23):
+            c.step(t)
+            c.measure(t, modes=False)
+        ctxs.append(c)
+    a, b = ctxs
+    for tp in range(7, 23):
+        for w in "Fr":
+            assert rel(b.get(w, 22, tp), a.get(w, 22, tp)) <= 1e-14
+
+
+RUNS = [  # uneven members: lattice, window, coupling, O(N), dt
+    dict(Nx=4, sdim=3, NstepsinMemory=5, dt=0.1, Lambda=0.7, ONgroup=1, Mass=1.0, Mass2=1.0),
+    dict(Nx=6, sdim=3, NstepsinMemory=9, dt=0.1, Lambda=1.6, ONgroup=1, Mass=1.0, Mass2=1.0),
+    dict(Nx=8, sdim=3, NstepsinMemory=6, dt=0.05, Lambda=1.1, ONgroup=4, Mass=0.8, Mass2=0.64),
+    dict(Nx=5, sdim=2, NstepsinMemory=7, dt=0.1, Lambda=2.0, ONgroup=2, Mass=1.0, Mass2=1.0),
+    dict(Nx=1, sdim=3, NstepsinMemory=4, dt=0.1, Lambda=1.0, ONgroup=1, Mass=1.0, Mass2=1.0),
+]
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_ensemble_members_match_oracle(graph):
+    from qftkb import KBEnsemble
+    print(f"\n[case] ensemble of {len(RUNS)} uneven runs, graph={graph}", flush=True)
+    ens = KBEnsemble(RUNS)
+    ens.use_graph(graph)
+    oracles = []
+    for r, m in zip(RUNS, ens.members):
+        o = Oracle(r["Nx"], r["sdim"], r["NstepsinMemory"], r["dt"], r["Lambda"], r["ONgroup"], r["Mass"],
+                   r["Mass2"])
+        n = thermal(o.kL2(), m=r["Mass"])
+        o.init_particles(n, r["Mass"])
+        m.init_particles(n, r["Mass"])
+        oracles.append(o)
+    for t in range(3, 26):
+        ens.step(t)
+        for o in oracles:
+            o.evolve(t)
+        if t % 5 == 0:
+            for o, m in zip(oracles, ens.members):
+                mo, mg = o.measure(t), m.measure(t)
+                assert abs(mg["E"] - mo["E"]) <= RTOL * abs(mo["E"])
+                assert rel(mg["n"], mo["n"]) <= RTOL
+    for o, m in zip(oracles, ens.members):
+        check_window(o, m)
+    # members cannot be stepped on their own
+    from qftkb import KBError
+    with pytest.raises(KBError):
+        ens.members[0].step(26)
+    ens.close()
