@@ -88,6 +88,84 @@ def cpu_baseline(w, budget_s=30.0):
                       f"32^3 N_mem={N} window, {len(times)} timed step(s) after fill{note}"}
 
 
+ENSEMBLE_SEED = 20240611
+ENSEMBLE_NX = (16, 16, 24, 24, 32, 32, 40, 40)
+
+
+def ensemble_runs(seed=ENSEMBLE_SEED, nmem=300):
+    """Config 3: lattices {16,16,24,24,32,32,40,40}^3, lambda ~ U[0.5, 2],
+    box occupations n0 ~ U[1, 4], Q ~ U[0.5, 1.5], drawn from one PCG64 seed."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lam = rng.uniform(0.5, 2.0, len(ENSEMBLE_NX))
+    n0 = rng.uniform(1.0, 4.0, len(ENSEMBLE_NX))
+    Q = rng.uniform(0.5, 1.5, len(ENSEMBLE_NX))
+    runs = [dict(Nx=nx, sdim=3, NstepsinMemory=nmem, dt=0.1, Lambda=float(l), ONgroup=1, Mass=1.0, Mass2=1.0)
+            for nx, l in zip(ENSEMBLE_NX, lam)]
+    return runs, [dict(n0=float(a), Q=float(b)) for a, b in zip(n0, Q)]
+
+
+def ensemble_bench(args):
+    from qftkb import KBContext, KBEnsemble, lattice
+
+    runs, occ = ensemble_runs(nmem=args.nmem if args.nmem != WORKLOAD["NstepsinMemory"] else 300)
+    N = runs[0]["NstepsinMemory"]
+    steps, warm = args.steps, args.warmup
+    # all runs at once
+    ens = KBEnsemble(runs)
+    for m, o in zip(ens.members, occ):
+        m.fill_free_window(N, lattice.n_box(m.kL2(), o["n0"], o["Q"]), 1.0)
+    ens.use_graph(True)
+    t = N + 1
+    for _ in range(warm):
+        ens.step(t)
+        t += 1
+    ens.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ens.step(t)
+        t += 1
+    ens.synchronize()
+    t_ens = time.perf_counter() - t0
+    for m in ens.members:
+        m.check_finite()
+    ens.close()
+    print(f"[bench] ensemble: {steps} steps of {len(runs)} runs in {t_ens:.4f} s", file=sys.stderr, flush=True)
+    # the same runs one after another, each a plain context (graph on too)
+    t_seq = 0.0
+    per_run = []
+    for r, o in zip(runs, occ):
+        c = KBContext(**r)
+        c.fill_free_window(N, lattice.n_box(c.kL2(), o["n0"], o["Q"]), 1.0)
+        c.use_graph(True)
+        t = N + 1
+        for _ in range(warm):
+            c.step(t)
+            t += 1
+        c.synchronize()
+        a = time.perf_counter()
+        for _ in range(steps):
+            c.step(t)
+            t += 1
+        c.synchronize()
+        dt_run = time.perf_counter() - a
+        per_run.append(dt_run / steps * 1e3)
+        t_seq += dt_run
+        c.close()
+    line = {
+        "metric": "ensemble KB steps/s (8 uneven runs, config 3)",
+        "value": len(runs) * steps / t_ens, "unit": "member KB steps/s",
+        "ensemble_steps_per_s": steps / t_ens,
+        "sequential_member_steps_per_s": len(runs) * steps / t_seq,
+        "speedup_vs_sequential": t_seq / t_ens,
+        "ms_per_ensemble_step": 1e3 * t_ens / steps,
+        "sequential_ms_per_step_by_run": dict(zip([f"{r['Nx']}^3 #{i}" for i, r in enumerate(runs)], per_run)),
+        "steps": steps, "warmup": warm, "dtype": "f64",
+        "config": {"workload": "config 3 ensemble", "Nx": list(ENSEMBLE_NX), "NstepsinMemory": N,
+                   "seed": ENSEMBLE_SEED, "Lambda": [r["Lambda"] for r in runs], "box": occ},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +174,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--nmem", type=int, default=WORKLOAD["NstepsinMemory"])
     ap.add_argument("--nx", type=int, default=WORKLOAD["Nx"])
+    ap.add_argument("--ensemble", action="store_true",
+                    help="config 3: 8 uneven runs advanced as one ensemble vs the same runs one after another")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 2 GiB device copy (torch) to scale rocprofv3 FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
@@ -112,6 +192,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not supported "
                                    "(multi-GPU sharding is design-only)"}))
         sys.exit(2)
+    if args.ensemble:
+        ensemble_bench(args)
+        return
 
     from qftkb import KBContext
 
