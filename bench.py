@@ -24,6 +24,9 @@ sys.path.insert(0, os.path.join(ROOT, "qftdynamicsm.jl_amd"))
 import numpy as np  # noqa: E402
 
 WORKLOAD = dict(Nx=32, sdim=3, NstepsinMemory=400, dt=0.1, Lambda=1.0, ONgroup=1, Mass=1.0, T=2.0)
+# differentiation workload: gradients of L = sum_p exp(-|p|^2) n(p, t_f)
+ADJ_WORKLOAD = dict(Nx=24, sdim=3, NstepsinMemory=200, dt=0.1, Lambda=1.0, ONgroup=1, Mass=1.0, T=2.0,
+                    steps=500, checkpoint_every=50, nbins=16, h=1e-5)
 HBM_PEAK_GBS = 8000.0
 
 
@@ -166,6 +169,111 @@ def ensemble_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def radial_bins(kL2, sdim, nbins):
+    edges = np.linspace(0.0, np.sqrt(4.0 * sdim), nbins + 1)
+    return np.minimum(np.searchsorted(edges, np.sqrt(kL2), side="right") - 1, nbins - 1)
+
+
+def adjoint_bench(fd_bins=False):
+    """Cost of dL/dlambda and dL/dn(p,0) by the adjoint kernels relative to
+    one forward run, at 24^3, N_mem = 200, 500 steps, checkpoints every 50
+    (and with every dropped column taped), plus the check against central
+    finite differences (h = 1e-5) of the forward solver."""
+    from qftkb import KBContext
+
+    a = ADJ_WORKLOAD
+    Nx, N, dt, lam, ON, M = a["Nx"], a["NstepsinMemory"], a["dt"], a["Lambda"], a["ONgroup"], a["Mass"]
+    T = 2 + a["steps"]
+
+    def make(lmb):
+        return KBContext(Nx, a["sdim"], N, dt, lmb, ON, M, M * M)
+
+    def forward(c, graph):
+        c.use_graph(graph)
+        for t in range(3, T + 1):
+            c.step(t)
+        c.synchronize()
+
+    # load every kernel once
+    w0 = KBContext(6, 3, 5, dt, lam, ON, M, M * M)
+    w0.init_particles(np.full(216, 0.5), M)
+    w0.adj_record(4, 12)
+    for t in range(3, 13):
+        w0.step(t)
+    w0.adj_gradient(np.ones(216))
+    w0.close()
+
+    c = make(lam)
+    k2 = c.kL2()
+    n0 = thermal_n(k2, M, a["T"])
+    wgt = np.exp(-k2)
+    c.init_particles(n0, M)
+    t0 = time.perf_counter()
+    forward(c, True)
+    t_fwd = time.perf_counter() - t0
+    L = float((wgt * c.measure(T)["n"]).sum())
+    c.close()
+    print(f"[bench] adjoint workload: forward {a['steps']} steps in {t_fwd:.3f} s", file=sys.stderr, flush=True)
+    out = {"workload": f"{Nx}^3 lattice, N_mem={N}, {a['steps']} steps, thermal T=2, lambda=1, "
+                       f"L = sum_p exp(-|p|^2) n(p, t_f)",
+           "forward_s": t_fwd, "loss": L}
+    grads = {}
+    for C in (a["checkpoint_every"], 0):
+        g = make(lam)
+        g.init_particles(n0, M)
+        g.adj_record(C, T)
+        t0 = time.perf_counter()
+        forward(g, False)
+        t1 = time.perf_counter()
+        gr = g.adj_gradient(wgt)
+        t2 = time.perf_counter()
+        info = g.adj_info()
+        g.close()
+        key = f"checkpoint_every_{C}" if C else "tape_every_column"
+        out[key] = {"record_forward_s": t1 - t0, "reverse_s": t2 - t1, "gradient_s": t2 - t0,
+                    "ratio": (t2 - t0) / t_fwd, "reverse_ratio": (t2 - t1) / t_fwd,
+                    "recomputed_steps": info["recomputed_steps"], "adjoint_device_gb": info["device_bytes"] / 1e9}
+        grads[C] = gr
+        print(f"[bench] adjoint C={C}: record {t1 - t0:.3f} s, reverse {t2 - t1:.3f} s, "
+              f"ratio {(t2 - t0) / t_fwd:.2f}", file=sys.stderr, flush=True)
+    gr = grads[a["checkpoint_every"]]
+    out["ratio"] = out[f"checkpoint_every_{a['checkpoint_every']}"]["ratio"]
+    out["dL_dlambda"] = gr["lambda"]
+    bins = radial_bins(k2, a["sdim"], a["nbins"])
+    out["dL_dn0_bins"] = [float(gr["n0"][bins == b].sum()) for b in range(a["nbins"])]
+    out["tape_vs_checkpoint_max_rel"] = float(np.max(np.abs(grads[0]["n0"] - gr["n0"])) / np.max(np.abs(gr["n0"])))
+
+    h = a["h"]
+
+    def loss(lmb, n):
+        c = make(lmb)
+        c.init_particles(n, M)
+        forward(c, True)
+        v = float((wgt * c.measure(T)["n"]).sum())
+        c.close()
+        return v
+
+    t0 = time.perf_counter()
+    fd = (loss(lam + h, n0) - loss(lam - h, n0)) / (2 * h)
+    out["fd_dL_dlambda"] = fd
+    out["fd_rel_err_lambda"] = abs(gr["lambda"] - fd) / abs(fd)
+    if fd_bins:
+        fdb = []
+        for b in range(a["nbins"]):
+            e = np.where(bins == b, h, 0.0)
+            fdb.append((loss(lam, n0 + e) - loss(lam, n0 - e)) / (2 * h) if np.any(bins == b) else 0.0)
+        scale = max(abs(x) for x in fdb)
+        out["fd_dL_dn0_bins"] = fdb
+        out["fd_rel_err_bins"] = max(abs(x - y) for x, y in zip(out["dL_dn0_bins"], fdb)) / scale
+    out["fd_s"] = time.perf_counter() - t0
+    out["fd_cost_ratio"] = 2 * (1 + (a["nbins"] if fd_bins else 0)) * 1.0  # forward runs per gradient, FD
+    gold = load_pmc(os.path.join(ROOT, "tests", "golden", "cfg4_fd.json"))
+    if gold and "dL_dlambda_fd" in gold:
+        out["oracle_fd_dL_dlambda"] = gold["dL_dlambda_fd"]
+        out["oracle_fd_rel_err_lambda"] = abs(gr["lambda"] - gold["dL_dlambda_fd"]) / abs(gold["dL_dlambda_fd"])
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,6 +284,9 @@ def main():
     ap.add_argument("--nx", type=int, default=WORKLOAD["Nx"])
     ap.add_argument("--ensemble", action="store_true",
                     help="config 3: 8 uneven runs advanced as one ensemble vs the same runs one after another")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost-ratio leg")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="only the differentiation workload, with finite differences of all 16 radial bins")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 2 GiB device copy (torch) to scale rocprofv3 FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
@@ -194,6 +305,9 @@ def main():
         sys.exit(2)
     if args.ensemble:
         ensemble_bench(args)
+        return
+    if args.adjoint:
+        print(json.dumps({"adjoint": adjoint_bench(fd_bins=True)}), flush=True)
         return
 
     from qftkb import KBContext
@@ -261,7 +375,10 @@ def main():
                      "avg_kernel_ms": avg_mi_s * 1e3, "sigma_stage_ms": ms_fft / max(n_fft, 1),
                      "hartree_ms": ms_h / max(n_mi, 1)},
         "cpu_baseline": None,
+        "adjoint": None,
     }
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_bench()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(line), flush=True)

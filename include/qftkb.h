@@ -149,12 +149,53 @@ int kb_synchronize(kb_ctx *ctx);
 int kb_set_stream(kb_ctx *ctx, void *hip_stream);
 /* HIP-event timing of the kernels of kb_step, recorded on the launch
  * stream.  stage: 0 = memory integral + leapfrog kernel, 1 = self-energy
- * FFT stage (all its passes), 2 = Hartree reduction.  kb_timing
+ * FFT stage (all its passes), 2 = Hartree reduction, 3 = adjoint memory
+ * integral, 4 = the other kernels of the adjoint sweep.  kb_timing
  * synchronises and returns the accumulated milliseconds and launch count
  * since the last reset. */
 int kb_enable_timing(kb_ctx *ctx, int on);
 int kb_timing(kb_ctx *ctx, int stage, double *total_ms, long long *count);
 int kb_reset_timing(kb_ctx *ctx);
+
+/* ---- adjoint ---------------------------------------------------------------
+ * Gradients of L = sum_p wgt(p) n(p, T) (n as in kb_measure at T = tmax,
+ * TwoPIScalar_Measurement.jl:38-59) with respect to the coupling Lambda and
+ * the initial state, by hand-written reverse kernels of the whole KB step
+ * (evolve!, TwoPIScalar_Evolution.jl:924-981).  The reference has no
+ * derivative code; these entry points are what a ChainRulesCore rrule of
+ * the back-end's evolve loop calls.  Mass2 is held fixed. */
+
+/* Start recording right after kb_init_particles / kb_init_rows (window
+ * [1,2]).  Later kb_step calls run eagerly and keep what the reverse sweep
+ * needs for up to max_steps:
+ *   checkpoint_every = 0: every column that a step drops from the window
+ *                         (N_mem slot pairs, V sites) is kept on a tape;
+ *   checkpoint_every = C: the window is kept after steps 2, 2+C, 2+2C, ...
+ *                         and each C-step segment is recomputed (taping its
+ *                         columns) before it is reversed.
+ * Fails with KB_ERR_OOM if the tape or checkpoints do not fit. */
+int kb_adj_record(kb_ctx *ctx, int checkpoint_every, int max_steps);
+/* Reverse sweep from T = tmax down to the initial rows.  wgt[V] are the
+ * loss weights.  Outputs (any may be NULL): dL/dLambda; dL/dn(p,0) per site
+ * (needs kb_init_particles; n enters F(1,1) = F(2,1) = (n+1/2)/w and
+ * F(2,2) = (n+1/2)(w dt^2 + 1/w), Initialization.jl:8-29); dL/dF of the
+ * initial rows as 3 x V doubles (F(1,1), F(2,1), F(2,2)).  Afterwards the
+ * window is the initial state [1,2] again and recording continues. */
+int kb_adj_gradient(kb_ctx *ctx, const double *wgt, double *dL_dlambda,
+                    double *dL_dn0, double *dL_dF0);
+/* Device bytes held for the adjoint and forward steps recomputed by the
+ * last kb_adj_gradient. */
+int kb_adj_info(kb_ctx *ctx, long long *device_bytes,
+                long long *recomputed_steps);
+/* Stop recording and free the adjoint state. */
+int kb_adj_stop(kb_ctx *ctx);
+/* Test access.  kb_adj_debug_stop makes the next kb_adj_gradient return
+ * right after reversing step t_stop (window of step t_stop - 1, no outputs
+ * written); kb_adj_get_entry then reads the adjoint of F(i,j) (which = 0)
+ * or of rho(i,j) (which = 1) accumulated so far.  The context must be
+ * re-initialised afterwards. */
+int kb_adj_debug_stop(kb_ctx *ctx, int t_stop);
+int kb_adj_get_entry(kb_ctx *ctx, int which, int i, int j, double *out);
 
 /* ---- ensembles ------------------------------------------------------------
  * Many independent KB runs (uneven lattices, couplings, memory windows)

@@ -1,0 +1,932 @@
+// kb_adjoint.hip -- reverse-mode sweep of the 2PI Kadanoff-Baym step.
+//
+// The reference has no derivative code; this is the adjoint of its evolve!
+// (TwoPIScalar_Evolution.jl:924-981) as restated by the forward kernels of
+// this library, for the loss
+//     L = sum_p wgt(p) n(p, T),
+// n measured at T as createMeasurement does (TwoPIScalar_Measurement.jl:
+// 38-59), with respect to the coupling lambda (self-energy prefactors and
+// the Hartree term) and the three initial rows F(1,1), F(2,1), F(2,2) of
+// initSimData! (TwoPIScalar_Initialization.jl:8-29), hence the initial
+// occupations n(p, 0).
+//
+// The adjoint window Hbar has the layout of the history H (rho_bar in the
+// same sign-folded convention).  Reverse step t, with the primal window of
+// step t resident:
+//   1. recompute Sigma(t-1, .) (saving the transforms Fx, rx) and M^2(t);
+//   2. adj_row_kernel: adjoint of the diagonal F(t,t) and of the leapfrog
+//      row t: R_bar_F, R_bar_r per slot, pushes into rows t-1 and t-2,
+//      the Omega adjoint per site block; it first settles the Hartree
+//      adjoint of step t+1 into F_bar(t,t);
+//   3. adj_memint_kernel: adjoint of the memory integrals, one read of H
+//      and one read-modify-write of Hbar per stored entry, with the
+//      self-energy adjoints Sigma_bar reduced per slot in LDS;
+//   4. the self-energy stage run backwards (kb_sigma.hip, ADJ) adds the
+//      sunset adjoint into row t-1 of Hbar;
+//   5. adj_slide_kernel: row t of Hbar is cleared and the column t-N that
+//      row t overwrote is put back into H from the tape.
+// The tape holds, for every forward step that dropped a column, that
+// column's N slot pairs.  With checkpointing, only the window at every C-th
+// step is kept and each segment's steps are recomputed (taping) before it
+// is reversed.
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "kb_internal.h"
+
+using namespace qftkb;
+using namespace qftkb_host;
+
+namespace qftkb {
+
+// Per-step arguments of the reverse kernels.
+struct AdjArgs {
+  const double* H;
+  double* Hbar;
+  const double* S;     // Sigma(t-1, .) ring
+  double* Rb;          // (R_bar_F, R_bar_r) ring, self-energy layout
+  double* Sbar;        // (SigF_bar, Sigr_bar) ring
+  const double* kL2;
+  const double* m2r;   // [parity][2]: M^2(t), sum_p F(t-1,t-1;p)
+  double* P;           // [parity][NB]: Omega adjoint summed over a site block
+  double* LP;          // [NB]: running dL/dlambda per site block
+  const int2* items;
+  int nitems;
+  long NP;
+  int N, V, NB;
+  double dt;
+  double hl_ivol;      // lambda (N+2)/(6N) / V: dM^2 / dF
+  double h_ivol;       // (N+2)/(6N) / V:        dM^2 / dlambda per unit sum F
+  double two_over_lambda;
+  int integral;
+  int first;           // t == T: no Hartree adjoint pending
+  StepState st;        // the forward state of step t
+};
+
+namespace {
+
+__device__ __forceinline__ double2 ld2(const double* p) { return *reinterpret_cast<const double2*>(p); }
+__device__ __forceinline__ void st2(double* p, double2 v) { *reinterpret_cast<double2*>(p) = v; }
+
+typedef double kb_a2v __attribute__((ext_vector_type(2)));
+// streamed once per step: keep them out of the way of the coefficient rings
+__device__ __forceinline__ double2 ld2_nt(const double* p) {
+  const kb_a2v v = __builtin_nontemporal_load(reinterpret_cast<const kb_a2v*>(p));
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void st2_nt(double* p, double2 v) {
+  kb_a2v w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<kb_a2v*>(p));
+}
+
+// sum of n values by a whole workgroup (fixed order: identical in every
+// workgroup that sums the same array)
+template <int NT>
+__device__ double block_sum(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = NT / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// tape: the N slot pairs (s, x) of every site block <-> one tape column
+// ---------------------------------------------------------------------------
+__global__ void tape_copy_kernel(double* __restrict__ H, double* __restrict__ tape, long NP, int N, int s,
+                                 int to_tape) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N * MB) return;
+  const int x = idx / MB, lane = idx % MB, mb = blockIdx.y;
+  double* h = H + hoff(mb, pair_off(s, x, N), lane, NP);
+  double* tp = tape + ((((size_t)mb * N) + x) * MB + lane) * 2;
+  if (to_tape)
+    st2(tp, ld2(h));
+  else
+    st2(h, ld2(tp));
+}
+
+// clear the adjoint of the slot pairs (s, x) and, with a tape column, put
+// the primal column back
+__global__ void adj_slide_kernel(double* __restrict__ H, double* __restrict__ Hbar, const double* __restrict__ tape,
+                                 long NP, int N, int s) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N * MB) return;
+  const int x = idx / MB, lane = idx % MB, mb = blockIdx.y;
+  const size_t o = hoff(mb, pair_off(s, x, N), lane, NP);
+  st2(Hbar + o, make_double2(0.0, 0.0));
+  if (tape) st2(H + o, ld2(tape + ((((size_t)mb * N) + x) * MB + lane) * 2));
+}
+
+__global__ void adj_state_kernel(StepState* st, int t, int N) { *st = state_of_step(t, N); }
+
+// M^2 = Mass2 + hfac sum_p F(pair) and the plain sum
+__global__ __launch_bounds__(1024) void adj_hartree_kernel(const double* __restrict__ H, long NP, int V, long pair,
+                                                           double hfac, double Mass2, double* __restrict__ out) {
+  __shared__ double red[1024];
+  double acc = 0.0;
+  for (int p = threadIdx.x; p < V; p += blockDim.x) acc += H[hoff(p / MB, pair, p % MB, NP)];
+  const double s = block_sum<1024>(acc, red);
+  if (threadIdx.x == 0) {
+    out[0] = s * hfac + Mass2;
+    out[1] = s;
+  }
+}
+
+// dL/dF of F(T-1,T-1), F(T,T-1), F(T,T) for L = sum wgt n(T); n as
+// createMeasurement (Measurement.jl:38-59) with its stability clamp
+__global__ void adj_seed_kernel(const double* __restrict__ H, double* __restrict__ Hbar,
+                                const double* __restrict__ wgt, long NP, int V, int N, int sT, int sTm1, double dt) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= V) return;
+  const int mb = p / MB, lane = p % MB;
+  const double Ftt = H[hoff(mb, pair_off(sT, sT, N), lane, NP)];
+  const double Fmt = H[hoff(mb, pair_off(sT, sTm1, N), lane, NP)];
+  const double F = H[hoff(mb, pair_off(sTm1, sTm1, N), lane, NP)];
+  const double ddF = (Ftt - 2.0 * Fmt + F) / (dt * dt);
+  const double om2 = (ddF / F) > 0 ? ddF / F : 0.0;
+  const bool stable = 0.25 * (dt * dt) * om2 < 1.0;
+  double dn_dF = 0.0, dn_dd = 0.0;
+  if (ddF * F > 0) {
+    const double D = stable ? ddF * F - 0.25 * dt * dt * ddF * ddF : ddF * F;
+    const double sq = sqrt(D);
+    dn_dF = ddF / (2.0 * sq);
+    dn_dd = stable ? (F - 0.5 * dt * dt * ddF) / (2.0 * sq) : F / (2.0 * sq);
+  }
+  const double w = wgt[p];
+  const double g = w * dn_dd / (dt * dt);
+  Hbar[hoff(mb, pair_off(sT, sT, N), lane, NP)] += g;
+  Hbar[hoff(mb, pair_off(sT, sTm1, N), lane, NP)] += -2.0 * g;
+  Hbar[hoff(mb, pair_off(sTm1, sTm1, N), lane, NP)] += w * dn_dF + g;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint of the diagonal and of the leapfrog row t (evolve_F!, evolve_r!,
+// Evolution.jl:890-915, 971-974), one workgroup per site block.
+//   F(t,t)  = (2-Om) F(t,t-1) - F(t,t-2) + dt^3 R_F(t)
+//   F(t,b)  = (2-Om) F(t-1,b) - F(t-2,b) + dt^3 R_F(b),  b <= t-1
+//   rho(t,b)= (2-Om) rho(t-1,b) - rho(t-2,b) + dt^3 R_r(b), b <= t-2
+// with R_F(t) = -sum_{z<=t-2} w Sr(z) F(z,t) + sum_{z<=t-1} w SF(z) rho(z,t).
+// The terms of R_F(t) that multiply row-t entries are folded in here; its
+// self-energy terms are left to adj_memint_kernel through R_bar_F(t).
+// ---------------------------------------------------------------------------
+constexpr int ROW_THREADS = 256;
+
+__global__ __launch_bounds__(ROW_THREADS) void adj_row_kernel(AdjArgs a) {
+  __shared__ double red[ROW_THREADS];
+  __shared__ double fl[2][MB];  // f_bar at t-1 and t-2
+  const StepState& st = a.st;
+  const int N = a.N;
+  const int t = st.t;
+  const int mb = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int lane = tid & (MB - 1);
+  const int worker = tid >> 3;
+  const int nworkers = ROW_THREADS >> 3;
+  const int p = mb * MB + lane;
+  const bool real = p < a.V;
+  const double dt = a.dt, dt2 = dt * dt, dt3 = dt2 * dt;
+
+  // Hartree adjoint of step t+1: M2_bar = dt^2 sum_p Om_bar(p)
+  double Mb1 = 0.0;
+  if (!a.first) {
+    const double* Pn = a.P + (size_t)((t + 1) & 1) * a.NB;
+    double acc = 0.0;
+    for (int i = tid; i < a.NB; i += ROW_THREADS) acc += Pn[i];
+    Mb1 = dt2 * block_sum<ROW_THREADS>(acc, red);
+    if (mb == 0 && tid == 0) a.LP[0] += a.h_ivol * a.m2r[((t + 1) & 1) * 2 + 1] * Mb1;
+  }
+  const double M2 = a.m2r[(t & 1) * 2];
+  const double Om = (a.kL2[p] + M2) * dt2;
+  const double* Hs = a.H + hoff(mb, 0, lane, a.NP);
+  double* Hbs = a.Hbar + hoff(mb, 0, lane, a.NP);
+  const double2* Ss = reinterpret_cast<const double2*>(a.S + soff(mb, 0, lane, N));
+  double2* Rs = reinterpret_cast<double2*>(a.Rb + soff(mb, 0, lane, N));
+  const size_t pstride = (size_t)MB * 2;  // doubles per slot pair of one site block
+
+  const double Fbtt = Hbs[pair_off(st.s_t, st.s_t, N) * pstride] + a.hl_ivol * Mb1;
+  const double RFt = dt3 * Fbtt;
+  double omb = 0.0;
+  for (int q = worker; q < st.L; q += nworkers) {
+    const int tx = st.tmin + q;
+    const int x = (tx - 1) % N;
+    const double w = q == 0 ? 0.5 : 1.0;
+    const double m = tx <= t - 2 ? 1.0 : 0.0;
+    const double2 hb = ld2(Hbs + pair_off(st.s_t, x, N) * pstride);
+    const double2 h1 = ld2(Hs + pair_off(st.s_tm1, x, N) * pstride);
+    const double2 sg = a.integral ? Ss[(size_t)x * MB] : make_double2(0.0, 0.0);
+    double fb = hb.x;
+    if (tx == t - 1) fb += (2.0 - Om) * Fbtt;
+    if (tx == t - 2) fb -= Fbtt;
+    fb -= m * w * sg.y * RFt;
+    double rp = 0.0;
+    if (tx <= t - 2) rp = hb.y * (st.s_t >= x ? 1.0 : -1.0) - w * sg.x * RFt;
+    const double s1 = st.s_tm1 >= x ? 1.0 : -1.0;
+    omb -= h1.x * fb + h1.y * s1 * rp;
+    if (real) {
+      Rs[(size_t)x * MB] = make_double2(dt3 * fb, dt3 * rp);
+      // leapfrog pushes; the pair {t-1, t-2} is reached from b = t-1 and
+      // b = t-2 and is updated once below
+      if (tx != t - 2) {
+        double* e = Hbs + pair_off(st.s_tm1, x, N) * pstride;
+        double2 v = ld2(e);
+        v.x += (2.0 - Om) * fb;
+        if (tx <= t - 3) v.y += s1 * (2.0 - Om) * rp;
+        st2(e, v);
+      }
+      if (tx != t - 1) {
+        double* e = Hbs + pair_off(st.s_tm2, x, N) * pstride;
+        double2 v = ld2(e);
+        v.x -= fb;
+        if (tx <= t - 3) v.y -= (st.s_tm2 >= x ? 1.0 : -1.0) * rp;
+        st2(e, v);
+      }
+    } else {
+      Rs[(size_t)x * MB] = make_double2(0.0, 0.0);
+    }
+    if (tx == t - 1) fl[0][lane] = fb;
+    if (tx == t - 2) fl[1][lane] = fb;
+  }
+  __syncthreads();
+  if (tid < MB) {
+    // F(t,t) depends on Om through (2-Om) F(t,t-1)
+    omb -= Hs[pair_off(st.s_t, st.s_tm1, N) * pstride] * Fbtt;
+    if (real) {
+      double* e = Hbs + pair_off(st.s_tm1, st.s_tm2, N) * pstride;
+      e[0] += (2.0 - Om) * fl[1][lane] - fl[0][lane];
+      Rs[(size_t)st.s_t * MB] = make_double2(RFt, 0.0);
+    } else {
+      Rs[(size_t)st.s_t * MB] = make_double2(0.0, 0.0);
+    }
+  }
+  const double tot = block_sum<ROW_THREADS>(real ? omb : 0.0, red);
+  if (tid == 0) a.P[(size_t)(t & 1) * a.NB + mb] = tot;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint of the memory integrals (getRHS_F!, getRHS_r!, Evolution.jl:
+// 766-782, 846-854) over the window [tmin, t], one workgroup per site block.
+// Forward, per stored entry (a,b), a>b, with u = m w Sr, v = w SF, ms = m Sr:
+//   R_F(a) -= u(b) F + v(b) rho,  R_F(b) -= u(a) F,  R_r(b) -= ms(a) rho,
+// and (a,a): R_F(a) -= u(a) F.  With R_bar = dt^3 (adjoint of row t), per
+// entry:
+//   F_bar(a,b)   -= u(a) R_bar_F(b) + u(b) R_bar_F(a)
+//   rho_bar(a,b) -= v(b) R_bar_F(a) + ms(a) R_bar_r(b)
+//   Sr_bar(z) = m(z) [ w(z) u_bar(z) + ms_bar(z) ],  SF_bar(z) = w(z) v_bar(z)
+//   u_bar(z)  = -sum_b F(z,b) R_bar_F(b)           (b over [tmin, t])
+//   v_bar(z)  = -sum_{a>z} rho(a,z) R_bar_F(a)
+//   ms_bar(z) = -sum_{b<z} rho(z,b) R_bar_r(b)
+// Row t enters with R_bar_F(t) = dt^3 F_bar(t,t) (the diagonal's memory
+// terms) and u = v = ms = 0; its own entries are cleared afterwards.
+// The slot triangle is walked like the forward kernel: 8-lane workers take
+// 64-row chunks of 4-column blocks, two 2-row batches of H and Hbar lines
+// in flight, column sums in registers, row sums in LDS.
+// ---------------------------------------------------------------------------
+constexpr int AM_THREADS = 256;
+constexpr int AM_B = 1;  // rows per batch (two batches in flight)
+
+struct SlotCoef {
+  double u, v, ms, rf, rr, w, m;
+};
+
+__device__ __forceinline__ SlotCoef slot_coef(const AdjArgs& a, const double2* Ss, const double2* Rs, int x) {
+  const StepState& st = a.st;
+  const int N = a.N;
+  const int rel = (x - st.s_min + N) % N;
+  const int tx = st.tmin + rel;
+  SlotCoef k;
+  k.w = rel == 0 ? 0.5 : 1.0;
+  k.m = tx <= st.t - 2 ? 1.0 : 0.0;
+  const double2 sg = tx <= st.t - 1 ? Ss[(size_t)x * MB] : make_double2(0.0, 0.0);
+  const double2 rb = Rs[(size_t)x * MB];
+  k.u = k.m * k.w * sg.y;
+  k.v = k.w * sg.x;
+  k.ms = k.m * sg.y;
+  k.rf = rb.x;
+  k.rr = rb.y;
+  return k;
+}
+
+struct AdjCols {
+  double u[CB], v[CB], ms[CB], rf[CB], rr[CB];
+  double t1[CB], t2[CB], t3[CB];  // sum F rf(r); sum_{neg} P rr(r); sum_{not neg} P rf(r)
+  unsigned phi;
+};
+
+struct AdjBatch {
+  double2 e[AM_B][CB], eb[AM_B][CB];
+  double2 sg[AM_B], rb[AM_B];
+};
+
+__device__ __forceinline__ void adj_load(AdjBatch& b, const double* Hs, const double* Hbs, const double2* Ss,
+                                         const double2* Rs, long cbase, int c0, int r, int rlast) {
+#pragma unroll
+  for (int h = 0; h < AM_B; ++h) {
+    const int rr = min(r + h, rlast);
+    const long pb = cbase + (long)(rr - c0) * CB;
+#pragma unroll
+    for (int jc = 0; jc < CB; ++jc) {
+      b.e[h][jc] = ld2_nt(Hs + (size_t)(pb + jc) * MB * 2);
+      b.eb[h][jc] = ld2_nt(Hbs + (size_t)(pb + jc) * MB * 2);
+    }
+    b.sg[h] = Ss[(size_t)rr * MB];
+    b.rb[h] = Rs[(size_t)rr * MB];
+  }
+}
+
+// one off-diagonal entry; row r (coefficients *r), column jc; n: time
+// order reversed (time(r) < time(c))
+__device__ __forceinline__ void adj_pair(double2 e, double2& eb, bool n, const SlotCoef& r, AdjCols& cs, int jc,
+                                         double& s1, double& s2, double& s3) {
+  const double Fv = e.x, Pv = e.y;
+  const double nf = n ? 1.0 : 0.0, pf = 1.0 - nf;
+  s1 += Fv * cs.rf[jc];
+  cs.t1[jc] += Fv * r.rf;
+  s2 += pf * Pv * cs.rr[jc];
+  cs.t3[jc] += pf * Pv * r.rf;
+  s3 += nf * Pv * cs.rf[jc];
+  cs.t2[jc] += nf * Pv * r.rr;
+  eb.x -= r.u * cs.rf[jc] + cs.u[jc] * r.rf;
+  eb.y += nf * (r.v * cs.rf[jc] + cs.ms[jc] * r.rr) - pf * (cs.v[jc] * r.rf + r.ms * cs.rr[jc]);
+}
+
+__device__ __forceinline__ void row_flush(const SlotCoef& r, int rr, double s1, double s2, double s3, double* accF,
+                                          double* accR, int lane) {
+  atomicAdd(&accR[rr * MB + lane], r.m * (-r.w * s1 - s2));
+  atomicAdd(&accF[rr * MB + lane], r.w * s3);
+}
+
+// full rows r .. r+AM_B-1 of one column block
+__device__ __forceinline__ void adj_batch(const AdjArgs& a, AdjBatch& b, int r, int r1, long cbase, int c0,
+                                          AdjCols& cs, double* Hbs, double* accF, double* accR, int lane) {
+  const StepState& st = a.st;
+  const int N = a.N;
+#pragma unroll
+  for (int h = 0; h < AM_B; ++h) {
+    const int rr = r + h;
+    if (rr < r1) {
+      SlotCoef k;
+      const int rel = (rr - st.s_min + N) % N;
+      const int tx = st.tmin + rel;
+      k.w = rel == 0 ? 0.5 : 1.0;
+      k.m = tx <= st.t - 2 ? 1.0 : 0.0;
+      const double2 sg = tx <= st.t - 1 ? b.sg[h] : make_double2(0.0, 0.0);
+      k.u = k.m * k.w * sg.y;
+      k.v = k.w * sg.x;
+      k.ms = k.m * sg.y;
+      k.rf = b.rb[h].x;
+      k.rr = b.rb[h].y;
+      const unsigned neg = rr >= st.s_min ? cs.phi : 0u;
+      const long pb = cbase + (long)(rr - c0) * CB;
+      double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+      for (int jc = 0; jc < CB; ++jc) {
+        adj_pair(b.e[h][jc], b.eb[h][jc], (neg >> jc) & 1u, k, cs, jc, s1, s2, s3);
+        st2_nt(Hbs + (size_t)(pb + jc) * MB * 2, b.eb[h][jc]);
+      }
+      row_flush(k, rr, s1, s2, s3, accF, accR, lane);
+    }
+  }
+}
+
+__global__ __launch_bounds__(AM_THREADS, 2) void adj_memint_kernel(AdjArgs a) {
+  extern __shared__ double smem[];
+  const int N = a.N;
+  double* accF = smem;           // SigF_bar per slot, lane
+  double* accR = accF + N * MB;  // Sigr_bar
+  __shared__ double red[AM_THREADS];
+  __shared__ int next_item;
+  const StepState& st = a.st;
+  const int t = st.t;
+  const int Rv = t > N ? N : t;  // slots in use: the window [tmin, t]
+  const int smin = st.s_min;
+  const int mb = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int lane = tid & (MB - 1);
+  const int worker = tid >> 3;
+  const int nworkers = AM_THREADS >> 3;
+  for (int i = tid; i < N * MB; i += AM_THREADS) {
+    accF[i] = 0.0;
+    accR[i] = 0.0;
+  }
+  if (tid == 0) next_item = nworkers;
+  __syncthreads();
+  const double* Hs = a.H + hoff(mb, 0, lane, a.NP);
+  double* Hbs = a.Hbar + hoff(mb, 0, lane, a.NP);
+  const double2* Ss = reinterpret_cast<const double2*>(a.S + soff(mb, 0, lane, N));
+  const double2* Rs = reinterpret_cast<const double2*>(a.Rb + soff(mb, 0, lane, N));
+
+  int item = worker;
+  while (item < a.nitems) {
+    const int2 jk = a.items[item];
+    const int c0 = jk.x * CB;
+    const int r0 = c0 + jk.y * MEMINT_ROWS;
+    const int r1 = min(r0 + MEMINT_ROWS, Rv);
+    if (c0 < Rv && r0 < r1) {
+      const long cbase = colblock_offset(jk.x, N);
+      AdjCols cs;
+      cs.phi = 0;
+#pragma unroll
+      for (int jc = 0; jc < CB; ++jc) {
+        const int c = c0 + jc;
+        const SlotCoef k = slot_coef(a, Ss, Rs, min(c, Rv - 1));
+        const bool ok = c < Rv;
+        cs.u[jc] = ok ? k.u : 0.0;
+        cs.v[jc] = ok ? k.v : 0.0;
+        cs.ms[jc] = ok ? k.ms : 0.0;
+        cs.rf[jc] = ok ? k.rf : 0.0;
+        cs.rr[jc] = ok ? k.rr : 0.0;
+        cs.t1[jc] = cs.t2[jc] = cs.t3[jc] = 0.0;
+        cs.phi |= (c < smin ? 1u : 0u) << jc;
+      }
+      int r = r0;
+      // rows crossing the block diagonal: columns c0 .. r
+      for (; r < r1 && r < c0 + CB; ++r) {
+        const SlotCoef k = slot_coef(a, Ss, Rs, r);
+        const unsigned neg = r >= smin ? cs.phi : 0u;
+        const long pb = cbase + (long)(r - c0) * CB;
+        double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        for (int jc = 0; jc < CB; ++jc) {
+          const int c = c0 + jc;
+          if (c > r) break;
+          const double2 e = ld2(Hs + (size_t)(pb + jc) * MB * 2);
+          double* pe = Hbs + (size_t)(pb + jc) * MB * 2;
+          double2 eb = ld2(pe);
+          if (c == r) {
+            s1 += e.x * k.rf;
+            eb.x -= k.u * k.rf;
+          } else {
+            adj_pair(e, eb, (neg >> jc) & 1u, k, cs, jc, s1, s2, s3);
+          }
+          st2(pe, eb);
+        }
+        row_flush(k, r, s1, s2, s3, accF, accR, lane);
+      }
+      if (r < r1) {
+        const int rlast = r1 - 1;
+        AdjBatch A, B;
+        adj_load(A, Hs, Hbs, Ss, Rs, cbase, c0, r, rlast);
+        while (true) {
+          adj_load(B, Hs, Hbs, Ss, Rs, cbase, c0, r + AM_B, rlast);
+          adj_batch(a, A, r, r1, cbase, c0, cs, Hbs, accF, accR, lane);
+          r += AM_B;
+          if (r >= r1) break;
+          adj_load(A, Hs, Hbs, Ss, Rs, cbase, c0, r + AM_B, rlast);
+          adj_batch(a, B, r, r1, cbase, c0, cs, Hbs, accF, accR, lane);
+          r += AM_B;
+          if (r >= r1) break;
+        }
+      }
+#pragma unroll
+      for (int jc = 0; jc < CB; ++jc) {
+        const int c = c0 + jc;
+        if (c < Rv) {
+          const int rel = (c - smin + N) % N;
+          const double w = rel == 0 ? 0.5 : 1.0;
+          const double m = st.tmin + rel <= t - 2 ? 1.0 : 0.0;
+          atomicAdd(&accR[c * MB + lane], m * (-w * cs.t1[jc] + cs.t2[jc]));
+          atomicAdd(&accF[c * MB + lane], -w * cs.t3[jc]);
+        }
+      }
+    }
+    int nxt = 0;
+    if (lane == 0) nxt = atomicAdd(&next_item, 1);
+    item = __shfl(nxt, tid & ~(MB - 1) & 63);
+  }
+  __syncthreads();
+
+  // Sigma_bar of the live times t' in [tmin, t-1]; dL/dlambda through the
+  // lambda^2 prefactors of the sunset
+  const int p = mb * MB + lane;
+  const bool real = p < a.V;
+  double2* Sb = reinterpret_cast<double2*>(a.Sbar + soff(mb, 0, lane, N));
+  double lam = 0.0;
+  for (int q = worker; q < st.L; q += nworkers) {
+    const int x = (st.tmin + q - 1) % N;
+    const double2 sb = make_double2(accF[x * MB + lane], accR[x * MB + lane]);
+    Sb[(size_t)x * MB] = real ? sb : make_double2(0.0, 0.0);
+    const double2 sg = Ss[(size_t)x * MB];
+    if (real) lam += sg.x * sb.x + sg.y * sb.y;
+  }
+  const double tot = block_sum<AM_THREADS>(lam, red);
+  if (tid == 0) a.LP[mb] += a.two_over_lambda * tot;
+}
+
+// the Hartree adjoint of step 3 and the initial-row gradients:
+// F(1,1) = F(2,1) = (n+1/2)/w, F(2,2) = (n+1/2)(w dt^2 + 1/w)
+__global__ __launch_bounds__(1024) void adj_pending_kernel(const double* __restrict__ P, int NB, double dt,
+                                                           double h_ivol, const double* __restrict__ m2r,
+                                                           double* __restrict__ LP, double* __restrict__ Mb_out) {
+  __shared__ double red[1024];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < NB; i += blockDim.x) acc += P[(size_t)(3 & 1) * NB + i];
+  const double Mb = dt * dt * block_sum<1024>(acc, red);
+  if (threadIdx.x == 0) {
+    Mb_out[0] = Mb;
+    LP[0] += h_ivol * m2r[(3 & 1) * 2 + 1] * Mb;
+  }
+}
+
+__global__ void adj_final_kernel(const double* __restrict__ Hbar, long NP, int V, int N, const double* kL2,
+                                 double initmass, int have_mass, double dt, double hl_ivol,
+                                 const double* __restrict__ Mb, double* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= V) return;
+  const int mb = p / MB, lane = p % MB;
+  const int s1 = 0, s2 = 1 % N;
+  const double f11 = Hbar[hoff(mb, pair_off(s1, s1, N), lane, NP)];
+  const double f21 = Hbar[hoff(mb, pair_off(s2, s1, N), lane, NP)];
+  const double f22 = Hbar[hoff(mb, pair_off(s2, s2, N), lane, NP)] + (Mb ? hl_ivol * Mb[0] : 0.0);
+  out[p] = f11;
+  out[V + p] = f21;
+  out[2 * V + p] = f22;
+  if (have_mass) {
+    const double w = sqrt(kL2[p] + initmass * initmass);
+    out[3 * V + p] = f11 / w + f21 / w + f22 * (w * dt * dt + 1.0 / w);
+  }
+}
+
+}  // namespace qftkb
+
+// ===========================================================================
+// host side
+// ===========================================================================
+namespace qftkb_host {
+
+struct AdjState {
+  int C = 0;          // checkpoint spacing in steps; 0: tape every dropped column
+  int max_steps = 0;  // largest T that may be recorded
+  double* Hbar = nullptr;
+  double* Rb = nullptr;
+  double* Sbar = nullptr;
+  double2* X = nullptr;
+  double* P = nullptr;    // [2][NB]
+  double* LP = nullptr;   // [NB]
+  double* m2r = nullptr;  // [2][2]
+  double* Mb = nullptr;   // [1]
+  double* out = nullptr;  // 4 x V
+  double* tape = nullptr;
+  int tape_cols = 0;
+  size_t col_doubles = 0;  // one tape column
+  std::vector<double*> ckpt;
+  size_t hbytes = 0;
+  size_t bytes = 0;
+  long long recomputed = 0;
+  int stop_after = 0;  // tests: end the next sweep after this reverse step
+};
+
+namespace {
+
+int amalloc(kb_ctx* c, AdjState* s, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes ? bytes : 8);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return seterr(c, KB_ERR_OOM, "adjoint: hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+  }
+  s->bytes += bytes;
+  return KB_OK;
+}
+
+void free_state(AdjState* s) {
+  if (!s) return;
+  for (void* p : {(void*)s->Hbar, (void*)s->Rb, (void*)s->Sbar, (void*)s->X, (void*)s->P, (void*)s->LP,
+                  (void*)s->m2r, (void*)s->Mb, (void*)s->out, (void*)s->tape})
+    if (p) (void)hipFree(p);
+  for (double* p : s->ckpt)
+    if (p) (void)hipFree(p);
+  delete s;
+}
+
+// first step of the segment that contains step t (checkpoint at seg - 1)
+inline int seg_first(int t, int C) { return 3 + C * ((t - 3) / C); }
+
+inline int tape_index(const kb_ctx* c, int t) {
+  const AdjState* s = c->adj;
+  return s->C > 0 ? t - seg_first(t, s->C) : t - c->N - 1;
+}
+
+int save_column(kb_ctx* c, int t) {
+  AdjState* s = c->adj;
+  const StepState st = state_of_step(t, c->N);
+  if (st.dead < 0) return KB_OK;
+  const int k = tape_index(c, t);
+  if (k < 0 || k >= s->tape_cols) return seterr(c, KB_ERR_STATE, "adjoint: tape column %d out of range", k);
+  const dim3 grid((c->N * MB + 255) / 256, c->NB);
+  hipLaunchKernelGGL(tape_copy_kernel, grid, dim3(256), 0, c->stream, c->H, s->tape + (size_t)k * s->col_doubles,
+                     c->NP, c->N, st.s_t, 1);
+  LAUNCHCHK(c);
+  return KB_OK;
+}
+
+AdjArgs adj_args(kb_ctx* c, int t, int T) {
+  AdjState* s = c->adj;
+  AdjArgs a{};
+  a.H = c->H;
+  a.Hbar = s->Hbar;
+  a.S = c->S;
+  a.Rb = s->Rb;
+  a.Sbar = s->Sbar;
+  a.kL2 = c->kL2;
+  a.m2r = s->m2r;
+  a.P = s->P;
+  a.LP = s->LP;
+  a.items = c->items;
+  a.nitems = c->nitems;
+  a.NP = c->NP;
+  a.N = c->N;
+  a.V = c->V;
+  a.NB = c->NB;
+  a.dt = c->p.dt;
+  const double h = (double)(c->p.ONgroup + 2) / (6.0 * (double)c->p.ONgroup);
+  a.h_ivol = h / (double)c->V;
+  a.hl_ivol = c->p.Lambda * h / (double)c->V;
+  a.two_over_lambda = c->p.Lambda != 0.0 ? 2.0 / c->p.Lambda : 0.0;
+  a.integral = c->p.pexp == KB_PEXP_LAMBDA_NLO;
+  a.first = t == T;
+  a.st = state_of_step(t, c->N);
+  return a;
+}
+
+int launch_sigma_adjoint(kb_ctx* c) {
+  AdjState* s = c->adj;
+  const double ivol = 1.0 / (double)c->V;
+  const double lam = c->p.Lambda, ON = (double)c->p.ONgroup;
+  SigmaArgs a{};
+  a.H = c->H;
+  a.W = c->W;
+  a.tw = c->tw;
+  a.plan = c->plan;
+  a.NP = c->NP;
+  a.N = c->N;
+  a.V = c->V;
+  a.n = c->p.Nx;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.nz = c->nz;
+  a.st = c->st_dev;
+  a.preF = ivol * ivol * ivol * (-(lam * lam) * (ON + 2.0) / (18.0 * ON * ON));
+  a.preR = ivol * ivol * ivol * (-(lam * lam) * (ON + 2.0) / (6.0 * ON * ON));
+  a.X = s->X;
+  a.Sbar = s->Sbar;
+  a.Hbar = s->Hbar;
+  hipError_t e = launch_sigma_planes_adjoint(a, c->N - 1, c->stream);
+  if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "adjoint self-energy launch failed: %s", hipGetErrorString(e));
+  return KB_OK;
+}
+
+// reverse step t: the primal window of step t is in H
+int reverse_step(kb_ctx* c, int t, int T) {
+  AdjState* s = c->adj;
+  const int N = c->N;
+  const bool nlo = c->p.pexp == KB_PEXP_LAMBDA_NLO;
+  const AdjArgs a = adj_args(c, t, T);
+  int rc;
+  {
+    StageTimer tm(c, 4);
+    hipLaunchKernelGGL(adj_state_kernel, dim3(1), dim3(1), 0, c->stream, c->st_dev, t, N);
+    LAUNCHCHK(c);
+    if (nlo && (rc = launch_sigma(c, s->X))) return rc;
+    const double H = c->p.Lambda * (double)(c->p.ONgroup + 2) / (6.0 * (double)c->p.ONgroup);
+    hipLaunchKernelGGL(adj_hartree_kernel, dim3(1), dim3(1024), 0, c->stream, c->H, c->NP, c->V,
+                       pair_off(a.st.s_tm1, a.st.s_tm1, N), H / (double)c->V, c->p.Mass2, s->m2r + (t & 1) * 2);
+    LAUNCHCHK(c);
+    hipLaunchKernelGGL(adj_row_kernel, dim3(c->NB), dim3(ROW_THREADS), 0, c->stream, a);
+    LAUNCHCHK(c);
+  }
+  if (nlo) {
+    StageTimer tm(c, 3);
+    hipLaunchKernelGGL(adj_memint_kernel, dim3(c->NB), dim3(AM_THREADS), memint_lds(N), c->stream, a);
+    LAUNCHCHK(c);
+  }
+  {
+    StageTimer tm(c, 4);
+    if (nlo && (rc = launch_sigma_adjoint(c))) return rc;
+    const double* col = nullptr;
+    if (a.st.dead >= 0) col = s->tape + (size_t)tape_index(c, t) * s->col_doubles;
+    const dim3 grid((N * MB + 255) / 256, c->NB);
+    hipLaunchKernelGGL(adj_slide_kernel, grid, dim3(256), 0, c->stream, c->H, s->Hbar, col, c->NP, N, a.st.s_t);
+    LAUNCHCHK(c);
+  }
+  return KB_OK;
+}
+
+// restore the checkpoint before step `first` and recompute steps
+// first .. last, keeping their dropped columns on the tape
+int recompute_segment(kb_ctx* c, int first, int last) {
+  AdjState* s = c->adj;
+  const int k = (first - 3) / s->C;
+  HIPCHK(c, hipMemcpyAsync(c->H, s->ckpt[k], s->hbytes, hipMemcpyDeviceToDevice, c->stream));
+  c->st = state_of_step(first - 1, c->N);
+  int rc;
+  if ((rc = upload_state(c))) return rc;
+  for (int t = first; t <= last; ++t) {
+    advance_state(c->st, c->N);
+    if ((rc = save_column(c, t))) return rc;
+    if ((rc = enqueue_step(c))) return rc;
+    s->recomputed += 1;
+  }
+  return KB_OK;
+}
+
+}  // namespace
+
+int adj_before_step(kb_ctx* c) {
+  AdjState* s = c->adj;
+  const int t = c->st.t;
+  if (t > s->max_steps)
+    return seterr(c, KB_ERR_ARG, "adjoint recording: step %d beyond max_steps %d", t, s->max_steps);
+  return save_column(c, t);
+}
+
+int adj_after_step(kb_ctx* c) {
+  AdjState* s = c->adj;
+  const int t = c->st.t;
+  if (s->C > 0 && (t - 2) % s->C == 0) {
+    const size_t k = (size_t)((t - 2) / s->C);
+    if (k < s->ckpt.size())
+      HIPCHK(c, hipMemcpyAsync(s->ckpt[k], c->H, s->hbytes, hipMemcpyDeviceToDevice, c->stream));
+  }
+  return KB_OK;
+}
+
+void adj_free(kb_ctx* c) {
+  if (c && c->adj) {
+    c->bytes -= c->adj->bytes;
+    free_state(c->adj);
+    c->adj = nullptr;
+  }
+}
+
+}  // namespace qftkb_host
+
+extern "C" {
+
+int kb_adj_record(kb_ctx* c, int checkpoint_every, int max_steps) {
+  if (!c || checkpoint_every < 0 || max_steps < 2) return KB_ERR_ARG;
+  if (!c->initialised || c->st.tmax != 2)
+    return seterr(c, KB_ERR_STATE, "adjoint recording starts right after initialisation (window [1,2])");
+  if (c->in_ensemble) return seterr(c, KB_ERR_STATE, "adjoint: not available for ensemble members");
+  if (c->p.pexp == KB_PEXP_LAMBDA_NLO && !sigma_planes_supported(c->p.Nx))
+    return seterr(c, KB_ERR_ARG, "adjoint: Nx=%d above the plane self-energy limit", c->p.Nx);
+  {
+    // per function, shared by every context: the largest supported window
+    const size_t lds = memint_lds(640);
+    hipError_t e = hipFuncSetAttribute((const void*)adj_memint_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess)
+      return seterr(c, KB_ERR_ARG, "adjoint: %zu bytes of LDS rejected: %s", lds, hipGetErrorString(e));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  adj_free(c);
+  drop_graph(c);
+  AdjState* s = new (std::nothrow) AdjState();
+  if (!s) return KB_ERR_OOM;
+  s->C = checkpoint_every;
+  s->max_steps = max_steps;
+  const size_t hcount = (size_t)c->NB * (size_t)c->NP * MB * 2;
+  s->hbytes = hcount * sizeof(double);
+  const size_t ring = (size_t)c->NB * c->N * MB * 2;
+  s->col_doubles = ring;
+  if (s->C > 0)
+    s->tape_cols = s->C;
+  else
+    s->tape_cols = max_steps > c->N ? max_steps - c->N : 0;
+  int rc = KB_OK;
+  auto fail = [&](int code) {
+    free_state(s);
+    return code;
+  };
+  if ((rc = amalloc(c, s, (void**)&s->Hbar, s->hbytes)) || (rc = amalloc(c, s, (void**)&s->Rb, ring * 8)) ||
+      (rc = amalloc(c, s, (void**)&s->Sbar, ring * 8)) ||
+      (rc = amalloc(c, s, (void**)&s->X, (size_t)c->N * c->V * sizeof(double2))) ||
+      (rc = amalloc(c, s, (void**)&s->P, 2 * (size_t)c->NB * 8)) ||
+      (rc = amalloc(c, s, (void**)&s->LP, (size_t)c->NB * 8)) || (rc = amalloc(c, s, (void**)&s->m2r, 4 * 8)) ||
+      (rc = amalloc(c, s, (void**)&s->Mb, 8)) || (rc = amalloc(c, s, (void**)&s->out, 4 * (size_t)c->V * 8)) ||
+      (rc = amalloc(c, s, (void**)&s->tape, (size_t)s->tape_cols * ring * 8)))
+    return fail(rc);
+  if (s->C > 0) {
+    const int nck = (max_steps - 2) / s->C + 1;  // windows at t = 2, 2 + C, ...
+    for (int k = 0; k < nck; ++k) {
+      double* p = nullptr;
+      if ((rc = amalloc(c, s, (void**)&p, s->hbytes))) return fail(rc);
+      s->ckpt.push_back(p);
+    }
+    if (hipMemcpyAsync(s->ckpt[0], c->H, s->hbytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+      return fail(seterr(c, KB_ERR_HIP, "adjoint: checkpoint copy failed"));
+  }
+  if (hipMemsetAsync(s->Hbar, 0, s->hbytes, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return fail(seterr(c, KB_ERR_HIP, "adjoint: initialisation failed"));
+  c->adj = s;
+  c->bytes += s->bytes;
+  return KB_OK;
+}
+
+int kb_adj_stop(kb_ctx* c) {
+  if (!c) return KB_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  adj_free(c);
+  return KB_OK;
+}
+
+int kb_adj_info(kb_ctx* c, long long* device_bytes, long long* recomputed_steps) {
+  if (!c) return KB_ERR_ARG;
+  if (!c->adj) return seterr(c, KB_ERR_STATE, "adjoint: not recording");
+  if (device_bytes) *device_bytes = (long long)c->adj->bytes;
+  if (recomputed_steps) *recomputed_steps = c->adj->recomputed;
+  return KB_OK;
+}
+
+int kb_adj_debug_stop(kb_ctx* c, int t_stop) {
+  if (!c || t_stop < 3) return KB_ERR_ARG;
+  if (!c->adj) return seterr(c, KB_ERR_STATE, "adjoint: not recording");
+  c->adj->stop_after = t_stop;
+  return KB_OK;
+}
+
+int kb_adj_get_entry(kb_ctx* c, int which, int i, int j, double* out) {
+  if (!c || !out || (which != 0 && which != 1)) return KB_ERR_ARG;
+  if (!c->adj) return seterr(c, KB_ERR_STATE, "adjoint: not recording");
+  if (i < c->st.tmin || i > c->st.tmax || j < c->st.tmin || j > c->st.tmax)
+    return seterr(c, KB_ERR_WINDOW, "adjoint get: (%d,%d) outside window [%d,%d]", i, j, c->st.tmin, c->st.tmax);
+  const int si = (i - 1) % c->N, sj = (j - 1) % c->N;
+  const double sg = which ? (si >= sj ? 1.0 : -1.0) : 1.0;
+  hipLaunchKernelGGL(gather_pair, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->adj->Hbar, c->NP, c->V,
+                     pair_off(si, sj, c->N), which, sg, c->vbuf);
+  LAUNCHCHK(c);
+  return download(c, out, c->vbuf, c->V);
+}
+
+int kb_adj_gradient(kb_ctx* c, const double* wgt, double* dL_dlambda, double* dL_dn0, double* dL_dF0) {
+  if (!c || !wgt) return KB_ERR_ARG;
+  AdjState* s = c->adj;
+  if (!s) return seterr(c, KB_ERR_STATE, "adjoint: kb_adj_record was not called");
+  if (dL_dn0 && !c->init_particles)
+    return seterr(c, KB_ERR_STATE, "adjoint: dL/dn0 needs initialisation by kb_init_particles");
+  const int T = c->st.tmax;
+  const int N = c->N;
+  if (T < 2 || T > s->max_steps) return seterr(c, KB_ERR_STATE, "adjoint: window end %d not recorded", T);
+  int rc;
+  s->recomputed = 0;
+  HIPCHK(c, hipMemsetAsync(s->Hbar, 0, s->hbytes, c->stream));
+  HIPCHK(c, hipMemsetAsync(s->P, 0, 2 * (size_t)c->NB * 8, c->stream));
+  HIPCHK(c, hipMemsetAsync(s->LP, 0, (size_t)c->NB * 8, c->stream));
+  HIPCHK(c, hipMemsetAsync(s->m2r, 0, 4 * 8, c->stream));
+  if ((rc = upload(c, c->vbuf, wgt, c->V))) return rc;
+  const int blocks = (c->V + 255) / 256;
+  hipLaunchKernelGGL(adj_seed_kernel, dim3(blocks), dim3(256), 0, c->stream, c->H, s->Hbar, c->vbuf, c->NP, c->V,
+                     N, (T - 1) % N, (T - 2) % N, c->p.dt);
+  LAUNCHCHK(c);
+  // the tape holds the last segment's columns already (or every column)
+  int have_from = s->C > 0 ? seg_first(T, s->C) : 3;
+  const int stop = s->stop_after;
+  s->stop_after = 0;
+  for (int t = T; t >= 3; --t) {
+    if (t < have_from) {
+      const int first = seg_first(t, s->C);
+      if ((rc = recompute_segment(c, first, t))) return rc;
+      have_from = first;
+    }
+    if ((rc = reverse_step(c, t, T))) return rc;
+    if (t == stop) {
+      // left mid-sweep for inspection: the window of step t-1
+      c->st = state_of_step(t - 1, N);
+      if ((rc = upload_state(c))) return rc;
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      return KB_OK;
+    }
+  }
+  const double h = (double)(c->p.ONgroup + 2) / (6.0 * (double)c->p.ONgroup);
+  const bool pending = T >= 3;
+  if (pending) {
+    hipLaunchKernelGGL(adj_pending_kernel, dim3(1), dim3(1024), 0, c->stream, s->P, c->NB, c->p.dt,
+                       h / (double)c->V, s->m2r, s->LP, s->Mb);
+    LAUNCHCHK(c);
+  }
+  hipLaunchKernelGGL(adj_final_kernel, dim3(blocks), dim3(256), 0, c->stream, s->Hbar, c->NP, c->V, N, c->kL2,
+                     c->initmass, c->init_particles ? 1 : 0, c->p.dt, c->p.Lambda * h / (double)c->V,
+                     pending ? s->Mb : nullptr, s->out);
+  LAUNCHCHK(c);
+  // the window is back at the initial state [1, 2]
+  c->st = state_of_step(2, N);
+  if ((rc = upload_state(c))) return rc;
+  std::vector<double> lp((size_t)c->NB);
+  if ((rc = download(c, lp.data(), s->LP, lp.size()))) return rc;
+  double lam = 0.0;
+  for (double v : lp) lam += v;
+  if (dL_dlambda) *dL_dlambda = lam;
+  if (dL_dF0 && (rc = download(c, dL_dF0, s->out, 3 * (size_t)c->V))) return rc;
+  if (dL_dn0 && (rc = download(c, dL_dn0, s->out + 3 * (size_t)c->V, c->V))) return rc;
+  // the initial window is checkpoint 0 again; a new forward run may be recorded
+  return KB_OK;
+}
+
+}  // extern "C"

@@ -12,62 +12,12 @@
 #include <vector>
 
 #include "../../include/qftkb.h"
-#include "kb_kernels.h"
+#include "kb_internal.h"
 
 using namespace qftkb;
+using namespace qftkb_host;
 
-namespace {
-
-struct EventPair {
-  hipEvent_t a, b;
-};
-
-}  // namespace
-
-struct kb_ctx {
-  kb_params p{};
-  int V = 0, Vpad = 0, NB = 0, N = 0;
-  long NP = 0;
-  int nx = 1, ny = 1, nz = 1;
-  StepState st{};          // host mirror of the window (simdata.indices)
-  bool initialised = false;
-  bool in_ensemble = false;
-  // device memory
-  double* H = nullptr;     // history
-  double* S = nullptr;     // self-energy ring
-  double2* W = nullptr;    // FFT workspace
-  double* kL2 = nullptr;
-  double* m2 = nullptr;    // [0] current Hartree mass^2
-  double* partial = nullptr;
-  double* modes = nullptr; // 6 x V measurement outputs
-  double* vbuf = nullptr;  // 4 x V staging
-  double2* tw = nullptr;
-  int* flag = nullptr;
-  int2* items = nullptr;  // memory-integral work items
-  int nitems = 0;
-  // device-resident step state and launch tables (graph-capturable steps)
-  StepState* st_dev = nullptr;
-  StepState** stptr_dev = nullptr;  // {st_dev}
-  int* N_dev = nullptr;
-  MemintArgs* margs_dev = nullptr;
-  int* wg0_dev = nullptr;  // {0}
-  bool use_graph = false;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  size_t bytes = 0;
-  FFTPlan plan{};
-  int fftP = 1;
-  hipStream_t own = nullptr;
-  hipStream_t stream = nullptr;
-  bool timing = false;
-  std::vector<EventPair> pending[3];
-  std::vector<hipEvent_t> freeEv;
-  double tsum[3] = {0, 0, 0};
-  long long tcount[3] = {0, 0, 0};
-  std::string err;
-};
-
-namespace {
+namespace qftkb_host {
 
 int seterr(kb_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
@@ -78,22 +28,6 @@ int seterr(kb_ctx* c, int code, const char* fmt, ...) {
   if (c) c->err = buf;
   return code;
 }
-
-#define HIPCHK(ctx, call)                                                                     \
-  do {                                                                                        \
-    hipError_t e_ = (call);                                                                   \
-    if (e_ != hipSuccess)                                                                     \
-      return seterr((ctx), KB_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
-                    __FILE__, __LINE__);                                                      \
-  } while (0)
-
-#define LAUNCHCHK(ctx)                                                                          \
-  do {                                                                                          \
-    hipError_t e_ = hipGetLastError();                                                          \
-    if (e_ != hipSuccess)                                                                       \
-      return seterr((ctx), KB_ERR_HIP, "kernel launch failed: %s (%s:%d)", hipGetErrorString(e_), \
-                    __FILE__, __LINE__);                                                        \
-  } while (0)
 
 inline int slot_of(const kb_ctx* c, int t) { return (t - 1) % c->N; }
 inline bool inwin(const kb_ctx* c, int t) { return c->st.tmin <= t && t <= c->st.tmax; }
@@ -119,28 +53,24 @@ hipEvent_t take_event(kb_ctx* c) {
   return e;
 }
 
-struct StageTimer {
-  kb_ctx* c;
-  int stage;
-  EventPair ev{nullptr, nullptr};
-  StageTimer(kb_ctx* c_, int s) : c(c_), stage(s) {
-    if (c->timing) {
-      ev.a = take_event(c);
-      ev.b = take_event(c);
-      if (ev.a) (void)hipEventRecord(ev.a, c->stream);
-    }
+StageTimer::StageTimer(kb_ctx* c_, int s) : c(c_), stage(s) {
+  if (c->timing) {
+    ev.a = take_event(c);
+    ev.b = take_event(c);
+    if (ev.a) (void)hipEventRecord(ev.a, c->stream);
   }
-  ~StageTimer() {
-    if (c->timing && ev.a && ev.b) {
-      (void)hipEventRecord(ev.b, c->stream);
-      c->pending[stage].push_back(ev);
-    }
+}
+
+StageTimer::~StageTimer() {
+  if (c->timing && ev.a && ev.b) {
+    (void)hipEventRecord(ev.b, c->stream);
+    c->pending[stage].push_back(ev);
   }
-};
+}
 
 int collect_timing(kb_ctx* c) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  for (int s = 0; s < 3; ++s) {
+  for (int s = 0; s < NSTAGE; ++s) {
     for (auto& ev : c->pending[s]) {
       float ms = 0.f;
       HIPCHK(c, hipEventElapsedTime(&ms, ev.a, ev.b));
@@ -179,7 +109,7 @@ int launch_hartree(kb_ctx* c, int i, int j) {
   return KB_OK;
 }
 
-int launch_sigma(kb_ctx* c) {
+int launch_sigma(kb_ctx* c, double2* X) {
   const int rows = c->N - 1;  // grid rows; rows q >= L exit at once
   const double ivol = 1.0 / (double)c->V;
   const double lam = c->p.Lambda, ON = (double)c->p.ONgroup;
@@ -201,6 +131,7 @@ int launch_sigma(kb_ctx* c) {
   a.st = c->st_dev;
   a.preF = ivol * ivol * ivol * (-(lam * lam) * (ON + 2.0) / (18.0 * ON * ON));
   a.preR = ivol * ivol * ivol * (-(lam * lam) * (ON + 2.0) / (6.0 * ON * ON));
+  a.X = X;
   const int L = rows;
   if (sigma_planes_supported(c->p.Nx)) {
     hipError_t e = launch_sigma_planes(a, L, c->stream);
@@ -303,7 +234,8 @@ void free_all(kb_ctx* c) {
                     (void*)c->modes, (void*)c->vbuf, (void*)c->tw, (void*)c->flag, (void*)c->items,
                     (void*)c->st_dev, (void*)c->stptr_dev, (void*)c->N_dev, (void*)c->margs_dev, (void*)c->wg0_dev})
     if (ptr) (void)hipFree(ptr);
-  for (int s = 0; s < 3; ++s)
+  adj_free(c);
+  for (int s = 0; s < NSTAGE; ++s)
     for (auto& ev : c->pending[s]) {
       (void)hipEventDestroy(ev.a);
       (void)hipEventDestroy(ev.b);
@@ -512,6 +444,8 @@ int kb_init_particles(kb_ctx* c, const double* n, double initmass) {
   if ((rc = launch_hartree(c, 2, 2))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->initialised = true;
+  c->initmass = initmass;
+  c->init_particles = true;
   return KB_OK;
 }
 
@@ -522,6 +456,13 @@ int kb_step(kb_ctx* c, int t) {
   if (t != c->st.tmax + 1 || t < 3)
     return seterr(c, KB_ERR_WINDOW, "evolve: t=%d but window is [%d,%d]", t, c->st.tmin, c->st.tmax);
   expand(c);  // the device state is advanced by the step's first kernel
+  if (c->adj) {
+    // recording for the adjoint: eager launches with the tape kept
+    int rc;
+    if ((rc = adj_before_step(c))) return rc;
+    if ((rc = enqueue_step(c))) return rc;
+    return adj_after_step(c);
+  }
   if (c->use_graph && !c->timing) {
     if (!c->gexec) {
       HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -733,7 +674,7 @@ int kb_enable_timing(kb_ctx* c, int on) {
 }
 
 int kb_timing(kb_ctx* c, int stage, double* total_ms, long long* count) {
-  if (!c || stage < 0 || stage > 2) return KB_ERR_ARG;
+  if (!c || stage < 0 || stage >= NSTAGE) return KB_ERR_ARG;
   int rc = collect_timing(c);
   if (rc) return rc;
   if (total_ms) *total_ms = c->tsum[stage];
@@ -745,7 +686,7 @@ int kb_reset_timing(kb_ctx* c) {
   if (!c) return KB_ERR_ARG;
   int rc = collect_timing(c);
   if (rc) return rc;
-  for (int s = 0; s < 3; ++s) {
+  for (int s = 0; s < NSTAGE; ++s) {
     c->tsum[s] = 0;
     c->tcount[s] = 0;
   }

@@ -1,0 +1,122 @@
+// kb_internal.h -- host-side context of libqftkb shared by the forward
+// (kb_api.hip) and adjoint (kb_adjoint.hip) translation units.  Nothing here
+// is part of the C ABI; every symbol is hidden from the shared library's
+// dynamic symbol table.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/qftkb.h"
+#include "kb_kernels.h"
+
+#pragma GCC visibility push(hidden)
+
+namespace qftkb_host {
+
+struct EventPair {
+  hipEvent_t a, b;
+};
+
+// stages of kb_timing: 0 memory integral, 1 self-energy, 2 Hartree,
+// 3 adjoint memory integral, 4 the other kernels of the adjoint sweep
+constexpr int NSTAGE = 5;
+
+struct AdjState;  // kb_adjoint.hip
+
+}  // namespace qftkb_host
+
+struct kb_ctx {
+  kb_params p{};
+  int V = 0, Vpad = 0, NB = 0, N = 0;
+  long NP = 0;
+  int nx = 1, ny = 1, nz = 1;
+  qftkb::StepState st{};  // host mirror of the window (simdata.indices)
+  bool initialised = false;
+  bool in_ensemble = false;
+  double initmass = 0.0;      // set by kb_init_particles
+  bool init_particles = false;
+  // device memory
+  double* H = nullptr;     // history
+  double* S = nullptr;     // self-energy ring
+  double2* W = nullptr;    // FFT workspace
+  double* kL2 = nullptr;
+  double* m2 = nullptr;    // [0] current Hartree mass^2
+  double* partial = nullptr;
+  double* modes = nullptr; // 6 x V measurement outputs
+  double* vbuf = nullptr;  // 4 x V staging
+  double2* tw = nullptr;
+  int* flag = nullptr;
+  int2* items = nullptr;  // memory-integral work items
+  int nitems = 0;
+  // device-resident step state and launch tables (graph-capturable steps)
+  qftkb::StepState* st_dev = nullptr;
+  qftkb::StepState** stptr_dev = nullptr;  // {st_dev}
+  int* N_dev = nullptr;
+  qftkb::MemintArgs* margs_dev = nullptr;
+  int* wg0_dev = nullptr;  // {0}
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  size_t bytes = 0;
+  qftkb::FFTPlan plan{};
+  int fftP = 1;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  bool timing = false;
+  std::vector<qftkb_host::EventPair> pending[qftkb_host::NSTAGE];
+  std::vector<hipEvent_t> freeEv;
+  double tsum[qftkb_host::NSTAGE] = {};
+  long long tcount[qftkb_host::NSTAGE] = {};
+  qftkb_host::AdjState* adj = nullptr;  // recording / adjoint state
+  std::string err;
+};
+
+#define HIPCHK(ctx, call)                                                                     \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return seterr((ctx), KB_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                    __FILE__, __LINE__);                                                      \
+  } while (0)
+
+#define LAUNCHCHK(ctx)                                                                          \
+  do {                                                                                          \
+    hipError_t e_ = hipGetLastError();                                                          \
+    if (e_ != hipSuccess)                                                                       \
+      return seterr((ctx), KB_ERR_HIP, "kernel launch failed: %s (%s:%d)", hipGetErrorString(e_), \
+                    __FILE__, __LINE__);                                                        \
+  } while (0)
+
+namespace qftkb_host {
+
+int seterr(kb_ctx* c, int code, const char* fmt, ...);
+int upload(kb_ctx* c, double* dst, const double* src, size_t n);
+int download(kb_ctx* c, double* dst, const double* src, size_t n);
+int dmalloc(kb_ctx* c, void** ptr, size_t bytes);
+int upload_state(kb_ctx* c);
+void drop_graph(kb_ctx* c);
+// self-energy stage of the step in c->st_dev; X, if set, receives (Fx, rx)
+int launch_sigma(kb_ctx* c, double2* X = nullptr);
+// the kernels of one forward step (the window in c->st_dev is advanced first)
+int enqueue_step(kb_ctx* c);
+size_t memint_lds(int N);
+
+// times one stage of kb_timing while in scope
+struct StageTimer {
+  kb_ctx* c;
+  int stage;
+  EventPair ev{nullptr, nullptr};
+  StageTimer(kb_ctx* c_, int s);
+  ~StageTimer();
+};
+
+// adjoint hooks (kb_adjoint.hip)
+int adj_before_step(kb_ctx* c);  // after the host window has advanced to step t
+int adj_after_step(kb_ctx* c);
+void adj_free(kb_ctx* c);
+
+}  // namespace qftkb_host
+
+#pragma GCC visibility pop

@@ -54,6 +54,24 @@ __host__ __device__ inline void advance_state(StepState& s, int N) {
   s.s_min = (s.tmin - 1) % N;
 }
 
+// the state advance_state reaches for step t, from t alone (used to walk
+// the window backwards in the adjoint sweep)
+__host__ __device__ inline StepState state_of_step(int t, int N) {
+  StepState s{};
+  s.tmax = t;
+  s.tmin = t > N ? t - N + 1 : 1;
+  const bool full = t > N;
+  s.t = t;
+  s.L = t - s.tmin;
+  s.Rmax = full ? N : t - 1;
+  s.s_t = (t - 1) % N;
+  s.dead = full ? s.s_t : -1;
+  s.s_tm1 = (t - 2) % N;
+  s.s_tm2 = (t - 3) % N;
+  s.s_min = (s.tmin - 1) % N;
+  return s;
+}
+
 struct SigmaArgs {
   const double* H;
   double* S;
@@ -65,6 +83,9 @@ struct SigmaArgs {
   int axis, flags, P, npencils;
   const StepState* st;  // rows q < st->L (t' = tmin + q) are live
   double preF, preR;
+  double2* X;           // forward: if set, receives (Fx, rx) per row and site; adjoint: read
+  const double* Sbar;   // adjoint input (SigF_bar, Sigr_bar), self-energy ring layout
+  double* Hbar;         // adjoint output: history-layout adjoint, row t-1 is added to
 };
 
 // Per-run arguments of memint_step; step-dependent values come from *st.
@@ -121,6 +142,7 @@ __global__ void finite_kernel(const double* H, size_t count, int* flag);
 bool sigma_planes_supported(int Nx);
 hipError_t sigma_planes_prepare(int n);
 hipError_t launch_sigma_planes(const SigmaArgs& a, int rows, hipStream_t st);
+hipError_t launch_sigma_planes_adjoint(const SigmaArgs& a, int rows, hipStream_t st);
 
 #ifndef KB_MEMINT_THREADS
 #define KB_MEMINT_THREADS 256

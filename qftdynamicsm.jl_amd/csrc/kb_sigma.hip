@@ -16,6 +16,14 @@
 //       plane_inv  inverse 2D transform, scale, write SigF/Sigr.
 //   2D / 1D: plane_fwd does everything for its row in one workgroup.
 //
+// The same passes run the adjoint of the stage (ADJ = true): the input is
+// the adjoint self-energy pair (SigF_bar, Sigr_bar) per t', the pointwise
+// step is the transpose of the sunset's Jacobian at the saved forward
+// transforms (Fx, rx), and the output is added to the adjoint history row
+// t-1.  The cosine transform C (real part of the DFT of an even lattice) is
+// symmetric, so Sigma = pre C g(C F, C rho) has the adjoint
+//   g_bar = pre C Sigma_bar,  F_bar += C (dg/dFx)^T g_bar,  rho_bar += C (dg/drx)^T g_bar.
+//
 // All transforms are unnormalised Stockham FFTs in LDS; lines of a plane are
 // padded to Nx+1 elements so that column access does not hit one bank.
 // The FFT length is a template parameter for the common sizes (0 = runtime).
@@ -190,11 +198,47 @@ __device__ __forceinline__ double2 sunset(double2 z) {
   return make_double2(Fx * (FF - 0.75 * rr), rx * (FF - (1.0 / 12.0) * rr));
 }
 
+// transpose of the sunset's Jacobian at (Fx, rx) applied to
+// (gF, gr) = (preF, preR) * (C SigF_bar, C Sigr_bar)
+__device__ __forceinline__ double2 sunset_adj(double2 cs, double2 x, double preF, double preR) {
+  const double gF = preF * cs.x, gr = preR * cs.y;
+  const double Fx = x.x, rx = x.y;
+  const double FF = Fx * Fx, rr = rx * rx;
+  return make_double2(gF * (3.0 * FF - 0.75 * rr) + gr * (2.0 * rx * Fx), gF * (-1.5 * Fx * rx) + gr * (FF - 0.25 * rr));
+}
+
+// the pointwise step of either direction; the forward saves (Fx, rx) when X is set
+template <bool ADJ>
+__device__ __forceinline__ double2 pointwise(const SigmaArgs& a, double2 z, size_t xi) {
+  if constexpr (ADJ) {
+    return sunset_adj(z, a.X[xi], a.preF, a.preR);
+  } else {
+    if (a.X) a.X[xi] = z;
+    return sunset(z);
+  }
+}
+
+// output of the stage for row q (time t'), site s: Sigma into the ring, or
+// the adjoint added to (F_bar, rho_bar)(t-1, t')
+template <bool ADJ>
+__device__ __forceinline__ void emit(const SigmaArgs& a, int s, int slot, double2 z) {
+  if constexpr (ADJ) {
+    const int s_tm1 = a.st->s_tm1;
+    double* o = a.Hbar + hoff(s / MB, pair_off(s_tm1, slot, a.N), s % MB, a.NP);
+    o[0] += z.x;
+    if (slot != s_tm1) o[1] += (s_tm1 >= slot ? 1.0 : -1.0) * z.y;  // rho(t-1,t-1) is not a variable
+  } else {
+    double* o = a.S + soff(s / MB, slot, s % MB, a.N);
+    o[0] = a.preF * z.x;
+    o[1] = a.preR * z.y;
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
 // blockIdx.x: plane iz (3D: 0 .. nz/2), blockIdx.y: row q (t' = tmin + q)
-template <int NT>
+template <int NT, bool ADJ>
 __global__ __launch_bounds__(256) void sigma_plane_fwd(SigmaArgs a) {
   extern __shared__ double2 lds[];
   const int n = NT > 0 ? NT : a.n;
@@ -214,15 +258,21 @@ __global__ __launch_bounds__(256) void sigma_plane_fwd(SigmaArgs a) {
   const size_t rowoff = (size_t)q * a.V;
   const int nplane = n * ny;
 
-  // physical F(t-1,t'), rho(t-1,t') of plane iz and the reflected plane -iz,
-  // inversion-symmetrised
+  // physical F(t-1,t'), rho(t-1,t') (adjoint: SigF_bar, Sigr_bar at t') of
+  // plane iz and the reflected plane -iz, inversion-symmetrised
   for (int e = threadIdx.x; e < nplane; e += blockDim.x) {
     const int iy = e / n, ix = e - iy * n;
     const int s1 = ix + n * (iy + ny * iz);
     const int s2 = (n - ix) % n + n * ((ny - iy) % ny + ny * izn);
-    const double* h1 = a.H + hoff(s1 / MB, pr, s1 % MB, a.NP);
-    const double* h2 = a.H + hoff(s2 / MB, pr, s2 % MB, a.NP);
-    A[iy * ls + ix] = make_double2(0.5 * (h1[0] + h2[0]), 0.5 * (h1[1] + h2[1]) * sg);
+    if constexpr (ADJ) {
+      const double* h1 = a.Sbar + soff(s1 / MB, slot, s1 % MB, a.N);
+      const double* h2 = a.Sbar + soff(s2 / MB, slot, s2 % MB, a.N);
+      A[iy * ls + ix] = make_double2(0.5 * (h1[0] + h2[0]), 0.5 * (h1[1] + h2[1]));
+    } else {
+      const double* h1 = a.H + hoff(s1 / MB, pr, s1 % MB, a.NP);
+      const double* h2 = a.H + hoff(s2 / MB, pr, s2 % MB, a.NP);
+      A[iy * ls + ix] = make_double2(0.5 * (h1[0] + h2[0]), 0.5 * (h1[1] + h2[1]) * sg);
+    }
   }
   __syncthreads();
   double2* res = fft_lines<NT>(A, B, a.plan, ny, ls, 1, a.tw, false);
@@ -235,7 +285,7 @@ __global__ __launch_bounds__(256) void sigma_plane_fwd(SigmaArgs a) {
     // 1D / 2D: the whole lattice is in LDS -- sunset, inverse, write Sigma
     for (int e = threadIdx.x; e < nplane; e += blockDim.x) {
       const int iy = e / n, ix = e - iy * n;
-      res[iy * ls + ix] = sunset(res[iy * ls + ix]);
+      res[iy * ls + ix] = pointwise<ADJ>(a, res[iy * ls + ix], rowoff + e);
     }
     __syncthreads();
     if (ny > 1) {
@@ -245,11 +295,7 @@ __global__ __launch_bounds__(256) void sigma_plane_fwd(SigmaArgs a) {
     res = fft_lines<NT>(res, oth, a.plan, ny, ls, 1, a.tw, true);
     for (int e = threadIdx.x; e < nplane; e += blockDim.x) {
       const int iy = e / n, ix = e - iy * n;
-      const int s = ix + n * iy;
-      const double2 z = res[iy * ls + ix];
-      double* o = a.S + soff(s / MB, slot, s % MB, a.N);
-      o[0] = a.preF * z.x;
-      o[1] = a.preR * z.y;
+      emit<ADJ>(a, ix + n * iy, slot, res[iy * ls + ix]);
     }
     return;
   }
@@ -264,7 +310,7 @@ __global__ __launch_bounds__(256) void sigma_plane_fwd(SigmaArgs a) {
 }
 
 // blockIdx.x: y row iy, blockIdx.y: row q.  Lines along z for every x.
-template <int NT>
+template <int NT, bool ADJ>
 __global__ __launch_bounds__(256) void sigma_zpass(SigmaArgs a) {
   extern __shared__ double2 lds[];
   const int n = NT > 0 ? NT : a.n;
@@ -285,7 +331,7 @@ __global__ __launch_bounds__(256) void sigma_zpass(SigmaArgs a) {
   double2* oth = res == A ? B : A;
   for (int e = threadIdx.x; e < n * nz; e += blockDim.x) {
     const int ix = e / nz, iz = e - ix * nz;
-    res[ix * ls + iz] = sunset(res[ix * ls + iz]);
+    res[ix * ls + iz] = pointwise<ADJ>(a, res[ix * ls + iz], rowoff + ix + n * (iy + ny * iz));
   }
   __syncthreads();
   res = fft_lines<NT>(res, oth, a.plan, n, ls, 1, a.tw, true);
@@ -296,7 +342,7 @@ __global__ __launch_bounds__(256) void sigma_zpass(SigmaArgs a) {
 }
 
 // blockIdx.x: plane iz, blockIdx.y: row q.  Inverse 2D transform + scatter.
-template <int NT>
+template <int NT, bool ADJ>
 __global__ __launch_bounds__(256) void sigma_plane_inv(SigmaArgs a) {
   extern __shared__ double2 lds[];
   const int n = NT > 0 ? NT : a.n;
@@ -321,45 +367,48 @@ __global__ __launch_bounds__(256) void sigma_plane_inv(SigmaArgs a) {
   res = fft_lines<NT>(res, oth, a.plan, ny, ls, 1, a.tw, true);
   for (int e = threadIdx.x; e < nplane; e += blockDim.x) {
     const int iy = e / n, ix = e - iy * n;
-    const int s = ix + n * (iy + ny * iz);
-    const double2 z = res[iy * ls + ix];
-    double* o = a.S + soff(s / MB, slot, s % MB, a.N);
-    o[0] = a.preF * z.x;
-    o[1] = a.preR * z.y;
+    emit<ADJ>(a, ix + n * (iy + ny * iz), slot, res[iy * ls + ix]);
   }
 }
 
 // ---------------------------------------------------------------------------
 // host-side launch of the whole stage
 // ---------------------------------------------------------------------------
-template <int NT>
+template <int NT, bool ADJ>
 static hipError_t launch_planes_t(const SigmaArgs& a, int L, hipStream_t st) {  // L: grid rows
   const int n = a.n;
   const size_t lds_plane = sizeof(double2) * 2 * (size_t)a.ny * (n + 1);
   if (a.nz == 1) {
-    hipLaunchKernelGGL(sigma_plane_fwd<NT>, dim3(1, L), dim3(256), lds_plane, st, a);
+    hipLaunchKernelGGL((sigma_plane_fwd<NT, ADJ>), dim3(1, L), dim3(256), lds_plane, st, a);
     return hipGetLastError();
   }
   const size_t lds_z = sizeof(double2) * 2 * (size_t)n * (a.nz + 1);
-  hipLaunchKernelGGL(sigma_plane_fwd<NT>, dim3(a.nz / 2 + 1, L), dim3(256), lds_plane, st, a);
+  hipLaunchKernelGGL((sigma_plane_fwd<NT, ADJ>), dim3(a.nz / 2 + 1, L), dim3(256), lds_plane, st, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(sigma_zpass<NT>, dim3(a.ny, L), dim3(256), lds_z, st, a);
+  hipLaunchKernelGGL((sigma_zpass<NT, ADJ>), dim3(a.ny, L), dim3(256), lds_z, st, a);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(sigma_plane_inv<NT>, dim3(a.nz, L), dim3(256), lds_plane, st, a);
+  hipLaunchKernelGGL((sigma_plane_inv<NT, ADJ>), dim3(a.nz, L), dim3(256), lds_plane, st, a);
   return hipGetLastError();
 }
 
 bool sigma_planes_supported(int Nx) { return Nx >= 1 && Nx <= 64; }
 
+template <int NT, bool ADJ>
+static hipError_t prepare_one(size_t lds) {
+  hipError_t e = hipFuncSetAttribute((const void*)sigma_plane_fwd<NT, ADJ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)sigma_zpass<NT, ADJ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void*)sigma_plane_inv<NT, ADJ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
 template <int NT>
 static hipError_t prepare_t(size_t lds) {
-  hipError_t e = hipFuncSetAttribute((const void*)sigma_plane_fwd<NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipError_t e = prepare_one<NT, false>(lds);
   if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void*)sigma_zpass<NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void*)sigma_plane_inv<NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  return prepare_one<NT, true>(lds);
 }
 
 // allow the dynamic LDS of the plane kernels for lattice size n (the
@@ -379,17 +428,24 @@ hipError_t sigma_planes_prepare(int n) {
   }
 }
 
-hipError_t launch_sigma_planes(const SigmaArgs& a, int L, hipStream_t st) {  // L: grid rows
+template <bool ADJ>
+static hipError_t launch_planes_n(const SigmaArgs& a, int L, hipStream_t st) {  // L: grid rows
   switch (a.n) {
-    case 8: return launch_planes_t<8>(a, L, st);
-    case 16: return launch_planes_t<16>(a, L, st);
-    case 24: return launch_planes_t<24>(a, L, st);
-    case 32: return launch_planes_t<32>(a, L, st);
-    case 40: return launch_planes_t<40>(a, L, st);
-    case 48: return launch_planes_t<48>(a, L, st);
-    case 64: return launch_planes_t<64>(a, L, st);
-    default: return launch_planes_t<0>(a, L, st);
+    case 8: return launch_planes_t<8, ADJ>(a, L, st);
+    case 16: return launch_planes_t<16, ADJ>(a, L, st);
+    case 24: return launch_planes_t<24, ADJ>(a, L, st);
+    case 32: return launch_planes_t<32, ADJ>(a, L, st);
+    case 40: return launch_planes_t<40, ADJ>(a, L, st);
+    case 48: return launch_planes_t<48, ADJ>(a, L, st);
+    case 64: return launch_planes_t<64, ADJ>(a, L, st);
+    default: return launch_planes_t<0, ADJ>(a, L, st);
   }
+}
+
+hipError_t launch_sigma_planes(const SigmaArgs& a, int L, hipStream_t st) { return launch_planes_n<false>(a, L, st); }
+
+hipError_t launch_sigma_planes_adjoint(const SigmaArgs& a, int L, hipStream_t st) {
+  return launch_planes_n<true>(a, L, st);
 }
 
 }  // namespace qftkb

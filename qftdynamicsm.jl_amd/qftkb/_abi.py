@@ -34,6 +34,7 @@ EXPORTED = (
     "kb_fill_free_window", "kb_synchronize", "kb_set_stream", "kb_enable_timing", "kb_timing",
     "kb_reset_timing", "kb_use_graph", "kb_ens_create", "kb_ens_destroy", "kb_ens_last_error", "kb_ens_member",
     "kb_ens_step", "kb_ens_use_graph", "kb_ens_synchronize", "kb_ens_enable_timing", "kb_ens_timing",
+    "kb_adj_record", "kb_adj_gradient", "kb_adj_info", "kb_adj_stop", "kb_adj_debug_stop", "kb_adj_get_entry",
 )
 
 
@@ -100,6 +101,12 @@ def load_library(path: str = LIB_PATH):
                             ctypes.POINTER(ctypes.c_longlong)]
     L.kb_reset_timing.argtypes = [vp]
     L.kb_use_graph.argtypes = [vp, ctypes.c_int]
+    L.kb_adj_record.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.kb_adj_gradient.argtypes = [vp, _dp, _dp, _dp, _dp]
+    L.kb_adj_info.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
+    L.kb_adj_stop.argtypes = [vp]
+    L.kb_adj_debug_stop.argtypes = [vp, ctypes.c_int]
+    L.kb_adj_get_entry.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
     L.kb_ens_create.argtypes = [ctypes.POINTER(kb_params), ctypes.c_int, ctypes.POINTER(vp)]
     L.kb_ens_destroy.argtypes = [vp]
     L.kb_ens_last_error.restype = ctypes.c_char_p
@@ -263,6 +270,49 @@ class KBContext:
 
     def reset_timing(self):
         self._chk(self._L.kb_reset_timing(self._h))
+
+    # ---- adjoint ------------------------------------------------------
+    def adj_record(self, checkpoint_every=0, max_steps=None):
+        """Record the coming kb_step calls for the reverse sweep (call right
+        after initialisation).  checkpoint_every = 0 keeps a tape of every
+        dropped column; C > 0 keeps the window every C steps and recomputes."""
+        if max_steps is None:
+            raise ValueError("max_steps is required")
+        self._chk(self._L.kb_adj_record(self._h, int(checkpoint_every), int(max_steps)))
+
+    def adj_gradient(self, wgt, n0=True, F0=False):
+        """Reverse sweep for L = sum wgt * n(tmax).  Returns a dict with
+        'lambda' (dL/dLambda) and, on request, 'n0' (dL/dn(p,0) per site) and
+        'F0' (3 x V: dL/dF(1,1), dL/dF(2,1), dL/dF(2,2))."""
+        w = _f64(wgt, self.V)
+        lam = ctypes.c_double()
+        dn = np.empty(self.V) if n0 else None
+        dF = np.empty(3 * self.V) if F0 else None
+        self._chk(self._L.kb_adj_gradient(self._h, _ptr(w), ctypes.byref(lam), _ptr(dn), _ptr(dF)))
+        out = {"lambda": lam.value}
+        if n0:
+            out["n0"] = dn
+        if F0:
+            out["F0"] = dF.reshape(3, self.V)
+        return out
+
+    def adj_info(self):
+        b, r = ctypes.c_longlong(), ctypes.c_longlong()
+        self._chk(self._L.kb_adj_info(self._h, ctypes.byref(b), ctypes.byref(r)))
+        return {"device_bytes": b.value, "recomputed_steps": r.value}
+
+    def adj_stop(self):
+        self._chk(self._L.kb_adj_stop(self._h))
+
+    def adj_debug_stop(self, t_stop):
+        self._chk(self._L.kb_adj_debug_stop(self._h, int(t_stop)))
+
+    def adj_get(self, which, i, j):
+        """Adjoint of F(i,j) ('F') or rho(i,j) ('r') after a stopped sweep."""
+        w = {"F": 0, "r": 1, 0: 0, 1: 1}[which]
+        out = np.empty(self.V)
+        self._chk(self._L.kb_adj_get_entry(self._h, w, int(i), int(j), _ptr(out)))
+        return out
 
 
 class KBEnsemble:

@@ -77,10 +77,12 @@ def loss(kb, T, wgt):
     return float((wgt * kb.measure(T)["n"]).sum())
 
 
-def reverse(kb, T, wgt, initmass, record=None):
+def reverse(kb, T, wgt, initmass, record=None, stop=None):
     """Reverse sweep over steps T .. 3.  Returns (dL/dlambda, dL/dn0,
     (dF11, dF21, dF22)).  `record`, if a dict, receives per-step
-    intermediates keyed by t for kernel-level comparison."""
+    intermediates keyed by t for kernel-level comparison.  With `stop`, the
+    sweep ends after reversing step `stop` and returns the adjoint entries
+    accumulated so far as (Fb, rb) dicts keyed (a, b), a >= b."""
     V, dt, lam = kb.V, kb.dt, kb.lam
     dt3 = dt ** 3
     H = kb.H()
@@ -193,6 +195,8 @@ def reverse(kb, T, wgt, initmass, record=None):
         for b in range(tmin, t + 1):
             Fb.pop((t, b), None)
             rb.pop((t, b), None)
+        if stop is not None and t == stop:
+            return dict(Fb), dict(rb)
     dF11, dF21, dF22 = Fb[(1, 1)], Fb[(2, 1)], Fb[(2, 2)]
     om = np.sqrt(kb.kL2 + initmass ** 2)
     dn0 = dF11 / om + dF21 / om + dF22 * (om * dt ** 2 + 1.0 / om)

@@ -1,0 +1,186 @@
+"""Hand-written adjoint kernels through the C ABI: gradients of
+L = sum_p wgt(p) n(p, T) against the numpy reverse-mode twin
+(tests/adjoint_twin.py, itself pinned against oracle finite differences)
+and against central finite differences (h = 1e-5) of the C oracle.
+
+Tolerances: adjoint vs twin 1e-10 (both exact, FP64 rounding only);
+adjoint vs finite differences 1e-6 relative to the largest component."""
+import numpy as np
+import pytest
+
+from oracle.kb_numpy import kL2_table
+from oracle.oracle import Oracle
+from tests.adjoint_twin import gradient, run_forward, reverse
+
+pytestmark = pytest.mark.gpu
+
+H = 1e-5
+FD_TOL = 1e-6
+TWIN_TOL = 1e-10
+
+
+def thermal(k2, T=2.0, m=1.0):
+    return 1.0 / (np.exp(np.sqrt(k2 + m * m) / (T * m)) - 1.0)
+
+
+def rel(a, b):
+    a, b = np.asarray(a, dtype=float), np.asarray(b, dtype=float)
+    s = np.max(np.abs(b))
+    return np.max(np.abs(a - b)) / s if s > 0 else np.max(np.abs(a - b))
+
+
+def gpu_gradient(args, n0, m, T, wgt, C=0):
+    from qftkb import KBContext
+    c = KBContext(*args[:8], pexp=args[8] if len(args) > 8 else 1)
+    c.init_particles(n0, m)
+    c.adj_record(C, T)
+    for t in range(3, T + 1):
+        c.step(t)
+    L = float((wgt * c.measure(T)["n"]).sum())
+    g = c.adj_gradient(wgt, n0=True, F0=True)
+    assert c.window() == (1, 2)
+    return c, L, g
+
+
+def oracle_loss(args, n0, m, T, wgt):
+    o = Oracle(*args)
+    o.init_particles(np.ascontiguousarray(n0, dtype=np.float64), m)
+    for t in range(3, T + 1):
+        o.evolve(t)
+    L = float((wgt * o.measure(T)["n"]).sum())
+    o.close()
+    return L
+
+
+def with_lambda(args, lam):
+    a = list(args)
+    a[4] = lam
+    return tuple(a)
+
+
+def setup(args):
+    k2 = kL2_table(args[0], args[1])
+    m = args[6]
+    return thermal(k2, m=m), m, np.exp(-k2)
+
+
+CASES = [
+    # (Nx, sdim, Nmem, dt, lambda, ON, Mass, Mass2, pexp), T
+    pytest.param((4, 3, 5, 0.1, 1.3, 1, 1.0, 1.0, 1), 14, id="4^3-sliding"),
+    pytest.param((4, 3, 16, 0.1, 1.0, 1, 1.0, 1.0, 1), 12, id="4^3-full-memory"),
+    pytest.param((6, 2, 6, 0.08, 1.7, 4, 0.9, 0.81, 1), 15, id="6^2-O(4)"),
+    pytest.param((5, 1, 4, 0.1, 2.0, 2, 1.0, 1.0, 1), 11, id="5-1d"),
+    pytest.param((5, 3, 7, 0.1, 1.2, 1, 1.0, 1.0, 1), 25, id="5^3-ragged-blocks"),
+    pytest.param((4, 3, 5, 0.1, 1.0, 1, 1.0, 1.0, 0), 14, id="4^3-LO"),
+]
+
+
+@pytest.mark.parametrize("args,T", CASES)
+def test_adjoint_matches_twin_and_oracle_fd(args, T):
+    print(f"\n[case] adjoint {args} T={T}", flush=True)
+    n0, m, wgt = setup(args)
+    c, L, g = gpu_gradient(args, n0, m, T, wgt)
+    Lt, lt, dnt, dFt = gradient(args, n0, m, T, wgt)
+    assert L == pytest.approx(Lt, rel=1e-12)
+    assert g["lambda"] == pytest.approx(lt, rel=TWIN_TOL, abs=TWIN_TOL * abs(Lt))
+    assert rel(g["n0"], dnt) <= TWIN_TOL
+    for k in range(3):
+        assert rel(g["F0"][k], dFt[k]) <= TWIN_TOL, k
+    lam = args[4]
+    fd = (oracle_loss(with_lambda(args, lam + H), n0, m, T, wgt)
+          - oracle_loss(with_lambda(args, lam - H), n0, m, T, wgt)) / (2 * H)
+    assert abs(g["lambda"] - fd) <= FD_TOL * max(abs(fd), 1e-3 * abs(L)), (g["lambda"], fd)
+    for p in (0, 1, n0.size // 2, n0.size - 1):
+        e = np.zeros_like(n0)
+        e[p] = H
+        fdp = (oracle_loss(args, n0 + e, m, T, wgt) - oracle_loss(args, n0 - e, m, T, wgt)) / (2 * H)
+        assert abs(g["n0"][p] - fdp) <= FD_TOL * np.max(np.abs(g["n0"])), (p, g["n0"][p], fdp)
+    c.close()
+
+
+def test_adjoint_entries_after_partial_sweep():
+    """Stop the sweep after a few reverse steps and compare every adjoint
+    entry of the window with the twin's (the Hartree adjoint of the last
+    reversed step is applied one step later on the device)."""
+    from qftkb import KBContext
+    args, T, stop = (4, 3, 5, 0.1, 1.3, 1, 1.0, 1.0, 1), 14, 11
+    print(f"\n[case] partial sweep {args} T={T} stop={stop}", flush=True)
+    n0, m, wgt = setup(args)
+    kb = run_forward(args, n0, m, T)
+    rec = {}
+    Fb, rb = reverse(kb, T, wgt, m, record=rec, stop=stop)
+    c = KBContext(*args[:8])
+    c.init_particles(n0, m)
+    c.adj_record(0, T)
+    for t in range(3, T + 1):
+        c.step(t)
+    c.adj_debug_stop(stop)
+    c.adj_gradient(wgt)
+    tmin, tmax = c.window()
+    assert tmax == stop - 1
+    hart = (1 + 2) / 6.0 * args[4] / n0.size * rec[stop]["M2b"]
+    for a in range(tmin, tmax + 1):
+        for b in range(tmin, a + 1):
+            want = Fb.get((a, b), np.zeros(n0.size))
+            if (a, b) == (stop - 1, stop - 1):
+                want = want - hart
+            assert rel(c.adj_get("F", a, b), want) <= TWIN_TOL, ("F", a, b)
+            if a > b + 1:
+                assert rel(c.adj_get("r", a, b), rb.get((a, b), np.zeros(n0.size))) <= TWIN_TOL, ("r", a, b)
+    c.close()
+
+
+@pytest.mark.parametrize("C", [1, 4, 7])
+def test_checkpointed_sweep_equals_taped_sweep(C):
+    args, T = (6, 3, 8, 0.1, 1.4, 1, 1.0, 1.0, 1), 40
+    print(f"\n[case] checkpoints every {C}: {args} T={T}", flush=True)
+    n0, m, wgt = setup(args)
+    c0, L0, g0 = gpu_gradient(args, n0, m, T, wgt, C=0)
+    c1, L1, g1 = gpu_gradient(args, n0, m, T, wgt, C=C)
+    assert L1 == pytest.approx(L0, rel=1e-13)
+    assert g1["lambda"] == pytest.approx(g0["lambda"], rel=1e-11)
+    assert rel(g1["n0"], g0["n0"]) <= 1e-11
+    info = c1.adj_info()
+    assert info["recomputed_steps"] > 0
+    c0.close()
+    c1.close()
+
+
+def test_second_forward_and_gradient_repeat():
+    """After a gradient the context is back at the initial state; the same
+    forward run and sweep give the same numbers."""
+    from qftkb import KBContext
+    args, T = (6, 3, 8, 0.1, 1.4, 1, 1.0, 1.0, 1), 30
+    print(f"\n[case] repeat {args} T={T}", flush=True)
+    n0, m, wgt = setup(args)
+    c = KBContext(*args[:8])
+    c.init_particles(n0, m)
+    c.adj_record(5, T)
+    out = []
+    for _ in range(2):
+        for t in range(3, T + 1):
+            c.step(t)
+        out.append(c.adj_gradient(wgt))
+    assert out[1]["lambda"] == pytest.approx(out[0]["lambda"], rel=1e-12)
+    assert rel(out[1]["n0"], out[0]["n0"]) <= 1e-12
+    c.close()
+
+
+def test_adjoint_vs_oracle_fd_at_16cubed():
+    """A larger lattice with a sliding window: GPU adjoint vs oracle FD."""
+    args, T = (16, 3, 12, 0.1, 1.0, 1, 1.0, 1.0, 1), 40
+    print(f"\n[case] 16^3 N_mem=12 T={T}", flush=True)
+    n0, m, wgt = setup(args)
+    c, L, g = gpu_gradient(args, n0, m, T, wgt, C=10)
+    lam = args[4]
+    fd = (oracle_loss(with_lambda(args, lam + H), n0, m, T, wgt)
+          - oracle_loss(with_lambda(args, lam - H), n0, m, T, wgt)) / (2 * H)
+    assert abs(g["lambda"] - fd) <= FD_TOL * abs(fd), (g["lambda"], fd)
+    # one radial shell of occupations moved together
+    k2 = kL2_table(16, 3)
+    shell = (k2 > 1.0) & (k2 <= 1.5)
+    e = np.where(shell, H, 0.0)
+    fds = (oracle_loss(args, n0 + e, m, T, wgt) - oracle_loss(args, n0 - e, m, T, wgt)) / (2 * H)
+    ad = float(g["n0"][shell].sum())
+    assert abs(ad - fds) <= FD_TOL * abs(fds), (ad, fds)
+    c.close()
