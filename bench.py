@@ -174,7 +174,7 @@ def radial_bins(kL2, sdim, nbins):
     return np.minimum(np.searchsorted(edges, np.sqrt(kL2), side="right") - 1, nbins - 1)
 
 
-def adjoint_bench(fd_bins=False):
+def adjoint_bench(fd_bins=False, fd=True):
     """Cost of dL/dlambda and dL/dn(p,0) by the adjoint kernels relative to
     one forward run, at 24^3, N_mem = 200, 500 steps, checkpoints every 50
     (and with every dropped column taped), plus the check against central
@@ -243,6 +243,8 @@ def adjoint_bench(fd_bins=False):
     out["dL_dn0_bins"] = [float(gr["n0"][bins == b].sum()) for b in range(a["nbins"])]
     out["tape_vs_checkpoint_max_rel"] = float(np.max(np.abs(grads[0]["n0"] - gr["n0"])) / np.max(np.abs(gr["n0"])))
 
+    if not fd:
+        return out
     h = a["h"]
 
     def loss(lmb, n):
@@ -287,6 +289,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost-ratio leg")
     ap.add_argument("--adjoint", action="store_true",
                     help="only the differentiation workload, with finite differences of all 16 radial bins")
+    ap.add_argument("--no-fd", action="store_true", help="with --adjoint: skip the finite differences")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 2 GiB device copy (torch) to scale rocprofv3 FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
@@ -307,7 +310,7 @@ def main():
         ensemble_bench(args)
         return
     if args.adjoint:
-        print(json.dumps({"adjoint": adjoint_bench(fd_bins=True)}), flush=True)
+        print(json.dumps({"adjoint": adjoint_bench(fd_bins=True, fd=not args.no_fd)}), flush=True)
         return
 
     from qftkb import KBContext

@@ -18,12 +18,14 @@
 //      row t: R_bar_F, R_bar_r per slot, pushes into rows t-1 and t-2,
 //      the Omega adjoint per site block; it first settles the Hartree
 //      adjoint of step t+1 into F_bar(t,t);
-//   3. adj_memint_kernel: adjoint of the memory integrals, one read of H
-//      and one read-modify-write of Hbar per stored entry, with the
-//      self-energy adjoints Sigma_bar reduced per slot in LDS;
+//   3. adj_reduce_kernel: the self-energy adjoints Sigma_bar of the memory
+//      integrals, reduced per slot in LDS from one read of H;
 //   4. the self-energy stage run backwards (kb_sigma.hip, ADJ) adds the
 //      sunset adjoint into row t-1 of Hbar;
-//   5. adj_slide_kernel: row t of Hbar is cleared and the column t-N that
+//   5. every AK reverse steps, adj_apply_kernel adds the memory integrals'
+//      entry adjoints of those steps to Hbar in one read-modify-write
+//      (adj_row_kernel adds the deferred terms of the row it consumes);
+//   6. adj_slide_kernel: row t of Hbar is cleared and the column t-N that
 //      row t overwrote is put back into H from the tape.
 // The tape holds, for every forward step that dropped a column, that
 // column's N slot pairs.  With checkpointing, only the window at every C-th
@@ -39,6 +41,10 @@ using namespace qftkb;
 using namespace qftkb_host;
 
 namespace qftkb {
+
+// deferred reverse steps whose memory-integral entry updates are applied
+// together
+constexpr int AK = 4;
 
 // Per-step arguments of the reverse kernels.
 struct AdjArgs {
@@ -62,6 +68,13 @@ struct AdjArgs {
   int integral;
   int first;           // t == T: no Hartree adjoint pending
   StepState st;        // the forward state of step t
+  // deferred memory-integral updates: steps pk[j] with windows starting at
+  // ptmin[j], their self-energy and R_bar rings (row kernel: the steps after
+  // t not yet applied; apply kernel: the steps being applied)
+  int npend;
+  int pk[AK], ptmin[AK];
+  const double* Sj[AK];
+  const double* Rj[AK];
 };
 
 namespace {
@@ -177,8 +190,82 @@ __global__ void adj_seed_kernel(const double* __restrict__ H, double* __restrict
 //   rho(t,b)= (2-Om) rho(t-1,b) - rho(t-2,b) + dt^3 R_r(b), b <= t-2
 // with R_F(t) = -sum_{z<=t-2} w Sr(z) F(z,t) + sum_{z<=t-1} w SF(z) rho(z,t).
 // The terms of R_F(t) that multiply row-t entries are folded in here; its
-// self-energy terms are left to adj_memint_kernel through R_bar_F(t).
+// self-energy terms are left to the memory-integral kernels through
+// R_bar_F(t).
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Adjoint of the memory integrals (getRHS_F!, getRHS_r!, Evolution.jl:
+// 766-782, 846-854) over the window [tmin, t], one workgroup per site block.
+// Forward, per stored entry (a,b), a>b, with u = m w Sr, v = w SF, ms = m Sr:
+//   R_F(a) -= u(b) F + v(b) rho,  R_F(b) -= u(a) F,  R_r(b) -= ms(a) rho,
+// and (a,a): R_F(a) -= u(a) F.  With R_bar = dt^3 (adjoint of row t):
+//   Sr_bar(z) = m(z) [ w(z) u_bar(z) + ms_bar(z) ],  SF_bar(z) = w(z) v_bar(z)
+//   u_bar(z)  = -sum_b F(z,b) R_bar_F(b)           (b over [tmin, t])
+//   v_bar(z)  = -sum_{a>z} rho(a,z) R_bar_F(a)
+//   ms_bar(z) = -sum_{b<z} rho(z,b) R_bar_r(b)
+// (adj_reduce_kernel, a read of the primal window only), and per entry
+//   F_bar(a,b)   -= u(a) R_bar_F(b) + u(b) R_bar_F(a)
+//   rho_bar(a,b) -= v(b) R_bar_F(a) + ms(a) R_bar_r(b).
+// The entry updates are rank-2 per step.  They are deferred: the coefficient
+// rings of up to AK consecutive reverse steps are kept and adj_apply_kernel
+// adds all of them in one read-modify-write of the adjoint window; a row
+// that is consumed in between (row t at reverse step t) gets the deferred
+// terms added on the fly by adj_row_kernel.  Entries of a deferred step's
+// update are those of its own window: times >= its tmin.
+// Row t enters with R_bar_F(t) = dt^3 F_bar(t,t) (the diagonal's memory
+// terms) and u = v = ms = 0; its own entries are cleared afterwards.
+// The slot triangle is walked like the forward kernel: 8-lane workers take
+// 64-row chunks of 4-column blocks, two row batches in flight, column sums
+// in registers, row sums in LDS.
+// ---------------------------------------------------------------------------
+constexpr int AM_THREADS = 256;
+constexpr int AM_B = 4;  // reduce: rows per batch (two batches in flight)
+
+struct SlotCoef {
+  double u, v, ms, rf, rr, w, m;
+};
+
+__device__ __forceinline__ SlotCoef slot_coef(const AdjArgs& a, const double2* Ss, const double2* Rs, int x) {
+  const StepState& st = a.st;
+  const int N = a.N;
+  const int rel = (x - st.s_min + N) % N;
+  const int tx = st.tmin + rel;
+  SlotCoef k;
+  k.w = rel == 0 ? 0.5 : 1.0;
+  k.m = tx <= st.t - 2 ? 1.0 : 0.0;
+  const double2 sg = tx <= st.t - 1 ? Ss[(size_t)x * MB] : make_double2(0.0, 0.0);
+  const double2 rb = Rs[(size_t)x * MB];
+  k.u = k.m * k.w * sg.y;
+  k.v = k.w * sg.x;
+  k.ms = k.m * sg.y;
+  k.rf = rb.x;
+  k.rr = rb.y;
+  return k;
+}
+
+// coefficients of deferred step j at slot x whose time (in the current
+// window) is tx; zero outside that step's window
+struct PCoef {
+  double u, v, ms, rf, rr;
+};
+
+__device__ __forceinline__ PCoef pend_coef(const AdjArgs& a, int j, int mb, int lane, int x, int tx) {
+  PCoef c{0.0, 0.0, 0.0, 0.0, 0.0};
+  if (tx < a.ptmin[j]) return c;
+  const int k = a.pk[j];
+  const double w = tx == a.ptmin[j] ? 0.5 : 1.0;
+  const double m = tx <= k - 2 ? 1.0 : 0.0;
+  const size_t o = soff(mb, x, lane, a.N);
+  const double2 sg = tx <= k - 1 ? ld2(a.Sj[j] + o) : make_double2(0.0, 0.0);
+  const double2 rb = ld2(a.Rj[j] + o);
+  c.u = m * w * sg.y;
+  c.v = w * sg.x;
+  c.ms = m * sg.y;
+  c.rf = rb.x;
+  c.rr = rb.y;
+  return c;
+}
+
 constexpr int ROW_THREADS = 256;
 
 __global__ __launch_bounds__(ROW_THREADS) void adj_row_kernel(AdjArgs a) {
@@ -213,7 +300,17 @@ __global__ __launch_bounds__(ROW_THREADS) void adj_row_kernel(AdjArgs a) {
   double2* Rs = reinterpret_cast<double2*>(a.Rb + soff(mb, 0, lane, N));
   const size_t pstride = (size_t)MB * 2;  // doubles per slot pair of one site block
 
-  const double Fbtt = Hbs[pair_off(st.s_t, st.s_t, N) * pstride] + a.hl_ivol * Mb1;
+  // deferred memory-integral terms of the steps after t that touch row t
+  PCoef pt[AK];
+  double Fbtt = Hbs[pair_off(st.s_t, st.s_t, N) * pstride] + a.hl_ivol * Mb1;
+#pragma unroll
+  for (int j = 0; j < AK; ++j) {
+    pt[j] = PCoef{0.0, 0.0, 0.0, 0.0, 0.0};
+    if (j < a.npend) {
+      pt[j] = pend_coef(a, j, mb, lane, st.s_t, t);
+      Fbtt -= pt[j].u * pt[j].rf;
+    }
+  }
   const double RFt = dt3 * Fbtt;
   double omb = 0.0;
   for (int q = worker; q < st.L; q += nworkers) {
@@ -221,7 +318,19 @@ __global__ __launch_bounds__(ROW_THREADS) void adj_row_kernel(AdjArgs a) {
     const int x = (tx - 1) % N;
     const double w = q == 0 ? 0.5 : 1.0;
     const double m = tx <= t - 2 ? 1.0 : 0.0;
-    const double2 hb = ld2(Hbs + pair_off(st.s_t, x, N) * pstride);
+    double2 hb = ld2(Hbs + pair_off(st.s_t, x, N) * pstride);
+    {
+      // complete F_bar(t,tx), rho_bar(t,tx) (stored sign) with the deferred terms
+      const double sgt = st.s_t >= x ? 1.0 : -1.0;
+#pragma unroll
+      for (int j = 0; j < AK; ++j) {
+        if (j < a.npend) {
+          const PCoef px = pend_coef(a, j, mb, lane, x, tx);
+          hb.x -= pt[j].u * px.rf + px.u * pt[j].rf;
+          hb.y -= sgt * (px.v * pt[j].rf + pt[j].ms * px.rr);
+        }
+      }
+    }
     const double2 h1 = ld2(Hs + pair_off(st.s_tm1, x, N) * pstride);
     const double2 sg = a.integral ? Ss[(size_t)x * MB] : make_double2(0.0, 0.0);
     double fb = hb.x;
@@ -272,133 +381,74 @@ __global__ __launch_bounds__(ROW_THREADS) void adj_row_kernel(AdjArgs a) {
   if (tid == 0) a.P[(size_t)(t & 1) * a.NB + mb] = tot;
 }
 
-// ---------------------------------------------------------------------------
-// Adjoint of the memory integrals (getRHS_F!, getRHS_r!, Evolution.jl:
-// 766-782, 846-854) over the window [tmin, t], one workgroup per site block.
-// Forward, per stored entry (a,b), a>b, with u = m w Sr, v = w SF, ms = m Sr:
-//   R_F(a) -= u(b) F + v(b) rho,  R_F(b) -= u(a) F,  R_r(b) -= ms(a) rho,
-// and (a,a): R_F(a) -= u(a) F.  With R_bar = dt^3 (adjoint of row t), per
-// entry:
-//   F_bar(a,b)   -= u(a) R_bar_F(b) + u(b) R_bar_F(a)
-//   rho_bar(a,b) -= v(b) R_bar_F(a) + ms(a) R_bar_r(b)
-//   Sr_bar(z) = m(z) [ w(z) u_bar(z) + ms_bar(z) ],  SF_bar(z) = w(z) v_bar(z)
-//   u_bar(z)  = -sum_b F(z,b) R_bar_F(b)           (b over [tmin, t])
-//   v_bar(z)  = -sum_{a>z} rho(a,z) R_bar_F(a)
-//   ms_bar(z) = -sum_{b<z} rho(z,b) R_bar_r(b)
-// Row t enters with R_bar_F(t) = dt^3 F_bar(t,t) (the diagonal's memory
-// terms) and u = v = ms = 0; its own entries are cleared afterwards.
-// The slot triangle is walked like the forward kernel: 8-lane workers take
-// 64-row chunks of 4-column blocks, two 2-row batches of H and Hbar lines
-// in flight, column sums in registers, row sums in LDS.
-// ---------------------------------------------------------------------------
-constexpr int AM_THREADS = 256;
-constexpr int AM_B = 1;  // rows per batch (two batches in flight)
-
-struct SlotCoef {
-  double u, v, ms, rf, rr, w, m;
-};
-
-__device__ __forceinline__ SlotCoef slot_coef(const AdjArgs& a, const double2* Ss, const double2* Rs, int x) {
-  const StepState& st = a.st;
-  const int N = a.N;
-  const int rel = (x - st.s_min + N) % N;
-  const int tx = st.tmin + rel;
-  SlotCoef k;
-  k.w = rel == 0 ? 0.5 : 1.0;
-  k.m = tx <= st.t - 2 ? 1.0 : 0.0;
-  const double2 sg = tx <= st.t - 1 ? Ss[(size_t)x * MB] : make_double2(0.0, 0.0);
-  const double2 rb = Rs[(size_t)x * MB];
-  k.u = k.m * k.w * sg.y;
-  k.v = k.w * sg.x;
-  k.ms = k.m * sg.y;
-  k.rf = rb.x;
-  k.rr = rb.y;
-  return k;
-}
-
-struct AdjCols {
-  double u[CB], v[CB], ms[CB], rf[CB], rr[CB];
+// ---- reduce: Sigma_bar from one read of the primal window -----------------
+struct RedCols {
+  double rf[CB], rr[CB];
   double t1[CB], t2[CB], t3[CB];  // sum F rf(r); sum_{neg} P rr(r); sum_{not neg} P rf(r)
   unsigned phi;
 };
 
-struct AdjBatch {
-  double2 e[AM_B][CB], eb[AM_B][CB];
-  double2 sg[AM_B], rb[AM_B];
+struct RedBatch {
+  double2 e[AM_B][CB];
+  double2 rb[AM_B];
 };
 
-__device__ __forceinline__ void adj_load(AdjBatch& b, const double* Hs, const double* Hbs, const double2* Ss,
-                                         const double2* Rs, long cbase, int c0, int r, int rlast) {
+__device__ __forceinline__ void red_load(RedBatch& b, const double* Hs, const double2* Rs, long cbase, int c0, int r,
+                                         int rlast) {
 #pragma unroll
   for (int h = 0; h < AM_B; ++h) {
     const int rr = min(r + h, rlast);
     const long pb = cbase + (long)(rr - c0) * CB;
 #pragma unroll
-    for (int jc = 0; jc < CB; ++jc) {
-      b.e[h][jc] = ld2_nt(Hs + (size_t)(pb + jc) * MB * 2);
-      b.eb[h][jc] = ld2_nt(Hbs + (size_t)(pb + jc) * MB * 2);
-    }
-    b.sg[h] = Ss[(size_t)rr * MB];
+    for (int jc = 0; jc < CB; ++jc) b.e[h][jc] = ld2_nt(Hs + (size_t)(pb + jc) * MB * 2);
     b.rb[h] = Rs[(size_t)rr * MB];
   }
 }
 
-// one off-diagonal entry; row r (coefficients *r), column jc; n: time
-// order reversed (time(r) < time(c))
-__device__ __forceinline__ void adj_pair(double2 e, double2& eb, bool n, const SlotCoef& r, AdjCols& cs, int jc,
-                                         double& s1, double& s2, double& s3) {
+// one off-diagonal entry; n: time order reversed (time(r) < time(c))
+__device__ __forceinline__ void red_pair(double2 e, bool n, double rfr, double rrr, RedCols& cs, int jc, double& s1,
+                                         double& s2, double& s3) {
   const double Fv = e.x, Pv = e.y;
   const double nf = n ? 1.0 : 0.0, pf = 1.0 - nf;
   s1 += Fv * cs.rf[jc];
-  cs.t1[jc] += Fv * r.rf;
+  cs.t1[jc] += Fv * rfr;
   s2 += pf * Pv * cs.rr[jc];
-  cs.t3[jc] += pf * Pv * r.rf;
+  cs.t3[jc] += pf * Pv * rfr;
   s3 += nf * Pv * cs.rf[jc];
-  cs.t2[jc] += nf * Pv * r.rr;
-  eb.x -= r.u * cs.rf[jc] + cs.u[jc] * r.rf;
-  eb.y += nf * (r.v * cs.rf[jc] + cs.ms[jc] * r.rr) - pf * (cs.v[jc] * r.rf + r.ms * cs.rr[jc]);
+  cs.t2[jc] += nf * Pv * rrr;
 }
 
-__device__ __forceinline__ void row_flush(const SlotCoef& r, int rr, double s1, double s2, double s3, double* accF,
-                                          double* accR, int lane) {
-  atomicAdd(&accR[rr * MB + lane], r.m * (-r.w * s1 - s2));
-  atomicAdd(&accF[rr * MB + lane], r.w * s3);
+__device__ __forceinline__ void slot_wm(const StepState& st, int N, int x, double& w, double& m) {
+  const int rel = (x - st.s_min + N) % N;
+  w = rel == 0 ? 0.5 : 1.0;
+  m = st.tmin + rel <= st.t - 2 ? 1.0 : 0.0;
 }
 
-// full rows r .. r+AM_B-1 of one column block
-__device__ __forceinline__ void adj_batch(const AdjArgs& a, AdjBatch& b, int r, int r1, long cbase, int c0,
-                                          AdjCols& cs, double* Hbs, double* accF, double* accR, int lane) {
-  const StepState& st = a.st;
-  const int N = a.N;
+__device__ __forceinline__ void row_flush(const StepState& st, int N, int rr, double s1, double s2, double s3,
+                                          double* accF, double* accR, int lane) {
+  double w, m;
+  slot_wm(st, N, rr, w, m);
+  atomicAdd(&accR[rr * MB + lane], m * (-w * s1 - s2));
+  atomicAdd(&accF[rr * MB + lane], w * s3);
+}
+
+__device__ __forceinline__ void red_batch(const AdjArgs& a, const RedBatch& b, int r, int r1, RedCols& cs,
+                                          double* accF, double* accR, int lane) {
 #pragma unroll
   for (int h = 0; h < AM_B; ++h) {
     const int rr = r + h;
     if (rr < r1) {
-      SlotCoef k;
-      const int rel = (rr - st.s_min + N) % N;
-      const int tx = st.tmin + rel;
-      k.w = rel == 0 ? 0.5 : 1.0;
-      k.m = tx <= st.t - 2 ? 1.0 : 0.0;
-      const double2 sg = tx <= st.t - 1 ? b.sg[h] : make_double2(0.0, 0.0);
-      k.u = k.m * k.w * sg.y;
-      k.v = k.w * sg.x;
-      k.ms = k.m * sg.y;
-      k.rf = b.rb[h].x;
-      k.rr = b.rb[h].y;
-      const unsigned neg = rr >= st.s_min ? cs.phi : 0u;
-      const long pb = cbase + (long)(rr - c0) * CB;
+      const unsigned neg = rr >= a.st.s_min ? cs.phi : 0u;
       double s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
-      for (int jc = 0; jc < CB; ++jc) {
-        adj_pair(b.e[h][jc], b.eb[h][jc], (neg >> jc) & 1u, k, cs, jc, s1, s2, s3);
-        st2_nt(Hbs + (size_t)(pb + jc) * MB * 2, b.eb[h][jc]);
-      }
-      row_flush(k, rr, s1, s2, s3, accF, accR, lane);
+      for (int jc = 0; jc < CB; ++jc)
+        red_pair(b.e[h][jc], (neg >> jc) & 1u, b.rb[h].x, b.rb[h].y, cs, jc, s1, s2, s3);
+      row_flush(a.st, a.N, rr, s1, s2, s3, accF, accR, lane);
     }
   }
 }
 
-__global__ __launch_bounds__(AM_THREADS, 2) void adj_memint_kernel(AdjArgs a) {
+__global__ __launch_bounds__(AM_THREADS, 2) void adj_reduce_kernel(AdjArgs a) {
   extern __shared__ double smem[];
   const int N = a.N;
   double* accF = smem;           // SigF_bar per slot, lane
@@ -421,7 +471,6 @@ __global__ __launch_bounds__(AM_THREADS, 2) void adj_memint_kernel(AdjArgs a) {
   if (tid == 0) next_item = nworkers;
   __syncthreads();
   const double* Hs = a.H + hoff(mb, 0, lane, a.NP);
-  double* Hbs = a.Hbar + hoff(mb, 0, lane, a.NP);
   const double2* Ss = reinterpret_cast<const double2*>(a.S + soff(mb, 0, lane, N));
   const double2* Rs = reinterpret_cast<const double2*>(a.Rb + soff(mb, 0, lane, N));
 
@@ -433,25 +482,22 @@ __global__ __launch_bounds__(AM_THREADS, 2) void adj_memint_kernel(AdjArgs a) {
     const int r1 = min(r0 + MEMINT_ROWS, Rv);
     if (c0 < Rv && r0 < r1) {
       const long cbase = colblock_offset(jk.x, N);
-      AdjCols cs;
+      RedCols cs;
       cs.phi = 0;
 #pragma unroll
       for (int jc = 0; jc < CB; ++jc) {
         const int c = c0 + jc;
-        const SlotCoef k = slot_coef(a, Ss, Rs, min(c, Rv - 1));
+        const double2 rb = Rs[(size_t)min(c, Rv - 1) * MB];
         const bool ok = c < Rv;
-        cs.u[jc] = ok ? k.u : 0.0;
-        cs.v[jc] = ok ? k.v : 0.0;
-        cs.ms[jc] = ok ? k.ms : 0.0;
-        cs.rf[jc] = ok ? k.rf : 0.0;
-        cs.rr[jc] = ok ? k.rr : 0.0;
+        cs.rf[jc] = ok ? rb.x : 0.0;
+        cs.rr[jc] = ok ? rb.y : 0.0;
         cs.t1[jc] = cs.t2[jc] = cs.t3[jc] = 0.0;
         cs.phi |= (c < smin ? 1u : 0u) << jc;
       }
       int r = r0;
       // rows crossing the block diagonal: columns c0 .. r
       for (; r < r1 && r < c0 + CB; ++r) {
-        const SlotCoef k = slot_coef(a, Ss, Rs, r);
+        const double2 rbr = Rs[(size_t)r * MB];
         const unsigned neg = r >= smin ? cs.phi : 0u;
         const long pb = cbase + (long)(r - c0) * CB;
         double s1 = 0.0, s2 = 0.0, s3 = 0.0;
@@ -459,29 +505,24 @@ __global__ __launch_bounds__(AM_THREADS, 2) void adj_memint_kernel(AdjArgs a) {
           const int c = c0 + jc;
           if (c > r) break;
           const double2 e = ld2(Hs + (size_t)(pb + jc) * MB * 2);
-          double* pe = Hbs + (size_t)(pb + jc) * MB * 2;
-          double2 eb = ld2(pe);
-          if (c == r) {
-            s1 += e.x * k.rf;
-            eb.x -= k.u * k.rf;
-          } else {
-            adj_pair(e, eb, (neg >> jc) & 1u, k, cs, jc, s1, s2, s3);
-          }
-          st2(pe, eb);
+          if (c == r)
+            s1 += e.x * rbr.x;
+          else
+            red_pair(e, (neg >> jc) & 1u, rbr.x, rbr.y, cs, jc, s1, s2, s3);
         }
-        row_flush(k, r, s1, s2, s3, accF, accR, lane);
+        row_flush(st, N, r, s1, s2, s3, accF, accR, lane);
       }
       if (r < r1) {
         const int rlast = r1 - 1;
-        AdjBatch A, B;
-        adj_load(A, Hs, Hbs, Ss, Rs, cbase, c0, r, rlast);
+        RedBatch A, B;
+        red_load(A, Hs, Rs, cbase, c0, r, rlast);
         while (true) {
-          adj_load(B, Hs, Hbs, Ss, Rs, cbase, c0, r + AM_B, rlast);
-          adj_batch(a, A, r, r1, cbase, c0, cs, Hbs, accF, accR, lane);
+          red_load(B, Hs, Rs, cbase, c0, r + AM_B, rlast);
+          red_batch(a, A, r, r1, cs, accF, accR, lane);
           r += AM_B;
           if (r >= r1) break;
-          adj_load(A, Hs, Hbs, Ss, Rs, cbase, c0, r + AM_B, rlast);
-          adj_batch(a, B, r, r1, cbase, c0, cs, Hbs, accF, accR, lane);
+          red_load(A, Hs, Rs, cbase, c0, r + AM_B, rlast);
+          red_batch(a, B, r, r1, cs, accF, accR, lane);
           r += AM_B;
           if (r >= r1) break;
         }
@@ -490,9 +531,8 @@ __global__ __launch_bounds__(AM_THREADS, 2) void adj_memint_kernel(AdjArgs a) {
       for (int jc = 0; jc < CB; ++jc) {
         const int c = c0 + jc;
         if (c < Rv) {
-          const int rel = (c - smin + N) % N;
-          const double w = rel == 0 ? 0.5 : 1.0;
-          const double m = st.tmin + rel <= t - 2 ? 1.0 : 0.0;
+          double w, m;
+          slot_wm(st, N, c, w, m);
           atomicAdd(&accR[c * MB + lane], m * (-w * cs.t1[jc] + cs.t2[jc]));
           atomicAdd(&accF[c * MB + lane], -w * cs.t3[jc]);
         }
@@ -519,6 +559,104 @@ __global__ __launch_bounds__(AM_THREADS, 2) void adj_memint_kernel(AdjArgs a) {
   }
   const double tot = block_sum<AM_THREADS>(lam, red);
   if (tid == 0) a.LP[mb] += a.two_over_lambda * tot;
+}
+
+// ---- apply: the deferred entry updates of npend steps ----------------------
+// Per column and deferred step: v, ms, rf, rr, and u = w ms with w = 1/2
+// flagged in `half`.
+struct ApCols {
+  double v[AK][CB], ms[AK][CB], rf[AK][CB], rr[AK][CB];
+  unsigned half;  // bit j*CB + jc: w = 1/2 for column jc in step j's window
+  unsigned phi;
+};
+
+__global__ __launch_bounds__(AM_THREADS, 2) void adj_apply_kernel(AdjArgs a) {
+  __shared__ int next_item;
+  const StepState& st = a.st;
+  const int N = a.N;
+  const int t = st.t;
+  const int Rv = t > N ? N : t;
+  const int smin = st.s_min;
+  const int mb = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int lane = tid & (MB - 1);
+  const int worker = tid >> 3;
+  const int nworkers = AM_THREADS >> 3;
+  if (tid == 0) next_item = nworkers;
+  __syncthreads();
+  double* Hbs = a.Hbar + hoff(mb, 0, lane, a.NP);
+  int item = worker;
+  while (item < a.nitems) {
+    const int2 jk = a.items[item];
+    const int c0 = jk.x * CB;
+    const int r0 = c0 + jk.y * MEMINT_ROWS;
+    const int r1 = min(r0 + MEMINT_ROWS, Rv);
+    if (c0 < Rv && r0 < r1) {
+      const long cbase = colblock_offset(jk.x, N);
+      ApCols cs;
+      cs.phi = 0;
+      cs.half = 0;
+#pragma unroll
+      for (int jc = 0; jc < CB; ++jc) {
+        const int c = c0 + jc;
+        const int cc = min(c, Rv - 1);
+        const int tx = st.tmin + (cc - smin + N) % N;
+#pragma unroll
+        for (int j = 0; j < AK; ++j) {
+          PCoef k{0.0, 0.0, 0.0, 0.0, 0.0};
+          if (j < a.npend && c < Rv) k = pend_coef(a, j, mb, lane, cc, tx);
+          cs.v[j][jc] = k.v;
+          cs.ms[j][jc] = k.ms;
+          cs.rf[j][jc] = k.rf;
+          cs.rr[j][jc] = k.rr;
+          if (j < a.npend && tx == a.ptmin[j]) cs.half |= 1u << (j * CB + jc);
+        }
+        cs.phi |= (c < smin ? 1u : 0u) << jc;
+      }
+      double2 nxt[CB];
+      const long pb0 = cbase + (long)(r0 - c0) * CB;
+#pragma unroll
+      for (int jc = 0; jc < CB; ++jc) nxt[jc] = ld2_nt(Hbs + (size_t)(pb0 + jc) * MB * 2);
+      for (int r = r0; r < r1; ++r) {
+        double2 cur[CB];
+#pragma unroll
+        for (int jc = 0; jc < CB; ++jc) cur[jc] = nxt[jc];
+        if (r + 1 < r1) {
+          const long pbn = cbase + (long)(r + 1 - c0) * CB;
+#pragma unroll
+          for (int jc = 0; jc < CB; ++jc) nxt[jc] = ld2_nt(Hbs + (size_t)(pbn + jc) * MB * 2);
+        }
+        const int tx = st.tmin + (r - smin + N) % N;
+        const unsigned neg = r >= smin ? cs.phi : 0u;
+#pragma unroll
+        for (int j = 0; j < AK; ++j) {
+          if (j < a.npend) {
+            const PCoef rc = pend_coef(a, j, mb, lane, r, tx);
+#pragma unroll
+            for (int jc = 0; jc < CB; ++jc) {
+              const int c = c0 + jc;
+              if (c < r) {
+                const double nf = ((neg >> jc) & 1u) ? 1.0 : 0.0, pf = 1.0 - nf;
+                const double uc = ((cs.half >> (j * CB + jc)) & 1u ? 0.5 : 1.0) * cs.ms[j][jc];
+                cur[jc].x -= rc.u * cs.rf[j][jc] + uc * rc.rf;
+                cur[jc].y += nf * (rc.v * cs.rf[j][jc] + cs.ms[j][jc] * rc.rr) -
+                             pf * (cs.v[j][jc] * rc.rf + rc.ms * cs.rr[j][jc]);
+              } else if (c == r) {
+                cur[jc].x -= rc.u * rc.rf;
+              }
+            }
+          }
+        }
+        const long pb = cbase + (long)(r - c0) * CB;
+#pragma unroll
+        for (int jc = 0; jc < CB; ++jc)
+          if (c0 + jc <= r) st2_nt(Hbs + (size_t)(pb + jc) * MB * 2, cur[jc]);
+      }
+    }
+    int nx = 0;
+    if (lane == 0) nx = atomicAdd(&next_item, 1);
+    item = __shfl(nx, tid & ~(MB - 1) & 63);
+  }
 }
 
 // the Hartree adjoint of step 3 and the initial-row gradients:
@@ -566,7 +704,8 @@ struct AdjState {
   int C = 0;          // checkpoint spacing in steps; 0: tape every dropped column
   int max_steps = 0;  // largest T that may be recorded
   double* Hbar = nullptr;
-  double* Rb = nullptr;
+  double* Sring[AK] = {};  // Sigma(k-1, .) of reverse step k, at k % AK
+  double* Rring[AK] = {};  // R_bar of reverse step k, at k % AK
   double* Sbar = nullptr;
   double2* X = nullptr;
   double* P = nullptr;    // [2][NB]
@@ -598,9 +737,13 @@ int amalloc(kb_ctx* c, AdjState* s, void** p, size_t bytes) {
 
 void free_state(AdjState* s) {
   if (!s) return;
-  for (void* p : {(void*)s->Hbar, (void*)s->Rb, (void*)s->Sbar, (void*)s->X, (void*)s->P, (void*)s->LP,
-                  (void*)s->m2r, (void*)s->Mb, (void*)s->out, (void*)s->tape})
+  for (void* p : {(void*)s->Hbar, (void*)s->Sbar, (void*)s->X, (void*)s->P, (void*)s->LP, (void*)s->m2r,
+                  (void*)s->Mb, (void*)s->out, (void*)s->tape})
     if (p) (void)hipFree(p);
+  for (int j = 0; j < AK; ++j) {
+    if (s->Sring[j]) (void)hipFree(s->Sring[j]);
+    if (s->Rring[j]) (void)hipFree(s->Rring[j]);
+  }
   for (double* p : s->ckpt)
     if (p) (void)hipFree(p);
   delete s;
@@ -632,8 +775,8 @@ AdjArgs adj_args(kb_ctx* c, int t, int T) {
   AdjArgs a{};
   a.H = c->H;
   a.Hbar = s->Hbar;
-  a.S = c->S;
-  a.Rb = s->Rb;
+  a.S = s->Sring[t % AK];
+  a.Rb = s->Rring[t % AK];
   a.Sbar = s->Sbar;
   a.kL2 = c->kL2;
   a.m2r = s->m2r;
@@ -683,33 +826,58 @@ int launch_sigma_adjoint(kb_ctx* c) {
   return KB_OK;
 }
 
-// reverse step t: the primal window of step t is in H
-int reverse_step(kb_ctx* c, int t, int T) {
+// deferred steps k = k0, k0+1, .. k0+n-1 into the argument block
+void set_pending(kb_ctx* c, AdjArgs& a, int k0, int n) {
+  AdjState* s = c->adj;
+  a.npend = n;
+  for (int j = 0; j < n; ++j) {
+    const int k = k0 + j;
+    a.pk[j] = k;
+    a.ptmin[j] = k > c->N ? k - c->N + 1 : 1;
+    a.Sj[j] = s->Sring[k % AK];
+    a.Rj[j] = s->Rring[k % AK];
+  }
+}
+
+// reverse step t: the primal window of step t is in H.  npend: steps
+// t+1 .. t+npend whose entry updates are deferred; apply: add them and this
+// step's to the adjoint window now.
+int reverse_step(kb_ctx* c, int t, int T, int npend, bool apply) {
   AdjState* s = c->adj;
   const int N = c->N;
   const bool nlo = c->p.pexp == KB_PEXP_LAMBDA_NLO;
-  const AdjArgs a = adj_args(c, t, T);
+  AdjArgs a = adj_args(c, t, T);
   int rc;
   {
     StageTimer tm(c, 4);
     hipLaunchKernelGGL(adj_state_kernel, dim3(1), dim3(1), 0, c->stream, c->st_dev, t, N);
     LAUNCHCHK(c);
-    if (nlo && (rc = launch_sigma(c, s->X))) return rc;
+    if (nlo && (rc = launch_sigma(c, s->X, s->Sring[t % AK]))) return rc;
     const double H = c->p.Lambda * (double)(c->p.ONgroup + 2) / (6.0 * (double)c->p.ONgroup);
     hipLaunchKernelGGL(adj_hartree_kernel, dim3(1), dim3(1024), 0, c->stream, c->H, c->NP, c->V,
                        pair_off(a.st.s_tm1, a.st.s_tm1, N), H / (double)c->V, c->p.Mass2, s->m2r + (t & 1) * 2);
     LAUNCHCHK(c);
+    set_pending(c, a, t + 1, nlo ? npend : 0);
     hipLaunchKernelGGL(adj_row_kernel, dim3(c->NB), dim3(ROW_THREADS), 0, c->stream, a);
     LAUNCHCHK(c);
   }
   if (nlo) {
     StageTimer tm(c, 3);
-    hipLaunchKernelGGL(adj_memint_kernel, dim3(c->NB), dim3(AM_THREADS), memint_lds(N), c->stream, a);
+    hipLaunchKernelGGL(adj_reduce_kernel, dim3(c->NB), dim3(AM_THREADS), memint_lds(N), c->stream, a);
     LAUNCHCHK(c);
   }
   {
     StageTimer tm(c, 4);
     if (nlo && (rc = launch_sigma_adjoint(c))) return rc;
+  }
+  if (nlo && apply) {
+    StageTimer tm(c, 3);
+    set_pending(c, a, t, npend + 1);
+    hipLaunchKernelGGL(adj_apply_kernel, dim3(c->NB), dim3(AM_THREADS), 0, c->stream, a);
+    LAUNCHCHK(c);
+  }
+  {
+    StageTimer tm(c, 4);
     const double* col = nullptr;
     if (a.st.dead >= 0) col = s->tape + (size_t)tape_index(c, t) * s->col_doubles;
     const dim3 grid((N * MB + 255) / 256, c->NB);
@@ -780,7 +948,7 @@ int kb_adj_record(kb_ctx* c, int checkpoint_every, int max_steps) {
   {
     // per function, shared by every context: the largest supported window
     const size_t lds = memint_lds(640);
-    hipError_t e = hipFuncSetAttribute((const void*)adj_memint_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = hipFuncSetAttribute((const void*)adj_reduce_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess)
       return seterr(c, KB_ERR_ARG, "adjoint: %zu bytes of LDS rejected: %s", lds, hipGetErrorString(e));
   }
@@ -804,8 +972,10 @@ int kb_adj_record(kb_ctx* c, int checkpoint_every, int max_steps) {
     free_state(s);
     return code;
   };
-  if ((rc = amalloc(c, s, (void**)&s->Hbar, s->hbytes)) || (rc = amalloc(c, s, (void**)&s->Rb, ring * 8)) ||
-      (rc = amalloc(c, s, (void**)&s->Sbar, ring * 8)) ||
+  for (int j = 0; j < AK; ++j)
+    if ((rc = amalloc(c, s, (void**)&s->Sring[j], ring * 8)) || (rc = amalloc(c, s, (void**)&s->Rring[j], ring * 8)))
+      return fail(rc);
+  if ((rc = amalloc(c, s, (void**)&s->Hbar, s->hbytes)) || (rc = amalloc(c, s, (void**)&s->Sbar, ring * 8)) ||
       (rc = amalloc(c, s, (void**)&s->X, (size_t)c->N * c->V * sizeof(double2))) ||
       (rc = amalloc(c, s, (void**)&s->P, 2 * (size_t)c->NB * 8)) ||
       (rc = amalloc(c, s, (void**)&s->LP, (size_t)c->NB * 8)) || (rc = amalloc(c, s, (void**)&s->m2r, 4 * 8)) ||
@@ -889,13 +1059,16 @@ int kb_adj_gradient(kb_ctx* c, const double* wgt, double* dL_dlambda, double* dL
   int have_from = s->C > 0 ? seg_first(T, s->C) : 3;
   const int stop = s->stop_after;
   s->stop_after = 0;
+  int npend = 0;  // reverse steps whose entry updates are deferred
   for (int t = T; t >= 3; --t) {
     if (t < have_from) {
       const int first = seg_first(t, s->C);
       if ((rc = recompute_segment(c, first, t))) return rc;
       have_from = first;
     }
-    if ((rc = reverse_step(c, t, T))) return rc;
+    const bool apply = npend + 1 == AK || t == 3 || t == stop;
+    if ((rc = reverse_step(c, t, T, npend, apply))) return rc;
+    npend = apply ? 0 : npend + 1;
     if (t == stop) {
       // left mid-sweep for inspection: the window of step t-1
       c->st = state_of_step(t - 1, N);

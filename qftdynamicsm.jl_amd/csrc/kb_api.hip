@@ -109,13 +109,13 @@ int launch_hartree(kb_ctx* c, int i, int j) {
   return KB_OK;
 }
 
-int launch_sigma(kb_ctx* c, double2* X) {
+int launch_sigma(kb_ctx* c, double2* X, double* S_out) {
   const int rows = c->N - 1;  // grid rows; rows q >= L exit at once
   const double ivol = 1.0 / (double)c->V;
   const double lam = c->p.Lambda, ON = (double)c->p.ONgroup;
   SigmaArgs a{};
   a.H = c->H;
-  a.S = c->S;
+  a.S = S_out ? S_out : c->S;
   a.W = c->W;
   a.tw = c->tw;
   a.plan = c->plan;

@@ -97,8 +97,9 @@ int download(kb_ctx* c, double* dst, const double* src, size_t n);
 int dmalloc(kb_ctx* c, void** ptr, size_t bytes);
 int upload_state(kb_ctx* c);
 void drop_graph(kb_ctx* c);
-// self-energy stage of the step in c->st_dev; X, if set, receives (Fx, rx)
-int launch_sigma(kb_ctx* c, double2* X = nullptr);
+// self-energy stage of the step in c->st_dev into S_out (default: the
+// context's ring); X, if set, receives the transforms (Fx, rx)
+int launch_sigma(kb_ctx* c, double2* X = nullptr, double* S_out = nullptr);
 // the kernels of one forward step (the window in c->st_dev is advanced first)
 int enqueue_step(kb_ctx* c);
 size_t memint_lds(int N);

@@ -184,3 +184,36 @@ def test_adjoint_vs_oracle_fd_at_16cubed():
     ad = float(g["n0"][shell].sum())
     assert abs(ad - fds) <= FD_TOL * abs(fds), (ad, fds)
     c.close()
+
+
+def test_differentiation_workload_vs_oracle_fd_fixture():
+    """24^3, N_mem = 200, 500 steps, checkpoints every 50: the adjoint
+    gradient against the oracle's central finite differences committed in
+    tests/golden/cfg4_fd.json (made by tests/golden/make_cfg4_fd.py)."""
+    import json
+    import os
+    from qftkb import KBContext
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "cfg4_fd.json")))
+    print("\n[case] differentiation workload 24^3 N_mem=200 500 steps, C=50", flush=True)
+    c = KBContext(gold["Nx"], gold["sdim"], gold["N_mem"], gold["dt"], gold["Lambda"], gold["ONgroup"],
+                  gold["Mass"], gold["Mass"] ** 2)
+    k2 = c.kL2()
+    n0 = thermal(k2, m=gold["Mass"])
+    wgt = np.exp(-k2)
+    T = gold["T_final"]
+    c.init_particles(n0, gold["Mass"])
+    c.adj_record(50, T)
+    for t in range(3, T + 1):
+        c.step(t)
+    L = float((wgt * c.measure(T)["n"]).sum())
+    assert L == pytest.approx(0.5 * (gold["L_plus"] + gold["L_minus"]), rel=1e-9)
+    g = c.adj_gradient(wgt)
+    fd = gold["dL_dlambda_fd"]
+    assert abs(g["lambda"] - fd) <= FD_TOL * abs(fd), (g["lambda"], fd)
+    fdb = gold.get("dL_dn0_bins_fd")
+    if fdb and all(v is not None for v in fdb):
+        edges = np.linspace(0.0, np.sqrt(12.0), 17)
+        b = np.minimum(np.searchsorted(edges, np.sqrt(k2), side="right") - 1, 15)
+        ad = np.array([g["n0"][b == i].sum() for i in range(16)])
+        assert np.max(np.abs(ad - np.array(fdb))) <= FD_TOL * np.max(np.abs(fdb))
+    c.close()
