@@ -99,13 +99,12 @@ __device__ __forceinline__ void st2_nt(double* p, double2 v) {
 // workgroup that sums the same array)
 template <int NT>
 __device__ double block_sum(double v, double* red) {
-  red[threadIdx.x] = v;
+  static_assert(NT % 64 == 0, "whole waves");
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int w = NT / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  const double r = red[0];
+  double r = 0.0;
+  for (int w = 0; w < NT / 64; ++w) r += red[w];
   __syncthreads();
   return r;
 }
@@ -145,7 +144,7 @@ __global__ void adj_state_kernel(StepState* st, int t, int N) { *st = state_of_s
 // M^2 = Mass2 + hfac sum_p F(pair) and the plain sum
 __global__ __launch_bounds__(1024) void adj_hartree_kernel(const double* __restrict__ H, long NP, int V, long pair,
                                                            double hfac, double Mass2, double* __restrict__ out) {
-  __shared__ double red[1024];
+  __shared__ double red[1024 / 64];
   double acc = 0.0;
   for (int p = threadIdx.x; p < V; p += blockDim.x) acc += H[hoff(p / MB, pair, p % MB, NP)];
   const double s = block_sum<1024>(acc, red);
@@ -182,17 +181,6 @@ __global__ void adj_seed_kernel(const double* __restrict__ H, double* __restrict
   Hbar[hoff(mb, pair_off(sTm1, sTm1, N), lane, NP)] += w * dn_dF + g;
 }
 
-// ---------------------------------------------------------------------------
-// Adjoint of the diagonal and of the leapfrog row t (evolve_F!, evolve_r!,
-// Evolution.jl:890-915, 971-974), one workgroup per site block.
-//   F(t,t)  = (2-Om) F(t,t-1) - F(t,t-2) + dt^3 R_F(t)
-//   F(t,b)  = (2-Om) F(t-1,b) - F(t-2,b) + dt^3 R_F(b),  b <= t-1
-//   rho(t,b)= (2-Om) rho(t-1,b) - rho(t-2,b) + dt^3 R_r(b), b <= t-2
-// with R_F(t) = -sum_{z<=t-2} w Sr(z) F(z,t) + sum_{z<=t-1} w SF(z) rho(z,t).
-// The terms of R_F(t) that multiply row-t entries are folded in here; its
-// self-energy terms are left to the memory-integral kernels through
-// R_bar_F(t).
-// ---------------------------------------------------------------------------
 // ---------------------------------------------------------------------------
 // Adjoint of the memory integrals (getRHS_F!, getRHS_r!, Evolution.jl:
 // 766-782, 846-854) over the window [tmin, t], one workgroup per site block.
@@ -250,26 +238,41 @@ struct PCoef {
 };
 
 __device__ __forceinline__ PCoef pend_coef(const AdjArgs& a, int j, int mb, int lane, int x, int tx) {
-  PCoef c{0.0, 0.0, 0.0, 0.0, 0.0};
-  if (tx < a.ptmin[j]) return c;
+  // branch-free: the ring addresses are valid for every j < AK and slot x;
+  // the values are selected (not multiplied) away, so stale ring contents
+  // never reach the result
   const int k = a.pk[j];
-  const double w = tx == a.ptmin[j] ? 0.5 : 1.0;
-  const double m = tx <= k - 2 ? 1.0 : 0.0;
   const size_t o = soff(mb, x, lane, a.N);
-  const double2 sg = tx <= k - 1 ? ld2(a.Sj[j] + o) : make_double2(0.0, 0.0);
+  const double2 sg = ld2(a.Sj[j] + o);
   const double2 rb = ld2(a.Rj[j] + o);
-  c.u = m * w * sg.y;
-  c.v = w * sg.x;
-  c.ms = m * sg.y;
-  c.rf = rb.x;
-  c.rr = rb.y;
+  const bool on = j < a.npend && tx >= a.ptmin[j];
+  const double w = tx == a.ptmin[j] ? 0.5 : 1.0;
+  const bool m = on && tx <= k - 2;
+  const bool s = on && tx <= k - 1;
+  PCoef c;
+  c.u = m ? w * sg.y : 0.0;
+  c.v = s ? w * sg.x : 0.0;
+  c.ms = m ? sg.y : 0.0;
+  c.rf = on ? rb.x : 0.0;
+  c.rr = on ? rb.y : 0.0;
   return c;
 }
 
-constexpr int ROW_THREADS = 256;
+// ---------------------------------------------------------------------------
+// Adjoint of the diagonal and of the leapfrog row t (evolve_F!, evolve_r!,
+// Evolution.jl:890-915, 971-974), one workgroup per site block.
+//   F(t,t)  = (2-Om) F(t,t-1) - F(t,t-2) + dt^3 R_F(t)
+//   F(t,b)  = (2-Om) F(t-1,b) - F(t-2,b) + dt^3 R_F(b),  b <= t-1
+//   rho(t,b)= (2-Om) rho(t-1,b) - rho(t-2,b) + dt^3 R_r(b), b <= t-2
+// with R_F(t) = -sum_{z<=t-2} w Sr(z) F(z,t) + sum_{z<=t-1} w SF(z) rho(z,t).
+// The terms of R_F(t) that multiply row-t entries are folded in here; its
+// self-energy terms are left to the memory-integral kernels through
+// R_bar_F(t).
+// ---------------------------------------------------------------------------
+constexpr int ROW_THREADS = 1024;
 
 __global__ __launch_bounds__(ROW_THREADS) void adj_row_kernel(AdjArgs a) {
-  __shared__ double red[ROW_THREADS];
+  __shared__ double red[ROW_THREADS / 64];
   __shared__ double fl[2][MB];  // f_bar at t-1 and t-2
   const StepState& st = a.st;
   const int N = a.N;
@@ -305,11 +308,8 @@ __global__ __launch_bounds__(ROW_THREADS) void adj_row_kernel(AdjArgs a) {
   double Fbtt = Hbs[pair_off(st.s_t, st.s_t, N) * pstride] + a.hl_ivol * Mb1;
 #pragma unroll
   for (int j = 0; j < AK; ++j) {
-    pt[j] = PCoef{0.0, 0.0, 0.0, 0.0, 0.0};
-    if (j < a.npend) {
-      pt[j] = pend_coef(a, j, mb, lane, st.s_t, t);
-      Fbtt -= pt[j].u * pt[j].rf;
-    }
+    pt[j] = pend_coef(a, j, mb, lane, st.s_t, t);
+    Fbtt -= pt[j].u * pt[j].rf;
   }
   const double RFt = dt3 * Fbtt;
   double omb = 0.0;
@@ -318,21 +318,28 @@ __global__ __launch_bounds__(ROW_THREADS) void adj_row_kernel(AdjArgs a) {
     const int x = (tx - 1) % N;
     const double w = q == 0 ? 0.5 : 1.0;
     const double m = tx <= t - 2 ? 1.0 : 0.0;
+    // every load of the iteration first: none of them is written by another
+    // thread of this kernel
+    double* e1p = Hbs + pair_off(st.s_tm1, x, N) * pstride;
+    double* e2p = Hbs + pair_off(st.s_tm2, x, N) * pstride;
+    const bool push1 = real && tx != t - 2, push2 = real && tx != t - 1;
     double2 hb = ld2(Hbs + pair_off(st.s_t, x, N) * pstride);
+    const double2 h1 = ld2(Hs + pair_off(st.s_tm1, x, N) * pstride);
+    const double2 sg = a.integral ? Ss[(size_t)x * MB] : make_double2(0.0, 0.0);
+    double2 v1 = push1 ? ld2(e1p) : make_double2(0.0, 0.0);
+    double2 v2 = push2 ? ld2(e2p) : make_double2(0.0, 0.0);
+    PCoef px[AK];
+#pragma unroll
+    for (int j = 0; j < AK; ++j) px[j] = pend_coef(a, j, mb, lane, x, tx);
     {
       // complete F_bar(t,tx), rho_bar(t,tx) (stored sign) with the deferred terms
       const double sgt = st.s_t >= x ? 1.0 : -1.0;
 #pragma unroll
       for (int j = 0; j < AK; ++j) {
-        if (j < a.npend) {
-          const PCoef px = pend_coef(a, j, mb, lane, x, tx);
-          hb.x -= pt[j].u * px.rf + px.u * pt[j].rf;
-          hb.y -= sgt * (px.v * pt[j].rf + pt[j].ms * px.rr);
-        }
+        hb.x -= pt[j].u * px[j].rf + px[j].u * pt[j].rf;
+        hb.y -= sgt * (px[j].v * pt[j].rf + pt[j].ms * px[j].rr);
       }
     }
-    const double2 h1 = ld2(Hs + pair_off(st.s_tm1, x, N) * pstride);
-    const double2 sg = a.integral ? Ss[(size_t)x * MB] : make_double2(0.0, 0.0);
     double fb = hb.x;
     if (tx == t - 1) fb += (2.0 - Om) * Fbtt;
     if (tx == t - 2) fb -= Fbtt;
@@ -341,26 +348,18 @@ __global__ __launch_bounds__(ROW_THREADS) void adj_row_kernel(AdjArgs a) {
     if (tx <= t - 2) rp = hb.y * (st.s_t >= x ? 1.0 : -1.0) - w * sg.x * RFt;
     const double s1 = st.s_tm1 >= x ? 1.0 : -1.0;
     omb -= h1.x * fb + h1.y * s1 * rp;
-    if (real) {
-      Rs[(size_t)x * MB] = make_double2(dt3 * fb, dt3 * rp);
-      // leapfrog pushes; the pair {t-1, t-2} is reached from b = t-1 and
-      // b = t-2 and is updated once below
-      if (tx != t - 2) {
-        double* e = Hbs + pair_off(st.s_tm1, x, N) * pstride;
-        double2 v = ld2(e);
-        v.x += (2.0 - Om) * fb;
-        if (tx <= t - 3) v.y += s1 * (2.0 - Om) * rp;
-        st2(e, v);
-      }
-      if (tx != t - 1) {
-        double* e = Hbs + pair_off(st.s_tm2, x, N) * pstride;
-        double2 v = ld2(e);
-        v.x -= fb;
-        if (tx <= t - 3) v.y -= (st.s_tm2 >= x ? 1.0 : -1.0) * rp;
-        st2(e, v);
-      }
-    } else {
-      Rs[(size_t)x * MB] = make_double2(0.0, 0.0);
+    Rs[(size_t)x * MB] = real ? make_double2(dt3 * fb, dt3 * rp) : make_double2(0.0, 0.0);
+    // leapfrog pushes; the pair {t-1, t-2} is reached from b = t-1 and
+    // b = t-2 and is updated once below
+    if (push1) {
+      v1.x += (2.0 - Om) * fb;
+      if (tx <= t - 3) v1.y += s1 * (2.0 - Om) * rp;
+      st2(e1p, v1);
+    }
+    if (push2) {
+      v2.x -= fb;
+      if (tx <= t - 3) v2.y -= (st.s_tm2 >= x ? 1.0 : -1.0) * rp;
+      st2(e2p, v2);
     }
     if (tx == t - 1) fl[0][lane] = fb;
     if (tx == t - 2) fl[1][lane] = fb;
@@ -453,7 +452,7 @@ __global__ __launch_bounds__(AM_THREADS, 2) void adj_reduce_kernel(AdjArgs a) {
   const int N = a.N;
   double* accF = smem;           // SigF_bar per slot, lane
   double* accR = accF + N * MB;  // Sigr_bar
-  __shared__ double red[AM_THREADS];
+  __shared__ double red[AM_THREADS / 64];
   __shared__ int next_item;
   const StepState& st = a.st;
   const int t = st.t;
@@ -562,15 +561,21 @@ __global__ __launch_bounds__(AM_THREADS, 2) void adj_reduce_kernel(AdjArgs a) {
 }
 
 // ---- apply: the deferred entry updates of npend steps ----------------------
-// Per column and deferred step: v, ms, rf, rr, and u = w ms with w = 1/2
-// flagged in `half`.
-struct ApCols {
-  double v[AK][CB], ms[AK][CB], rf[AK][CB], rr[AK][CB];
-  unsigned half;  // bit j*CB + jc: w = 1/2 for column jc in step j's window
-  unsigned phi;
-};
+// One workgroup per site block walks the triangle in chunks of AP_ROWS
+// absolute slot rows.  The row coefficients of the chunk (v, ms, rf, rr per
+// deferred step; u = w ms with w = 1/2 at that step's tmin) are staged in
+// LDS once; 8-lane workers then take the chunk's 4-column blocks, keep the
+// columns' coefficients in registers and stream the rows (two rows in
+// flight) with one read-modify-write of each adjoint line.
+constexpr int AP_THREADS = 256;
+constexpr int AP_ROWS = 64;
 
-__global__ __launch_bounds__(AM_THREADS, 2) void adj_apply_kernel(AdjArgs a) {
+__device__ __forceinline__ size_t ap_lds(int row, int j, int q, int lane) {  // q: 0 v, 1 ms, 2 rf, 3 rr
+  return (((size_t)row * AK + j) * 4 + q) * MB + lane;
+}
+
+__global__ __launch_bounds__(AP_THREADS, 2) void adj_apply_kernel(AdjArgs a) {
+  extern __shared__ double rco[];  // AP_ROWS x AK x 4 x MB
   __shared__ int next_item;
   const StepState& st = a.st;
   const int N = a.N;
@@ -581,68 +586,81 @@ __global__ __launch_bounds__(AM_THREADS, 2) void adj_apply_kernel(AdjArgs a) {
   const int tid = threadIdx.x;
   const int lane = tid & (MB - 1);
   const int worker = tid >> 3;
-  const int nworkers = AM_THREADS >> 3;
-  if (tid == 0) next_item = nworkers;
-  __syncthreads();
+  const int nworkers = AP_THREADS >> 3;
+  const int np = a.npend;
   double* Hbs = a.Hbar + hoff(mb, 0, lane, a.NP);
-  int item = worker;
-  while (item < a.nitems) {
-    const int2 jk = a.items[item];
-    const int c0 = jk.x * CB;
-    const int r0 = c0 + jk.y * MEMINT_ROWS;
-    const int r1 = min(r0 + MEMINT_ROWS, Rv);
-    if (c0 < Rv && r0 < r1) {
-      const long cbase = colblock_offset(jk.x, N);
-      ApCols cs;
-      cs.phi = 0;
-      cs.half = 0;
+  for (int R0 = 0; R0 < Rv; R0 += AP_ROWS) {
+    const int R1 = min(R0 + AP_ROWS, Rv);
+    // stage the chunk's row coefficients
+    for (int i = worker; i < (R1 - R0) * AK; i += nworkers) {
+      const int row = i / AK, j = i - row * AK;
+      const int r = R0 + row;
+      const PCoef k = pend_coef(a, j, mb, lane, r, st.tmin + (r - smin + N) % N);
+      rco[ap_lds(row, j, 0, lane)] = k.v;
+      rco[ap_lds(row, j, 1, lane)] = k.ms;
+      rco[ap_lds(row, j, 2, lane)] = k.rf;
+      rco[ap_lds(row, j, 3, lane)] = k.rr;
+    }
+    if (tid == 0) next_item = nworkers;
+    __syncthreads();
+    const int nJ = (R1 - 1) / CB + 1;  // column blocks with rows in the chunk
+    int J = worker;
+    while (J < nJ) {
+      const int c0 = J * CB;
+      const int r0 = max(R0, c0);
+      const long cbase = colblock_offset(J, N);
+      double cv[AK][CB], cms[AK][CB], crf[AK][CB], crr[AK][CB];
+      unsigned half = 0, phi = 0;
 #pragma unroll
       for (int jc = 0; jc < CB; ++jc) {
         const int c = c0 + jc;
         const int cc = min(c, Rv - 1);
-        const int tx = st.tmin + (cc - smin + N) % N;
+        const int txc = st.tmin + (cc - smin + N) % N;
 #pragma unroll
         for (int j = 0; j < AK; ++j) {
-          PCoef k{0.0, 0.0, 0.0, 0.0, 0.0};
-          if (j < a.npend && c < Rv) k = pend_coef(a, j, mb, lane, cc, tx);
-          cs.v[j][jc] = k.v;
-          cs.ms[j][jc] = k.ms;
-          cs.rf[j][jc] = k.rf;
-          cs.rr[j][jc] = k.rr;
-          if (j < a.npend && tx == a.ptmin[j]) cs.half |= 1u << (j * CB + jc);
+          const PCoef k = pend_coef(a, j, mb, lane, cc, txc);
+          const bool ok = c < Rv;
+          if (txc == a.ptmin[j]) half |= 1u << (j * CB + jc);
+          cv[j][jc] = ok ? k.v : 0.0;
+          cms[j][jc] = ok ? k.ms : 0.0;
+          crf[j][jc] = ok ? k.rf : 0.0;
+          crr[j][jc] = ok ? k.rr : 0.0;
         }
-        cs.phi |= (c < smin ? 1u : 0u) << jc;
+        phi |= (c < smin ? 1u : 0u) << jc;
       }
       double2 nxt[CB];
-      const long pb0 = cbase + (long)(r0 - c0) * CB;
 #pragma unroll
-      for (int jc = 0; jc < CB; ++jc) nxt[jc] = ld2_nt(Hbs + (size_t)(pb0 + jc) * MB * 2);
-      for (int r = r0; r < r1; ++r) {
+      for (int jc = 0; jc < CB; ++jc) nxt[jc] = ld2_nt(Hbs + (size_t)(cbase + (long)(r0 - c0) * CB + jc) * MB * 2);
+      for (int r = r0; r < R1; ++r) {
         double2 cur[CB];
 #pragma unroll
         for (int jc = 0; jc < CB; ++jc) cur[jc] = nxt[jc];
-        if (r + 1 < r1) {
-          const long pbn = cbase + (long)(r + 1 - c0) * CB;
+        if (r + 1 < R1) {
 #pragma unroll
-          for (int jc = 0; jc < CB; ++jc) nxt[jc] = ld2_nt(Hbs + (size_t)(pbn + jc) * MB * 2);
+          for (int jc = 0; jc < CB; ++jc)
+            nxt[jc] = ld2_nt(Hbs + (size_t)(cbase + (long)(r + 1 - c0) * CB + jc) * MB * 2);
         }
         const int tx = st.tmin + (r - smin + N) % N;
-        const unsigned neg = r >= smin ? cs.phi : 0u;
+        const unsigned neg = r >= smin ? phi : 0u;
+        const int row = r - R0;
 #pragma unroll
         for (int j = 0; j < AK; ++j) {
-          if (j < a.npend) {
-            const PCoef rc = pend_coef(a, j, mb, lane, r, tx);
+          if (j < np) {
+            const double rv = rco[ap_lds(row, j, 0, lane)];
+            const double rms = rco[ap_lds(row, j, 1, lane)];
+            const double rrf = rco[ap_lds(row, j, 2, lane)];
+            const double rrr = rco[ap_lds(row, j, 3, lane)];
+            const double ru = (tx == a.ptmin[j] ? 0.5 : 1.0) * rms;
 #pragma unroll
             for (int jc = 0; jc < CB; ++jc) {
               const int c = c0 + jc;
               if (c < r) {
                 const double nf = ((neg >> jc) & 1u) ? 1.0 : 0.0, pf = 1.0 - nf;
-                const double uc = ((cs.half >> (j * CB + jc)) & 1u ? 0.5 : 1.0) * cs.ms[j][jc];
-                cur[jc].x -= rc.u * cs.rf[j][jc] + uc * rc.rf;
-                cur[jc].y += nf * (rc.v * cs.rf[j][jc] + cs.ms[j][jc] * rc.rr) -
-                             pf * (cs.v[j][jc] * rc.rf + rc.ms * cs.rr[j][jc]);
+                const double uc = ((half >> (j * CB + jc)) & 1u ? 0.5 : 1.0) * cms[j][jc];
+                cur[jc].x -= ru * crf[j][jc] + uc * rrf;
+                cur[jc].y += nf * (rv * crf[j][jc] + cms[j][jc] * rrr) - pf * (cv[j][jc] * rrf + rms * crr[j][jc]);
               } else if (c == r) {
-                cur[jc].x -= rc.u * rc.rf;
+                cur[jc].x -= ru * rrf;
               }
             }
           }
@@ -652,19 +670,22 @@ __global__ __launch_bounds__(AM_THREADS, 2) void adj_apply_kernel(AdjArgs a) {
         for (int jc = 0; jc < CB; ++jc)
           if (c0 + jc <= r) st2_nt(Hbs + (size_t)(pb + jc) * MB * 2, cur[jc]);
       }
+      int nx = 0;
+      if (lane == 0) nx = atomicAdd(&next_item, 1);
+      J = __shfl(nx, tid & ~(MB - 1) & 63);
     }
-    int nx = 0;
-    if (lane == 0) nx = atomicAdd(&next_item, 1);
-    item = __shfl(nx, tid & ~(MB - 1) & 63);
+    __syncthreads();
   }
 }
+
+size_t apply_lds() { return sizeof(double) * (size_t)AP_ROWS * AK * 4 * MB; }
 
 // the Hartree adjoint of step 3 and the initial-row gradients:
 // F(1,1) = F(2,1) = (n+1/2)/w, F(2,2) = (n+1/2)(w dt^2 + 1/w)
 __global__ __launch_bounds__(1024) void adj_pending_kernel(const double* __restrict__ P, int NB, double dt,
                                                            double h_ivol, const double* __restrict__ m2r,
                                                            double* __restrict__ LP, double* __restrict__ Mb_out) {
-  __shared__ double red[1024];
+  __shared__ double red[1024 / 64];
   double acc = 0.0;
   for (int i = threadIdx.x; i < NB; i += blockDim.x) acc += P[(size_t)(3 & 1) * NB + i];
   const double Mb = dt * dt * block_sum<1024>(acc, red);
@@ -830,7 +851,8 @@ int launch_sigma_adjoint(kb_ctx* c) {
 void set_pending(kb_ctx* c, AdjArgs& a, int k0, int n) {
   AdjState* s = c->adj;
   a.npend = n;
-  for (int j = 0; j < n; ++j) {
+  for (int j = 0; j < AK; ++j) {
+    // slots past n point at valid rings too; the kernels mask them out
     const int k = k0 + j;
     a.pk[j] = k;
     a.ptmin[j] = k > c->N ? k - c->N + 1 : 1;
@@ -873,7 +895,7 @@ int reverse_step(kb_ctx* c, int t, int T, int npend, bool apply) {
   if (nlo && apply) {
     StageTimer tm(c, 3);
     set_pending(c, a, t, npend + 1);
-    hipLaunchKernelGGL(adj_apply_kernel, dim3(c->NB), dim3(AM_THREADS), 0, c->stream, a);
+    hipLaunchKernelGGL(adj_apply_kernel, dim3(c->NB), dim3(AP_THREADS), apply_lds(), c->stream, a);
     LAUNCHCHK(c);
   }
   {
@@ -949,6 +971,9 @@ int kb_adj_record(kb_ctx* c, int checkpoint_every, int max_steps) {
     // per function, shared by every context: the largest supported window
     const size_t lds = memint_lds(640);
     hipError_t e = hipFuncSetAttribute((const void*)adj_reduce_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute((const void*)adj_apply_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)apply_lds());
     if (e != hipSuccess)
       return seterr(c, KB_ERR_ARG, "adjoint: %zu bytes of LDS rejected: %s", lds, hipGetErrorString(e));
   }
@@ -992,8 +1017,14 @@ int kb_adj_record(kb_ctx* c, int checkpoint_every, int max_steps) {
     if (hipMemcpyAsync(s->ckpt[0], c->H, s->hbytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
       return fail(seterr(c, KB_ERR_HIP, "adjoint: checkpoint copy failed"));
   }
-  if (hipMemsetAsync(s->Hbar, 0, s->hbytes, c->stream) != hipSuccess ||
-      hipStreamSynchronize(c->stream) != hipSuccess)
+  bool ok = hipMemsetAsync(s->Hbar, 0, s->hbytes, c->stream) == hipSuccess &&
+            hipMemsetAsync(s->Sbar, 0, ring * 8, c->stream) == hipSuccess;
+  // the coefficient rings are read for every slot and lane (padding lanes and
+  // slots outside a window are selected away): keep them finite
+  for (int j = 0; j < AK && ok; ++j)
+    ok = hipMemsetAsync(s->Sring[j], 0, ring * 8, c->stream) == hipSuccess &&
+         hipMemsetAsync(s->Rring[j], 0, ring * 8, c->stream) == hipSuccess;
+  if (!ok || hipStreamSynchronize(c->stream) != hipSuccess)
     return fail(seterr(c, KB_ERR_HIP, "adjoint: initialisation failed"));
   c->adj = s;
   c->bytes += s->bytes;
