@@ -276,6 +276,34 @@ def adjoint_bench(fd_bins=False, fd=True):
     return out
 
 
+AUTOTUNE_WORKLOADS = [(32, 3, 400), (24, 3, 200), (48, 3, 300), (16, 3, 202), (40, 3, 300)]
+
+
+def autotune_bench(out_path):
+    """Pick the memory-integral tile shape per lattice and window: fill the
+    window, take two steps, re-time the last step's integral in every shape."""
+    from qftkb import KBContext
+    from qftkb._abi import tiles_key
+
+    w = WORKLOAD
+    res = {"method": "kb_autotune: 3 timed launches per tile shape on a filled window (HIP events)",
+           "choices": {}}
+    for Nx, sd, N in AUTOTUNE_WORKLOADS:
+        c = KBContext(Nx, sd, N, w["dt"], w["Lambda"], w["ONgroup"], w["Mass"], w["Mass"] ** 2, tiles=None)
+        c.fill_free_window(N, thermal_n(c.kL2(), w["Mass"], w["T"]), w["Mass"])
+        c.step(N + 1)
+        c.step(N + 2)
+        r = c.autotune(reps=3)
+        r["shapes"] = c.tile_shapes()
+        res["choices"][tiles_key(Nx, sd, N)] = r
+        print(f"[bench] autotune {tiles_key(Nx, sd, N)}: rows {r['rows']} batch {r['batch']} "
+              f"ms {[round(x, 3) for x in r['ms']]}", file=sys.stderr, flush=True)
+        c.close()
+    with open(out_path, "w") as f:
+        json.dump(res, f, indent=1)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -290,6 +318,8 @@ def main():
     ap.add_argument("--adjoint", action="store_true",
                     help="only the differentiation workload, with finite differences of all 16 radial bins")
     ap.add_argument("--no-fd", action="store_true", help="with --adjoint: skip the finite differences")
+    ap.add_argument("--autotune", metavar="OUT_JSON",
+                    help="pick the memory-integral tile shape for the standard workloads and write the choices")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 2 GiB device copy (torch) to scale rocprofv3 FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
@@ -312,6 +342,9 @@ def main():
     if args.adjoint:
         print(json.dumps({"adjoint": adjoint_bench(fd_bins=True, fd=not args.no_fd)}), flush=True)
         return
+    if args.autotune:
+        print(json.dumps({"autotune": autotune_bench(args.autotune)}), flush=True)
+        return
 
     from qftkb import KBContext
 
@@ -321,6 +354,7 @@ def main():
     N = w["NstepsinMemory"]
     V = w["Nx"] ** w["sdim"]
     ctx = KBContext(w["Nx"], w["sdim"], N, w["dt"], w["Lambda"], w["ONgroup"], w["Mass"], w["Mass"] ** 2)
+    tile_shape = ctx.tile_shapes()[ctx.tiles()]
     n = thermal_n(ctx.kL2(), w["Mass"], w["T"])
     ctx.fill_free_window(N, n, w["Mass"])
     print(f"[bench] window filled: {w['Nx']}^3 N_mem={N}, device bytes {ctx.device_bytes() / 1e9:.2f} GB",
@@ -376,6 +410,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "memint_step", "algorithmic_bytes": B,
                      "avg_kernel_ms": avg_mi_s * 1e3, "sigma_stage_ms": ms_fft / max(n_fft, 1),
+                     "tile_rows_batch": list(tile_shape),
                      "hartree_ms": ms_h / max(n_mi, 1)},
         "cpu_baseline": None,
         "adjoint": None,

@@ -157,6 +157,19 @@ int kb_enable_timing(kb_ctx *ctx, int on);
 int kb_timing(kb_ctx *ctx, int stage, double *total_ms, long long *count);
 int kb_reset_timing(kb_ctx *ctx);
 
+/* ---- tile autotuning -------------------------------------------------------
+ * The memory-integral kernel exists in several tile shapes (slot rows per
+ * work item, rows per batch).  kb_autotune re-times the last step's memory
+ * integral in every shape (reps launches each, HIP events on the context's
+ * stream; each launch recomputes row t from the same inputs, so row t may
+ * change in the last bits) and keeps the fastest; ms, if not NULL, receives
+ * one average per shape.  Shapes can also be set from a stored choice.  No
+ * counterpart in the reference (launch configuration). */
+int kb_tile_shape(int variant, int *rows, int *batch); /* KB_ERR_ARG past the last */
+int kb_set_tiles(kb_ctx *ctx, int variant);
+int kb_get_tiles(const kb_ctx *ctx, int *variant);
+int kb_autotune(kb_ctx *ctx, int reps, int *best, double *ms);
+
 /* ---- adjoint ---------------------------------------------------------------
  * Gradients of L = sum_p wgt(p) n(p, T) (n as in kb_measure at T = tmax,
  * TwoPIScalar_Measurement.jl:38-59) with respect to the coupling Lambda and

@@ -175,10 +175,19 @@ MemintArgs memint_args(const kb_ctx* c) {
   a.V = c->V;
   a.dt = c->p.dt;
   a.integral = c->p.pexp == KB_PEXP_LAMBDA_NLO;
-  a.items = c->items;
-  a.nitems = c->nitems;
+  int ri = 1;
+  for (int i = 0; i < 3; ++i)
+    if (MEMINT_ROW_CHOICES[i] == kMemintVariants[c->tile].rows) ri = i;
+  a.items = c->items_r[ri];
+  a.nitems = c->nitems_r[ri];
   a.st = c->st_dev;
   return a;
+}
+
+hipError_t launch_memint(int tile, unsigned grid, size_t lds, hipStream_t s, const MemintArgs* args,
+                         const int* wg_start, int nruns) {
+  void* kargs[] = {(void*)&args, (void*)&wg_start, (void*)&nruns};
+  return hipLaunchKernel(kMemintVariants[tile].fn, dim3(grid), dim3(MEMINT_THREADS), kargs, lds, s);
 }
 
 size_t memint_lds(int N) { return sizeof(double) * 2 * (size_t)N * MB; }
@@ -208,9 +217,8 @@ int enqueue_step(kb_ctx* c) {
   }
   {
     StageTimer st(c, 0);
-    hipLaunchKernelGGL(memint_step, dim3(c->NB), dim3(MEMINT_THREADS), memint_lds(c->N), c->stream, c->margs_dev,
-                       c->wg0_dev, 1);
-    LAUNCHCHK(c);
+    hipError_t e = launch_memint(c->tile, c->NB, memint_lds(c->N), c->stream, c->margs_dev, c->wg0_dev, 1);
+    if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "memory-integral launch failed: %s", hipGetErrorString(e));
   }
   return KB_OK;
 }
@@ -231,7 +239,8 @@ void drop_graph(kb_ctx* c);
 void free_all(kb_ctx* c) {
   drop_graph(c);
   for (void* ptr : {(void*)c->H, (void*)c->S, (void*)c->W, (void*)c->kL2, (void*)c->m2, (void*)c->partial,
-                    (void*)c->modes, (void*)c->vbuf, (void*)c->tw, (void*)c->flag, (void*)c->items,
+                    (void*)c->modes, (void*)c->vbuf, (void*)c->tw, (void*)c->flag, (void*)c->items_r[0],
+                    (void*)c->items_r[1], (void*)c->items_r[2],
                     (void*)c->st_dev, (void*)c->stptr_dev, (void*)c->N_dev, (void*)c->margs_dev, (void*)c->wg0_dev})
     if (ptr) (void)hipFree(ptr);
   adj_free(c);
@@ -308,13 +317,20 @@ int kb_create(const kb_params* p, kb_ctx** out) {
   if ((rc = dmalloc(c, (void**)&c->flag, sizeof(int) * 4))) return fail(rc);
   // memory-integral work items: chunks of MEMINT_ROWS rows of each column
   // block, in storage order
-  std::vector<int2> items;
-  for (int J = 0; J * CB < c->N; ++J)
-    for (int k = 0; J * CB + k * MEMINT_ROWS < c->N; ++k) items.push_back(make_int2(J, k));
-  c->nitems = (int)items.size();
-  if ((rc = dmalloc(c, (void**)&c->items, sizeof(int2) * items.size()))) return fail(rc);
-  if (hipMemcpy(c->items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice) != hipSuccess)
-    return fail(KB_ERR_HIP);
+  for (int i = 0; i < 3; ++i) {
+    const int rows = MEMINT_ROW_CHOICES[i];
+    std::vector<int2> items;
+    for (int J = 0; J * CB < c->N; ++J)
+      for (int k = 0; J * CB + k * rows < c->N; ++k) items.push_back(make_int2(J, k));
+    c->nitems_r[i] = (int)items.size();
+    if ((rc = dmalloc(c, (void**)&c->items_r[i], sizeof(int2) * items.size()))) return fail(rc);
+    if (hipMemcpy(c->items_r[i], items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(KB_ERR_HIP);
+    if (rows == MEMINT_ROWS) {
+      c->items = c->items_r[i];
+      c->nitems = c->nitems_r[i];
+    }
+  }
   if ((rc = dmalloc(c, (void**)&c->st_dev, sizeof(StepState)))) return fail(rc);
   if ((rc = dmalloc(c, (void**)&c->stptr_dev, sizeof(StepState*)))) return fail(rc);
   if ((rc = dmalloc(c, (void**)&c->N_dev, sizeof(int)))) return fail(rc);
@@ -338,7 +354,9 @@ int kb_create(const kb_params* p, kb_ctx** out) {
     // the attribute is per function, shared by every context: allow the
     // largest supported window, not this context's
     const size_t lds = memint_lds(640);
-    hipError_t e = hipFuncSetAttribute((const void*)memint_step, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = hipSuccess;
+    for (int v = 0; v < NMEMINT_VARIANTS && e == hipSuccess; ++v)
+      e = hipFuncSetAttribute(kMemintVariants[v].fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) {
       seterr(c, KB_ERR_ARG, "memory-integral kernel: %zu bytes of LDS rejected: %s", lds, hipGetErrorString(e));
       return fail(KB_ERR_ARG);
@@ -682,6 +700,67 @@ int kb_timing(kb_ctx* c, int stage, double* total_ms, long long* count) {
   return KB_OK;
 }
 
+int kb_tile_shape(int variant, int* rows, int* batch) {
+  if (variant < 0 || variant >= NMEMINT_VARIANTS) return KB_ERR_ARG;
+  if (rows) *rows = kMemintVariants[variant].rows;
+  if (batch) *batch = kMemintVariants[variant].batch;
+  return KB_OK;
+}
+
+int kb_set_tiles(kb_ctx* c, int variant) {
+  if (!c || variant < 0 || variant >= NMEMINT_VARIANTS) return KB_ERR_ARG;
+  if (c->in_ensemble) return seterr(c, KB_ERR_STATE, "tiles: ensemble members use the default shape");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->tile = variant;
+  const MemintArgs ma = memint_args(c);
+  HIPCHK(c, hipMemcpy(c->margs_dev, &ma, sizeof(MemintArgs), hipMemcpyHostToDevice));
+  drop_graph(c);
+  return KB_OK;
+}
+
+int kb_get_tiles(const kb_ctx* c, int* variant) {
+  if (!c || !variant) return KB_ERR_ARG;
+  *variant = c->tile;
+  return KB_OK;
+}
+
+int kb_autotune(kb_ctx* c, int reps, int* best, double* ms) {
+  if (!c || reps < 1) return KB_ERR_ARG;
+  if (!c->initialised || c->st.tmax < 3)
+    return seterr(c, KB_ERR_STATE, "autotune: take a step first (the last step's integral is re-timed)");
+  if (c->in_ensemble || c->adj) return seterr(c, KB_ERR_STATE, "autotune: not for ensemble members or while recording");
+  std::vector<double> t(NMEMINT_VARIANTS, 0.0);
+  hipEvent_t a = take_event(c), b = take_event(c);
+  if (!a || !b) return seterr(c, KB_ERR_HIP, "autotune: no events");
+  int rc = KB_OK;
+  for (int v = 0; v < NMEMINT_VARIANTS && rc == KB_OK; ++v) {
+    if ((rc = kb_set_tiles(c, v))) break;
+    // re-evaluates row t of the last step from the same inputs
+    for (int k = 0; k <= reps && rc == KB_OK; ++k) {
+      if (k == 1 && hipEventRecord(a, c->stream) != hipSuccess) rc = seterr(c, KB_ERR_HIP, "autotune: event");
+      if (launch_memint(v, c->NB, memint_lds(c->N), c->stream, c->margs_dev, c->wg0_dev, 1) != hipSuccess)
+        rc = seterr(c, KB_ERR_HIP, "autotune: launch failed");
+    }
+    if (rc) break;
+    float e_ms = 0.f;
+    if (hipEventRecord(b, c->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+        hipEventElapsedTime(&e_ms, a, b) != hipSuccess)
+      rc = seterr(c, KB_ERR_HIP, "autotune: timing failed");
+    t[v] = e_ms / reps;
+  }
+  c->freeEv.push_back(a);
+  c->freeEv.push_back(b);
+  if (rc) return rc;
+  int bv = 0;
+  for (int v = 1; v < NMEMINT_VARIANTS; ++v)
+    if (t[v] < t[bv]) bv = v;
+  if ((rc = kb_set_tiles(c, bv))) return rc;
+  if (best) *best = bv;
+  if (ms)
+    for (int v = 0; v < NMEMINT_VARIANTS; ++v) ms[v] = t[v];
+  return KB_OK;
+}
+
 int kb_reset_timing(kb_ctx* c) {
   if (!c) return KB_ERR_ARG;
   int rc = collect_timing(c);
@@ -767,9 +846,8 @@ int ens_enqueue(kb_ens* e) {
       return ens_err(e, rc, "member self-energy: %s", c->err.c_str());
     if ((rc = launch_hartree_step(c))) return ens_err(e, rc, "member Hartree: %s", c->err.c_str());
   }
-  hipLaunchKernelGGL(memint_step, dim3(e->total_wg), dim3(MEMINT_THREADS), memint_lds(e->maxN), e->stream,
-                     e->margs_dev, e->wg_dev, nr);
-  err = hipGetLastError();
+  // members keep the default tile shape (their work items are 64-row chunks)
+  err = launch_memint(0, e->total_wg, memint_lds(e->maxN), e->stream, e->margs_dev, e->wg_dev, nr);
   if (err != hipSuccess) return ens_err(e, KB_ERR_HIP, "memint launch failed: %s", hipGetErrorString(err));
   return KB_OK;
 }

@@ -48,8 +48,11 @@ struct kb_ctx {
   double* vbuf = nullptr;  // 4 x V staging
   double2* tw = nullptr;
   int* flag = nullptr;
-  int2* items = nullptr;  // memory-integral work items
+  int2* items = nullptr;  // memory-integral work items of 64-row chunks (adjoint kernels, ensembles)
   int nitems = 0;
+  int2* items_r[3] = {};  // work items per row-chunk choice (MEMINT_ROW_CHOICES)
+  int nitems_r[3] = {};
+  int tile = 0;           // memory-integral tile shape (kMemintVariants index)
   // device-resident step state and launch tables (graph-capturable steps)
   qftkb::StepState* st_dev = nullptr;
   qftkb::StepState** stptr_dev = nullptr;  // {st_dev}
@@ -103,6 +106,9 @@ int launch_sigma(kb_ctx* c, double2* X = nullptr, double* S_out = nullptr);
 // the kernels of one forward step (the window in c->st_dev is advanced first)
 int enqueue_step(kb_ctx* c);
 size_t memint_lds(int N);
+// launch the forward memory-integral kernel of tile shape `tile` on args/wg tables
+hipError_t launch_memint(int tile, unsigned grid, size_t lds, hipStream_t s, const qftkb::MemintArgs* args,
+                         const int* wg_start, int nruns);
 
 // times one stage of kb_timing while in scope
 struct StageTimer {

@@ -123,7 +123,18 @@ __global__ void hartree_kernel(const double* H, long NP, int V, long pair, doubl
 __global__ void hartree_step_kernel(const double* H, long NP, int V, int N, const StepState* st, double hfac,
                                     double Mass2, double* out);
 // one launch over all runs: workgroups [wg_start[r], wg_start[r+1]) belong to run r
+template <int ROWS, int B>
 __global__ void memint_step(const MemintArgs* runs, const int* wg_start, int nruns);
+
+// memory-integral tile shapes: ROWS slot rows per work item, B rows per
+// batch (two batches in flight); fn is the kernel to launch
+struct MemintVariant {
+  int rows, batch;
+  const void* fn;
+};
+constexpr int NMEMINT_VARIANTS = 6;
+extern const MemintVariant kMemintVariants[NMEMINT_VARIANTS];
+constexpr int MEMINT_ROW_CHOICES[3] = {32, 64, 128};  // item tables kept per context
 __global__ void advance_kernel(StepState* const* states, const int* Ns, int nruns);
 __global__ void measure_kernel(MeasArgs a);
 __global__ void history_kernel(const double* H, const double* S, long NP, int N, int tmin, int t, double* out);
@@ -148,7 +159,7 @@ hipError_t launch_sigma_planes_adjoint(const SigmaArgs& a, int rows, hipStream_t
 #define KB_MEMINT_THREADS 256
 #endif
 constexpr int MEMINT_THREADS = KB_MEMINT_THREADS;
-constexpr int MEMINT_ROWS = 64;  // rows per work item
+constexpr int MEMINT_ROWS = 64;  // rows per work item of the adjoint kernels
 constexpr int MEAS_THREADS = 256;
 
 }  // namespace qftkb

@@ -316,8 +316,9 @@ __global__ void advance_kernel(StepState* const* states, const int* Ns, int nrun
 // In slot space, pair (hi, lo) has time(hi) < time(lo) exactly when
 // lo < s_min <= hi; those pairs swap the roles of a and b.
 // ---------------------------------------------------------------------------
-constexpr int MI_ROWS = MEMINT_ROWS;  // slot rows per work item (one column block chunk)
-constexpr int MI_B = 4;      // rows per batch; two batches are in flight
+// Tile shape (template parameters, picked per lattice and window by the
+// autotuner): ROWS slot rows per work item (one chunk of a column block),
+// B rows per batch, two batches in flight.
 
 __device__ __forceinline__ void slot_weights(int x, int smin, int N, int L, double& w, double& m) {
   const int rel = (x - smin + N) % N;
@@ -340,17 +341,19 @@ __device__ __forceinline__ double2 ld_stream(const double* p) {
 #endif
 }
 
+template <int B>
 struct RowBatch {
-  double2 e[MI_B][CB];  // (F, r) of slots (row, c0 + jc)
-  double2 fs[MI_B];     // (SigF, Sigr) of the row slot
+  double2 e[B][CB];  // (F, r) of slots (row, c0 + jc)
+  double2 fs[B];     // (SigF, Sigr) of the row slot
 };
 
 // rows r .. r+MI_B-1 of the column block starting at storage slot cbase;
 // rows past rlast are clamped to rlast (re-read, not used)
-__device__ __forceinline__ void load_batch(RowBatch& b, const double* __restrict__ Hb,
+template <int B>
+__device__ __forceinline__ void load_batch(RowBatch<B>& b, const double* __restrict__ Hb,
                                            const double2* __restrict__ Sb, long cbase, int c0, int r, int rlast) {
 #pragma unroll
-  for (int h = 0; h < MI_B; ++h) {
+  for (int h = 0; h < B; ++h) {
     const int rr = min(r + h, rlast);
     const long pb = cbase + (long)(rr - c0) * CB;
 #pragma unroll
@@ -375,10 +378,11 @@ __device__ __forceinline__ void pair_update(double Fv, double Pv, bool n, double
   cR -= (n ? 0.0 : msr) * Pv;
 }
 
-__device__ __forceinline__ void process_batch(const RowBatch& b, int r, int r1, int dead, int smin, int N, int L,
+template <int B>
+__device__ __forceinline__ void process_batch(const RowBatch<B>& b, int r, int r1, int dead, int smin, int N, int L,
                                               ColState& cs, double* accF, double* accR, int lane) {
 #pragma unroll
-  for (int h = 0; h < MI_B; ++h) {
+  for (int h = 0; h < B; ++h) {
     const int rr = r + h;
     if (rr < r1) {
       const bool okr = rr != dead;
@@ -399,6 +403,7 @@ __device__ __forceinline__ void process_batch(const RowBatch& b, int r, int r1, 
   }
 }
 
+template <int ROWS, int B>
 __global__ __launch_bounds__(MEMINT_THREADS, 2) void memint_step(const MemintArgs* __restrict__ runs,
                                                                  const int* __restrict__ wg_start, int nruns) {
   extern __shared__ double smem[];
@@ -444,8 +449,8 @@ __global__ __launch_bounds__(MEMINT_THREADS, 2) void memint_step(const MemintArg
       const int2 jk = a.items[item];
       const int J = jk.x;
       const int c0 = J * CB;
-      const int r0 = c0 + jk.y * MI_ROWS;
-      const int r1 = min(r0 + MI_ROWS, Rmax);
+      const int r0 = c0 + jk.y * ROWS;
+      const int r1 = min(r0 + ROWS, Rmax);
       if (c0 < Rmax && r0 < r1) {
         const long cbase = colblock_offset(J, N);
         ColState cs;
@@ -492,16 +497,16 @@ __global__ __launch_bounds__(MEMINT_THREADS, 2) void memint_step(const MemintArg
         // full rows: two batches of MI_B rows in flight
         if (r < r1) {
           const int rlast = r1 - 1;
-          RowBatch A, B;
-          load_batch(A, Hb, Sb, cbase, c0, r, rlast);
+          RowBatch<B> X, Y;
+          load_batch(X, Hb, Sb, cbase, c0, r, rlast);
           while (true) {
-            load_batch(B, Hb, Sb, cbase, c0, r + MI_B, rlast);
-            process_batch(A, r, r1, dead, smin, N, L, cs, accF, accR, lane);
-            r += MI_B;
+            load_batch(Y, Hb, Sb, cbase, c0, r + B, rlast);
+            process_batch(X, r, r1, dead, smin, N, L, cs, accF, accR, lane);
+            r += B;
             if (r >= r1) break;
-            load_batch(A, Hb, Sb, cbase, c0, r + MI_B, rlast);
-            process_batch(B, r, r1, dead, smin, N, L, cs, accF, accR, lane);
-            r += MI_B;
+            load_batch(X, Hb, Sb, cbase, c0, r + B, rlast);
+            process_batch(Y, r, r1, dead, smin, N, L, cs, accF, accR, lane);
+            r += B;
             if (r >= r1) break;
           }
         }
@@ -588,6 +593,13 @@ __global__ __launch_bounds__(MEMINT_THREADS, 2) void memint_step(const MemintArg
     }
   }
 }
+
+// the tile shapes the autotuner chooses from; entry 0 is the default
+const MemintVariant kMemintVariants[NMEMINT_VARIANTS] = {
+    {64, 4, (const void*)memint_step<64, 4>},  {32, 4, (const void*)memint_step<32, 4>},
+    {128, 4, (const void*)memint_step<128, 4>}, {64, 2, (const void*)memint_step<64, 2>},
+    {128, 2, (const void*)memint_step<128, 2>}, {32, 2, (const void*)memint_step<32, 2>},
+};
 
 // ---------------------------------------------------------------------------
 // Measurement (createMeasurement / getE_kin / getE_pot / getE_mem)

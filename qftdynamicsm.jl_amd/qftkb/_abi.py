@@ -7,6 +7,7 @@ fail loudly.
 from __future__ import annotations
 
 import ctypes
+import json
 import os
 
 import numpy as np
@@ -35,6 +36,7 @@ EXPORTED = (
     "kb_reset_timing", "kb_use_graph", "kb_ens_create", "kb_ens_destroy", "kb_ens_last_error", "kb_ens_member",
     "kb_ens_step", "kb_ens_use_graph", "kb_ens_synchronize", "kb_ens_enable_timing", "kb_ens_timing",
     "kb_adj_record", "kb_adj_gradient", "kb_adj_info", "kb_adj_stop", "kb_adj_debug_stop", "kb_adj_get_entry",
+    "kb_tile_shape", "kb_set_tiles", "kb_get_tiles", "kb_autotune",
 )
 
 
@@ -101,6 +103,10 @@ def load_library(path: str = LIB_PATH):
                             ctypes.POINTER(ctypes.c_longlong)]
     L.kb_reset_timing.argtypes = [vp]
     L.kb_use_graph.argtypes = [vp, ctypes.c_int]
+    L.kb_tile_shape.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.kb_set_tiles.argtypes = [vp, ctypes.c_int]
+    L.kb_get_tiles.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    L.kb_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp]
     L.kb_adj_record.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.kb_adj_gradient.argtypes = [vp, _dp, _dp, _dp, _dp]
     L.kb_adj_info.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
@@ -119,6 +125,25 @@ def load_library(path: str = LIB_PATH):
     L.kb_ens_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)]
     _lib = L
     return L
+
+
+TILES_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tiles.json")
+
+
+def tiles_key(Nx, sdim, NstepsinMemory):
+    return f"{int(Nx)}^{int(sdim)} N_mem={int(NstepsinMemory)}"
+
+
+def committed_tiles(Nx, sdim, NstepsinMemory):
+    """Tile shape chosen by the autotuner for this lattice and window
+    (tiles.json beside the library, written from `bench.py --autotune`)."""
+    try:
+        with open(TILES_PATH) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = table.get("choices", {}).get(tiles_key(Nx, sdim, NstepsinMemory))
+    return None if e is None else int(e["variant"])
 
 
 def _ptr(a):
@@ -144,7 +169,7 @@ class KBContext:
     """One device-resident KB simulation (a kb_ctx)."""
 
     def __init__(self, Nx, sdim, NstepsinMemory, dt, Lambda, ONgroup, Mass, Mass2,
-                 pexp=KB_PEXP_LAMBDA_NLO, device=0, _handle=None, _owner=None):
+                 pexp=KB_PEXP_LAMBDA_NLO, device=0, tiles="auto", _handle=None, _owner=None):
         self._L = load_library()
         p = make_params(Nx, sdim, NstepsinMemory, dt, Lambda, ONgroup, Mass, Mass2, pexp, device)
         if _handle is None:
@@ -158,6 +183,10 @@ class KBContext:
         self._owner = _owner  # ensemble members are destroyed with their ensemble
         self.params = p
         self.V = int(Nx) ** int(sdim)
+        if _owner is None and tiles is not None:
+            v = committed_tiles(Nx, sdim, NstepsinMemory) if tiles == "auto" else int(tiles)
+            if v is not None:
+                self.set_tiles(v)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -270,6 +299,33 @@ class KBContext:
 
     def reset_timing(self):
         self._chk(self._L.kb_reset_timing(self._h))
+
+    # ---- tile shapes --------------------------------------------------
+    def tile_shapes(self):
+        """[(rows per work item, rows per batch)] of the memory-integral kernel."""
+        out = []
+        r, b = ctypes.c_int(), ctypes.c_int()
+        while self._L.kb_tile_shape(len(out), ctypes.byref(r), ctypes.byref(b)) == 0:
+            out.append((r.value, b.value))
+        return out
+
+    def set_tiles(self, variant):
+        self._chk(self._L.kb_set_tiles(self._h, int(variant)))
+
+    def tiles(self):
+        v = ctypes.c_int()
+        self._chk(self._L.kb_get_tiles(self._h, ctypes.byref(v)))
+        return v.value
+
+    def autotune(self, reps=3):
+        """Re-time the last step's memory integral in every tile shape and keep
+        the fastest.  Returns {"variant", "rows", "batch", "ms": [...]}."""
+        shapes = self.tile_shapes()
+        ms = np.zeros(len(shapes))
+        best = ctypes.c_int()
+        self._chk(self._L.kb_autotune(self._h, int(reps), ctypes.byref(best), _ptr(ms)))
+        r, b = shapes[best.value]
+        return {"variant": best.value, "rows": r, "batch": b, "ms": ms.tolist()}
 
     # ---- adjoint ------------------------------------------------------
     def adj_record(self, checkpoint_every=0, max_steps=None):
