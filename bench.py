@@ -295,9 +295,12 @@ def autotune_bench(out_path):
         c.step(N + 2)
         r = c.autotune(reps=3)
         r["shapes"] = c.tile_shapes()
+        r["fft"] = c.autotune_fft(reps=3)
+        r["fft"]["shapes"] = c.fft_shapes()
         res["choices"][tiles_key(Nx, sd, N)] = r
         print(f"[bench] autotune {tiles_key(Nx, sd, N)}: rows {r['rows']} batch {r['batch']} "
-              f"ms {[round(x, 3) for x in r['ms']]}", file=sys.stderr, flush=True)
+              f"ms {[round(x, 3) for x in r['ms']]}; fft threads {r['fft']['threads']} "
+              f"ms {[round(x, 3) for x in r['fft']['ms']]}", file=sys.stderr, flush=True)
         c.close()
     with open(out_path, "w") as f:
         json.dump(res, f, indent=1)
@@ -354,7 +357,7 @@ def main():
     N = w["NstepsinMemory"]
     V = w["Nx"] ** w["sdim"]
     ctx = KBContext(w["Nx"], w["sdim"], N, w["dt"], w["Lambda"], w["ONgroup"], w["Mass"], w["Mass"] ** 2)
-    tile_shape = ctx.tile_shapes()[ctx.tiles()]
+    tile_shape = list(ctx.tile_shapes()[ctx.tiles()]) + [ctx.fft_shapes()[ctx.fft_tiles()]]
     n = thermal_n(ctx.kL2(), w["Mass"], w["T"])
     ctx.fill_free_window(N, n, w["Mass"])
     print(f"[bench] window filled: {w['Nx']}^3 N_mem={N}, device bytes {ctx.device_bytes() / 1e9:.2f} GB",
@@ -410,7 +413,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "memint_step", "algorithmic_bytes": B,
                      "avg_kernel_ms": avg_mi_s * 1e3, "sigma_stage_ms": ms_fft / max(n_fft, 1),
-                     "tile_rows_batch": list(tile_shape),
+                     "tiles_rows_batch_fftthreads": tile_shape,
                      "hartree_ms": ms_h / max(n_mi, 1)},
         "cpu_baseline": None,
         "adjoint": None,

@@ -169,6 +169,12 @@ int kb_tile_shape(int variant, int *rows, int *batch); /* KB_ERR_ARG past the la
 int kb_set_tiles(kb_ctx *ctx, int variant);
 int kb_get_tiles(const kb_ctx *ctx, int *variant);
 int kb_autotune(kb_ctx *ctx, int reps, int *best, double *ms);
+/* The same for the self-energy stage's plane kernels (threads per
+ * workgroup); re-running the stage recomputes the same self-energies. */
+int kb_fft_shape(int variant, int *threads); /* KB_ERR_ARG past the last */
+int kb_set_fft_tiles(kb_ctx *ctx, int variant);
+int kb_get_fft_tiles(const kb_ctx *ctx, int *variant);
+int kb_autotune_fft(kb_ctx *ctx, int reps, int *best, double *ms);
 
 /* ---- adjoint ---------------------------------------------------------------
  * Gradients of L = sum_p wgt(p) n(p, T) (n as in kb_measure at T = tmax,

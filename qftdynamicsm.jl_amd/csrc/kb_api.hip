@@ -134,7 +134,7 @@ int launch_sigma(kb_ctx* c, double2* X, double* S_out) {
   a.X = X;
   const int L = rows;
   if (sigma_planes_supported(c->p.Nx)) {
-    hipError_t e = launch_sigma_planes(a, L, c->stream);
+    hipError_t e = launch_sigma_planes(a, L, c->stream, c->fft_tile);
     if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "self-energy launch failed: %s", hipGetErrorString(e));
     return KB_OK;
   }
@@ -758,6 +758,64 @@ int kb_autotune(kb_ctx* c, int reps, int* best, double* ms) {
   if (best) *best = bv;
   if (ms)
     for (int v = 0; v < NMEMINT_VARIANTS; ++v) ms[v] = t[v];
+  return KB_OK;
+}
+
+int kb_fft_shape(int variant, int* threads) {
+  if (variant < 0 || variant >= NFFT_VARIANTS) return KB_ERR_ARG;
+  if (threads) *threads = kFFTThreads[variant];
+  return KB_OK;
+}
+
+int kb_set_fft_tiles(kb_ctx* c, int variant) {
+  if (!c || variant < 0 || variant >= NFFT_VARIANTS) return KB_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->fft_tile = variant;
+  drop_graph(c);
+  return KB_OK;
+}
+
+int kb_get_fft_tiles(const kb_ctx* c, int* variant) {
+  if (!c || !variant) return KB_ERR_ARG;
+  *variant = c->fft_tile;
+  return KB_OK;
+}
+
+int kb_autotune_fft(kb_ctx* c, int reps, int* best, double* ms) {
+  if (!c || reps < 1) return KB_ERR_ARG;
+  if (!c->initialised || c->st.tmax < 3)
+    return seterr(c, KB_ERR_STATE, "autotune: take a step first (the last step's self-energy is re-timed)");
+  if (c->in_ensemble || c->adj) return seterr(c, KB_ERR_STATE, "autotune: not for ensemble members or while recording");
+  if (c->p.pexp != KB_PEXP_LAMBDA_NLO || !sigma_planes_supported(c->p.Nx))
+    return seterr(c, KB_ERR_ARG, "autotune: no plane self-energy stage for this context");
+  std::vector<double> t(NFFT_VARIANTS, 0.0);
+  hipEvent_t a = take_event(c), b = take_event(c);
+  if (!a || !b) return seterr(c, KB_ERR_HIP, "autotune: no events");
+  int rc = KB_OK;
+  for (int v = 0; v < NFFT_VARIANTS && rc == KB_OK; ++v) {
+    c->fft_tile = v;
+    // recomputes Sigma(t-1, .) of the last step: the same values
+    for (int k = 0; k <= reps && rc == KB_OK; ++k) {
+      if (k == 1 && hipEventRecord(a, c->stream) != hipSuccess) rc = seterr(c, KB_ERR_HIP, "autotune: event");
+      if (rc == KB_OK) rc = launch_sigma(c);
+    }
+    if (rc) break;
+    float e_ms = 0.f;
+    if (hipEventRecord(b, c->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+        hipEventElapsedTime(&e_ms, a, b) != hipSuccess)
+      rc = seterr(c, KB_ERR_HIP, "autotune: timing failed");
+    t[v] = e_ms / reps;
+  }
+  c->freeEv.push_back(a);
+  c->freeEv.push_back(b);
+  if (rc) return rc;
+  int bv = 0;
+  for (int v = 1; v < NFFT_VARIANTS; ++v)
+    if (t[v] < t[bv]) bv = v;
+  if ((rc = kb_set_fft_tiles(c, bv))) return rc;
+  if (best) *best = bv;
+  if (ms)
+    for (int v = 0; v < NFFT_VARIANTS; ++v) ms[v] = t[v];
   return KB_OK;
 }
 

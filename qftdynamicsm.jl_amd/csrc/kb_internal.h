@@ -53,6 +53,7 @@ struct kb_ctx {
   int2* items_r[3] = {};  // work items per row-chunk choice (MEMINT_ROW_CHOICES)
   int nitems_r[3] = {};
   int tile = 0;           // memory-integral tile shape (kMemintVariants index)
+  int fft_tile = 0;       // self-energy plane kernels' block size (kFFTThreads index)
   // device-resident step state and launch tables (graph-capturable steps)
   qftkb::StepState* st_dev = nullptr;
   qftkb::StepState** stptr_dev = nullptr;  // {st_dev}

@@ -152,7 +152,10 @@ __global__ void finite_kernel(const double* H, size_t count, int* flag);
 // self-energy stage on whole planes (kb_sigma.hip), Nx <= 64
 bool sigma_planes_supported(int Nx);
 hipError_t sigma_planes_prepare(int n);
-hipError_t launch_sigma_planes(const SigmaArgs& a, int rows, hipStream_t st);
+// variant: index into kFFTThreads (threads per workgroup of the plane kernels)
+constexpr int NFFT_VARIANTS = 3;
+extern const int kFFTThreads[NFFT_VARIANTS];
+hipError_t launch_sigma_planes(const SigmaArgs& a, int rows, hipStream_t st, int variant = 0);
 hipError_t launch_sigma_planes_adjoint(const SigmaArgs& a, int rows, hipStream_t st);
 
 #ifndef KB_MEMINT_THREADS

@@ -238,8 +238,8 @@ __device__ __forceinline__ void emit(const SigmaArgs& a, int s, int slot, double
 
 // ---------------------------------------------------------------------------
 // blockIdx.x: plane iz (3D: 0 .. nz/2), blockIdx.y: row q (t' = tmin + q)
-template <int NT, bool ADJ>
-__global__ __launch_bounds__(256) void sigma_plane_fwd(SigmaArgs a) {
+template <int NT, bool ADJ, int TPB>
+__global__ __launch_bounds__(TPB) void sigma_plane_fwd(SigmaArgs a) {
   extern __shared__ double2 lds[];
   const int n = NT > 0 ? NT : a.n;
   const int ny = a.ny, nz = a.nz;
@@ -310,8 +310,8 @@ __global__ __launch_bounds__(256) void sigma_plane_fwd(SigmaArgs a) {
 }
 
 // blockIdx.x: y row iy, blockIdx.y: row q.  Lines along z for every x.
-template <int NT, bool ADJ>
-__global__ __launch_bounds__(256) void sigma_zpass(SigmaArgs a) {
+template <int NT, bool ADJ, int TPB>
+__global__ __launch_bounds__(TPB) void sigma_zpass(SigmaArgs a) {
   extern __shared__ double2 lds[];
   const int n = NT > 0 ? NT : a.n;
   const int ny = a.ny, nz = a.nz;
@@ -342,8 +342,8 @@ __global__ __launch_bounds__(256) void sigma_zpass(SigmaArgs a) {
 }
 
 // blockIdx.x: plane iz, blockIdx.y: row q.  Inverse 2D transform + scatter.
-template <int NT, bool ADJ>
-__global__ __launch_bounds__(256) void sigma_plane_inv(SigmaArgs a) {
+template <int NT, bool ADJ, int TPB>
+__global__ __launch_bounds__(TPB) void sigma_plane_inv(SigmaArgs a) {
   extern __shared__ double2 lds[];
   const int n = NT > 0 ? NT : a.n;
   const int ny = a.ny;
@@ -374,41 +374,43 @@ __global__ __launch_bounds__(256) void sigma_plane_inv(SigmaArgs a) {
 // ---------------------------------------------------------------------------
 // host-side launch of the whole stage
 // ---------------------------------------------------------------------------
-template <int NT, bool ADJ>
+template <int NT, bool ADJ, int TPB>
 static hipError_t launch_planes_t(const SigmaArgs& a, int L, hipStream_t st) {  // L: grid rows
   const int n = a.n;
   const size_t lds_plane = sizeof(double2) * 2 * (size_t)a.ny * (n + 1);
   if (a.nz == 1) {
-    hipLaunchKernelGGL((sigma_plane_fwd<NT, ADJ>), dim3(1, L), dim3(256), lds_plane, st, a);
+    hipLaunchKernelGGL((sigma_plane_fwd<NT, ADJ, TPB>), dim3(1, L), dim3(TPB), lds_plane, st, a);
     return hipGetLastError();
   }
   const size_t lds_z = sizeof(double2) * 2 * (size_t)n * (a.nz + 1);
-  hipLaunchKernelGGL((sigma_plane_fwd<NT, ADJ>), dim3(a.nz / 2 + 1, L), dim3(256), lds_plane, st, a);
+  hipLaunchKernelGGL((sigma_plane_fwd<NT, ADJ, TPB>), dim3(a.nz / 2 + 1, L), dim3(TPB), lds_plane, st, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((sigma_zpass<NT, ADJ>), dim3(a.ny, L), dim3(256), lds_z, st, a);
+  hipLaunchKernelGGL((sigma_zpass<NT, ADJ, TPB>), dim3(a.ny, L), dim3(TPB), lds_z, st, a);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((sigma_plane_inv<NT, ADJ>), dim3(a.nz, L), dim3(256), lds_plane, st, a);
+  hipLaunchKernelGGL((sigma_plane_inv<NT, ADJ, TPB>), dim3(a.nz, L), dim3(TPB), lds_plane, st, a);
   return hipGetLastError();
 }
 
 bool sigma_planes_supported(int Nx) { return Nx >= 1 && Nx <= 64; }
 
-template <int NT, bool ADJ>
+template <int NT, bool ADJ, int TPB>
 static hipError_t prepare_one(size_t lds) {
-  hipError_t e = hipFuncSetAttribute((const void*)sigma_plane_fwd<NT, ADJ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipError_t e = hipFuncSetAttribute((const void*)sigma_plane_fwd<NT, ADJ, TPB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void*)sigma_zpass<NT, ADJ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  e = hipFuncSetAttribute((const void*)sigma_zpass<NT, ADJ, TPB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void*)sigma_plane_inv<NT, ADJ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  return hipFuncSetAttribute((const void*)sigma_plane_inv<NT, ADJ, TPB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
 template <int NT>
 static hipError_t prepare_t(size_t lds) {
-  hipError_t e = prepare_one<NT, false>(lds);
+  hipError_t e = prepare_one<NT, false, 256>(lds);
+  if (e == hipSuccess) e = prepare_one<NT, false, 128>(lds);
+  if (e == hipSuccess) e = prepare_one<NT, false, 512>(lds);
   if (e != hipSuccess) return e;
-  return prepare_one<NT, true>(lds);
+  return prepare_one<NT, true, 256>(lds);
 }
 
 // allow the dynamic LDS of the plane kernels for lattice size n (the
@@ -428,24 +430,33 @@ hipError_t sigma_planes_prepare(int n) {
   }
 }
 
-template <bool ADJ>
+template <bool ADJ, int TPB>
 static hipError_t launch_planes_n(const SigmaArgs& a, int L, hipStream_t st) {  // L: grid rows
   switch (a.n) {
-    case 8: return launch_planes_t<8, ADJ>(a, L, st);
-    case 16: return launch_planes_t<16, ADJ>(a, L, st);
-    case 24: return launch_planes_t<24, ADJ>(a, L, st);
-    case 32: return launch_planes_t<32, ADJ>(a, L, st);
-    case 40: return launch_planes_t<40, ADJ>(a, L, st);
-    case 48: return launch_planes_t<48, ADJ>(a, L, st);
-    case 64: return launch_planes_t<64, ADJ>(a, L, st);
-    default: return launch_planes_t<0, ADJ>(a, L, st);
+    case 8: return launch_planes_t<8, ADJ, TPB>(a, L, st);
+    case 16: return launch_planes_t<16, ADJ, TPB>(a, L, st);
+    case 24: return launch_planes_t<24, ADJ, TPB>(a, L, st);
+    case 32: return launch_planes_t<32, ADJ, TPB>(a, L, st);
+    case 40: return launch_planes_t<40, ADJ, TPB>(a, L, st);
+    case 48: return launch_planes_t<48, ADJ, TPB>(a, L, st);
+    case 64: return launch_planes_t<64, ADJ, TPB>(a, L, st);
+    default: return launch_planes_t<0, ADJ, TPB>(a, L, st);
   }
 }
 
-hipError_t launch_sigma_planes(const SigmaArgs& a, int L, hipStream_t st) { return launch_planes_n<false>(a, L, st); }
+// the plane kernels' block sizes the autotuner chooses from; entry 0 is the default
+const int kFFTThreads[NFFT_VARIANTS] = {256, 128, 512};
+
+hipError_t launch_sigma_planes(const SigmaArgs& a, int L, hipStream_t st, int variant) {
+  switch (variant) {
+    case 1: return launch_planes_n<false, 128>(a, L, st);
+    case 2: return launch_planes_n<false, 512>(a, L, st);
+    default: return launch_planes_n<false, 256>(a, L, st);
+  }
+}
 
 hipError_t launch_sigma_planes_adjoint(const SigmaArgs& a, int L, hipStream_t st) {
-  return launch_planes_n<true>(a, L, st);
+  return launch_planes_n<true, 256>(a, L, st);
 }
 
 }  // namespace qftkb

@@ -37,6 +37,7 @@ EXPORTED = (
     "kb_ens_step", "kb_ens_use_graph", "kb_ens_synchronize", "kb_ens_enable_timing", "kb_ens_timing",
     "kb_adj_record", "kb_adj_gradient", "kb_adj_info", "kb_adj_stop", "kb_adj_debug_stop", "kb_adj_get_entry",
     "kb_tile_shape", "kb_set_tiles", "kb_get_tiles", "kb_autotune",
+    "kb_fft_shape", "kb_set_fft_tiles", "kb_get_fft_tiles", "kb_autotune_fft",
 )
 
 
@@ -107,6 +108,10 @@ def load_library(path: str = LIB_PATH):
     L.kb_set_tiles.argtypes = [vp, ctypes.c_int]
     L.kb_get_tiles.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     L.kb_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp]
+    L.kb_fft_shape.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.kb_set_fft_tiles.argtypes = [vp, ctypes.c_int]
+    L.kb_get_fft_tiles.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    L.kb_autotune_fft.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp]
     L.kb_adj_record.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.kb_adj_gradient.argtypes = [vp, _dp, _dp, _dp, _dp]
     L.kb_adj_info.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
@@ -141,9 +146,12 @@ def committed_tiles(Nx, sdim, NstepsinMemory):
         with open(TILES_PATH) as f:
             table = json.load(f)
     except (OSError, ValueError):
-        return None
+        return None, None
     e = table.get("choices", {}).get(tiles_key(Nx, sdim, NstepsinMemory))
-    return None if e is None else int(e["variant"])
+    if e is None:
+        return None, None
+    f = e.get("fft")
+    return int(e["variant"]), (None if f is None else int(f["variant"]))
 
 
 def _ptr(a):
@@ -184,9 +192,14 @@ class KBContext:
         self.params = p
         self.V = int(Nx) ** int(sdim)
         if _owner is None and tiles is not None:
-            v = committed_tiles(Nx, sdim, NstepsinMemory) if tiles == "auto" else int(tiles)
+            if tiles == "auto":
+                v, f = committed_tiles(Nx, sdim, NstepsinMemory)
+            else:
+                v, f = int(tiles), None
             if v is not None:
                 self.set_tiles(v)
+            if f is not None:
+                self.set_fft_tiles(f)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -326,6 +339,31 @@ class KBContext:
         self._chk(self._L.kb_autotune(self._h, int(reps), ctypes.byref(best), _ptr(ms)))
         r, b = shapes[best.value]
         return {"variant": best.value, "rows": r, "batch": b, "ms": ms.tolist()}
+
+    def fft_shapes(self):
+        """[threads per workgroup] of the self-energy plane kernels."""
+        out = []
+        n = ctypes.c_int()
+        while self._L.kb_fft_shape(len(out), ctypes.byref(n)) == 0:
+            out.append(n.value)
+        return out
+
+    def set_fft_tiles(self, variant):
+        self._chk(self._L.kb_set_fft_tiles(self._h, int(variant)))
+
+    def fft_tiles(self):
+        v = ctypes.c_int()
+        self._chk(self._L.kb_get_fft_tiles(self._h, ctypes.byref(v)))
+        return v.value
+
+    def autotune_fft(self, reps=3):
+        """Re-time the last step's self-energy stage for every plane-kernel
+        block size and keep the fastest.  Returns {"variant", "threads", "ms"}."""
+        shapes = self.fft_shapes()
+        ms = np.zeros(len(shapes))
+        best = ctypes.c_int()
+        self._chk(self._L.kb_autotune_fft(self._h, int(reps), ctypes.byref(best), _ptr(ms)))
+        return {"variant": best.value, "threads": shapes[best.value], "ms": ms.tolist()}
 
     # ---- adjoint ------------------------------------------------------
     def adj_record(self, checkpoint_every=0, max_steps=None):

@@ -72,3 +72,27 @@ def test_autotune_keeps_results(reference):
         for k, tp in enumerate(range(tmin, 146)):
             assert rel(g.get(w, 145, tp), rows[145][w][k]) <= RTOL
     g.close()
+
+
+def test_every_fft_block_size_matches_oracle(reference):
+    from qftkb import KBContext
+    n, rows = reference
+    probe = KBContext(*ARGS, tiles=None)
+    shapes = probe.fft_shapes()
+    probe.close()
+    assert len(shapes) >= 2
+    for v, threads in enumerate(shapes):
+        print(f"\n[case] plane kernels with {threads} threads", flush=True)
+        g = KBContext(*ARGS, tiles=None)
+        g.set_fft_tiles(v)
+        assert g.fft_tiles() == v
+        g.init_particles(n, 1.0)
+        for t in range(3, 61):
+            g.step(t)
+        tmin = g.window()[0]
+        for w in "Fr":
+            for k, tp in enumerate(range(tmin, 61)):
+                assert rel(g.get(w, 60, tp), rows[60][w][k]) <= RTOL, (v, w, tp)
+        r = g.autotune_fft(reps=2)
+        assert 0 <= r["variant"] < len(shapes) and all(x > 0 for x in r["ms"])
+        g.close()
