@@ -276,7 +276,8 @@ def adjoint_bench(fd_bins=False, fd=True):
     return out
 
 
-AUTOTUNE_WORKLOADS = [(32, 3, 400), (24, 3, 200), (48, 3, 300), (16, 3, 202), (40, 3, 300)]
+AUTOTUNE_WORKLOADS = [(32, 3, 400), (24, 3, 200), (48, 3, 300), (16, 3, 202), (40, 3, 300),
+                      (16, 3, 300), (24, 3, 300), (32, 3, 300)]
 
 
 def autotune_bench(out_path):
@@ -299,6 +300,7 @@ def autotune_bench(out_path):
         r["fft"]["shapes"] = c.fft_shapes()
         res["choices"][tiles_key(Nx, sd, N)] = r
         print(f"[bench] autotune {tiles_key(Nx, sd, N)}: rows {r['rows']} batch {r['batch']} "
+              f"threads {r['threads']} "
               f"ms {[round(x, 3) for x in r['ms']]}; fft threads {r['fft']['threads']} "
               f"ms {[round(x, 3) for x in r['fft']['ms']]}", file=sys.stderr, flush=True)
         c.close()
@@ -413,7 +415,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "memint_step", "algorithmic_bytes": B,
                      "avg_kernel_ms": avg_mi_s * 1e3, "sigma_stage_ms": ms_fft / max(n_fft, 1),
-                     "tiles_rows_batch_fftthreads": tile_shape,
+                     "tiles_rows_batch_threads_fftthreads": tile_shape,
                      "hartree_ms": ms_h / max(n_mi, 1)},
         "cpu_baseline": None,
         "adjoint": None,

@@ -159,13 +159,14 @@ int kb_reset_timing(kb_ctx *ctx);
 
 /* ---- tile autotuning -------------------------------------------------------
  * The memory-integral kernel exists in several tile shapes (slot rows per
- * work item, rows per batch).  kb_autotune re-times the last step's memory
+ * work item, rows per batch, threads per workgroup).  kb_autotune re-times the last step's memory
  * integral in every shape (reps launches each, HIP events on the context's
  * stream; each launch recomputes row t from the same inputs, so row t may
  * change in the last bits) and keeps the fastest; ms, if not NULL, receives
  * one average per shape.  Shapes can also be set from a stored choice.  No
  * counterpart in the reference (launch configuration). */
-int kb_tile_shape(int variant, int *rows, int *batch); /* KB_ERR_ARG past the last */
+/* rows per work item, rows per batch, threads per workgroup of a shape */
+int kb_tile_shape(int variant, int *rows, int *batch, int *threads); /* KB_ERR_ARG past the last */
 int kb_set_tiles(kb_ctx *ctx, int variant);
 int kb_get_tiles(const kb_ctx *ctx, int *variant);
 int kb_autotune(kb_ctx *ctx, int reps, int *best, double *ms);

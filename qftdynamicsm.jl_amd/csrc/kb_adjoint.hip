@@ -271,6 +271,9 @@ __device__ __forceinline__ PCoef pend_coef(const AdjArgs& a, int j, int mb, int 
 // ---------------------------------------------------------------------------
 constexpr int ROW_THREADS = 1024;
 
+// NP: number of deferred steps after t (a compile-time count, so only
+// their rings are read)
+template <int NP>
 __global__ __launch_bounds__(ROW_THREADS) void adj_row_kernel(AdjArgs a) {
   __shared__ double red[ROW_THREADS / 64];
   __shared__ double fl[2][MB];  // f_bar at t-1 and t-2
@@ -304,10 +307,10 @@ __global__ __launch_bounds__(ROW_THREADS) void adj_row_kernel(AdjArgs a) {
   const size_t pstride = (size_t)MB * 2;  // doubles per slot pair of one site block
 
   // deferred memory-integral terms of the steps after t that touch row t
-  PCoef pt[AK];
+  PCoef pt[NP > 0 ? NP : 1];
   double Fbtt = Hbs[pair_off(st.s_t, st.s_t, N) * pstride] + a.hl_ivol * Mb1;
 #pragma unroll
-  for (int j = 0; j < AK; ++j) {
+  for (int j = 0; j < NP; ++j) {
     pt[j] = pend_coef(a, j, mb, lane, st.s_t, t);
     Fbtt -= pt[j].u * pt[j].rf;
   }
@@ -328,14 +331,14 @@ __global__ __launch_bounds__(ROW_THREADS) void adj_row_kernel(AdjArgs a) {
     const double2 sg = a.integral ? Ss[(size_t)x * MB] : make_double2(0.0, 0.0);
     double2 v1 = push1 ? ld2(e1p) : make_double2(0.0, 0.0);
     double2 v2 = push2 ? ld2(e2p) : make_double2(0.0, 0.0);
-    PCoef px[AK];
+    PCoef px[NP > 0 ? NP : 1];
 #pragma unroll
-    for (int j = 0; j < AK; ++j) px[j] = pend_coef(a, j, mb, lane, x, tx);
+    for (int j = 0; j < NP; ++j) px[j] = pend_coef(a, j, mb, lane, x, tx);
     {
       // complete F_bar(t,tx), rho_bar(t,tx) (stored sign) with the deferred terms
       const double sgt = st.s_t >= x ? 1.0 : -1.0;
 #pragma unroll
-      for (int j = 0; j < AK; ++j) {
+      for (int j = 0; j < NP; ++j) {
         hb.x -= pt[j].u * px[j].rf + px[j].u * pt[j].rf;
         hb.y -= sgt * (px[j].v * pt[j].rf + pt[j].ms * px[j].rr);
       }
@@ -564,14 +567,75 @@ __global__ __launch_bounds__(AM_THREADS, 2) void adj_reduce_kernel(AdjArgs a) {
 // One workgroup per site block walks the triangle in chunks of AP_ROWS
 // absolute slot rows.  The row coefficients of the chunk (v, ms, rf, rr per
 // deferred step; u = w ms with w = 1/2 at that step's tmin) are staged in
-// LDS once; 8-lane workers then take the chunk's 4-column blocks, keep the
-// columns' coefficients in registers and stream the rows (two rows in
-// flight) with one read-modify-write of each adjoint line.
+// LDS once.  8-lane workers then take half column blocks (2 of the 4
+// columns), keep those columns' coefficients in registers and stream the
+// rows with AP_D rows in flight, one read-modify-write per adjoint line.
 constexpr int AP_THREADS = 256;
 constexpr int AP_ROWS = 64;
+constexpr int AP_C = 2;  // columns per worker
+constexpr int AP_D = 4;  // rows per batch (two batches in flight)
 
 __device__ __forceinline__ size_t ap_lds(int row, int j, int q, int lane) {  // q: 0 v, 1 ms, 2 rf, 3 rr
   return (((size_t)row * AK + j) * 4 + q) * MB + lane;
+}
+
+struct ApCols {
+  double v[AK][AP_C], ms[AK][AP_C], rf[AK][AP_C], rr[AK][AP_C];
+  unsigned half;  // bit j*AP_C + k: w = 1/2 for column k in step j's window
+  unsigned phi;   // bit k: column k lies before s_min
+};
+
+__device__ __forceinline__ void ap_load(double2 (&e)[AP_D][AP_C], const double* Hbs, long cbase, int c0, int k0,
+                                        int r, int rlast) {
+#pragma unroll
+  for (int d = 0; d < AP_D; ++d) {
+    const int rr = min(r + d, rlast);
+#pragma unroll
+    for (int k = 0; k < AP_C; ++k)
+      e[d][k] = ld2_nt(Hbs + (size_t)(cbase + (long)(rr - c0) * CB + k0 + k) * MB * 2);
+  }
+}
+
+__device__ __forceinline__ void ap_rows(const AdjArgs& a, double2 (&e)[AP_D][AP_C], double* Hbs, const double* rco,
+                                        long cbase, int c0, int k0, int r, int r1, int R0, const ApCols& cs,
+                                        int lane) {
+  const StepState& st = a.st;
+  const int N = a.N;
+#pragma unroll
+  for (int d = 0; d < AP_D; ++d) {
+    const int rr = r + d;
+    if (rr < r1) {
+      const int tx = st.tmin + (rr - st.s_min + N) % N;
+      const unsigned neg = rr >= st.s_min ? cs.phi : 0u;
+      const int row = rr - R0;
+#pragma unroll
+      for (int j = 0; j < AK; ++j) {
+        if (j < a.npend) {
+          const double rv = rco[ap_lds(row, j, 0, lane)];
+          const double rms = rco[ap_lds(row, j, 1, lane)];
+          const double rrf = rco[ap_lds(row, j, 2, lane)];
+          const double rrr = rco[ap_lds(row, j, 3, lane)];
+          const double ru = (tx == a.ptmin[j] ? 0.5 : 1.0) * rms;
+#pragma unroll
+          for (int k = 0; k < AP_C; ++k) {
+            const int c = c0 + k0 + k;
+            if (c < rr) {
+              const double nf = ((neg >> k) & 1u) ? 1.0 : 0.0, pf = 1.0 - nf;
+              const double uc = ((cs.half >> (j * AP_C + k)) & 1u ? 0.5 : 1.0) * cs.ms[j][k];
+              e[d][k].x -= ru * cs.rf[j][k] + uc * rrf;
+              e[d][k].y += nf * (rv * cs.rf[j][k] + cs.ms[j][k] * rrr) - pf * (cs.v[j][k] * rrf + rms * cs.rr[j][k]);
+            } else if (c == rr) {
+              e[d][k].x -= ru * rrf;
+            }
+          }
+        }
+      }
+      const long pb = cbase + (long)(rr - c0) * CB + k0;
+#pragma unroll
+      for (int k = 0; k < AP_C; ++k)
+        if (c0 + k0 + k <= rr) st2_nt(Hbs + (size_t)(pb + k) * MB * 2, e[d][k]);
+    }
+  }
 }
 
 __global__ __launch_bounds__(AP_THREADS, 2) void adj_apply_kernel(AdjArgs a) {
@@ -587,7 +651,6 @@ __global__ __launch_bounds__(AP_THREADS, 2) void adj_apply_kernel(AdjArgs a) {
   const int lane = tid & (MB - 1);
   const int worker = tid >> 3;
   const int nworkers = AP_THREADS >> 3;
-  const int np = a.npend;
   double* Hbs = a.Hbar + hoff(mb, 0, lane, a.NP);
   for (int R0 = 0; R0 < Rv; R0 += AP_ROWS) {
     const int R1 = min(R0 + AP_ROWS, Rv);
@@ -604,75 +667,53 @@ __global__ __launch_bounds__(AP_THREADS, 2) void adj_apply_kernel(AdjArgs a) {
     if (tid == 0) next_item = nworkers;
     __syncthreads();
     const int nJ = (R1 - 1) / CB + 1;  // column blocks with rows in the chunk
-    int J = worker;
-    while (J < nJ) {
+    const int nitem = nJ * (CB / AP_C);
+    int item = worker;
+    while (item < nitem) {
+      const int J = item / (CB / AP_C);
+      const int k0 = (item - J * (CB / AP_C)) * AP_C;
       const int c0 = J * CB;
-      const int r0 = max(R0, c0);
-      const long cbase = colblock_offset(J, N);
-      double cv[AK][CB], cms[AK][CB], crf[AK][CB], crr[AK][CB];
-      unsigned half = 0, phi = 0;
+      const int r0 = max(R0, c0 + k0);  // rows from the diagonal of the first column on
+      if (c0 + k0 < Rv && r0 < R1) {
+        const long cbase = colblock_offset(J, N);
+        ApCols cs;
+        cs.half = 0;
+        cs.phi = 0;
 #pragma unroll
-      for (int jc = 0; jc < CB; ++jc) {
-        const int c = c0 + jc;
-        const int cc = min(c, Rv - 1);
-        const int txc = st.tmin + (cc - smin + N) % N;
+        for (int k = 0; k < AP_C; ++k) {
+          const int c = c0 + k0 + k;
+          const int cc = min(c, Rv - 1);
+          const int txc = st.tmin + (cc - smin + N) % N;
 #pragma unroll
-        for (int j = 0; j < AK; ++j) {
-          const PCoef k = pend_coef(a, j, mb, lane, cc, txc);
-          const bool ok = c < Rv;
-          if (txc == a.ptmin[j]) half |= 1u << (j * CB + jc);
-          cv[j][jc] = ok ? k.v : 0.0;
-          cms[j][jc] = ok ? k.ms : 0.0;
-          crf[j][jc] = ok ? k.rf : 0.0;
-          crr[j][jc] = ok ? k.rr : 0.0;
-        }
-        phi |= (c < smin ? 1u : 0u) << jc;
-      }
-      double2 nxt[CB];
-#pragma unroll
-      for (int jc = 0; jc < CB; ++jc) nxt[jc] = ld2_nt(Hbs + (size_t)(cbase + (long)(r0 - c0) * CB + jc) * MB * 2);
-      for (int r = r0; r < R1; ++r) {
-        double2 cur[CB];
-#pragma unroll
-        for (int jc = 0; jc < CB; ++jc) cur[jc] = nxt[jc];
-        if (r + 1 < R1) {
-#pragma unroll
-          for (int jc = 0; jc < CB; ++jc)
-            nxt[jc] = ld2_nt(Hbs + (size_t)(cbase + (long)(r + 1 - c0) * CB + jc) * MB * 2);
-        }
-        const int tx = st.tmin + (r - smin + N) % N;
-        const unsigned neg = r >= smin ? phi : 0u;
-        const int row = r - R0;
-#pragma unroll
-        for (int j = 0; j < AK; ++j) {
-          if (j < np) {
-            const double rv = rco[ap_lds(row, j, 0, lane)];
-            const double rms = rco[ap_lds(row, j, 1, lane)];
-            const double rrf = rco[ap_lds(row, j, 2, lane)];
-            const double rrr = rco[ap_lds(row, j, 3, lane)];
-            const double ru = (tx == a.ptmin[j] ? 0.5 : 1.0) * rms;
-#pragma unroll
-            for (int jc = 0; jc < CB; ++jc) {
-              const int c = c0 + jc;
-              if (c < r) {
-                const double nf = ((neg >> jc) & 1u) ? 1.0 : 0.0, pf = 1.0 - nf;
-                const double uc = ((half >> (j * CB + jc)) & 1u ? 0.5 : 1.0) * cms[j][jc];
-                cur[jc].x -= ru * crf[j][jc] + uc * rrf;
-                cur[jc].y += nf * (rv * crf[j][jc] + cms[j][jc] * rrr) - pf * (cv[j][jc] * rrf + rms * crr[j][jc]);
-              } else if (c == r) {
-                cur[jc].x -= ru * rrf;
-              }
-            }
+          for (int j = 0; j < AK; ++j) {
+            const PCoef q = pend_coef(a, j, mb, lane, cc, txc);
+            const bool ok = c < Rv;
+            if (txc == a.ptmin[j]) cs.half |= 1u << (j * AP_C + k);
+            cs.v[j][k] = ok ? q.v : 0.0;
+            cs.ms[j][k] = ok ? q.ms : 0.0;
+            cs.rf[j][k] = ok ? q.rf : 0.0;
+            cs.rr[j][k] = ok ? q.rr : 0.0;
           }
+          cs.phi |= (c < smin ? 1u : 0u) << k;
         }
-        const long pb = cbase + (long)(r - c0) * CB;
-#pragma unroll
-        for (int jc = 0; jc < CB; ++jc)
-          if (c0 + jc <= r) st2_nt(Hbs + (size_t)(pb + jc) * MB * 2, cur[jc]);
+        const int rlast = R1 - 1;
+        double2 X[AP_D][AP_C], Y[AP_D][AP_C];
+        int r = r0;
+        ap_load(X, Hbs, cbase, c0, k0, r, rlast);
+        while (true) {
+          ap_load(Y, Hbs, cbase, c0, k0, r + AP_D, rlast);
+          ap_rows(a, X, Hbs, rco, cbase, c0, k0, r, R1, R0, cs, lane);
+          r += AP_D;
+          if (r >= R1) break;
+          ap_load(X, Hbs, cbase, c0, k0, r + AP_D, rlast);
+          ap_rows(a, Y, Hbs, rco, cbase, c0, k0, r, R1, R0, cs, lane);
+          r += AP_D;
+          if (r >= R1) break;
+        }
       }
       int nx = 0;
       if (lane == 0) nx = atomicAdd(&next_item, 1);
-      J = __shfl(nx, tid & ~(MB - 1) & 63);
+      item = __shfl(nx, tid & ~(MB - 1) & 63);
     }
     __syncthreads();
   }
@@ -880,7 +921,13 @@ int reverse_step(kb_ctx* c, int t, int T, int npend, bool apply) {
                        pair_off(a.st.s_tm1, a.st.s_tm1, N), H / (double)c->V, c->p.Mass2, s->m2r + (t & 1) * 2);
     LAUNCHCHK(c);
     set_pending(c, a, t + 1, nlo ? npend : 0);
-    hipLaunchKernelGGL(adj_row_kernel, dim3(c->NB), dim3(ROW_THREADS), 0, c->stream, a);
+    static_assert(AK == 4, "adj_row_kernel instances cover 0 .. AK-1 deferred steps");
+    switch (a.npend) {
+      case 0: hipLaunchKernelGGL(adj_row_kernel<0>, dim3(c->NB), dim3(ROW_THREADS), 0, c->stream, a); break;
+      case 1: hipLaunchKernelGGL(adj_row_kernel<1>, dim3(c->NB), dim3(ROW_THREADS), 0, c->stream, a); break;
+      case 2: hipLaunchKernelGGL(adj_row_kernel<2>, dim3(c->NB), dim3(ROW_THREADS), 0, c->stream, a); break;
+      default: hipLaunchKernelGGL(adj_row_kernel<3>, dim3(c->NB), dim3(ROW_THREADS), 0, c->stream, a); break;
+    }
     LAUNCHCHK(c);
   }
   if (nlo) {

@@ -187,7 +187,7 @@ MemintArgs memint_args(const kb_ctx* c) {
 hipError_t launch_memint(int tile, unsigned grid, size_t lds, hipStream_t s, const MemintArgs* args,
                          const int* wg_start, int nruns) {
   void* kargs[] = {(void*)&args, (void*)&wg_start, (void*)&nruns};
-  return hipLaunchKernel(kMemintVariants[tile].fn, dim3(grid), dim3(MEMINT_THREADS), kargs, lds, s);
+  return hipLaunchKernel(kMemintVariants[tile].fn, dim3(grid), dim3(kMemintVariants[tile].threads), kargs, lds, s);
 }
 
 size_t memint_lds(int N) { return sizeof(double) * 2 * (size_t)N * MB; }
@@ -700,10 +700,11 @@ int kb_timing(kb_ctx* c, int stage, double* total_ms, long long* count) {
   return KB_OK;
 }
 
-int kb_tile_shape(int variant, int* rows, int* batch) {
+int kb_tile_shape(int variant, int* rows, int* batch, int* threads) {
   if (variant < 0 || variant >= NMEMINT_VARIANTS) return KB_ERR_ARG;
   if (rows) *rows = kMemintVariants[variant].rows;
   if (batch) *batch = kMemintVariants[variant].batch;
+  if (threads) *threads = kMemintVariants[variant].threads;
   return KB_OK;
 }
 

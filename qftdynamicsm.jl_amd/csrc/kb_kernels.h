@@ -123,16 +123,17 @@ __global__ void hartree_kernel(const double* H, long NP, int V, long pair, doubl
 __global__ void hartree_step_kernel(const double* H, long NP, int V, int N, const StepState* st, double hfac,
                                     double Mass2, double* out);
 // one launch over all runs: workgroups [wg_start[r], wg_start[r+1]) belong to run r
-template <int ROWS, int B>
+template <int ROWS, int B, int TPB>
 __global__ void memint_step(const MemintArgs* runs, const int* wg_start, int nruns);
 
 // memory-integral tile shapes: ROWS slot rows per work item, B rows per
-// batch (two batches in flight); fn is the kernel to launch
+// batch (two batches in flight), threads per workgroup (one workgroup per
+// site block); fn is the kernel to launch
 struct MemintVariant {
-  int rows, batch;
+  int rows, batch, threads;
   const void* fn;
 };
-constexpr int NMEMINT_VARIANTS = 6;
+constexpr int NMEMINT_VARIANTS = 9;
 extern const MemintVariant kMemintVariants[NMEMINT_VARIANTS];
 constexpr int MEMINT_ROW_CHOICES[3] = {32, 64, 128};  // item tables kept per context
 __global__ void advance_kernel(StepState* const* states, const int* Ns, int nruns);
@@ -158,10 +159,7 @@ extern const int kFFTThreads[NFFT_VARIANTS];
 hipError_t launch_sigma_planes(const SigmaArgs& a, int rows, hipStream_t st, int variant = 0);
 hipError_t launch_sigma_planes_adjoint(const SigmaArgs& a, int rows, hipStream_t st);
 
-#ifndef KB_MEMINT_THREADS
-#define KB_MEMINT_THREADS 256
-#endif
-constexpr int MEMINT_THREADS = KB_MEMINT_THREADS;
+constexpr int MEMINT_THREADS = 256;  // default workgroup of the memory-integral kernels
 constexpr int MEMINT_ROWS = 64;  // rows per work item of the adjoint kernels
 constexpr int MEAS_THREADS = 256;
 

@@ -403,8 +403,8 @@ __device__ __forceinline__ void process_batch(const RowBatch<B>& b, int r, int r
   }
 }
 
-template <int ROWS, int B>
-__global__ __launch_bounds__(MEMINT_THREADS, 2) void memint_step(const MemintArgs* __restrict__ runs,
+template <int ROWS, int B, int TPB>
+__global__ __launch_bounds__(TPB, 512 / TPB) void memint_step(const MemintArgs* __restrict__ runs,
                                                                  const int* __restrict__ wg_start, int nruns) {
   extern __shared__ double smem[];
   // which run this workgroup belongs to (runs are few; wg_start is uniform)
@@ -415,7 +415,7 @@ __global__ __launch_bounds__(MEMINT_THREADS, 2) void memint_step(const MemintArg
   const int N = a.N;
   double* accF = smem;           // R_F per slot, lane
   double* accR = accF + N * MB;  // R_r
-  __shared__ double red[MEMINT_THREADS / 64][MB];
+  __shared__ double red[TPB / 64][MB];
   __shared__ double diagF[2][MB];
   __shared__ int next_item;
 
@@ -596,9 +596,11 @@ __global__ __launch_bounds__(MEMINT_THREADS, 2) void memint_step(const MemintArg
 
 // the tile shapes the autotuner chooses from; entry 0 is the default
 const MemintVariant kMemintVariants[NMEMINT_VARIANTS] = {
-    {64, 4, (const void*)memint_step<64, 4>},  {32, 4, (const void*)memint_step<32, 4>},
-    {128, 4, (const void*)memint_step<128, 4>}, {64, 2, (const void*)memint_step<64, 2>},
-    {128, 2, (const void*)memint_step<128, 2>}, {32, 2, (const void*)memint_step<32, 2>},
+    {64, 2, 256, (const void*)memint_step<64, 2, 256>},   {32, 4, 256, (const void*)memint_step<32, 4, 256>},
+    {128, 4, 256, (const void*)memint_step<128, 4, 256>}, {64, 4, 256, (const void*)memint_step<64, 4, 256>},
+    {128, 2, 256, (const void*)memint_step<128, 2, 256>}, {32, 2, 256, (const void*)memint_step<32, 2, 256>},
+    {64, 4, 512, (const void*)memint_step<64, 4, 512>},   {128, 2, 512, (const void*)memint_step<128, 2, 512>},
+    {32, 4, 512, (const void*)memint_step<32, 4, 512>},
 };
 
 // ---------------------------------------------------------------------------

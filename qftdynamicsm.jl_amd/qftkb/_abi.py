@@ -104,7 +104,8 @@ def load_library(path: str = LIB_PATH):
                             ctypes.POINTER(ctypes.c_longlong)]
     L.kb_reset_timing.argtypes = [vp]
     L.kb_use_graph.argtypes = [vp, ctypes.c_int]
-    L.kb_tile_shape.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.kb_tile_shape.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                ctypes.POINTER(ctypes.c_int)]
     L.kb_set_tiles.argtypes = [vp, ctypes.c_int]
     L.kb_get_tiles.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     L.kb_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp]
@@ -140,8 +141,9 @@ def tiles_key(Nx, sdim, NstepsinMemory):
 
 
 def committed_tiles(Nx, sdim, NstepsinMemory):
-    """Tile shape chosen by the autotuner for this lattice and window
-    (tiles.json beside the library, written from `bench.py --autotune`)."""
+    """(memory-integral variant, self-energy variant) chosen by the autotuner
+    for this lattice and window (tiles.json beside the library, written by
+    `bench.py --autotune`); shapes are matched by value, None if absent."""
     try:
         with open(TILES_PATH) as f:
             table = json.load(f)
@@ -150,8 +152,22 @@ def committed_tiles(Nx, sdim, NstepsinMemory):
     e = table.get("choices", {}).get(tiles_key(Nx, sdim, NstepsinMemory))
     if e is None:
         return None, None
-    f = e.get("fft")
-    return int(e["variant"]), (None if f is None else int(f["variant"]))
+    L = load_library()
+    v = f = None
+    r, b, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    i = 0
+    while L.kb_tile_shape(i, ctypes.byref(r), ctypes.byref(b), ctypes.byref(n)) == 0:
+        if (r.value, b.value, n.value) == (e.get("rows"), e.get("batch"), e.get("threads", 256)):
+            v = i
+        i += 1
+    ff = e.get("fft")
+    if ff is not None:
+        i = 0
+        while L.kb_fft_shape(i, ctypes.byref(n)) == 0:
+            if n.value == ff.get("threads"):
+                f = i
+            i += 1
+    return v, f
 
 
 def _ptr(a):
@@ -315,11 +331,11 @@ class KBContext:
 
     # ---- tile shapes --------------------------------------------------
     def tile_shapes(self):
-        """[(rows per work item, rows per batch)] of the memory-integral kernel."""
+        """[(rows per work item, rows per batch, threads)] of the memory-integral kernel."""
         out = []
-        r, b = ctypes.c_int(), ctypes.c_int()
-        while self._L.kb_tile_shape(len(out), ctypes.byref(r), ctypes.byref(b)) == 0:
-            out.append((r.value, b.value))
+        r, b, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        while self._L.kb_tile_shape(len(out), ctypes.byref(r), ctypes.byref(b), ctypes.byref(n)) == 0:
+            out.append((r.value, b.value, n.value))
         return out
 
     def set_tiles(self, variant):
@@ -337,8 +353,8 @@ class KBContext:
         ms = np.zeros(len(shapes))
         best = ctypes.c_int()
         self._chk(self._L.kb_autotune(self._h, int(reps), ctypes.byref(best), _ptr(ms)))
-        r, b = shapes[best.value]
-        return {"variant": best.value, "rows": r, "batch": b, "ms": ms.tolist()}
+        r, b, n = shapes[best.value]
+        return {"variant": best.value, "rows": r, "batch": b, "threads": n, "ms": ms.tolist()}
 
     def fft_shapes(self):
         """[threads per workgroup] of the self-energy plane kernels."""
@@ -425,7 +441,13 @@ class KBEnsemble:
         for i, r in enumerate(runs):
             m = ctypes.c_void_p()
             self._chk(self._L.kb_ens_member(self._h, i, ctypes.byref(m)))
-            self.members.append(KBContext(_handle=m, _owner=self, device=device, **r))
+            member = KBContext(_handle=m, _owner=self, device=device, **r)
+            # members share the default memory-integral shape (one launch over
+            # all of them); the self-energy stage is per member
+            _, f = committed_tiles(r["Nx"], r["sdim"], r["NstepsinMemory"])
+            if f is not None:
+                member.set_fft_tiles(f)
+            self.members.append(member)
 
     def _chk(self, rc):
         if rc != KB_OK:
