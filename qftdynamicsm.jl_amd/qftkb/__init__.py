@@ -11,7 +11,7 @@ problem / solution API, running the KB step on MI355X through libqftkb.so.
         ...
 """
 from ._abi import KBContext, KBEnsemble, KBError, load_library, LIB_PATH, EXPORTED  # noqa: F401
-from . import lattice  # noqa: F401
+from . import lattice, output  # noqa: F401
 from .problem import (  # noqa: F401
     TwoPIScalarfile, QFTdynamicsProblem, QFTdynamicsSolution, TwoPIScalarProblem, SimSetup,
     MeasurementTwoPIScalar, check_parameters, getParameterstring, splitter, initialize, evolve,
