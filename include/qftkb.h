@@ -83,6 +83,22 @@ const char *kb_last_error(const kb_ctx *ctx);
 /* Number of lattice sites V and bytes of device memory held. */
 int kb_info(const kb_ctx *ctx, long long *V, long long *device_bytes);
 
+/* Cubic-orbit storage: the same KB step with one stored value per orbit of
+ * the lattice's cubic group (axis permutations and reflections
+ * n_i -> Nx - n_i), about 48x fewer stored values in 3D (969 orbits for
+ * the 32768 sites of 32^3).  Exact for cubic-symmetric states (the step
+ * maps them to cubic-symmetric states); the self-energy transforms still
+ * run on the whole lattice.  Every entry point keeps exchanging whole
+ * lattices of V sites: inputs are read at each orbit's representative site
+ * (the caller supplies symmetric data), outputs are expanded over the
+ * orbits.  Nx <= 64; no adjoint, no ensembles.  The reference's CPUred
+ * storage (TwoPIScalar_SimDataStruct.jl:204-260) groups sites by FFTWhelper
+ * levels instead; this is the exact-symmetry version of that idea. */
+int kb_create_sym(const kb_params *p, kb_ctx **out);
+/* Stored sites (orbits, or V) and lattice sites V. */
+int kb_storage_info(const kb_ctx *ctx, long long *stored_sites,
+                    long long *lattice_sites);
+
 /* Per-site lattice momentum squared kL2 (SimDataStruct.jl:673-679).  The
  * library fills it at creation with sum_i 4 sin^2(pi n_i / Nx); a caller
  * holding FFTWhelper levels may overwrite it. */

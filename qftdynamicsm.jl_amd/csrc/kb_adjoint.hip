@@ -918,7 +918,7 @@ int reverse_step(kb_ctx* c, int t, int T, int npend, bool apply) {
     if (nlo && (rc = launch_sigma(c, s->X, s->Sring[t % AK]))) return rc;
     const double H = c->p.Lambda * (double)(c->p.ONgroup + 2) / (6.0 * (double)c->p.ONgroup);
     hipLaunchKernelGGL(adj_hartree_kernel, dim3(1), dim3(1024), 0, c->stream, c->H, c->NP, c->V,
-                       pair_off(a.st.s_tm1, a.st.s_tm1, N), H / (double)c->V, c->p.Mass2, s->m2r + (t & 1) * 2);
+                       pair_off(a.st.s_tm1, a.st.s_tm1, N), H / (double)c->Vl, c->p.Mass2, s->m2r + (t & 1) * 2);
     LAUNCHCHK(c);
     set_pending(c, a, t + 1, nlo ? npend : 0);
     static_assert(AK == 4, "adj_row_kernel instances cover 0 .. AK-1 deferred steps");
@@ -1012,6 +1012,7 @@ int kb_adj_record(kb_ctx* c, int checkpoint_every, int max_steps) {
   if (!c->initialised || c->st.tmax != 2)
     return seterr(c, KB_ERR_STATE, "adjoint recording starts right after initialisation (window [1,2])");
   if (c->in_ensemble) return seterr(c, KB_ERR_STATE, "adjoint: not available for ensemble members");
+  if (c->sym) return seterr(c, KB_ERR_STATE, "adjoint: not available with cubic-orbit storage");
   if (c->p.pexp == KB_PEXP_LAMBDA_NLO && !sigma_planes_supported(c->p.Nx))
     return seterr(c, KB_ERR_ARG, "adjoint: Nx=%d above the plane self-energy limit", c->p.Nx);
   {

@@ -101,33 +101,35 @@ void drop_graph(kb_ctx* c) {
 
 int launch_hartree(kb_ctx* c, int i, int j) {
   const double H = c->p.Lambda * (double)(c->p.ONgroup + 2) / (6.0 * (double)c->p.ONgroup);
-  const double hfac = H * (1.0 / (double)c->V);
+  const double hfac = H * (1.0 / (double)c->Vl);
   const long pr = pair_off(slot_of(c, i), slot_of(c, j), c->N);
   hipLaunchKernelGGL(hartree_kernel, dim3(1), dim3(1024), 0, c->stream, c->H, c->NP, c->V, pr, hfac, c->p.Mass2,
-                     c->m2);
+                     c->wgt_dev, c->m2);
   LAUNCHCHK(c);
   return KB_OK;
 }
 
 int launch_sigma(kb_ctx* c, double2* X, double* S_out) {
   const int rows = c->N - 1;  // grid rows; rows q >= L exit at once
-  const double ivol = 1.0 / (double)c->V;
+  const double ivol = 1.0 / (double)c->Vl;
   const double lam = c->p.Lambda, ON = (double)c->p.ONgroup;
   SigmaArgs a{};
   a.H = c->H;
   a.S = S_out ? S_out : c->S;
+  a.smap = c->smap_dev;
+  a.rmap = c->rmap_dev;
   a.W = c->W;
   a.tw = c->tw;
   a.plan = c->plan;
   a.NP = c->NP;
   a.N = c->N;
-  a.V = c->V;
+  a.V = c->Vl;  // the transforms run on the whole lattice
   a.n = c->p.Nx;
   a.nx = c->nx;
   a.ny = c->ny;
   a.nz = c->nz;
   a.P = c->fftP;
-  a.npencils = c->V / c->p.Nx;
+  a.npencils = c->Vl / c->p.Nx;
   a.st = c->st_dev;
   a.preF = ivol * ivol * ivol * (-(lam * lam) * (ON + 2.0) / (18.0 * ON * ON));
   a.preR = ivol * ivol * ivol * (-(lam * lam) * (ON + 2.0) / (6.0 * ON * ON));
@@ -194,9 +196,9 @@ size_t memint_lds(int N) { return sizeof(double) * 2 * (size_t)N * MB; }
 
 int launch_hartree_step(kb_ctx* c) {
   const double H = c->p.Lambda * (double)(c->p.ONgroup + 2) / (6.0 * (double)c->p.ONgroup);
-  const double hfac = H * (1.0 / (double)c->V);
+  const double hfac = H * (1.0 / (double)c->Vl);
   hipLaunchKernelGGL(hartree_step_kernel, dim3(1), dim3(1024), 0, c->stream, c->H, c->NP, c->V, c->N, c->st_dev,
-                     hfac, c->p.Mass2, c->m2);
+                     hfac, c->p.Mass2, c->wgt_dev, c->m2);
   LAUNCHCHK(c);
   return KB_OK;
 }
@@ -234,13 +236,32 @@ int download(kb_ctx* c, double* dst, const double* src, size_t n) {
   return KB_OK;
 }
 
+int lat_upload(kb_ctx* c, double* dst, const double* src) {
+  if (!c->sym) return upload(c, dst, src, c->V);
+  c->lat_tmp.resize(c->V);
+  for (int r = 0; r < c->V; ++r) c->lat_tmp[r] = src[c->rep_site[r]];
+  HIPCHK(c, hipMemcpyAsync(dst, c->lat_tmp.data(), sizeof(double) * c->V, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));  // lat_tmp is reused by the next call
+  return KB_OK;
+}
+
+int lat_download(kb_ctx* c, double* dst, const double* src) {
+  if (!c->sym) return download(c, dst, src, c->V);
+  c->lat_tmp.resize(c->V);
+  int rc = download(c, c->lat_tmp.data(), src, c->V);
+  if (rc) return rc;
+  for (int s = 0; s < c->Vl; ++s) dst[s] = c->lat_tmp[c->orbit_of[s]];
+  return KB_OK;
+}
+
 void drop_graph(kb_ctx* c);
 
 void free_all(kb_ctx* c) {
   drop_graph(c);
   for (void* ptr : {(void*)c->H, (void*)c->S, (void*)c->W, (void*)c->kL2, (void*)c->m2, (void*)c->partial,
                     (void*)c->modes, (void*)c->vbuf, (void*)c->tw, (void*)c->flag, (void*)c->items_r[0],
-                    (void*)c->items_r[1], (void*)c->items_r[2],
+                    (void*)c->items_r[1], (void*)c->items_r[2], (void*)c->smap_dev, (void*)c->rmap_dev,
+                    (void*)c->wgt_dev,
                     (void*)c->st_dev, (void*)c->stptr_dev, (void*)c->N_dev, (void*)c->margs_dev, (void*)c->wg0_dev})
     if (ptr) (void)hipFree(ptr);
   adj_free(c);
@@ -271,19 +292,87 @@ extern "C" {
 
 const char* kb_last_error(const kb_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
-int kb_create(const kb_params* p, kb_ctx** out) {
+namespace {
+
+// Orbits of the cubic group of the lattice (axis permutations and
+// reflections n_i -> Nx - n_i) on the sites, numbered in order of first
+// appearance; the first site of an orbit is its representative.  kL2 and
+// the whole KB step are invariant under the group, so a cubic-symmetric
+// state is carried exactly by one value per orbit.
+void cubic_orbits(int Nx, int sdim, std::vector<int>& orbit_of, std::vector<int>& rep_site) {
+  int Vl = 1;
+  for (int d = 0; d < sdim; ++d) Vl *= Nx;
+  orbit_of.assign(Vl, -1);
+  rep_site.clear();
+  std::vector<int> key_to_orbit;  // canonical key (sorted folded coordinates) -> orbit
+  const int half = Nx / 2 + 1;
+  int nkeys = 1;
+  for (int d = 0; d < sdim; ++d) nkeys *= half;
+  key_to_orbit.assign(nkeys, -1);
+  for (int s = 0; s < Vl; ++s) {
+    int m[3] = {0, 0, 0}, rem = s;
+    for (int d = 0; d < sdim; ++d) {
+      const int n = rem % Nx;
+      rem /= Nx;
+      m[d] = n < Nx - n ? n : Nx - n;  // reflection
+    }
+    for (int i = 0; i < sdim; ++i)  // permutation: sort
+      for (int j = i + 1; j < sdim; ++j)
+        if (m[j] < m[i]) {
+          const int x = m[i];
+          m[i] = m[j];
+          m[j] = x;
+        }
+    int key = 0;
+    for (int d = sdim - 1; d >= 0; --d) key = key * half + m[d];
+    if (key_to_orbit[key] < 0) {
+      key_to_orbit[key] = (int)rep_site.size();
+      rep_site.push_back(s);
+    }
+    orbit_of[s] = key_to_orbit[key];
+  }
+}
+
+int create_ctx(const kb_params* p, bool sym, kb_ctx** out);
+
+}  // namespace
+
+int kb_create(const kb_params* p, kb_ctx** out) { return create_ctx(p, false, out); }
+
+int kb_create_sym(const kb_params* p, kb_ctx** out) { return create_ctx(p, true, out); }
+
+int kb_storage_info(const kb_ctx* c, long long* stored_sites, long long* lattice_sites) {
+  if (!c) return KB_ERR_ARG;
+  if (stored_sites) *stored_sites = c->V;
+  if (lattice_sites) *lattice_sites = c->Vl;
+  return KB_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
   if (!p || !out) return KB_ERR_ARG;
   *out = nullptr;
   if (p->sdim < 1 || p->sdim > 3 || p->Nx < 1 || p->Nx > 128 || p->NstepsinMemory < 3 ||
       p->NstepsinMemory > 640 || p->ONgroup < 1 || !(p->dt > 0))
     return KB_ERR_ARG;
+  if (sym && !sigma_planes_supported(p->Nx)) return KB_ERR_ARG;  // orbit storage uses the plane kernels
   kb_ctx* c = new (std::nothrow) kb_ctx();
   if (!c) return KB_ERR_OOM;
   c->p = *p;
   c->N = p->NstepsinMemory;
   c->NP = slot_pairs(c->N);
-  c->V = 1;
-  for (int d = 0; d < p->sdim; ++d) c->V *= p->Nx;
+  c->Vl = 1;
+  for (int d = 0; d < p->sdim; ++d) c->Vl *= p->Nx;
+  c->sym = sym;
+  if (sym) {
+    cubic_orbits(p->Nx, p->sdim, c->orbit_of, c->rep_site);
+    c->V = (int)c->rep_site.size();
+  } else {
+    c->V = c->Vl;
+  }
   c->nx = p->Nx;
   c->ny = p->sdim >= 2 ? p->Nx : 1;
   c->nz = p->sdim >= 3 ? p->Nx : 1;
@@ -292,7 +381,7 @@ int kb_create(const kb_params* p, kb_ctx** out) {
   plan_fft(c->plan, p->Nx);
   // pencils per workgroup: 16 complex elements per pencil row at least
   c->fftP = p->Nx >= 64 ? 4 : (p->Nx >= 16 ? 16 : 32);
-  if (c->fftP > c->V / p->Nx) c->fftP = c->V / p->Nx;
+  if (c->fftP > c->Vl / p->Nx) c->fftP = c->Vl / p->Nx;
   int rc;
   auto fail = [&](int code) {
     free_all(c);
@@ -305,7 +394,21 @@ int kb_create(const kb_params* p, kb_ctx** out) {
   const size_t hcount = (size_t)c->NB * (size_t)c->NP * MB * 2;
   if ((rc = dmalloc(c, (void**)&c->H, hcount * sizeof(double)))) return fail(rc);
   if ((rc = dmalloc(c, (void**)&c->S, (size_t)c->NB * c->N * MB * 2 * sizeof(double)))) return fail(rc);
-  if ((rc = dmalloc(c, (void**)&c->W, (size_t)c->N * c->V * sizeof(double2)))) return fail(rc);
+  if ((rc = dmalloc(c, (void**)&c->W, (size_t)c->N * c->Vl * sizeof(double2)))) return fail(rc);
+  if (sym) {
+    std::vector<int> rmap(c->Vl, -1);
+    std::vector<double> wgt(c->V, 0.0);
+    for (int s = 0; s < c->Vl; ++s) wgt[c->orbit_of[s]] += 1.0;
+    for (int r = 0; r < c->V; ++r) rmap[c->rep_site[r]] = r;
+    if ((rc = dmalloc(c, (void**)&c->smap_dev, sizeof(int) * c->Vl)) ||
+        (rc = dmalloc(c, (void**)&c->rmap_dev, sizeof(int) * c->Vl)) ||
+        (rc = dmalloc(c, (void**)&c->wgt_dev, sizeof(double) * c->V)))
+      return fail(rc);
+    if (hipMemcpy(c->smap_dev, c->orbit_of.data(), sizeof(int) * c->Vl, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->rmap_dev, rmap.data(), sizeof(int) * c->Vl, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->wgt_dev, wgt.data(), sizeof(double) * c->V, hipMemcpyHostToDevice) != hipSuccess)
+      return fail(KB_ERR_HIP);
+  }
   if ((rc = dmalloc(c, (void**)&c->kL2, (size_t)c->Vpad * sizeof(double)))) return fail(rc);
   if ((rc = dmalloc(c, (void**)&c->m2, 8 * sizeof(double)))) return fail(rc);
   if ((rc = dmalloc(c, (void**)&c->partial, (size_t)6 * meas_grid(c) * sizeof(double)))) return fail(rc);
@@ -374,7 +477,7 @@ int kb_create(const kb_params* p, kb_ctx** out) {
   }
   std::vector<double> k2(c->Vpad, 0.0);
   for (int i = 0; i < c->V; ++i) {
-    int rem = i;
+    int rem = sym ? c->rep_site[i] : i;
     double s2 = 0.0;
     for (int d = 0; d < p->sdim; ++d) {
       const int nd = rem % p->Nx;
@@ -393,6 +496,10 @@ int kb_create(const kb_params* p, kb_ctx** out) {
   return KB_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
 int kb_destroy(kb_ctx* ctx) {
   if (!ctx) return KB_OK;
   (void)hipSetDevice(ctx->p.device);
@@ -404,19 +511,19 @@ int kb_destroy(kb_ctx* ctx) {
 
 int kb_info(const kb_ctx* ctx, long long* V, long long* device_bytes) {
   if (!ctx) return KB_ERR_ARG;
-  if (V) *V = ctx->V;
+  if (V) *V = ctx->Vl;
   if (device_bytes) *device_bytes = (long long)ctx->bytes;
   return KB_OK;
 }
 
 int kb_get_kL2(kb_ctx* c, double* out) {
   if (!c || !out) return KB_ERR_ARG;
-  return download(c, out, c->kL2, c->V);
+  return lat_download(c, out, c->kL2);
 }
 
 int kb_set_kL2(kb_ctx* c, const double* in) {
   if (!c || !in) return KB_ERR_ARG;
-  int rc = upload(c, c->kL2, in, c->V);
+  int rc = lat_upload(c, c->kL2, in);
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return KB_OK;
@@ -427,8 +534,7 @@ int kb_init_rows(kb_ctx* c, const double* F11, const double* F21, const double* 
   if (c->st.tmax != 0) return seterr(c, KB_ERR_STATE, "init: context already initialised");
   int rc;
   double* d = c->vbuf;
-  if ((rc = upload(c, d, F11, c->V)) || (rc = upload(c, d + c->V, F21, c->V)) ||
-      (rc = upload(c, d + 2 * c->V, F22, c->V)))
+  if ((rc = lat_upload(c, d, F11)) || (rc = lat_upload(c, d + c->V, F21)) || (rc = lat_upload(c, d + 2 * c->V, F22)))
     return rc;
   expand(c);
   expand(c);
@@ -448,7 +554,7 @@ int kb_init_particles(kb_ctx* c, const double* n, double initmass) {
   if (c->st.tmax != 0) return seterr(c, KB_ERR_STATE, "init: context already initialised");
   int rc;
   double* d = c->vbuf;
-  if ((rc = upload(c, d + 3 * c->V, n, c->V))) return rc;
+  if ((rc = lat_upload(c, d + 3 * c->V, n))) return rc;
   const int blocks = (c->V + 255) / 256;
   hipLaunchKernelGGL(init_particles_kernel, dim3(blocks), dim3(256), 0, c->stream, c->kL2, d + 3 * c->V,
                      initmass, c->p.dt, c->V, d, d + c->V, d + 2 * c->V);
@@ -526,6 +632,7 @@ int kb_measure(kb_ctx* c, int t, kb_meas* out, double* F, double* ddF, double* r
   if (!c->initialised) return seterr(c, KB_ERR_STATE, "measure before init");
   if (t != c->st.tmax || !inwin(c, t - 1)) return seterr(c, KB_ERR_WINDOW, "measure: t=%d, window [%d,%d]", t, c->st.tmin, c->st.tmax);
   MeasArgs a{};
+  a.wgt = c->wgt_dev;
   a.H = c->H;
   a.S = c->S;
   a.kL2 = c->kL2;
@@ -561,7 +668,7 @@ int kb_measure(kb_ctx* c, int t, kb_meas* out, double* F, double* ddF, double* r
     for (int k = 0; k < 6; ++k) s[k] += part[(size_t)b * 6 + k];
   const double dt = c->p.dt;
   const double H = c->p.Lambda * (double)(c->p.ONgroup + 2) / (6.0 * (double)c->p.ONgroup);
-  const double ivol = 1.0 / (double)c->V;
+  const double ivol = 1.0 / (double)c->Vl;
   out->time = (double)(t - 1) * dt * c->p.Mass;
   out->E_kin = s[0];
   // E_pot = sum_p 1/2 (Mass2 + kL2 + 1/2 ivol H sum_q F) F  (Measurement.jl:135-149)
@@ -577,7 +684,7 @@ int kb_measure(kb_ctx* c, int t, kb_meas* out, double* F, double* ddF, double* r
   out->n_unstable = (int)s[5];
   double* dsts[6] = {F, ddF, r, n, omega, cc};
   for (int k = 0; k < 6; ++k)
-    if (dsts[k] && (rc = download(c, dsts[k], c->modes + (size_t)k * c->V, c->V))) return rc;
+    if (dsts[k] && (rc = lat_download(c, dsts[k], c->modes + (size_t)k * c->V))) return rc;
   return KB_OK;
 }
 
@@ -607,7 +714,7 @@ int kb_get_entry(kb_ctx* c, int which, int i, int j, double* out) {
   hipLaunchKernelGGL(gather_pair, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->H, c->NP, c->V,
                      pair_off(si, sj, c->N), which, sg, c->vbuf);
   LAUNCHCHK(c);
-  return download(c, out, c->vbuf, c->V);
+  return lat_download(c, out, c->vbuf);
 }
 
 int kb_set_entry(kb_ctx* c, int which, int i, int j, const double* in) {
@@ -617,7 +724,7 @@ int kb_set_entry(kb_ctx* c, int which, int i, int j, const double* in) {
   const int si = slot_of(c, i), sj = slot_of(c, j);
   const double sg = which ? (si >= sj ? 1.0 : -1.0) : 1.0;
   int rc;
-  if ((rc = upload(c, c->vbuf, in, c->V))) return rc;
+  if ((rc = lat_upload(c, c->vbuf, in))) return rc;
   hipLaunchKernelGGL(scatter_pair, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->H, c->NP, c->V,
                      pair_off(si, sj, c->N), which, sg, c->vbuf);
   LAUNCHCHK(c);
@@ -631,7 +738,7 @@ int kb_get_sigma(kb_ctx* c, int which, int tp, double* out) {
   hipLaunchKernelGGL(gather_sigma, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->S, c->N, c->V,
                      slot_of(c, tp), which, c->vbuf);
   LAUNCHCHK(c);
-  return download(c, out, c->vbuf, c->V);
+  return lat_download(c, out, c->vbuf);
 }
 
 int kb_check_finite(kb_ctx* c) {
@@ -653,7 +760,7 @@ int kb_fill_free_window(kb_ctx* c, int tmax, const double* n, double mass) {
   int rc;
   const double* dn = nullptr;
   if (n) {
-    if ((rc = upload(c, c->vbuf, n, c->V))) return rc;
+    if ((rc = lat_upload(c, c->vbuf, n))) return rc;
     dn = c->vbuf;
   }
   const long total = (long)c->NB * MB * c->N;

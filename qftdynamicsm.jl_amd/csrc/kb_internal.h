@@ -29,7 +29,17 @@ struct AdjState;  // kb_adjoint.hip
 
 struct kb_ctx {
   kb_params p{};
-  int V = 0, Vpad = 0, NB = 0, N = 0;
+  int V = 0, Vpad = 0, NB = 0, N = 0;  // V: stored sites (lattice sites, or cubic orbits)
+  int Vl = 0;                          // lattice sites Nx^sdim
+  // cubic-orbit storage: one stored site per orbit of the lattice's cubic
+  // group (axis permutations and reflections n -> Nx - n)
+  bool sym = false;
+  std::vector<int> orbit_of;  // lattice site -> stored site
+  std::vector<int> rep_site;  // stored site -> its representative lattice site
+  int* smap_dev = nullptr;     // orbit_of on the device
+  int* rmap_dev = nullptr;     // lattice site -> stored site if it is the representative, else -1
+  double* wgt_dev = nullptr;   // orbit sizes
+  std::vector<double> lat_tmp;
   long NP = 0;
   int nx = 1, ny = 1, nz = 1;
   qftkb::StepState st{};  // host mirror of the window (simdata.indices)
@@ -98,6 +108,10 @@ namespace qftkb_host {
 int seterr(kb_ctx* c, int code, const char* fmt, ...);
 int upload(kb_ctx* c, double* dst, const double* src, size_t n);
 int download(kb_ctx* c, double* dst, const double* src, size_t n);
+// host lattices (Nx^sdim sites) to / from stored sites: identity in full
+// storage; representatives / orbit expansion in cubic-orbit storage
+int lat_upload(kb_ctx* c, double* dst, const double* src);
+int lat_download(kb_ctx* c, double* dst, const double* src);
 int dmalloc(kb_ctx* c, void** ptr, size_t bytes);
 int upload_state(kb_ctx* c);
 void drop_graph(kb_ctx* c);

@@ -86,6 +86,11 @@ struct SigmaArgs {
   double2* X;           // forward: if set, receives (Fx, rx) per row and site; adjoint: read
   const double* Sbar;   // adjoint input (SigF_bar, Sigr_bar), self-energy ring layout
   double* Hbar;         // adjoint output: history-layout adjoint, row t-1 is added to
+  // orbit storage (NULL: every lattice site is stored): lattice site ->
+  // stored site, and lattice site -> stored site where it is the
+  // orbit's representative, else -1
+  const int* smap;
+  const int* rmap;
 };
 
 // Per-run arguments of memint_step; step-dependent values come from *st.
@@ -104,6 +109,7 @@ struct MemintArgs {
 };
 
 struct MeasArgs {
+  const double* wgt;  // sites each stored site stands for (orbit storage), NULL = 1
   const double* H;
   const double* S;
   const double* kL2;
@@ -118,10 +124,12 @@ struct MeasArgs {
 };
 
 __global__ void sigma_pass(SigmaArgs a);
-__global__ void hartree_kernel(const double* H, long NP, int V, long pair, double hfac, double Mass2, double* out);
+// wgt: sites each stored site stands for (orbit storage), NULL = 1
+__global__ void hartree_kernel(const double* H, long NP, int V, long pair, double hfac, double Mass2,
+                               const double* wgt, double* out);
 // Hartree mass of the step being taken: F(t-1,t-1) from *st
 __global__ void hartree_step_kernel(const double* H, long NP, int V, int N, const StepState* st, double hfac,
-                                    double Mass2, double* out);
+                                    double Mass2, const double* wgt, double* out);
 // one launch over all runs: workgroups [wg_start[r], wg_start[r+1]) belong to run r
 template <int ROWS, int B, int TPB>
 __global__ void memint_step(const MemintArgs* runs, const int* wg_start, int nruns);

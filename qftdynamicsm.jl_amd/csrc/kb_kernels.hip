@@ -249,10 +249,12 @@ __global__ __launch_bounds__(256) void sigma_pass(SigmaArgs a) {
 // Hartree mass: m2 = Mass2 + H ivol sum_p F(t-1,t-1;p)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void hartree_kernel(const double* __restrict__ H, long NP, int V, long pair,
-                                                       double hfac, double Mass2, double* __restrict__ out) {
+                                                       double hfac, double Mass2, const double* __restrict__ wgt,
+                                                       double* __restrict__ out) {
   __shared__ double red[1024];
   double acc = 0.0;
-  for (int p = threadIdx.x; p < V; p += blockDim.x) acc += H[hoff(p / MB, pair, p % MB, NP)];
+  for (int p = threadIdx.x; p < V; p += blockDim.x)
+    acc += (wgt ? wgt[p] : 1.0) * H[hoff(p / MB, pair, p % MB, NP)];
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int w = blockDim.x / 2; w > 0; w >>= 1) {
@@ -269,11 +271,13 @@ __global__ __launch_bounds__(1024) void hartree_kernel(const double* __restrict_
 
 __global__ __launch_bounds__(1024) void hartree_step_kernel(const double* __restrict__ H, long NP, int V, int N,
                                                             const StepState* __restrict__ st, double hfac,
-                                                            double Mass2, double* __restrict__ out) {
+                                                            double Mass2, const double* __restrict__ wgt,
+                                                            double* __restrict__ out) {
   __shared__ double red[1024];
   const long pair = pair_off(st->s_tm1, st->s_tm1, N);  // F(t-1,t-1), Evolution.jl:945
   double acc = 0.0;
-  for (int p = threadIdx.x; p < V; p += blockDim.x) acc += H[hoff(p / MB, pair, p % MB, NP)];
+  for (int p = threadIdx.x; p < V; p += blockDim.x)
+    acc += (wgt ? wgt[p] : 1.0) * H[hoff(p / MB, pair, p % MB, NP)];
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int w = blockDim.x / 2; w > 0; w >>= 1) {
@@ -638,10 +642,11 @@ __global__ __launch_bounds__(256) void measure_kernel(MeasArgs a) {
       a.outom[p] = om;
       a.outc[p] = c;
     }
-    v[0] = 0.5 * ddF;
-    v[1] = Fmm;
-    v[2] = a.kL2[p] * Fmm;
-    v[5] = bad;
+    const double g = a.wgt ? a.wgt[p] : 1.0;
+    v[0] = g * 0.5 * ddF;
+    v[1] = g * Fmm;
+    v[2] = g * a.kL2[p] * Fmm;
+    v[5] = g * bad;
     if (a.t != 2 && a.integral) {
       // E_mem integrand over t' in [tmin, t-2] against F(t',t-1), rho(t',t-1)
       for (int tp = a.tmin; tp <= a.t - 2; ++tp) {
@@ -649,7 +654,7 @@ __global__ __launch_bounds__(256) void measure_kernel(MeasArgs a) {
         const double* e = a.H + hoff(mb, pair_off(x, a.s_tm1, a.N), lane, a.NP);
         const double rr = e[1] * (x >= a.s_tm1 ? 1.0 : -1.0);
         const double* S = a.S + soff(mb, x, lane, a.N);
-        const double term = S[1] * e[0] - S[0] * rr;
+        const double term = g * (S[1] * e[0] - S[0] * rr);
         if (tp == a.tmin)
           v[3] += term;
         else

@@ -228,6 +228,10 @@ __device__ __forceinline__ void emit(const SigmaArgs& a, int s, int slot, double
     o[0] += z.x;
     if (slot != s_tm1) o[1] += (s_tm1 >= slot ? 1.0 : -1.0) * z.y;  // rho(t-1,t-1) is not a variable
   } else {
+    if (a.rmap) {
+      s = a.rmap[s];  // orbit storage: the representative writes its orbit's Sigma
+      if (s < 0) return;
+    }
     double* o = a.S + soff(s / MB, slot, s % MB, a.N);
     o[0] = a.preF * z.x;
     o[1] = a.preR * z.y;
@@ -269,8 +273,9 @@ __global__ __launch_bounds__(TPB) void sigma_plane_fwd(SigmaArgs a) {
       const double* h2 = a.Sbar + soff(s2 / MB, slot, s2 % MB, a.N);
       A[iy * ls + ix] = make_double2(0.5 * (h1[0] + h2[0]), 0.5 * (h1[1] + h2[1]));
     } else {
-      const double* h1 = a.H + hoff(s1 / MB, pr, s1 % MB, a.NP);
-      const double* h2 = a.H + hoff(s2 / MB, pr, s2 % MB, a.NP);
+      const int q1 = a.smap ? a.smap[s1] : s1, q2 = a.smap ? a.smap[s2] : s2;
+      const double* h1 = a.H + hoff(q1 / MB, pr, q1 % MB, a.NP);
+      const double* h2 = a.H + hoff(q2 / MB, pr, q2 % MB, a.NP);
       A[iy * ls + ix] = make_double2(0.5 * (h1[0] + h2[0]), 0.5 * (h1[1] + h2[1]) * sg);
     }
   }

@@ -38,6 +38,7 @@ EXPORTED = (
     "kb_adj_record", "kb_adj_gradient", "kb_adj_info", "kb_adj_stop", "kb_adj_debug_stop", "kb_adj_get_entry",
     "kb_tile_shape", "kb_set_tiles", "kb_get_tiles", "kb_autotune",
     "kb_fft_shape", "kb_set_fft_tiles", "kb_get_fft_tiles", "kb_autotune_fft",
+    "kb_create_sym", "kb_storage_info",
 )
 
 
@@ -79,6 +80,8 @@ def load_library(path: str = LIB_PATH):
     L = ctypes.CDLL(path)
     vp = ctypes.c_void_p
     L.kb_create.argtypes = [ctypes.POINTER(kb_params), ctypes.POINTER(vp)]
+    L.kb_create_sym.argtypes = [ctypes.POINTER(kb_params), ctypes.POINTER(vp)]
+    L.kb_storage_info.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
     L.kb_destroy.argtypes = [vp]
     L.kb_last_error.restype = ctypes.c_char_p
     L.kb_last_error.argtypes = [vp]
@@ -136,11 +139,11 @@ def load_library(path: str = LIB_PATH):
 TILES_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tiles.json")
 
 
-def tiles_key(Nx, sdim, NstepsinMemory):
-    return f"{int(Nx)}^{int(sdim)} N_mem={int(NstepsinMemory)}"
+def tiles_key(Nx, sdim, NstepsinMemory, symmetric=False):
+    return f"{int(Nx)}^{int(sdim)} N_mem={int(NstepsinMemory)}" + (" orbits" if symmetric else "")
 
 
-def committed_tiles(Nx, sdim, NstepsinMemory):
+def committed_tiles(Nx, sdim, NstepsinMemory, symmetric=False):
     """(memory-integral variant, self-energy variant) chosen by the autotuner
     for this lattice and window (tiles.json beside the library, written by
     `bench.py --autotune`); shapes are matched by value, None if absent."""
@@ -149,7 +152,7 @@ def committed_tiles(Nx, sdim, NstepsinMemory):
             table = json.load(f)
     except (OSError, ValueError):
         return None, None
-    e = table.get("choices", {}).get(tiles_key(Nx, sdim, NstepsinMemory))
+    e = table.get("choices", {}).get(tiles_key(Nx, sdim, NstepsinMemory, symmetric))
     if e is None:
         return None, None
     L = load_library()
@@ -193,12 +196,13 @@ class KBContext:
     """One device-resident KB simulation (a kb_ctx)."""
 
     def __init__(self, Nx, sdim, NstepsinMemory, dt, Lambda, ONgroup, Mass, Mass2,
-                 pexp=KB_PEXP_LAMBDA_NLO, device=0, tiles="auto", _handle=None, _owner=None):
+                 pexp=KB_PEXP_LAMBDA_NLO, device=0, tiles="auto", symmetric=False, _handle=None, _owner=None):
         self._L = load_library()
         p = make_params(Nx, sdim, NstepsinMemory, dt, Lambda, ONgroup, Mass, Mass2, pexp, device)
         if _handle is None:
             h = ctypes.c_void_p()
-            rc = self._L.kb_create(ctypes.byref(p), ctypes.byref(h))
+            create = self._L.kb_create_sym if symmetric else self._L.kb_create
+            rc = create(ctypes.byref(p), ctypes.byref(h))
             if rc != KB_OK:
                 raise KBError(rc, "kb_create failed (invalid parameters, no HIP device, or out of memory)")
         else:
@@ -207,8 +211,11 @@ class KBContext:
         self._owner = _owner  # ensemble members are destroyed with their ensemble
         self.params = p
         self.V = int(Nx) ** int(sdim)
+        self.symmetric = bool(symmetric)
         if _owner is None and tiles is not None:
-            if tiles == "auto":
+            if tiles == "auto" and symmetric:
+                v, f = committed_tiles(Nx, sdim, NstepsinMemory, symmetric=True)
+            elif tiles == "auto":
                 v, f = committed_tiles(Nx, sdim, NstepsinMemory)
             else:
                 v, f = int(tiles), None
@@ -234,6 +241,12 @@ class KBContext:
             raise KBError(rc, self._L.kb_last_error(self._h).decode())
 
     # ---- set-up -------------------------------------------------------
+    def storage_info(self):
+        """(stored sites, lattice sites): orbits in cubic-orbit storage."""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        self._chk(self._L.kb_storage_info(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def device_bytes(self):
         V, b = ctypes.c_longlong(), ctypes.c_longlong()
         self._chk(self._L.kb_info(self._h, ctypes.byref(V), ctypes.byref(b)))

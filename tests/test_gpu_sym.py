@@ -1,0 +1,97 @@
+"""Cubic-orbit storage (kb_create_sym): one stored value per orbit of the
+lattice's cubic group.  For cubic-symmetric initial data the KB step keeps
+the state symmetric, so orbit storage must reproduce full storage (and the
+oracle) exactly up to FP64 rounding: tolerance 1e-12 against full storage,
+1e-10 against the oracle."""
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    d = np.max(np.abs(np.asarray(a) - np.asarray(b)))
+    s = np.max(np.abs(np.asarray(b)))
+    return d / s if s > 0 else d
+
+
+def thermal(k2, T=2.0, m=1.0):
+    return 1.0 / (np.exp(np.sqrt(k2 + m * m) / (T * m)) - 1.0)
+
+
+@pytest.mark.parametrize("Nx,sdim,orbits", [(16, 3, 165), (32, 3, 969), (16, 2, 45), (16, 1, 9), (5, 3, 10)])
+def test_orbit_counts(Nx, sdim, orbits):
+    from qftkb import KBContext
+    c = KBContext(Nx, sdim, 4, 0.1, 1.0, 1, 1.0, 1.0, symmetric=True)
+    assert c.storage_info() == (orbits, Nx ** sdim)
+    c.close()
+
+
+CASES = [
+    # (Nx, sdim, Nmem, dt, lambda, ON, Mass, Mass2), pexp, steps
+    pytest.param((8, 3, 10, 0.1, 1.3, 1, 1.0, 1.0), 1, 40, id="8^3-sliding"),
+    pytest.param((6, 2, 7, 0.08, 1.7, 4, 0.9, 0.81), 1, 30, id="6^2-O(4)"),
+    pytest.param((10, 1, 6, 0.1, 2.0, 2, 1.0, 1.0), 1, 30, id="10-1d"),
+    pytest.param((16, 3, 20, 0.1, 1.0, 1, 1.0, 1.0), 1, 45, id="16^3"),
+    pytest.param((8, 3, 10, 0.1, 1.0, 1, 1.0, 1.0), 0, 30, id="8^3-LO"),
+]
+
+
+@pytest.mark.parametrize("args,pexp,T", CASES)
+def test_orbit_storage_equals_full_storage(args, pexp, T):
+    from qftkb import KBContext
+    print(f"\n[case] orbit storage {args} pexp={pexp} T={T}", flush=True)
+    full = KBContext(*args, pexp=pexp)
+    red = KBContext(*args, pexp=pexp, symmetric=True)
+    assert red.storage_info()[0] < full.storage_info()[0]
+    n = thermal(full.kL2(), m=args[6])
+    # the same terms summed in each site's own axis order: equal to rounding
+    np.testing.assert_allclose(red.kL2(), full.kL2(), rtol=1e-14, atol=1e-14)
+    full.init_particles(n, args[6])
+    red.init_particles(n, args[6])
+    for t in range(3, T + 1):
+        full.step(t)
+        red.step(t)
+        if t % 5 == 0 or t == T:
+            mf, mr = full.measure(t), red.measure(t)
+            for k in ("E", "E_kin", "E_pot", "E_mem", "M2"):
+                assert abs(mr[k] - mf[k]) <= 1e-12 * max(abs(mf[k]), 1e-300), (t, k, mr[k], mf[k])
+            assert rel(mr["n"], mf["n"]) <= 1e-12
+    tmin, tmax = full.window()
+    assert red.window() == (tmin, tmax)
+    for tp in range(tmin, tmax + 1):
+        for w in "Fr":
+            assert rel(red.get(w, tmax, tp), full.get(w, tmax, tp)) <= 1e-12, (w, tp)
+    if pexp:
+        # Sigma(t-1, t') over the window, against the scale of the whole ring
+        # (Sigr(t-1, t-1) is zero up to transform rounding in both storages)
+        for w in ("F", "r"):
+            sf = np.array([full.sigma(w, tp) for tp in range(tmin, tmax)])
+            sr = np.array([red.sigma(w, tp) for tp in range(tmin, tmax)])
+            assert np.max(np.abs(sr - sf)) <= 1e-12 * np.max(np.abs(sf)), w
+    full.close()
+    red.close()
+
+
+def test_orbit_storage_config1_box_against_oracle():
+    """The 16^3 box state (config 1 shape, 60 steps) in orbit storage against
+    the oracle, n(p,t) and E at every 10th step."""
+    from qftkb import KBContext
+    from qftkb.lattice import n_box
+    print("\n[case] 16^3 box, orbit storage vs oracle", flush=True)
+    args = (16, 3, 62, 0.1, 1.0, 1, 1.0, 1.0)
+    o = Oracle(*args)
+    g = KBContext(*args, symmetric=True)
+    n = n_box(o.kL2(), 2.0, 1.0)
+    o.init_particles(n, 1.0)
+    g.init_particles(n, 1.0)
+    for t in range(3, 61):
+        o.evolve(t)
+        g.step(t)
+        if t % 10 == 0:
+            mo, mg = o.measure(t), g.measure(t)
+            assert abs(mg["E"] - mo["E"]) <= 1e-10 * abs(mo["E"])
+            assert rel(mg["n"], mo["n"]) <= 1e-10
+    g.close()
