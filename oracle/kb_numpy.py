@@ -11,7 +11,15 @@ mistake:
     the x axis as Julia's rfft does;
   * the memory integrals are written as the mathematics of
     TwoPIScalar_Evolution.jl:766-782 (getRHS_F!) and :846-854 (getRHS_r!)
-    with rho antisymmetric and F symmetric.
+    with rho antisymmetric and F symmetric;
+  * the 1/N expansion (pexp = 3) follows the reference's momentum-space
+    formulation (the CPUred2 back-end: bubbles Fr, F2kr2 of calc_Fr_F2kr2!,
+    TwoPIScalar_SimDataStruct.jl:726-750; resummations Irk, IFk of
+    TwoPIScalar_Evolution.jl:252-292 and 539-597; self-energies :721-735),
+    a per-momentum recursion with no transform inside the time sums, where
+    the C oracle restates the CPUfull back-end (a transform pair per term).
+
+pexp: 0 LambdaLO, 1 LambdaNLO, 2 NinverseLO, 3 NinverseNLO.
 """
 from __future__ import annotations
 
@@ -75,7 +83,14 @@ class NumpyKB:
 
     # physics ---------------------------------------------------------
     def H(self):
+        """Hartreeprefactor, Initialization.jl:245-247."""
+        if self.pexp == 2:
+            return self.lam / 6.0
         return self.lam * (self.ON + 2) / (6.0 * self.ON)
+
+    @property
+    def nlo(self):
+        return self.pexp in (1, 3)
 
     def hartree(self, a, b):
         return self.Mass2 + self.H() / self.V * self.getF(a, b).sum()
@@ -106,13 +121,63 @@ class NumpyKB:
         gF = np.fft.irfftn(Fx * (Fx ** 2 - 0.75 * rx ** 2), s=s, axes=ax) * self.V
         return pF * gF.reshape(-1), pr * gr.reshape(-1)
 
+    def _cos(self, x):
+        """real(rfft(x)) on the full grid (the transform of the even part)."""
+        return np.fft.fftn(x.reshape(self.shape[::-1])).real.reshape(-1)
+
+    def bubbles(self, a, b):
+        """(F2kr2, Fr) of the pair (a, b): ivol^2 C[Fx^2 - rx^2/4] and
+        ivol^2 C[Fx rx] with Fx = C F(a,b), rx = C rho(a,b) (physical)."""
+        key = (a, b)
+        if key not in self._bub:
+            Fx, rx = self._cos(self.getF(a, b)), self._cos(self.getr(a, b))
+            iv2 = 1.0 / self.V ** 2
+            self._bub[key] = (iv2 * self._cos(Fx * Fx - 0.25 * rx * rx), iv2 * self._cos(Fx * rx))
+        return self._bub[key]
+
+    def resum(self, T):
+        """Momentum-space Ir(T, tp), IF(T, tp) for tp in [tmin, T]."""
+        self._bub = {}
+        tmin, lam, dt = self.tmin, self.lam, self.dt
+        Ir = {T: np.zeros(self.V)}
+        for tp in range(T - 1, tmin - 1, -1):
+            acc = np.zeros(self.V)
+            for z in range(tp + 1, T):
+                acc += self.bubbles(z, tp)[1] * Ir[z]
+            Ir[tp] = lam / 3.0 * (self.bubbles(T, tp)[1] - dt * acc)
+        IF = {}
+        for tp in range(tmin, T + 1):
+            acc1 = np.zeros(self.V)
+            for z in range(tmin, T):
+                w = 0.5 if z == tmin else 1.0
+                acc1 += w * Ir[z] * self.bubbles(z, tp)[0]
+            acc2 = np.zeros(self.V)
+            for z in range(tmin, tp):
+                w = 0.5 if z == tmin else 1.0
+                acc2 += w * IF[z] * self.bubbles(z, tp)[1]
+            IF[tp] = lam / 6.0 * (self.bubbles(T, tp)[0] - dt * acc1) + lam / 3.0 * dt * acc2
+        self.Irk, self.IFk = Ir, IF
+        return Ir, IF
+
+    def _ninv_sigma(self, T):
+        Ir, IF = self.resum(T)
+        ivol = 1.0 / self.V
+        pre = -self.lam / (3.0 * self.ON) * ivol
+        for tp in range(self.tmin, T + 1):
+            A, B = self._cos(self.getF(T, tp)), self._cos(self.getr(T, tp))
+            Irx, IFx = ivol * self._cos(Ir[tp]), ivol * self._cos(IF[tp])
+            self.Sigr[tp] = pre * self._cos(A * Irx + B * IFx)
+            self.SigF[tp] = pre * self._cos(A * IFx - 0.25 * B * Irx)
+
     def evolve(self, t):
         assert t == self.tmax + 1
         self.expand()
         tmin = self.tmin
-        if self.pexp:
+        if self.pexp == 1:
             for tp in range(tmin, t):
                 self.SigF[tp], self.Sigr[tp] = self._sunset(self.getF(t - 1, tp), self.getr(t - 1, tp))
+        elif self.pexp == 3:
+            self._ninv_sigma(t - 1)
         self.hartreemass2 = self.hartree(t - 1, t - 1)
         Om = (self.kL2 + self.hartreemass2) * self.dt ** 2
         dt3 = self.dt ** 3
@@ -120,7 +185,7 @@ class NumpyKB:
         for tp in range(tmin, t):
             RF = np.zeros(self.V)
             Rr = np.zeros(self.V)
-            if self.pexp:
+            if self.nlo:
                 for z in range(tp + 1, t - 1):
                     Rr -= self.Sigr[z] * self.getr(z, tp)
                 for z in range(tmin, t - 1):
@@ -139,7 +204,7 @@ class NumpyKB:
             self.setr(t, tp, newr[tp])
         self.setr(t, t, np.zeros(self.V))
         RF = np.zeros(self.V)
-        if self.pexp:
+        if self.nlo:
             for z in range(tmin, t - 1):
                 w = 0.5 if z == tmin else 1.0
                 RF -= w * self.Sigr[z] * self.getF(z, t)
@@ -161,7 +226,7 @@ class NumpyKB:
         hs = F.sum()
         Epot = (0.5 * (self.Mass2 + self.kL2 + 0.5 * self.H() / self.V * hs) * F).sum()
         Emem = 0.0
-        if t != 2 and self.pexp:
+        if t != 2 and self.nlo:
             tmin = self.tmin
             Emem = 0.125 * dt * (self.Sigr[tmin] * self.getF(tmin, t - 1)
                                  - self.SigF[tmin] * self.getr(tmin, t - 1)).sum()

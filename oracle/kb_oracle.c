@@ -136,6 +136,9 @@ struct or_ctx {
   double *F, *r;       /* N(N+1)/2 lattices each, slot triangle */
   size_t bigbytes;
   double *SigF, *Sigr; /* N lattices each */
+  /* NinverseNLO: Irx, IFx per slot (CyclVector of rfftlattice,
+   * SimDataStruct.jl:70-71), kept as the complex full-grid extension */
+  cplx *Irx, *IFx;
   double *kL2, *omega2;
   double hartreemass2;
   int nthreads;
@@ -206,7 +209,7 @@ static void reset_omega2(or_ctx *c) {
 or_ctx *or_create(int Nx, int sdim, int Nmem, double dt, double lambda,
                   int ONgroup, double Mass, double Mass2, int pexp,
                   int nthreads) {
-  if (Nx < 1 || sdim < 1 || sdim > 3 || Nmem < 3 || ONgroup < 1) return NULL;
+  if (Nx < 1 || sdim < 1 || sdim > 3 || Nmem < 3 || ONgroup < 1 || pexp < 0 || pexp > 3) return NULL;
   or_ctx *c = (or_ctx *)calloc(1, sizeof(or_ctx));
   c->Nx = Nx;
   c->sdim = sdim;
@@ -230,6 +233,14 @@ or_ctx *or_create(int Nx, int sdim, int Nmem, double dt, double lambda,
   c->r = (double *)big_alloc(c->bigbytes);
   c->SigF = (double *)calloc((size_t)Nmem * c->V, sizeof(double));
   c->Sigr = (double *)calloc((size_t)Nmem * c->V, sizeof(double));
+  if (pexp == 3) {
+    c->Irx = (cplx *)calloc((size_t)Nmem * c->V, sizeof(cplx));
+    c->IFx = (cplx *)calloc((size_t)Nmem * c->V, sizeof(cplx));
+    if (!c->Irx || !c->IFx) {
+      or_destroy(c);
+      return NULL;
+    }
+  }
   c->kL2 = (double *)malloc(sizeof(double) * c->V);
   c->omega2 = (double *)malloc(sizeof(double) * c->V);
   if (!c->F || !c->r || !c->SigF || !c->Sigr || !c->kL2 || !c->omega2) {
@@ -250,6 +261,8 @@ void or_destroy(or_ctx *c) {
   if (c->r) munmap(c->r, c->bigbytes);
   free(c->SigF);
   free(c->Sigr);
+  free(c->Irx);
+  free(c->IFx);
   free(c->kL2);
   free(c->omega2);
   if (c->plan.w) plan1d_free(&c->plan);
@@ -366,9 +379,13 @@ static void selfenergy_tp(or_ctx *c, int tmone, int tp, cplx *a, cplx *b,
   }
 }
 
-/* calcSelfEnergies!, Evolution.jl:16-23 (threaded over tp) */
+static int ninv_selfenergies(or_ctx *c, int tmone, int simple);
+
+/* calcSelfEnergies!, Evolution.jl:16-23 (threaded over tp); NinverseNLO:
+ * Evolution.jl:144-154 */
 int or_selfenergies(or_ctx *c, int tmone) {
-  if (c->pexp == 0) return 0; /* LambdaLO: Evolution.jl:8 */
+  if (c->pexp == 0 || c->pexp == 2) return 0; /* LO: Evolution.jl:8-10 */
+  if (c->pexp == 3) return ninv_selfenergies(c, tmone, 0);
   const int lo = c->tmin, hi = c->tmax - 1;
   if (!inwin(c, tmone)) return fail(c, "tmone %d outside window [%d,%d]", tmone, c->tmin, c->tmax);
 #pragma omp parallel num_threads(c->nthreads)
@@ -394,7 +411,7 @@ int or_selfenergies(or_ctx *c, int tmone) {
  * the role calcSelfEnergiessimple! (Evolution.jl:123-139) plays in the
  * reference's own fast-vs-simple test. */
 int or_selfenergies_direct(or_ctx *c, int tmone) {
-  if (c->pexp == 0) return 0;
+  if (c->pexp != 1) return fail(c, "direct self-energy: LambdaNLO only");
   const int V = c->V, Nx = c->Nx, sd = c->sdim;
   const double ivol = 1.0 / (double)V;
   const double lam = c->lambda, ON = (double)c->ON;
@@ -454,8 +471,340 @@ int or_selfenergies_direct(or_ctx *c, int tmone) {
   return 0;
 }
 
-/* Hartreeprefactor for lambda expansions, Initialization.jl:245 */
+/* ------------------------------------------------------------------ */
+/* 1/N expansion at NLO (NinverseNLO), CPUfull                         */
+/* ------------------------------------------------------------------ */
+/* The reference works with rfft (x axis halved) and brfft.  Here every
+ * lattice lives on the full grid: fwd(X) is the full complex DFT (sign -1),
+ * whose restriction to the half grid is rfft(X); bwd(Y) = real part of the
+ * unnormalised inverse DFT of the Hermitian extension Y, which is brfft(Y)
+ * for the half-grid Y.  For a real lattice X, fwd(X)(-k) = conj(fwd(X)(k)),
+ * and products of such arrays keep that symmetry, so the full-grid
+ * arithmetic equals the reference's half-grid arithmetic entry by entry.
+ * real(.) is taken exactly where the reference takes it. */
+typedef struct {
+  cplx *a, *b, *g, *h, *pen, *tmp;
+  double *RHS, *R2, *R3;
+} ninv_ws;
+
+static void ninv_ws_alloc(or_ctx *c, ninv_ws *w) {
+  const size_t V = (size_t)c->V;
+  w->a = (cplx *)malloc(sizeof(cplx) * V);
+  w->b = (cplx *)malloc(sizeof(cplx) * V);
+  w->g = (cplx *)malloc(sizeof(cplx) * V);
+  w->h = (cplx *)malloc(sizeof(cplx) * V);
+  w->pen = (cplx *)malloc(sizeof(cplx) * c->Nx);
+  w->tmp = (cplx *)malloc(sizeof(cplx) * c->Nx);
+  w->RHS = (double *)malloc(sizeof(double) * V);
+  w->R2 = (double *)malloc(sizeof(double) * V);
+  w->R3 = (double *)malloc(sizeof(double) * V);
+}
+
+static void ninv_ws_free(ninv_ws *w) {
+  free(w->a);
+  free(w->b);
+  free(w->g);
+  free(w->h);
+  free(w->pen);
+  free(w->tmp);
+  free(w->RHS);
+  free(w->R2);
+  free(w->R3);
+}
+
+/* out = rfft(x * s) on the full grid */
+static void nfwd(or_ctx *c, ninv_ws *w, const double *x, double s, cplx *out) {
+  for (int i = 0; i < c->V; ++i) {
+    out[i].re = x[i] * s;
+    out[i].im = 0.0;
+  }
+  dft3(c, out, w->pen, w->tmp, -1);
+}
+
+/* out = brfft(Y) (real lattice); Y is preserved */
+static void nbwd(or_ctx *c, ninv_ws *w, const cplx *Y, double *out) {
+  for (int i = 0; i < c->V; ++i) w->h[i] = Y[i];
+  dft3(c, w->h, w->pen, w->tmp, +1);
+  for (int i = 0; i < c->V; ++i) out[i] = w->h[i].re;
+}
+
+static cplx *ircyc(or_ctx *c, cplx *base, int i) { return base + (size_t)(mod1(i, c->N) - 1) * (size_t)c->V; }
+
+/* calcIrs! CPUfull, Evolution.jl:170-206 (tp counts down: Ir(tp) needs
+ * Ir(z) for z > tp) */
+static void ninv_Irs(or_ctx *c, ninv_ws *w, int tmone) {
+  const int V = c->V, N = c->N;
+  const double ivol = 1.0 / (double)V;
+  const double f1 = (c->lambda / 3.0) * ivol * ivol;
+  const double f2 = c->dt * (c->lambda / 3.0) * ivol;
+  const double f3 = ivol * ivol;
+  cplx *It = ircyc(c, c->Irx, tmone);
+  for (int i = 0; i < V; ++i) It[i].re = It[i].im = 0.0;
+  for (int tp = tmone - 1; tp >= c->tmin; --tp) {
+    nfwd(c, w, symptr(c, c->F, tmone, tp), 1.0, w->a);
+    nfwd(c, w, symptr(c, c->r, tmone, tp), 1.0, w->b);
+    const double s = thesign(tmone, tp, N);
+    cplx *I = ircyc(c, c->Irx, tp);
+    for (int i = 0; i < V; ++i) {
+      I[i].re = s * f1 * w->a[i].re * w->b[i].re;
+      I[i].im = 0.0;
+    }
+    for (int i = 0; i < V; ++i) w->RHS[i] = 0.0;
+    for (int z = tp + 1; z <= tmone - 1; ++z) {
+      nfwd(c, w, symptr(c, c->F, z, tp), 1.0, w->a);
+      nfwd(c, w, symptr(c, c->r, z, tp), 1.0, w->b);
+      const double sz = thesign(z, tp, N);
+      for (int i = 0; i < V; ++i) {
+        w->g[i].re = w->a[i].re * w->b[i].re;
+        w->g[i].im = 0.0;
+      }
+      nbwd(c, w, w->g, w->R2);
+      nbwd(c, w, ircyc(c, c->Irx, z), w->R3);
+      for (int i = 0; i < V; ++i) w->RHS[i] -= f3 * sz * w->R2[i] * w->R3[i];
+    }
+    nfwd(c, w, w->RHS, 1.0, w->g);
+    for (int i = 0; i < V; ++i) I[i].re += f2 * w->g[i].re;
+  }
+}
+
+/* calcIrssimple!, Evolution.jl:294-314 */
+static void ninv_Irs_simple(or_ctx *c, ninv_ws *w, int tmone) {
+  const int V = c->V, N = c->N;
+  const double ivol = 1.0 / (double)V;
+  const double lam = c->lambda;
+  cplx *It = ircyc(c, c->Irx, tmone);
+  for (int i = 0; i < V; ++i) It[i].re = It[i].im = 0.0;
+  for (int tp = tmone - 1; tp >= c->tmin; --tp) {
+    nfwd(c, w, symptr(c, c->F, tmone, tp), 1.0, w->a);
+    nfwd(c, w, symptr(c, c->r, tmone, tp), thesign(tmone, tp, N), w->b);
+    cplx *I = ircyc(c, c->Irx, tp);
+    for (int i = 0; i < V; ++i) {
+      /* real((lambda/3) ivol^2 rfft(F) .* rfft(r s)) */
+      I[i].re = (lam / 3.0) * ivol * ivol * (w->a[i].re * w->b[i].re - w->a[i].im * w->b[i].im);
+      I[i].im = 0.0;
+    }
+    for (int z = tp + 1; z <= tmone - 1; ++z) {
+      nfwd(c, w, symptr(c, c->F, z, tp), 1.0, w->a);
+      nfwd(c, w, symptr(c, c->r, z, tp), thesign(z, tp, N), w->b);
+      for (int i = 0; i < V; ++i) {
+        w->g[i].re = ivol * ivol * (w->a[i].re * w->b[i].re - w->a[i].im * w->b[i].im); /* Bx */
+        w->g[i].im = 0.0;
+      }
+      nbwd(c, w, ircyc(c, c->Irx, z), w->R3);
+      nbwd(c, w, w->g, w->R2);
+      for (int i = 0; i < V; ++i) w->RHS[i] = w->R3[i] * w->R2[i];
+      nfwd(c, w, w->RHS, 1.0, w->g);
+      for (int i = 0; i < V; ++i) I[i].re -= c->dt * (lam / 3.0) * ivol * w->g[i].re;
+    }
+  }
+}
+
+/* F2 = fwd(F)^2 - 1/4 fwd(r)^2 (complex), scaled by sc */
+static void ninv_F2r2(or_ctx *c, ninv_ws *w, int z, int tp, double sc, cplx *out) {
+  nfwd(c, w, symptr(c, c->F, z, tp), 1.0, w->a);
+  nfwd(c, w, symptr(c, c->r, z, tp), 1.0, w->b);
+  for (int i = 0; i < c->V; ++i) {
+    const double Fr = w->a[i].re * sc, Fi = w->a[i].im * sc, rr = w->b[i].re * sc, ri = w->b[i].im * sc;
+    out[i].re = (Fr * Fr - Fi * Fi) - 0.25 * (rr * rr - ri * ri);
+    out[i].im = 2.0 * Fr * Fi - 0.25 * 2.0 * rr * ri;
+  }
+}
+
+/* fwd(F) fwd(r) s (complex), scaled by sc^2 */
+static void ninv_Fr(or_ctx *c, ninv_ws *w, int z, int tp, double sc, double s, cplx *out) {
+  nfwd(c, w, symptr(c, c->F, z, tp), 1.0, w->a);
+  nfwd(c, w, symptr(c, c->r, z, tp), 1.0, w->b);
+  for (int i = 0; i < c->V; ++i) {
+    const double Fr = w->a[i].re * sc, Fi = w->a[i].im * sc, rr = w->b[i].re * sc, ri = w->b[i].im * sc;
+    out[i].re = s * (Fr * rr - Fi * ri);
+    out[i].im = s * (Fr * ri + Fi * rr);
+  }
+}
+
+/* calcIFs! CPUfull, Evolution.jl:354-436 (tp counts up: IF(tp) needs IF(z)
+ * for z < tp).  IFx[tp] += ivol * Fx is complex in the reference; its
+ * imaginary part is kept. */
+static void ninv_IFs(or_ctx *c, ninv_ws *w, int tmone) {
+  const int V = c->V, N = c->N, tmin = c->tmin;
+  const double ivol = 1.0 / (double)V;
+  const double lam = c->lambda, dt = c->dt;
+  const double f1 = lam / 6.0 * ivol * ivol, f2 = lam / 6.0 * dt, f3 = lam / 3.0 * dt;
+  cplx *Y = (cplx *)malloc(sizeof(cplx) * V);
+  for (int tp = tmin; tp <= tmone; ++tp) {
+    cplx *IF = ircyc(c, c->IFx, tp);
+    ninv_F2r2(c, w, tmone, tp, 1.0, Y);
+    for (int i = 0; i < V; ++i) {
+      IF[i].re = f1 * Y[i].re; /* real(factor1 * Fx) */
+      IF[i].im = 0.0;
+    }
+    for (int i = 0; i < V; ++i) w->RHS[i] = 0.0;
+    /* first integral, z = tmin+1 .. tp-1: Ir part (nr 1) and IF part (nr 2) */
+    for (int z = tmin + 1; z <= tp - 1; ++z) {
+      nbwd(c, w, ircyc(c, c->Irx, z), w->R3);
+      ninv_F2r2(c, w, z, tp, ivol, Y);
+      nbwd(c, w, Y, w->R2);
+      for (int i = 0; i < V; ++i) w->RHS[i] -= f2 * w->R3[i] * w->R2[i];
+      ninv_Fr(c, w, z, tp, ivol, 1.0, Y);
+      const double s = thesign(z, tp, N);
+      nbwd(c, w, Y, w->R3);
+      nbwd(c, w, ircyc(c, c->IFx, z), w->R2);
+      for (int i = 0; i < V; ++i) w->RHS[i] += f3 * s * w->R2[i] * w->R3[i];
+    }
+    /* second integral, z = max(tp, tmin+1) .. tmone-1: Ir part only */
+    const int beg = tp == tmin ? tmin + 1 : tp;
+    for (int z = beg; z <= tmone - 1; ++z) {
+      ninv_F2r2(c, w, z, tp, ivol, Y);
+      nbwd(c, w, Y, w->R3);
+      nbwd(c, w, ircyc(c, c->Irx, z), w->R2);
+      for (int i = 0; i < V; ++i) w->RHS[i] -= f2 * w->R2[i] * w->R3[i];
+    }
+    /* border z = tmin, trapezoid weight 1/2 */
+    ninv_F2r2(c, w, tmin, tp, 1.0, Y);
+    nbwd(c, w, Y, w->R3);
+    for (int i = 0; i < V; ++i) w->R3[i] *= ivol * ivol;
+    nbwd(c, w, ircyc(c, c->Irx, tmin), w->R2);
+    for (int i = 0; i < V; ++i) w->RHS[i] -= 0.5 * dt * lam / 6.0 * w->R2[i] * w->R3[i];
+    if (tp > tmin) {
+      ninv_Fr(c, w, tmin, tp, 1.0, thesign(tmin, tp, N), Y);
+      nbwd(c, w, Y, w->R3);
+      nbwd(c, w, ircyc(c, c->IFx, tmin), w->R2);
+      for (int i = 0; i < V; ++i) w->RHS[i] += 0.5 * dt * lam / 3.0 * ivol * ivol * w->R2[i] * w->R3[i];
+    }
+    nfwd(c, w, w->RHS, 1.0, w->g);
+    for (int i = 0; i < V; ++i) {
+      IF[i].re += ivol * w->g[i].re;
+      IF[i].im += ivol * w->g[i].im;
+    }
+  }
+  free(Y);
+}
+
+/* calcIFssimple!, Evolution.jl:599-624 */
+static void ninv_IFs_simple(or_ctx *c, ninv_ws *w, int tmone) {
+  const int V = c->V, N = c->N, tmin = c->tmin;
+  const double ivol = 1.0 / (double)V;
+  const double lam = c->lambda, dt = c->dt;
+  cplx *Y = (cplx *)malloc(sizeof(cplx) * V);
+  for (int tp = tmin; tp <= tmone; ++tp) {
+    cplx *IF = ircyc(c, c->IFx, tp);
+    ninv_F2r2(c, w, tmone, tp, 1.0, Y);
+    for (int i = 0; i < V; ++i) {
+      IF[i].re = (lam / 6.0) * ivol * ivol * Y[i].re;
+      IF[i].im = 0.0;
+    }
+    for (int z = tmin; z <= tmone - 1; ++z) {
+      const double wz = z == tmin ? 0.5 * dt : dt;
+      nbwd(c, w, ircyc(c, c->Irx, z), w->R3);
+      ninv_F2r2(c, w, z, tp, 1.0, Y);
+      for (int i = 0; i < V; ++i) {
+        Y[i].re *= ivol * ivol;
+        Y[i].im *= ivol * ivol;
+      }
+      nbwd(c, w, Y, w->R2);
+      for (int i = 0; i < V; ++i) w->RHS[i] = w->R3[i] * w->R2[i];
+      nfwd(c, w, w->RHS, 1.0, w->g);
+      for (int i = 0; i < V; ++i) IF[i].re -= wz * (lam / 6.0) * ivol * w->g[i].re;
+    }
+    for (int z = tmin; z <= tp - 1; ++z) {
+      const double wz = z == tmin ? 0.5 * dt : dt;
+      nbwd(c, w, ircyc(c, c->IFx, z), w->R3);
+      ninv_Fr(c, w, z, tp, 1.0, thesign(z, tp, N), Y);
+      for (int i = 0; i < V; ++i) {
+        Y[i].re *= ivol * ivol;
+        Y[i].im *= ivol * ivol;
+      }
+      nbwd(c, w, Y, w->R2);
+      for (int i = 0; i < V; ++i) w->RHS[i] = w->R3[i] * w->R2[i];
+      nfwd(c, w, w->RHS, 1.0, w->g);
+      for (int i = 0; i < V; ++i) IF[i].re += wz * (lam / 3.0) * ivol * w->g[i].re;
+    }
+  }
+  free(Y);
+}
+
+/* calcSelfEnergyattp! CPUfull NinverseNLO, Evolution.jl:683-698 (fast)
+ * and calcSelfEnergyattpsimple!, :737-742: both evaluate
+ *   Sigr = -lambda/(3N) ivol brfft(Fx Irx + rx IFx)
+ *   SigF = -lambda/(3N) ivol brfft(Fx IFx - 1/4 rx Irx)
+ * with Fx = rfft(F(tmone,tp)), rx = rfft(r(tmone,tp)) * thesign. */
+static void ninv_sigma_tp(or_ctx *c, ninv_ws *w, int tmone, int tp) {
+  const int V = c->V;
+  const double ivol = 1.0 / (double)V;
+  const double pre = -c->lambda / (3.0 * (double)c->ON) * ivol;
+  nfwd(c, w, symptr(c, c->F, tmone, tp), 1.0, w->a);
+  nfwd(c, w, symptr(c, c->r, tmone, tp), thesign(tmone, tp, c->N), w->b);
+  const cplx *Ir = ircyc(c, c->Irx, tp), *IF = ircyc(c, c->IFx, tp);
+  for (int i = 0; i < V; ++i) {
+    const cplx A = w->a[i], B = w->b[i], R = Ir[i], Q = IF[i];
+    w->g[i].re = (A.re * R.re - A.im * R.im) + (B.re * Q.re - B.im * Q.im);
+    w->g[i].im = (A.re * R.im + A.im * R.re) + (B.re * Q.im + B.im * Q.re);
+  }
+  nbwd(c, w, w->g, cycptr(c, c->Sigr, tp));
+  for (int i = 0; i < V; ++i) {
+    const cplx A = w->a[i], B = w->b[i], R = Ir[i], Q = IF[i];
+    w->g[i].re = (A.re * Q.re - A.im * Q.im) - 0.25 * (B.re * R.re - B.im * R.im);
+    w->g[i].im = (A.re * Q.im + A.im * Q.re) - 0.25 * (B.re * R.im + B.im * R.re);
+  }
+  nbwd(c, w, w->g, cycptr(c, c->SigF, tp));
+  double *Sr = cycptr(c, c->Sigr, tp), *SF = cycptr(c, c->SigF, tp);
+  for (int i = 0; i < V; ++i) {
+    Sr[i] *= pre;
+    SF[i] *= pre;
+  }
+}
+
+/* calcSelfEnergies! NinverseNLO, Evolution.jl:144-154: Ir and IF for all
+ * tp, then the self-energies for tp in [tmin, tmone] */
+static int ninv_selfenergies(or_ctx *c, int tmone, int simple) {
+  if (!inwin(c, tmone)) return fail(c, "tmone %d outside window [%d,%d]", tmone, c->tmin, c->tmax);
+  ninv_ws w;
+  ninv_ws_alloc(c, &w);
+  if (simple) {
+    ninv_Irs_simple(c, &w, tmone);
+    ninv_IFs_simple(c, &w, tmone);
+  } else {
+    ninv_Irs(c, &w, tmone);
+    ninv_IFs(c, &w, tmone);
+  }
+  for (int tp = c->tmin; tp <= c->tmax - 1; ++tp) ninv_sigma_tp(c, &w, tmone, tp);
+  ninv_ws_free(&w);
+  return 0;
+}
+
+int or_ninv_resum(or_ctx *c, int tmone, int simple) {
+  if (c->pexp != 3) return fail(c, "resummation: NinverseNLO only");
+  if (!inwin(c, tmone)) return fail(c, "tmone %d outside window [%d,%d]", tmone, c->tmin, c->tmax);
+  ninv_ws w;
+  ninv_ws_alloc(c, &w);
+  if (simple) {
+    ninv_Irs_simple(c, &w, tmone);
+    ninv_IFs_simple(c, &w, tmone);
+  } else {
+    ninv_Irs(c, &w, tmone);
+    ninv_IFs(c, &w, tmone);
+  }
+  ninv_ws_free(&w);
+  return 0;
+}
+
+int or_ninv_selfenergies(or_ctx *c, int tmone, int simple) {
+  if (c->pexp != 3) return fail(c, "1/N self-energies: NinverseNLO only");
+  return ninv_selfenergies(c, tmone, simple);
+}
+
+/* Irx / IFx at the slot of time tp (which = 0 IFx, 1 Irx), as V complex
+ * values (re, im interleaved) */
+int or_get_ninv(or_ctx *c, int which, int tp, double *out) {
+  if (c->pexp != 3) return fail(c, "no 1/N resummation in this context");
+  if (!inwin(c, tp)) return fail(c, "ninv: %d outside window", tp);
+  memcpy(out, ircyc(c, which ? c->Irx : c->IFx, tp), sizeof(cplx) * c->V);
+  return 0;
+}
+
+/* Hartreeprefactor, Initialization.jl:245-247: lambda (N+2)/(6N) for the
+ * lambda expansions and NinverseNLO, lambda/6 for NinverseLO */
 static double hartreeprefactor(const or_ctx *c) {
+  if (c->pexp == 2) return c->lambda * (1.0 / 6.0);
   return c->lambda * (double)(c->ON + 2) / (6.0 * (double)c->ON);
 }
 
@@ -487,7 +836,7 @@ static void setomega2(or_ctx *c) {
 static void rhs_r(or_ctx *c, double *Mem, int t, int tp) {
   const int V = c->V;
   for (int i = 0; i < V; ++i) Mem[i] = 0.0;
-  if (c->pexp == 0) return; /* LO: Evolution.jl:842-844 */
+  if (c->pexp == 0 || c->pexp == 2) return; /* LO: Evolution.jl:842-844 */
   for (int z = tp + 1; z <= t - 2; ++z) {
     const double *S = cycptr(c, c->Sigr, z);
     const double *R = symptr(c, c->r, z, tp);
@@ -500,7 +849,7 @@ static void rhs_r(or_ctx *c, double *Mem, int t, int tp) {
 static void rhs_F(or_ctx *c, double *Mem, int t, int tp) {
   const int V = c->V;
   const int tpmin = c->tmin;
-  if (c->pexp == 0) { /* LO: Evolution.jl:763-765 */
+  if (c->pexp == 0 || c->pexp == 2) { /* LO: Evolution.jl:763-765 */
     for (int i = 0; i < V; ++i) Mem[i] = 0.0;
     return;
   }

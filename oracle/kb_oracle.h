@@ -28,7 +28,10 @@ typedef struct or_ctx or_ctx;
 
 /* pexp: 0 = LambdaLO (no self-energy, zero memory integrals:
  *           Evolution.jl:8-9, 763-765, 842-844),
- *       1 = LambdaNLO quantum (Evolution.jl:25-31). */
+ *       1 = LambdaNLO quantum (Evolution.jl:25-31),
+ *       2 = NinverseLO (no self-energy, Hartree prefactor lambda/6:
+ *           Evolution.jl:10-11, Initialization.jl:246),
+ *       3 = NinverseNLO (resummed bubble chain, Evolution.jl:144-735). */
 or_ctx *or_create(int Nx, int sdim, int Nmem, double dt, double lambda,
                   int ONgroup, double Mass, double Mass2, int pexp,
                   int nthreads);
@@ -66,6 +69,16 @@ int or_selfenergies(or_ctx *c, int tmone);          /* Evolution.jl:16-31 */
 int or_selfenergies_direct(or_ctx *c, int tmone);   /* brute-force double
                                                        convolution twin */
 double or_hartree(or_ctx *c, int i, int j);          /* Initialization.jl:207-218 */
+/* NinverseNLO pieces: the resummed kernels Irx (calcIrs!, Evolution.jl:
+ * 170-206) and IFx (calcIFs!, :354-436) for every tp at tmone; simple = 1
+ * runs calcIrssimple!/calcIFssimple! (:294-314, :599-624) instead.
+ * or_ninv_selfenergies also evaluates calcSelfEnergyattp! (:683-698). */
+int or_ninv_resum(or_ctx *c, int tmone, int simple);
+int or_ninv_selfenergies(or_ctx *c, int tmone, int simple);
+/* IFx (which = 0) or Irx (which = 1) at the slot of time tp: V complex
+ * values, (re, im) interleaved, full grid (the Hermitian extension of the
+ * reference's rfft-layout array). */
+int or_get_ninv(or_ctx *c, int which, int tp, double *out);
 int or_set_hartree(or_ctx *c, double m2);
 double or_get_hartree(or_ctx *c);
 

@@ -56,6 +56,9 @@ def lib():
                                 ctypes.POINTER(ctypes.c_int)]
         L.or_selfenergies.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.or_selfenergies_direct.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_ninv_resum.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.or_ninv_selfenergies.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.or_get_ninv.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
         L.or_hartree.restype = ctypes.c_double
         L.or_hartree.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.or_set_hartree.argtypes = [ctypes.c_void_p, ctypes.c_double]
@@ -81,7 +84,8 @@ def _ptr(a: np.ndarray):
 class Oracle:
     """One reference-equivalent CPUfull simulation state.
 
-    pexp: 0 = LambdaLO, 1 = LambdaNLO (quantum sunset).
+    pexp: 0 = LambdaLO, 1 = LambdaNLO (quantum sunset), 2 = NinverseLO,
+    3 = NinverseNLO.
     """
 
     def __init__(self, Nx, sdim, Nmem, dt, lam, ONgroup, Mass, Mass2, pexp=1, nthreads=None):
@@ -145,6 +149,20 @@ class Oracle:
     def selfenergies(self, tmone, direct=False):
         f = self._L.or_selfenergies_direct if direct else self._L.or_selfenergies
         self._chk(f(self._c, int(tmone)))
+
+    def ninv_resum(self, tmone, simple=False):
+        """NinverseNLO: Irx and IFx for every tp at tmone (calcIrs!/calcIFs!,
+        or the simple variants)."""
+        self._chk(self._L.or_ninv_resum(self._c, int(tmone), int(bool(simple))))
+
+    def ninv_selfenergies(self, tmone, simple=False):
+        self._chk(self._L.or_ninv_selfenergies(self._c, int(tmone), int(bool(simple))))
+
+    def ninv(self, which, tp):
+        """IFx (which='F') or Irx (which='r') at tp: complex, full grid."""
+        out = np.empty(2 * self.V)
+        self._chk(self._L.or_get_ninv(self._c, 1 if which == "r" else 0, int(tp), _ptr(out)))
+        return out.view(np.complex128).copy()
 
     def hartree(self, i, j):
         return self._L.or_hartree(self._c, int(i), int(j))
