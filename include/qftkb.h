@@ -45,6 +45,8 @@ extern "C" {
 
 #define KB_PEXP_LAMBDA_LO 0  /* TwoPIScalarLambdaLO (Problemdefinition.jl:34) */
 #define KB_PEXP_LAMBDA_NLO 1 /* TwoPIScalarLambdaNLOquantum (:36)            */
+#define KB_PEXP_NINVERSE_LO 2  /* TwoPIScalarNinverseLO (:44)             */
+#define KB_PEXP_NINVERSE_NLO 3 /* TwoPIScalarNinverseNLO (:45)            */
 
 typedef struct kb_ctx kb_ctx;
 
@@ -149,6 +151,15 @@ int kb_get_entry(kb_ctx *ctx, int which, int i, int j, double *out);
 int kb_set_entry(kb_ctx *ctx, int which, int i, int j, const double *in);
 /* Self-energy lattices of the last step: which 0 = SigF, 1 = Sigr, at tp. */
 int kb_get_sigma(kb_ctx *ctx, int which, int tp, double *out);
+/* NinverseNLO internals, in momentum space (the CPUred2 quantities of the
+ * reference): which = 0 the bubble F2kr2(i,j), 1 the bubble Fr(i,j)
+ * (physical, antisymmetric), 2 the resummed kernel Ir(i,j) for i > j
+ * (calc_Fr_F2kr2!, SimDataStruct.jl:726-750; Irk, Evolution.jl:252-292);
+ * 3 IF(tmax-1, j) and 4 Ir(tmax-1, j) of the last step (Irk/IFk,
+ * Evolution.jl:252-292, 539-597; the reference keeps their transforms
+ * Irx/IFx, SimDataStruct.jl:70-71), i ignored.  KB_ERR_STATE for other
+ * expansions. */
+int kb_ninv_get(kb_ctx *ctx, int which, int i, int j, double *out);
 /* Scan the window for NaN/Inf; returns KB_ERR_NONFINITE if any. */
 int kb_check_finite(kb_ctx *ctx);
 

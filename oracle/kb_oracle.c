@@ -481,7 +481,12 @@ int or_selfenergies_direct(or_ctx *c, int tmone) {
  * for the half-grid Y.  For a real lattice X, fwd(X)(-k) = conj(fwd(X)(k)),
  * and products of such arrays keep that symmetry, so the full-grid
  * arithmetic equals the reference's half-grid arithmetic entry by entry.
- * real(.) is taken exactly where the reference takes it. */
+ * real(.) is taken exactly where the reference takes it.  Every lattice
+ * the dynamics produces from the reference's initial states is inversion-
+ * even (F(p) = F(-p)); for such lattices all transforms are real and the
+ * fast, simple and momentum-space (CPUred2) forms coincide.  For lattices
+ * with an odd part they do not even agree with each other (the fast form
+ * takes real(Fx) real(rx), the simple one real(Fx rx)). */
 typedef struct {
   cplx *a, *b, *g, *h, *pen, *tmp;
   double *RHS, *R2, *R3;

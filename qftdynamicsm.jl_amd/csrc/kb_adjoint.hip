@@ -40,6 +40,7 @@
 using namespace qftkb;
 using namespace qftkb_host;
 
+#pragma GCC visibility push(hidden)
 namespace qftkb {
 
 // deferred reverse steps whose memory-integral entry updates are applied
@@ -756,10 +757,12 @@ __global__ void adj_final_kernel(const double* __restrict__ Hbar, long NP, int V
 }
 
 }  // namespace qftkb
+#pragma GCC visibility pop
 
 // ===========================================================================
 // host side
 // ===========================================================================
+#pragma GCC visibility push(hidden)
 namespace qftkb_host {
 
 struct AdjState {
@@ -1004,6 +1007,7 @@ void adj_free(kb_ctx* c) {
 }
 
 }  // namespace qftkb_host
+#pragma GCC visibility pop
 
 extern "C" {
 
@@ -1013,6 +1017,8 @@ int kb_adj_record(kb_ctx* c, int checkpoint_every, int max_steps) {
     return seterr(c, KB_ERR_STATE, "adjoint recording starts right after initialisation (window [1,2])");
   if (c->in_ensemble) return seterr(c, KB_ERR_STATE, "adjoint: not available for ensemble members");
   if (c->sym) return seterr(c, KB_ERR_STATE, "adjoint: not available with cubic-orbit storage");
+  if (c->p.pexp != KB_PEXP_LAMBDA_LO && c->p.pexp != KB_PEXP_LAMBDA_NLO)
+    return seterr(c, KB_ERR_ARG, "adjoint: lambda expansions only");
   if (c->p.pexp == KB_PEXP_LAMBDA_NLO && !sigma_planes_supported(c->p.Nx))
     return seterr(c, KB_ERR_ARG, "adjoint: Nx=%d above the plane self-energy limit", c->p.Nx);
   {

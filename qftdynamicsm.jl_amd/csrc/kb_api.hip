@@ -17,6 +17,8 @@
 using namespace qftkb;
 using namespace qftkb_host;
 
+// host helpers: internal to the library, not part of its dynamic symbol table
+#pragma GCC visibility push(hidden)
 namespace qftkb_host {
 
 int seterr(kb_ctx* c, int code, const char* fmt, ...) {
@@ -99,8 +101,13 @@ void drop_graph(kb_ctx* c) {
   c->graph = nullptr;
 }
 
+double hartree_prefactor(const kb_ctx* c) {
+  if (c->p.pexp == KB_PEXP_NINVERSE_LO) return c->p.Lambda * (1.0 / 6.0);
+  return c->p.Lambda * (double)(c->p.ONgroup + 2) / (6.0 * (double)c->p.ONgroup);
+}
+
 int launch_hartree(kb_ctx* c, int i, int j) {
-  const double H = c->p.Lambda * (double)(c->p.ONgroup + 2) / (6.0 * (double)c->p.ONgroup);
+  const double H = hartree_prefactor(c);
   const double hfac = H * (1.0 / (double)c->Vl);
   const long pr = pair_off(slot_of(c, i), slot_of(c, j), c->N);
   hipLaunchKernelGGL(hartree_kernel, dim3(1), dim3(1024), 0, c->stream, c->H, c->NP, c->V, pr, hfac, c->p.Mass2,
@@ -166,6 +173,116 @@ int launch_sigma(kb_ctx* c, double2* X, double* S_out) {
   return KB_OK;
 }
 
+// 1/N expansion: the bubble pass of row t-1 of the state `st` (writes row
+// t-1 of the bubble triangles; X, if set, keeps the transforms), the
+// resummation sweeps, and the chain pass that turns (IF, Ir) into Sigma
+SigmaArgs ninv_sigma_args(kb_ctx* c, const StepState* st) {
+  const double ivol = 1.0 / (double)c->Vl;
+  SigmaArgs a{};
+  a.H = c->H;
+  a.S = c->S;
+  a.smap = c->smap_dev;
+  a.rmap = c->rmap_dev;
+  a.W = c->W;
+  a.tw = c->tw;
+  a.plan = c->plan;
+  a.NP = c->NP;
+  a.N = c->N;
+  a.V = c->Vl;
+  a.n = c->p.Nx;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.nz = c->nz;
+  a.P = c->fftP;
+  a.npencils = c->Vl / c->p.Nx;
+  a.st = st;
+  a.PF = c->PF;
+  a.PR = c->PR;
+  a.Iring = c->Iring;
+  a.ivol = ivol;
+  return a;
+}
+
+int launch_bubble(kb_ctx* c, const StepState* st, double2* X) {
+  SigmaArgs a = ninv_sigma_args(c, st);
+  const double ivol = 1.0 / (double)c->Vl;
+  a.preF = a.preR = ivol * ivol;
+  a.X = X;
+  hipError_t e = launch_bubble_planes(a, c->N - 1, c->stream);
+  if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "bubble launch failed: %s", hipGetErrorString(e));
+  return KB_OK;
+}
+
+int launch_resum(kb_ctx* c, const StepState* st, int passes) {
+  NinvArgs a{};
+  a.PF = c->PF;
+  a.PR = c->PR;
+  a.IR = c->IR;
+  a.Iring = c->Iring;
+  a.items = c->items;
+  a.nitems = c->nitems;
+  a.NP = c->NP;
+  a.N = c->N;
+  a.V = c->V;
+  a.dt = c->p.dt;
+  a.lam = c->p.Lambda;
+  a.passes = passes;
+  a.st = st;
+  hipLaunchKernelGGL(ninv_resum_kernel, dim3(c->NB), dim3(NINV_THREADS), ninv_lds(c->N), c->stream, a);
+  LAUNCHCHK(c);
+  return KB_OK;
+}
+
+int launch_chain(kb_ctx* c) {
+  SigmaArgs a = ninv_sigma_args(c, c->st_dev);
+  const double ivol = 1.0 / (double)c->Vl;
+  a.preF = a.preR = -c->p.Lambda / (3.0 * (double)c->p.ONgroup) * ivol;
+  a.X = c->Xn;
+  hipError_t e = launch_chain_planes(a, c->N - 1, c->stream);
+  if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "chain launch failed: %s", hipGetErrorString(e));
+  return KB_OK;
+}
+
+// calcSelfEnergies! NinverseNLO (Evolution.jl:144-154) for the step in st_dev
+int launch_ninv_stage(kb_ctx* c) {
+  int rc;
+  if ((rc = launch_bubble(c, c->st_dev, c->Xn))) return rc;
+  if ((rc = launch_resum(c, c->st_dev, NINV_PASS_ALL))) return rc;
+  return launch_chain(c);
+}
+
+// Rows tmin .. T-1 of the bubble and Ir triangles for the step the host
+// window has just been advanced to (T = t - 1), one row after the other:
+// needed after init and after any write to the window from outside a step.
+int ninv_rebuild(kb_ctx* c) {
+  const int T = c->st.t - 1;
+  const int tmin = c->st.tmin;
+  std::vector<StepState> fs;
+  for (int R = tmin; R <= T - 1; ++R) {
+    StepState f = c->st;
+    f.t = R + 1;
+    f.tmax = R + 1;
+    f.L = R + 1 - tmin;
+    f.s_t = R % c->N;
+    f.s_tm1 = (R - 1) % c->N;
+    f.s_tm2 = (R - 2 + c->N) % c->N;
+    f.dead = -1;
+    fs.push_back(f);
+  }
+  if (!fs.empty()) {
+    HIPCHK(c, hipMemcpyAsync(c->stfill_dev, fs.data(), sizeof(StepState) * fs.size(), hipMemcpyHostToDevice,
+                             c->stream));
+    int rc;
+    for (size_t k = 0; k < fs.size(); ++k) {
+      if ((rc = launch_bubble(c, c->stfill_dev + k, nullptr))) return rc;
+      if ((rc = launch_resum(c, c->stfill_dev + k, NINV_PASS_IR))) return rc;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  c->ninv_valid = true;
+  return KB_OK;
+}
+
 MemintArgs memint_args(const kb_ctx* c) {
   MemintArgs a{};
   a.H = c->H;
@@ -176,7 +293,7 @@ MemintArgs memint_args(const kb_ctx* c) {
   a.N = c->N;
   a.V = c->V;
   a.dt = c->p.dt;
-  a.integral = c->p.pexp == KB_PEXP_LAMBDA_NLO;
+  a.integral = has_sigma(c);
   int ri = 1;
   for (int i = 0; i < 3; ++i)
     if (MEMINT_ROW_CHOICES[i] == kMemintVariants[c->tile].rows) ri = i;
@@ -195,7 +312,7 @@ hipError_t launch_memint(int tile, unsigned grid, size_t lds, hipStream_t s, con
 size_t memint_lds(int N) { return sizeof(double) * 2 * (size_t)N * MB; }
 
 int launch_hartree_step(kb_ctx* c) {
-  const double H = c->p.Lambda * (double)(c->p.ONgroup + 2) / (6.0 * (double)c->p.ONgroup);
+  const double H = hartree_prefactor(c);
   const double hfac = H * (1.0 / (double)c->Vl);
   hipLaunchKernelGGL(hartree_step_kernel, dim3(1), dim3(1024), 0, c->stream, c->H, c->NP, c->V, c->N, c->st_dev,
                      hfac, c->p.Mass2, c->wgt_dev, c->m2);
@@ -212,6 +329,9 @@ int enqueue_step(kb_ctx* c) {
   if (c->p.pexp == KB_PEXP_LAMBDA_NLO) {
     StageTimer st(c, 1);
     if ((rc = launch_sigma(c))) return rc;
+  } else if (c->p.pexp == KB_PEXP_NINVERSE_NLO) {
+    StageTimer st(c, 1);
+    if ((rc = launch_ninv_stage(c))) return rc;
   }
   {
     StageTimer st(c, 2);
@@ -261,7 +381,8 @@ void free_all(kb_ctx* c) {
   for (void* ptr : {(void*)c->H, (void*)c->S, (void*)c->W, (void*)c->kL2, (void*)c->m2, (void*)c->partial,
                     (void*)c->modes, (void*)c->vbuf, (void*)c->tw, (void*)c->flag, (void*)c->items_r[0],
                     (void*)c->items_r[1], (void*)c->items_r[2], (void*)c->smap_dev, (void*)c->rmap_dev,
-                    (void*)c->wgt_dev,
+                    (void*)c->wgt_dev, (void*)c->PF, (void*)c->PR, (void*)c->IR, (void*)c->Iring, (void*)c->Xn,
+                    (void*)c->stfill_dev,
                     (void*)c->st_dev, (void*)c->stptr_dev, (void*)c->N_dev, (void*)c->margs_dev, (void*)c->wg0_dev})
     if (ptr) (void)hipFree(ptr);
   adj_free(c);
@@ -287,6 +408,7 @@ int dmalloc(kb_ctx* c, void** ptr, size_t bytes) {
 int meas_grid(const kb_ctx* c) { return (c->V + MEAS_THREADS - 1) / MEAS_THREADS; }
 
 }  // namespace
+#pragma GCC visibility pop
 
 extern "C" {
 
@@ -299,7 +421,7 @@ namespace {
 // appearance; the first site of an orbit is its representative.  kL2 and
 // the whole KB step are invariant under the group, so a cubic-symmetric
 // state is carried exactly by one value per orbit.
-void cubic_orbits(int Nx, int sdim, std::vector<int>& orbit_of, std::vector<int>& rep_site) {
+static void cubic_orbits(int Nx, int sdim, std::vector<int>& orbit_of, std::vector<int>& rep_site) {
   int Vl = 1;
   for (int d = 0; d < sdim; ++d) Vl *= Nx;
   orbit_of.assign(Vl, -1);
@@ -333,7 +455,7 @@ void cubic_orbits(int Nx, int sdim, std::vector<int>& orbit_of, std::vector<int>
   }
 }
 
-int create_ctx(const kb_params* p, bool sym, kb_ctx** out);
+static int create_ctx(const kb_params* p, bool sym, kb_ctx** out);
 
 }  // namespace
 
@@ -352,12 +474,15 @@ int kb_storage_info(const kb_ctx* c, long long* stored_sites, long long* lattice
 
 namespace {
 
-int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
+static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
   if (!p || !out) return KB_ERR_ARG;
   *out = nullptr;
   if (p->sdim < 1 || p->sdim > 3 || p->Nx < 1 || p->Nx > 128 || p->NstepsinMemory < 3 ||
-      p->NstepsinMemory > 640 || p->ONgroup < 1 || !(p->dt > 0))
+      p->NstepsinMemory > 640 || p->ONgroup < 1 || !(p->dt > 0) || p->pexp < KB_PEXP_LAMBDA_LO ||
+      p->pexp > KB_PEXP_NINVERSE_NLO)
     return KB_ERR_ARG;
+  // the 1/N stage runs on the plane kernels only
+  if (p->pexp == KB_PEXP_NINVERSE_NLO && !sigma_planes_supported(p->Nx)) return KB_ERR_ARG;
   if (sym && !sigma_planes_supported(p->Nx)) return KB_ERR_ARG;  // orbit storage uses the plane kernels
   kb_ctx* c = new (std::nothrow) kb_ctx();
   if (!c) return KB_ERR_OOM;
@@ -395,6 +520,17 @@ int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
   if ((rc = dmalloc(c, (void**)&c->H, hcount * sizeof(double)))) return fail(rc);
   if ((rc = dmalloc(c, (void**)&c->S, (size_t)c->NB * c->N * MB * 2 * sizeof(double)))) return fail(rc);
   if ((rc = dmalloc(c, (void**)&c->W, (size_t)c->N * c->Vl * sizeof(double2)))) return fail(rc);
+  const size_t pcount = (size_t)c->NB * (size_t)c->NP * MB;  // one value per pair and site
+  const size_t rcount = (size_t)c->NB * c->N * MB * 2;
+  if (p->pexp == KB_PEXP_NINVERSE_NLO) {
+    if ((rc = dmalloc(c, (void**)&c->PF, pcount * sizeof(double))) ||
+        (rc = dmalloc(c, (void**)&c->PR, pcount * sizeof(double))) ||
+        (rc = dmalloc(c, (void**)&c->IR, pcount * sizeof(double))) ||
+        (rc = dmalloc(c, (void**)&c->Iring, rcount * sizeof(double))) ||
+        (rc = dmalloc(c, (void**)&c->Xn, (size_t)c->N * c->Vl * sizeof(double2))) ||
+        (rc = dmalloc(c, (void**)&c->stfill_dev, (size_t)c->N * sizeof(StepState))))
+      return fail(rc);
+  }
   if (sym) {
     std::vector<int> rmap(c->Vl, -1);
     std::vector<double> wgt(c->V, 0.0);
@@ -469,6 +605,16 @@ int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
   if (hipMemsetAsync(c->S, 0, (size_t)c->NB * c->N * MB * 2 * sizeof(double), c->stream) != hipSuccess)
     return fail(KB_ERR_HIP);
   if (hipMemsetAsync(c->m2, 0, 8 * sizeof(double), c->stream) != hipSuccess) return fail(KB_ERR_HIP);
+  if (c->PF) {
+    for (double* ptr : {c->PF, c->PR, c->IR})
+      if (hipMemsetAsync(ptr, 0, pcount * sizeof(double), c->stream) != hipSuccess) return fail(KB_ERR_HIP);
+    if (hipMemsetAsync(c->Iring, 0, rcount * sizeof(double), c->stream) != hipSuccess) return fail(KB_ERR_HIP);
+    if (hipFuncSetAttribute((const void*)ninv_resum_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)ninv_lds(640)) != hipSuccess) {
+      seterr(c, KB_ERR_ARG, "resummation kernel: LDS attribute rejected");
+      return fail(KB_ERR_ARG);
+    }
+  }
   // twiddles exp(-2 pi i k / Nx) and the default k^2 table
   std::vector<double2> tw(p->Nx);
   for (int k = 0; k < p->Nx; ++k) {
@@ -546,6 +692,7 @@ int kb_init_rows(kb_ctx* c, const double* F11, const double* F21, const double* 
   if ((rc = launch_hartree(c, 2, 2))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->initialised = true;
+  c->ninv_valid = false;
   return KB_OK;
 }
 
@@ -568,6 +715,7 @@ int kb_init_particles(kb_ctx* c, const double* n, double initmass) {
   if ((rc = launch_hartree(c, 2, 2))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->initialised = true;
+  c->ninv_valid = false;
   c->initmass = initmass;
   c->init_particles = true;
   return KB_OK;
@@ -580,6 +728,10 @@ int kb_step(kb_ctx* c, int t) {
   if (t != c->st.tmax + 1 || t < 3)
     return seterr(c, KB_ERR_WINDOW, "evolve: t=%d but window is [%d,%d]", t, c->st.tmin, c->st.tmax);
   expand(c);  // the device state is advanced by the step's first kernel
+  if (c->p.pexp == KB_PEXP_NINVERSE_NLO && !c->ninv_valid) {
+    int rc = ninv_rebuild(c);
+    if (rc) return rc;
+  }
   if (c->adj) {
     // recording for the adjoint: eager launches with the tape kept
     int rc;
@@ -644,7 +796,7 @@ int kb_measure(kb_ctx* c, int t, kb_meas* out, double* F, double* ddF, double* r
   a.tmin = c->st.tmin;
   a.t = t;
   a.dt = c->p.dt;
-  a.integral = c->p.pexp == KB_PEXP_LAMBDA_NLO;
+  a.integral = has_sigma(c);
   const bool want = F || ddF || r || n || omega || cc;
   if (want) {
     a.outF = c->modes;
@@ -667,7 +819,7 @@ int kb_measure(kb_ctx* c, int t, kb_meas* out, double* F, double* ddF, double* r
   for (int b = 0; b < g; ++b)
     for (int k = 0; k < 6; ++k) s[k] += part[(size_t)b * 6 + k];
   const double dt = c->p.dt;
-  const double H = c->p.Lambda * (double)(c->p.ONgroup + 2) / (6.0 * (double)c->p.ONgroup);
+  const double H = hartree_prefactor(c);
   const double ivol = 1.0 / (double)c->Vl;
   out->time = (double)(t - 1) * dt * c->p.Mass;
   out->E_kin = s[0];
@@ -729,6 +881,7 @@ int kb_set_entry(kb_ctx* c, int which, int i, int j, const double* in) {
                      pair_off(si, sj, c->N), which, sg, c->vbuf);
   LAUNCHCHK(c);
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->ninv_valid = false;
   return KB_OK;
 }
 
@@ -737,6 +890,27 @@ int kb_get_sigma(kb_ctx* c, int which, int tp, double* out) {
   if (!inwin(c, tp)) return seterr(c, KB_ERR_WINDOW, "sigma: %d outside window [%d,%d]", tp, c->st.tmin, c->st.tmax);
   hipLaunchKernelGGL(gather_sigma, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->S, c->N, c->V,
                      slot_of(c, tp), which, c->vbuf);
+  LAUNCHCHK(c);
+  return lat_download(c, out, c->vbuf);
+}
+
+int kb_ninv_get(kb_ctx* c, int which, int i, int j, double* out) {
+  if (!c || !out || which < 0 || which > 4) return KB_ERR_ARG;
+  if (c->p.pexp != KB_PEXP_NINVERSE_NLO) return seterr(c, KB_ERR_STATE, "no 1/N resummation in this context");
+  if (which >= 3) {
+    if (!inwin(c, j) || c->st.tmax < 3) return seterr(c, KB_ERR_WINDOW, "ring: %d outside window [%d,%d]", j, c->st.tmin, c->st.tmax);
+    hipLaunchKernelGGL(gather_sigma, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->Iring, c->N, c->V,
+                       slot_of(c, j), which == 3 ? 0 : 1, c->vbuf);
+    LAUNCHCHK(c);
+    return lat_download(c, out, c->vbuf);
+  }
+  if (!inwin(c, i) || !inwin(c, j) || (which == 2 && i <= j))
+    return seterr(c, KB_ERR_WINDOW, "1/N entry (%d,%d) outside window [%d,%d]", i, j, c->st.tmin, c->st.tmax);
+  const int si = slot_of(c, i), sj = slot_of(c, j);
+  const double* src = which == 0 ? c->PF : which == 1 ? c->PR : c->IR;
+  const double sg = which == 1 ? (si >= sj ? 1.0 : -1.0) : 1.0;
+  hipLaunchKernelGGL(gather_tri, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, src, c->NP, c->V,
+                     pair_off(si, sj, c->N), sg, c->vbuf);
   LAUNCHCHK(c);
   return lat_download(c, out, c->vbuf);
 }
@@ -773,6 +947,7 @@ int kb_fill_free_window(kb_ctx* c, int tmax, const double* n, double mass) {
   c->st.tmax = tmax;
   if ((rc = upload_state(c))) return rc;
   c->initialised = true;
+  c->ninv_valid = false;
   if ((rc = launch_hartree(c, tmax, tmax))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return KB_OK;
@@ -1037,6 +1212,7 @@ int kb_ens_create(const kb_params* params, int nruns, kb_ens** out) {
   };
   for (int i = 0; i < nruns; ++i) {
     if (params[i].device != e->device) return fail(KB_ERR_ARG);
+    if (params[i].pexp == KB_PEXP_NINVERSE_NLO) return fail(KB_ERR_ARG);  // the 1/N stage runs per context
     kb_ctx* c = nullptr;
     int rc = kb_create(&params[i], &c);
     if (rc) return fail(rc);

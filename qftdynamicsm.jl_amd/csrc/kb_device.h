@@ -23,6 +23,11 @@
 //   one contiguous region that the memory-integral kernel sweeps in order.
 //
 // Self-energy ring: S[mb][slot][lane][comp], comp 0 = SigF, 1 = Sigr.
+//
+// 1/N expansion (NinverseNLO): three more slot triangles with one value per
+// pair and site, P[mb][pair_off][lane] -- the bubbles F2kr2 and Fr (Fr
+// sign-folded like rho) and the resummed kernel Ir(later, earlier) -- and
+// the ring of resummed kernels (IF, Ir)(t-1, t') in the self-energy layout.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -50,6 +55,12 @@ __host__ __device__ inline long pair_off(int a, int b, int N) {
 // double offset of (block, pair, lane), component F; component r is +1
 __host__ __device__ inline size_t hoff(int mb, long pair, int lane, long NP) {
   return ((((size_t)mb * (size_t)NP + (size_t)pair) * MB) + lane) * 2;
+}
+
+// one-component slot triangles (the 1/N bubbles and resummed kernel):
+// P[mb][pair_off][lane]
+__host__ __device__ inline size_t poff(int mb, long pair, int lane, long NP) {
+  return (((size_t)mb * (size_t)NP + (size_t)pair) * MB) + lane;
 }
 
 __host__ __device__ inline size_t soff(int mb, int slot, int lane, int N) {

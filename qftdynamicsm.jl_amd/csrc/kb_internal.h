@@ -84,6 +84,17 @@ struct kb_ctx {
   double tsum[qftkb_host::NSTAGE] = {};
   long long tcount[qftkb_host::NSTAGE] = {};
   qftkb_host::AdjState* adj = nullptr;  // recording / adjoint state
+  // 1/N expansion (NinverseNLO): bubble triangles F2kr2 and Fr, the
+  // resummed-kernel triangle Ir, the (IF, Ir) ring, the bubble pass's
+  // transforms (Fx, rx) for the chain pass, and one step state per row for
+  // rebuilding the rows of a window that was written from outside a step
+  double* PF = nullptr;
+  double* PR = nullptr;
+  double* IR = nullptr;
+  double* Iring = nullptr;
+  double2* Xn = nullptr;
+  qftkb::StepState* stfill_dev = nullptr;
+  bool ninv_valid = false;  // bubble and Ir rows cover the window
   std::string err;
 };
 
@@ -118,6 +129,12 @@ void drop_graph(kb_ctx* c);
 // self-energy stage of the step in c->st_dev into S_out (default: the
 // context's ring); X, if set, receives the transforms (Fx, rx)
 int launch_sigma(kb_ctx* c, double2* X = nullptr, double* S_out = nullptr);
+// Hartreeprefactor of the context's expansion (Initialization.jl:245-247)
+double hartree_prefactor(const kb_ctx* c);
+// expansions with a self-energy (TwoPIScalarNLO, Problemdefinition.jl:48)
+inline bool has_sigma(const kb_ctx* c) {
+  return c->p.pexp == KB_PEXP_LAMBDA_NLO || c->p.pexp == KB_PEXP_NINVERSE_NLO;
+}
 // the kernels of one forward step (the window in c->st_dev is advanced first)
 int enqueue_step(kb_ctx* c);
 size_t memint_lds(int N);

@@ -4,6 +4,9 @@
 
 #include "kb_device.h"
 
+// kernels and launch helpers are internal to the library
+#pragma GCC visibility push(hidden)
+
 namespace qftkb {
 
 constexpr int FFT_MAXF = 24;
@@ -72,6 +75,9 @@ __host__ __device__ inline StepState state_of_step(int t, int N) {
   return s;
 }
 
+// modes of the plane kernels (kb_sigma.hip)
+enum SigmaMode { SG_SUNSET = 0, SG_SUNSET_ADJ = 1, SG_BUBBLE = 2, SG_CHAIN = 3 };
+
 struct SigmaArgs {
   const double* H;
   double* S;
@@ -91,7 +97,35 @@ struct SigmaArgs {
   // orbit's representative, else -1
   const int* smap;
   const int* rmap;
+  // 1/N expansion: bubble triangles written by SG_BUBBLE, (IF, Ir) ring read
+  // by SG_CHAIN, 1/V
+  double* PF;
+  double* PR;
+  const double* Iring;
+  double ivol;
 };
+
+// The resummation sweeps of the 1/N expansion (kb_ninverse.hip): one
+// workgroup per site block; st->t - 1 is the time whose row of Ir and IF is
+// computed.
+struct NinvArgs {
+  const double* PF;  // bubble triangles
+  const double* PR;
+  double* IR;        // resummed-kernel triangle Ir(later, earlier)
+  double* Iring;     // (IF, Ir)(t-1, t') per slot, self-energy layout
+  const int2* items;
+  int nitems;
+  long NP;
+  int N, V;
+  double dt, lam;
+  int passes;        // NINV_PASS_* bits
+  const StepState* st;
+};
+enum NinvPasses { NINV_PASS_IR = 1, NINV_PASS_ALL = 7 };
+constexpr int NINV_THREADS = 256;
+constexpr int NINV_ROWS = 64;
+__global__ void ninv_resum_kernel(NinvArgs a);
+inline size_t ninv_lds(int N) { return sizeof(double) * 2 * (size_t)N * MB; }
 
 // Per-run arguments of memint_step; step-dependent values come from *st.
 struct MemintArgs {
@@ -150,6 +184,7 @@ __global__ void history_kernel(const double* H, const double* S, long NP, int N,
 __global__ void gather_pair(const double* H, long NP, int V, long pair, int comp, double sign, double* out);
 __global__ void scatter_pair(double* H, long NP, int V, long pair, int comp, double sign, const double* in);
 __global__ void gather_sigma(const double* S, int N, int V, int slot, int comp, double* out);
+__global__ void gather_tri(const double* P, long NP, int V, long pair, double sign, double* out);
 __global__ void init_rows_kernel(double* H, long NP, int V, int N, const double* F11, const double* F21,
                                  const double* F22, double dt);
 __global__ void init_particles_kernel(const double* kL2, const double* n, double m, double dt, int V, double* F11,
@@ -166,9 +201,13 @@ constexpr int NFFT_VARIANTS = 3;
 extern const int kFFTThreads[NFFT_VARIANTS];
 hipError_t launch_sigma_planes(const SigmaArgs& a, int rows, hipStream_t st, int variant = 0);
 hipError_t launch_sigma_planes_adjoint(const SigmaArgs& a, int rows, hipStream_t st);
+hipError_t launch_bubble_planes(const SigmaArgs& a, int rows, hipStream_t st);
+hipError_t launch_chain_planes(const SigmaArgs& a, int rows, hipStream_t st);
 
 constexpr int MEMINT_THREADS = 256;  // default workgroup of the memory-integral kernels
 constexpr int MEMINT_ROWS = 64;  // rows per work item of the adjoint kernels
 constexpr int MEAS_THREADS = 256;
 
 }  // namespace qftkb
+
+#pragma GCC visibility pop

@@ -709,6 +709,12 @@ __global__ void scatter_pair(double* __restrict__ H, long NP, int V, long pair, 
   if (p < V) H[hoff(p / MB, pair, p % MB, NP) + comp] = in[p] * sign;
 }
 
+__global__ void gather_tri(const double* __restrict__ P, long NP, int V, long pair, double sign,
+                           double* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < V) out[p] = P[poff(p / MB, pair, p % MB, NP)] * sign;
+}
+
 __global__ void gather_sigma(const double* __restrict__ S, int N, int V, int slot, int comp,
                              double* __restrict__ out) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;

@@ -16,13 +16,24 @@
 //       plane_inv  inverse 2D transform, scale, write SigF/Sigr.
 //   2D / 1D: plane_fwd does everything for its row in one workgroup.
 //
-// The same passes run the adjoint of the stage (ADJ = true): the input is
+// The same passes run the adjoint of the stage (SG_SUNSET_ADJ): the input is
 // the adjoint self-energy pair (SigF_bar, Sigr_bar) per t', the pointwise
 // step is the transpose of the sunset's Jacobian at the saved forward
 // transforms (Fx, rx), and the output is added to the adjoint history row
 // t-1.  The cosine transform C (real part of the DFT of an even lattice) is
 // symmetric, so Sigma = pre C g(C F, C rho) has the adjoint
 //   g_bar = pre C Sigma_bar,  F_bar += C (dg/dFx)^T g_bar,  rho_bar += C (dg/drx)^T g_bar.
+//
+// Two more modes serve the 1/N expansion (NinverseNLO):
+//   BUBBLE  gathers F(t-1,t'), rho(t-1,t') like the sunset, keeps (Fx, rx)
+//           in X and transforms Fx^2 - rx^2/4 and Fx rx back: the bubbles
+//           F2kr2 = ivol^2 C[Fx^2 - rx^2/4] and Fr = ivol^2 C[Fx rx] of the
+//           pair (t-1, t') (calc_Fr_F2kr2!, SimDataStruct.jl:726-750), stored
+//           as row t-1 of two slot triangles;
+//   CHAIN   gathers the resummed kernels (IF, Ir)(t') in momentum space,
+//           transforms them to Irx = ivol C Ir, IFx = ivol C IF and forms
+//           Sigr = pre C[Fx Irx + rx IFx], SigF = pre C[Fx IFx - rx Irx / 4]
+//           with the saved (Fx, rx) (calcSelfEnergyattp!, Evolution.jl:683-698).
 //
 // All transforms are unnormalised Stockham FFTs in LDS; lines of a plane are
 // padded to Nx+1 elements so that column access does not hit one bank.
@@ -207,34 +218,48 @@ __device__ __forceinline__ double2 sunset_adj(double2 cs, double2 x, double preF
   return make_double2(gF * (3.0 * FF - 0.75 * rr) + gr * (2.0 * rx * Fx), gF * (-1.5 * Fx * rx) + gr * (FF - 0.25 * rr));
 }
 
-// the pointwise step of either direction; the forward saves (Fx, rx) when X is set
-template <bool ADJ>
+// the pointwise step of each mode; the sunset and bubble save (Fx, rx) in X when it is set
+template <int MODE>
 __device__ __forceinline__ double2 pointwise(const SigmaArgs& a, double2 z, size_t xi) {
-  if constexpr (ADJ) {
+  if constexpr (MODE == SG_SUNSET_ADJ) {
     return sunset_adj(z, a.X[xi], a.preF, a.preR);
+  } else if constexpr (MODE == SG_BUBBLE) {
+    if (a.X) a.X[xi] = z;
+    return make_double2(z.x * z.x - 0.25 * (z.y * z.y), z.x * z.y);
+  } else if constexpr (MODE == SG_CHAIN) {
+    const double2 x = a.X[xi];  // (Fx, rx) of (t-1, t')
+    const double IFx = a.ivol * z.x, Irx = a.ivol * z.y;
+    return make_double2(x.x * IFx - 0.25 * (x.y * Irx), x.x * Irx + x.y * IFx);
   } else {
     if (a.X) a.X[xi] = z;
     return sunset(z);
   }
 }
 
-// output of the stage for row q (time t'), site s: Sigma into the ring, or
-// the adjoint added to (F_bar, rho_bar)(t-1, t')
-template <bool ADJ>
+// output of the stage for row q (time t'), site s: Sigma into the ring, the
+// adjoint added to (F_bar, rho_bar)(t-1, t'), or the bubbles of (t-1, t')
+template <int MODE>
 __device__ __forceinline__ void emit(const SigmaArgs& a, int s, int slot, double2 z) {
-  if constexpr (ADJ) {
+  if constexpr (MODE == SG_SUNSET_ADJ) {
     const int s_tm1 = a.st->s_tm1;
     double* o = a.Hbar + hoff(s / MB, pair_off(s_tm1, slot, a.N), s % MB, a.NP);
     o[0] += z.x;
     if (slot != s_tm1) o[1] += (s_tm1 >= slot ? 1.0 : -1.0) * z.y;  // rho(t-1,t-1) is not a variable
   } else {
     if (a.rmap) {
-      s = a.rmap[s];  // orbit storage: the representative writes its orbit's Sigma
+      s = a.rmap[s];  // orbit storage: the representative writes its orbit's value
       if (s < 0) return;
     }
-    double* o = a.S + soff(s / MB, slot, s % MB, a.N);
-    o[0] = a.preF * z.x;
-    o[1] = a.preR * z.y;
+    if constexpr (MODE == SG_BUBBLE) {
+      const int s_tm1 = a.st->s_tm1;
+      const size_t o = poff(s / MB, pair_off(s_tm1, slot, a.N), s % MB, a.NP);
+      a.PF[o] = a.preF * z.x;
+      a.PR[o] = (s_tm1 >= slot ? a.preR : -a.preR) * z.y;  // sign-folded like rho
+    } else {
+      double* o = a.S + soff(s / MB, slot, s % MB, a.N);
+      o[0] = a.preF * z.x;
+      o[1] = a.preR * z.y;
+    }
   }
 }
 
@@ -242,7 +267,7 @@ __device__ __forceinline__ void emit(const SigmaArgs& a, int s, int slot, double
 
 // ---------------------------------------------------------------------------
 // blockIdx.x: plane iz (3D: 0 .. nz/2), blockIdx.y: row q (t' = tmin + q)
-template <int NT, bool ADJ, int TPB>
+template <int NT, int MODE, int TPB>
 __global__ __launch_bounds__(TPB) void sigma_plane_fwd(SigmaArgs a) {
   extern __shared__ double2 lds[];
   const int n = NT > 0 ? NT : a.n;
@@ -268,9 +293,12 @@ __global__ __launch_bounds__(TPB) void sigma_plane_fwd(SigmaArgs a) {
     const int iy = e / n, ix = e - iy * n;
     const int s1 = ix + n * (iy + ny * iz);
     const int s2 = (n - ix) % n + n * ((ny - iy) % ny + ny * izn);
-    if constexpr (ADJ) {
-      const double* h1 = a.Sbar + soff(s1 / MB, slot, s1 % MB, a.N);
-      const double* h2 = a.Sbar + soff(s2 / MB, slot, s2 % MB, a.N);
+    if constexpr (MODE == SG_SUNSET_ADJ || MODE == SG_CHAIN) {
+      // a ring in self-energy layout: (SigF_bar, Sigr_bar) or (IF, Ir)
+      const double* ring = MODE == SG_CHAIN ? a.Iring : a.Sbar;
+      const int q1 = a.smap ? a.smap[s1] : s1, q2 = a.smap ? a.smap[s2] : s2;
+      const double* h1 = ring + soff(q1 / MB, slot, q1 % MB, a.N);
+      const double* h2 = ring + soff(q2 / MB, slot, q2 % MB, a.N);
       A[iy * ls + ix] = make_double2(0.5 * (h1[0] + h2[0]), 0.5 * (h1[1] + h2[1]));
     } else {
       const int q1 = a.smap ? a.smap[s1] : s1, q2 = a.smap ? a.smap[s2] : s2;
@@ -290,7 +318,7 @@ __global__ __launch_bounds__(TPB) void sigma_plane_fwd(SigmaArgs a) {
     // 1D / 2D: the whole lattice is in LDS -- sunset, inverse, write Sigma
     for (int e = threadIdx.x; e < nplane; e += blockDim.x) {
       const int iy = e / n, ix = e - iy * n;
-      res[iy * ls + ix] = pointwise<ADJ>(a, res[iy * ls + ix], rowoff + e);
+      res[iy * ls + ix] = pointwise<MODE>(a, res[iy * ls + ix], rowoff + e);
     }
     __syncthreads();
     if (ny > 1) {
@@ -300,7 +328,7 @@ __global__ __launch_bounds__(TPB) void sigma_plane_fwd(SigmaArgs a) {
     res = fft_lines<NT>(res, oth, a.plan, ny, ls, 1, a.tw, true);
     for (int e = threadIdx.x; e < nplane; e += blockDim.x) {
       const int iy = e / n, ix = e - iy * n;
-      emit<ADJ>(a, ix + n * iy, slot, res[iy * ls + ix]);
+      emit<MODE>(a, ix + n * iy, slot, res[iy * ls + ix]);
     }
     return;
   }
@@ -315,7 +343,7 @@ __global__ __launch_bounds__(TPB) void sigma_plane_fwd(SigmaArgs a) {
 }
 
 // blockIdx.x: y row iy, blockIdx.y: row q.  Lines along z for every x.
-template <int NT, bool ADJ, int TPB>
+template <int NT, int MODE, int TPB>
 __global__ __launch_bounds__(TPB) void sigma_zpass(SigmaArgs a) {
   extern __shared__ double2 lds[];
   const int n = NT > 0 ? NT : a.n;
@@ -336,7 +364,7 @@ __global__ __launch_bounds__(TPB) void sigma_zpass(SigmaArgs a) {
   double2* oth = res == A ? B : A;
   for (int e = threadIdx.x; e < n * nz; e += blockDim.x) {
     const int ix = e / nz, iz = e - ix * nz;
-    res[ix * ls + iz] = pointwise<ADJ>(a, res[ix * ls + iz], rowoff + ix + n * (iy + ny * iz));
+    res[ix * ls + iz] = pointwise<MODE>(a, res[ix * ls + iz], rowoff + ix + n * (iy + ny * iz));
   }
   __syncthreads();
   res = fft_lines<NT>(res, oth, a.plan, n, ls, 1, a.tw, true);
@@ -347,7 +375,7 @@ __global__ __launch_bounds__(TPB) void sigma_zpass(SigmaArgs a) {
 }
 
 // blockIdx.x: plane iz, blockIdx.y: row q.  Inverse 2D transform + scatter.
-template <int NT, bool ADJ, int TPB>
+template <int NT, int MODE, int TPB>
 __global__ __launch_bounds__(TPB) void sigma_plane_inv(SigmaArgs a) {
   extern __shared__ double2 lds[];
   const int n = NT > 0 ? NT : a.n;
@@ -372,50 +400,52 @@ __global__ __launch_bounds__(TPB) void sigma_plane_inv(SigmaArgs a) {
   res = fft_lines<NT>(res, oth, a.plan, ny, ls, 1, a.tw, true);
   for (int e = threadIdx.x; e < nplane; e += blockDim.x) {
     const int iy = e / n, ix = e - iy * n;
-    emit<ADJ>(a, ix + n * (iy + ny * iz), slot, res[iy * ls + ix]);
+    emit<MODE>(a, ix + n * (iy + ny * iz), slot, res[iy * ls + ix]);
   }
 }
 
 // ---------------------------------------------------------------------------
 // host-side launch of the whole stage
 // ---------------------------------------------------------------------------
-template <int NT, bool ADJ, int TPB>
+template <int NT, int MODE, int TPB>
 static hipError_t launch_planes_t(const SigmaArgs& a, int L, hipStream_t st) {  // L: grid rows
   const int n = a.n;
   const size_t lds_plane = sizeof(double2) * 2 * (size_t)a.ny * (n + 1);
   if (a.nz == 1) {
-    hipLaunchKernelGGL((sigma_plane_fwd<NT, ADJ, TPB>), dim3(1, L), dim3(TPB), lds_plane, st, a);
+    hipLaunchKernelGGL((sigma_plane_fwd<NT, MODE, TPB>), dim3(1, L), dim3(TPB), lds_plane, st, a);
     return hipGetLastError();
   }
   const size_t lds_z = sizeof(double2) * 2 * (size_t)n * (a.nz + 1);
-  hipLaunchKernelGGL((sigma_plane_fwd<NT, ADJ, TPB>), dim3(a.nz / 2 + 1, L), dim3(TPB), lds_plane, st, a);
+  hipLaunchKernelGGL((sigma_plane_fwd<NT, MODE, TPB>), dim3(a.nz / 2 + 1, L), dim3(TPB), lds_plane, st, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((sigma_zpass<NT, ADJ, TPB>), dim3(a.ny, L), dim3(TPB), lds_z, st, a);
+  hipLaunchKernelGGL((sigma_zpass<NT, MODE, TPB>), dim3(a.ny, L), dim3(TPB), lds_z, st, a);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((sigma_plane_inv<NT, ADJ, TPB>), dim3(a.nz, L), dim3(TPB), lds_plane, st, a);
+  hipLaunchKernelGGL((sigma_plane_inv<NT, MODE, TPB>), dim3(a.nz, L), dim3(TPB), lds_plane, st, a);
   return hipGetLastError();
 }
 
 bool sigma_planes_supported(int Nx) { return Nx >= 1 && Nx <= 64; }
 
-template <int NT, bool ADJ, int TPB>
+template <int NT, int MODE, int TPB>
 static hipError_t prepare_one(size_t lds) {
-  hipError_t e = hipFuncSetAttribute((const void*)sigma_plane_fwd<NT, ADJ, TPB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipError_t e = hipFuncSetAttribute((const void*)sigma_plane_fwd<NT, MODE, TPB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void*)sigma_zpass<NT, ADJ, TPB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  e = hipFuncSetAttribute((const void*)sigma_zpass<NT, MODE, TPB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void*)sigma_plane_inv<NT, ADJ, TPB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  return hipFuncSetAttribute((const void*)sigma_plane_inv<NT, MODE, TPB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
 template <int NT>
 static hipError_t prepare_t(size_t lds) {
-  hipError_t e = prepare_one<NT, false, 256>(lds);
-  if (e == hipSuccess) e = prepare_one<NT, false, 128>(lds);
-  if (e == hipSuccess) e = prepare_one<NT, false, 512>(lds);
+  hipError_t e = prepare_one<NT, SG_SUNSET, 256>(lds);
+  if (e == hipSuccess) e = prepare_one<NT, SG_SUNSET, 128>(lds);
+  if (e == hipSuccess) e = prepare_one<NT, SG_SUNSET, 512>(lds);
+  if (e == hipSuccess) e = prepare_one<NT, SG_BUBBLE, 256>(lds);
+  if (e == hipSuccess) e = prepare_one<NT, SG_CHAIN, 256>(lds);
   if (e != hipSuccess) return e;
-  return prepare_one<NT, true, 256>(lds);
+  return prepare_one<NT, SG_SUNSET_ADJ, 256>(lds);
 }
 
 // allow the dynamic LDS of the plane kernels for lattice size n (the
@@ -435,17 +465,17 @@ hipError_t sigma_planes_prepare(int n) {
   }
 }
 
-template <bool ADJ, int TPB>
+template <int MODE, int TPB>
 static hipError_t launch_planes_n(const SigmaArgs& a, int L, hipStream_t st) {  // L: grid rows
   switch (a.n) {
-    case 8: return launch_planes_t<8, ADJ, TPB>(a, L, st);
-    case 16: return launch_planes_t<16, ADJ, TPB>(a, L, st);
-    case 24: return launch_planes_t<24, ADJ, TPB>(a, L, st);
-    case 32: return launch_planes_t<32, ADJ, TPB>(a, L, st);
-    case 40: return launch_planes_t<40, ADJ, TPB>(a, L, st);
-    case 48: return launch_planes_t<48, ADJ, TPB>(a, L, st);
-    case 64: return launch_planes_t<64, ADJ, TPB>(a, L, st);
-    default: return launch_planes_t<0, ADJ, TPB>(a, L, st);
+    case 8: return launch_planes_t<8, MODE, TPB>(a, L, st);
+    case 16: return launch_planes_t<16, MODE, TPB>(a, L, st);
+    case 24: return launch_planes_t<24, MODE, TPB>(a, L, st);
+    case 32: return launch_planes_t<32, MODE, TPB>(a, L, st);
+    case 40: return launch_planes_t<40, MODE, TPB>(a, L, st);
+    case 48: return launch_planes_t<48, MODE, TPB>(a, L, st);
+    case 64: return launch_planes_t<64, MODE, TPB>(a, L, st);
+    default: return launch_planes_t<0, MODE, TPB>(a, L, st);
   }
 }
 
@@ -454,14 +484,22 @@ const int kFFTThreads[NFFT_VARIANTS] = {256, 128, 512};
 
 hipError_t launch_sigma_planes(const SigmaArgs& a, int L, hipStream_t st, int variant) {
   switch (variant) {
-    case 1: return launch_planes_n<false, 128>(a, L, st);
-    case 2: return launch_planes_n<false, 512>(a, L, st);
-    default: return launch_planes_n<false, 256>(a, L, st);
+    case 1: return launch_planes_n<SG_SUNSET, 128>(a, L, st);
+    case 2: return launch_planes_n<SG_SUNSET, 512>(a, L, st);
+    default: return launch_planes_n<SG_SUNSET, 256>(a, L, st);
   }
 }
 
 hipError_t launch_sigma_planes_adjoint(const SigmaArgs& a, int L, hipStream_t st) {
-  return launch_planes_n<true, 256>(a, L, st);
+  return launch_planes_n<SG_SUNSET_ADJ, 256>(a, L, st);
+}
+
+hipError_t launch_bubble_planes(const SigmaArgs& a, int L, hipStream_t st) {
+  return launch_planes_n<SG_BUBBLE, 256>(a, L, st);
+}
+
+hipError_t launch_chain_planes(const SigmaArgs& a, int L, hipStream_t st) {
+  return launch_planes_n<SG_CHAIN, 256>(a, L, st);
 }
 
 }  // namespace qftkb

@@ -26,12 +26,16 @@ KB_ERR_NONFINITE = 6
 
 KB_PEXP_LAMBDA_LO = 0
 KB_PEXP_LAMBDA_NLO = 1
+KB_PEXP_NINVERSE_LO = 2
+KB_PEXP_NINVERSE_NLO = 3
+PEXP_CODES = {"LambdaLO": KB_PEXP_LAMBDA_LO, "LambdaNLO": KB_PEXP_LAMBDA_NLO, "NinverseLO": KB_PEXP_NINVERSE_LO,
+              "NinverseNLO": KB_PEXP_NINVERSE_NLO}
 
 # every symbol include/qftkb.h declares
 EXPORTED = (
     "kb_create", "kb_destroy", "kb_last_error", "kb_info", "kb_get_kL2", "kb_set_kL2",
     "kb_init_particles", "kb_init_rows", "kb_step", "kb_window", "kb_hartree", "kb_measure",
-    "kb_measure_history", "kb_get_entry", "kb_set_entry", "kb_get_sigma", "kb_check_finite",
+    "kb_measure_history", "kb_get_entry", "kb_set_entry", "kb_get_sigma", "kb_ninv_get", "kb_check_finite",
     "kb_fill_free_window", "kb_synchronize", "kb_set_stream", "kb_enable_timing", "kb_timing",
     "kb_reset_timing", "kb_use_graph", "kb_ens_create", "kb_ens_destroy", "kb_ens_last_error", "kb_ens_member",
     "kb_ens_step", "kb_ens_use_graph", "kb_ens_synchronize", "kb_ens_enable_timing", "kb_ens_timing",
@@ -98,6 +102,7 @@ def load_library(path: str = LIB_PATH):
     L.kb_get_entry.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
     L.kb_set_entry.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
     L.kb_get_sigma.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp]
+    L.kb_ninv_get.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
     L.kb_check_finite.argtypes = [vp]
     L.kb_fill_free_window.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_double]
     L.kb_synchronize.argtypes = [vp]
@@ -325,6 +330,16 @@ class KBContext:
     def sigma(self, which, tp):
         out = np.empty(self.V)
         self._chk(self._L.kb_get_sigma(self._h, 1 if which == "r" else 0, int(tp), _ptr(out)))
+        return out
+
+    _NINV = {"F2kr2": 0, "Fr": 1, "Ir": 2, "IF_row": 3, "Ir_row": 4}
+
+    def ninv(self, which, i, j):
+        """NinverseNLO internals in momentum space: 'F2kr2' / 'Fr' bubbles of
+        the pair (i, j), 'Ir' resummed kernel Ir(i, j) (i > j), 'IF_row' /
+        'Ir_row' the last step's IF(t-1, j) / Ir(t-1, j) (i ignored)."""
+        out = np.empty(self.V)
+        self._chk(self._L.kb_ninv_get(self._h, self._NINV[which], int(i), int(j), _ptr(out)))
         return out
 
     def check_finite(self):

@@ -21,7 +21,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import lattice
-from ._abi import KB_PEXP_LAMBDA_LO, KB_PEXP_LAMBDA_NLO, KBContext
+from ._abi import PEXP_CODES, KBContext
 
 
 class TwoPIScalarfile:
@@ -180,8 +180,8 @@ def _make_problem(modelfile, parameters: dict) -> TwoPIScalarProblem:
         raise ValueError(f"unknown init {kind!r}")
     init = Init(kind, {k: parameters[k] for k in _INIT_KEYS[kind]})
     pexp = parameters["Pexp"]
-    if pexp not in ("LambdaLO", "LambdaNLO"):
-        raise NotImplementedError(f"Pexp={pexp!r}: this back-end implements LambdaLO and LambdaNLO")
+    if pexp not in PEXP_CODES:
+        raise ValueError(f"unknown Pexp {pexp!r}")
     reno = parameters["Reno"]
     if reno not in ("RNone", "RMass"):
         raise ValueError(f"unknown Reno {reno!r}")
@@ -238,7 +238,7 @@ class TwoPIScalarSimDataGPU:
 
     def __init__(self, problem: TwoPIScalarProblem, device: int = 0):
         d, m = problem.disc, problem.model
-        pexp = KB_PEXP_LAMBDA_NLO if problem.pexp == "LambdaNLO" else KB_PEXP_LAMBDA_LO
+        pexp = PEXP_CODES[problem.pexp]
         self.ctx = KBContext(d.Nx, d.sdim, d.NstepsinMemory, d.dt, m.Lambda, m.ONgroup, d.Mass, m.Mass2,
                              pexp=pexp, device=device)
         self.ctx.set_kL2(d.kL2)

@@ -70,8 +70,14 @@ def test_problem_factory_and_errors():
         _make_problem(TwoPIScalarfile(), base_params(Num="CPUfull"))
     with pytest.raises(NotImplementedError):
         _make_problem(TwoPIScalarfile(), base_params(init="Quench"))
-    with pytest.raises(NotImplementedError):
-        _make_problem(TwoPIScalarfile(), base_params(Pexp="NinverseNLO"))
+    with pytest.raises(ValueError):
+        _make_problem(TwoPIScalarfile(), base_params(Pexp="LambdaNNLO"))
+    for pexp in ("NinverseLO", "NinverseNLO"):
+        assert _make_problem(TwoPIScalarfile(), base_params(Pexp=pexp)).pexp == pexp
+    # Hartreeprefactor per expansion (Initialization.jl:245-247) in the RMass counterterm
+    lo = _make_problem(TwoPIScalarfile(), base_params(Pexp="NinverseLO", Reno="RMass", Lambda=3.0, ONgroup=4, Nx=4))
+    nlo = _make_problem(TwoPIScalarfile(), base_params(Pexp="NinverseNLO", Reno="RMass", Lambda=3.0, ONgroup=4, Nx=4))
+    assert (lo.model.Mass2 - 1.0) / (nlo.model.Mass2 - 1.0) == pytest.approx((1 / 6) / (6 / 24), rel=1e-12)
 
 
 def test_counterterm_rmass():
