@@ -41,6 +41,17 @@ def bytes_memint(N, V):
     return ((N - 1) * N + 2 * (N - 1) + 2 * N) * V * 8
 
 
+def bytes_resum(N, V):
+    """Algorithmic bytes of one 1/N resummation launch over a full window of
+    L = N-1 live times: sweep 1 reads the Ir triangle without row T
+    ((L-1)(L-2)/2 values per site), sweep 2 the F2kr2 triangle with its
+    diagonal (L(L+1)/2), sweep 3 the Ir triangle with row T (L(L-1)/2); plus
+    Fr row T, F2kr2 row T and Fr column tmin read, Ir row T and the (IF, Ir)
+    ring written (6L)."""
+    L = N - 1
+    return ((L - 1) * (L - 2) // 2 + L * (L + 1) // 2 + L * (L - 1) // 2 + 6 * L) * V * 8
+
+
 def load_pmc(path):
     try:
         with open(path) as f:
@@ -327,6 +338,9 @@ def main():
     ap.add_argument("--no-fd", action="store_true", help="with --adjoint: skip the finite differences")
     ap.add_argument("--autotune", metavar="OUT_JSON",
                     help="pick the memory-integral tile shape for the standard workloads and write the choices")
+    ap.add_argument("--ninverse", action="store_true",
+                    help="the same workload in the 1/N expansion at NLO (a separate metric; the roofline "
+                         "line is the resummation kernel's)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 2 GiB device copy (torch) to scale rocprofv3 FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
@@ -360,8 +374,9 @@ def main():
     w["Nx"] = args.nx
     N = w["NstepsinMemory"]
     V = w["Nx"] ** w["sdim"]
+    pexp = 3 if args.ninverse else 1
     ctx = KBContext(w["Nx"], w["sdim"], N, w["dt"], w["Lambda"], w["ONgroup"], w["Mass"], w["Mass"] ** 2,
-                    symmetric=args.sym)
+                    symmetric=args.sym, pexp=pexp)
     Vs = ctx.storage_info()[0]  # stored sites: V, or the cubic orbits
     tile_shape = list(ctx.tile_shapes()[ctx.tiles()]) + [ctx.fft_shapes()[ctx.fft_tiles()]]
     n = thermal_n(ctx.kL2(), w["Mass"], w["T"])
@@ -384,6 +399,7 @@ def main():
     ms_mi, n_mi = ctx.timing(0)
     ms_fft, n_fft = ctx.timing(1)
     ms_h, _ = ctx.timing(2)
+    ms_rs, n_rs = ctx.timing(5)
     ctx.enable_timing(False)
     ctx.check_finite()
     print(f"[bench] {args.steps} steps in {elapsed:.4f} s; memint {ms_mi / max(n_mi, 1):.3f} ms, "
@@ -398,10 +414,14 @@ def main():
     traffic = None
     if pmc and pmc.get("Nx") == w["Nx"] and pmc.get("N") == N and not args.sym:
         traffic = pmc.get("traffic_bytes")
-    if args.sym:
+    if args.sym or args.ninverse:
         args.no_adjoint = args.no_cpu = True
+    if args.ninverse:
+        traffic = None
     line = {
-        "metric": ("KB steps/s at 32^3 N_mem=400 FP64 (% HBM peak); adjoint/forward cost ratio" if not args.sym else
+        "metric": ("KB steps/s at 32^3 N_mem=400 FP64 (% HBM peak); adjoint/forward cost ratio"
+                   if not (args.sym or args.ninverse) else
+                   "KB steps/s at 32^3 N_mem=400 FP64, 1/N expansion at NLO (separate metric)" if args.ninverse else
                    "KB steps/s at 32^3 N_mem=400 FP64 with cubic-orbit storage (separate metric)"),
         "value": steps_per_s,
         "unit": "KB steps/s",
@@ -415,8 +435,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic (window filled with free thermal propagators, as the reference benchmark fills "
                 "its window without physics)",
-        "config": {"workload": f"2PI KB step, LambdaNLO sunset, phi^4 O(1), {w['Nx']}^3 lattice, "
-                               f"N_mem={N}, thermal T=2, lambda=1, m=1, dt=0.1",
+        "config": {"workload": f"2PI KB step, {'NinverseNLO bubble chain' if args.ninverse else 'LambdaNLO sunset'}, "
+                               f"phi^4 O(1), {w['Nx']}^3 lattice, N_mem={N}, thermal T=2, lambda=1, m=1, dt=0.1",
                    "Nx": w["Nx"], "sdim": 3, "NstepsinMemory": N, "modes": V, "stored_sites": Vs,
                    "storage": "cubic orbits" if args.sym else "every site"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -428,6 +448,20 @@ def main():
         "cpu_baseline": None,
         "adjoint": None,
     }
+    if args.ninverse:
+        avg_rs_s = ms_rs / max(n_rs, 1) / 1e3
+        Br = bytes_resum(N, Vs)
+        ach = Br / avg_rs_s / 1e9
+        line["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "ninv_resum_kernel",
+                            "algorithmic_bytes": Br, "avg_kernel_ms": avg_rs_s * 1e3,
+                            "memint_ms": avg_mi_s * 1e3, "memint_frac": achieved / HBM_PEAK_GBS,
+                            "self_energy_stage_ms": ms_fft / max(n_fft, 1),
+                            "fft_passes_ms": ms_fft / max(n_fft, 1) - avg_rs_s * 1e3,
+                            "hartree_ms": ms_h / max(n_mi, 1)}
+        line["cpu_baseline"] = {"value": None, "note": "the oracle restates the reference's CPUfull 1/N code, "
+                                "which transforms every term of the time sums (N_mem^2 lattice FFTs per step); "
+                                "not timed at this size"}
     if not args.no_adjoint:
         line["adjoint"] = adjoint_bench()
     if not args.no_cpu:

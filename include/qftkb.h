@@ -176,8 +176,9 @@ int kb_synchronize(kb_ctx *ctx);
 int kb_set_stream(kb_ctx *ctx, void *hip_stream);
 /* HIP-event timing of the kernels of kb_step, recorded on the launch
  * stream.  stage: 0 = memory integral + leapfrog kernel, 1 = self-energy
- * FFT stage (all its passes), 2 = Hartree reduction, 3 = adjoint memory
- * integral, 4 = the other kernels of the adjoint sweep.  kb_timing
+ * stage (all its FFT passes; for NinverseNLO also the resummation), 2 =
+ * Hartree reduction, 3 = adjoint memory integral, 4 = the other kernels of
+ * the adjoint sweep, 5 = the NinverseNLO resummation sweeps alone.  kb_timing
  * synchronises and returns the accumulated milliseconds and launch count
  * since the last reset. */
 int kb_enable_timing(kb_ctx *ctx, int on);

@@ -199,6 +199,7 @@ SigmaArgs ninv_sigma_args(kb_ctx* c, const StepState* st) {
   a.PF = c->PF;
   a.PR = c->PR;
   a.Iring = c->Iring;
+  a.PNP = c->PNP;
   a.ivol = ivol;
   return a;
 }
@@ -219,9 +220,9 @@ int launch_resum(kb_ctx* c, const StepState* st, int passes) {
   a.PR = c->PR;
   a.IR = c->IR;
   a.Iring = c->Iring;
-  a.items = c->items;
-  a.nitems = c->nitems;
-  a.NP = c->NP;
+  a.items = c->pitems;
+  a.nitems = c->npitems;
+  a.PNP = c->PNP;
   a.N = c->N;
   a.V = c->V;
   a.dt = c->p.dt;
@@ -247,7 +248,10 @@ int launch_chain(kb_ctx* c) {
 int launch_ninv_stage(kb_ctx* c) {
   int rc;
   if ((rc = launch_bubble(c, c->st_dev, c->Xn))) return rc;
-  if ((rc = launch_resum(c, c->st_dev, NINV_PASS_ALL))) return rc;
+  {
+    StageTimer st(c, 5);
+    if ((rc = launch_resum(c, c->st_dev, NINV_PASS_ALL))) return rc;
+  }
   return launch_chain(c);
 }
 
@@ -382,7 +386,7 @@ void free_all(kb_ctx* c) {
                     (void*)c->modes, (void*)c->vbuf, (void*)c->tw, (void*)c->flag, (void*)c->items_r[0],
                     (void*)c->items_r[1], (void*)c->items_r[2], (void*)c->smap_dev, (void*)c->rmap_dev,
                     (void*)c->wgt_dev, (void*)c->PF, (void*)c->PR, (void*)c->IR, (void*)c->Iring, (void*)c->Xn,
-                    (void*)c->stfill_dev,
+                    (void*)c->stfill_dev, (void*)c->pitems,
                     (void*)c->st_dev, (void*)c->stptr_dev, (void*)c->N_dev, (void*)c->margs_dev, (void*)c->wg0_dev})
     if (ptr) (void)hipFree(ptr);
   adj_free(c);
@@ -520,7 +524,8 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
   if ((rc = dmalloc(c, (void**)&c->H, hcount * sizeof(double)))) return fail(rc);
   if ((rc = dmalloc(c, (void**)&c->S, (size_t)c->NB * c->N * MB * 2 * sizeof(double)))) return fail(rc);
   if ((rc = dmalloc(c, (void**)&c->W, (size_t)c->N * c->Vl * sizeof(double2)))) return fail(rc);
-  const size_t pcount = (size_t)c->NB * (size_t)c->NP * MB;  // one value per pair and site
+  c->PNP = pslot_pairs(c->N);
+  const size_t pcount = (size_t)c->NB * (size_t)c->PNP * MB;  // one value per pair and site
   const size_t rcount = (size_t)c->NB * c->N * MB * 2;
   if (p->pexp == KB_PEXP_NINVERSE_NLO) {
     if ((rc = dmalloc(c, (void**)&c->PF, pcount * sizeof(double))) ||
@@ -530,6 +535,13 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
         (rc = dmalloc(c, (void**)&c->Xn, (size_t)c->N * c->Vl * sizeof(double2))) ||
         (rc = dmalloc(c, (void**)&c->stfill_dev, (size_t)c->N * sizeof(StepState))))
       return fail(rc);
+    std::vector<int2> items;
+    for (int J = 0; J * PCB < c->N; ++J)
+      for (int k = 0; J * PCB + k * NINV_ROWS < c->N; ++k) items.push_back(make_int2(J, k));
+    c->npitems = (int)items.size();
+    if ((rc = dmalloc(c, (void**)&c->pitems, sizeof(int2) * items.size()))) return fail(rc);
+    if (hipMemcpy(c->pitems, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(KB_ERR_HIP);
   }
   if (sym) {
     std::vector<int> rmap(c->Vl, -1);
@@ -909,8 +921,8 @@ int kb_ninv_get(kb_ctx* c, int which, int i, int j, double* out) {
   const int si = slot_of(c, i), sj = slot_of(c, j);
   const double* src = which == 0 ? c->PF : which == 1 ? c->PR : c->IR;
   const double sg = which == 1 ? (si >= sj ? 1.0 : -1.0) : 1.0;
-  hipLaunchKernelGGL(gather_tri, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, src, c->NP, c->V,
-                     pair_off(si, sj, c->N), sg, c->vbuf);
+  hipLaunchKernelGGL(gather_tri, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, src, c->PNP, c->V,
+                     ppair_off(si, sj, c->N), sg, c->vbuf);
   LAUNCHCHK(c);
   return lat_download(c, out, c->vbuf);
 }

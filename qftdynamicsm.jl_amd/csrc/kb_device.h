@@ -25,7 +25,7 @@
 // Self-energy ring: S[mb][slot][lane][comp], comp 0 = SigF, 1 = Sigr.
 //
 // 1/N expansion (NinverseNLO): three more slot triangles with one value per
-// pair and site, P[mb][pair_off][lane] -- the bubbles F2kr2 and Fr (Fr
+// pair and site (layout at poff below) -- the bubbles F2kr2 and Fr (Fr
 // sign-folded like rho) and the resummed kernel Ir(later, earlier) -- and
 // the ring of resummed kernels (IF, Ir)(t-1, t') in the self-energy layout.
 #pragma once
@@ -57,10 +57,30 @@ __host__ __device__ inline size_t hoff(int mb, long pair, int lane, long NP) {
   return ((((size_t)mb * (size_t)NP + (size_t)pair) * MB) + lane) * 2;
 }
 
-// one-component slot triangles (the 1/N bubbles and resummed kernel):
-// P[mb][pair_off][lane]
-__host__ __device__ inline size_t poff(int mb, long pair, int lane, long NP) {
-  return (((size_t)mb * (size_t)NP + (size_t)pair) * MB) + lane;
+// One-component slot triangles (the 1/N bubbles and resummed kernel) use
+// column blocks of PCB slots; inside one row of a column block, columns
+// 2k, 2k+1 of all MB lanes are stored together:
+//   P[mb][pp / PCB][(pp % PCB) / 2][lane][pp % 2],  pp = ppair_off(slot_i, slot_j),
+// so one 16-byte load per lane reads two columns and the MB lanes of a
+// worker read 128 contiguous bytes.
+constexpr int PCB = 4;
+
+__host__ __device__ inline long pcolblock_offset(int J, int N) {
+  return (long)PCB * ((long)N * J - (long)PCB * J * (J - 1) / 2);
+}
+__host__ __device__ inline long pslot_pairs(int N) { return pcolblock_offset((N + PCB - 1) / PCB, N); }
+__host__ __device__ inline long ppair_off(int a, int b, int N) {
+  const int hi = a > b ? a : b;
+  const int lo = a > b ? b : a;
+  const int J = lo / PCB;
+  return pcolblock_offset(J, N) + (long)(hi - J * PCB) * PCB + (lo - J * PCB);
+}
+// offset of pair pp relative to poff(mb, 0, lane, PNP)
+__host__ __device__ inline size_t prel(long pp) {
+  return (size_t)(pp & ~(long)(PCB - 1)) * MB + (size_t)((pp & (PCB - 1)) >> 1) * (2 * MB) + (size_t)(pp & 1);
+}
+__host__ __device__ inline size_t poff(int mb, long pp, int lane, long PNP) {
+  return (size_t)mb * (size_t)PNP * MB + (size_t)lane * 2 + prel(pp);
 }
 
 __host__ __device__ inline size_t soff(int mb, int slot, int lane, int N) {

@@ -20,8 +20,9 @@ struct EventPair {
 };
 
 // stages of kb_timing: 0 memory integral, 1 self-energy, 2 Hartree,
-// 3 adjoint memory integral, 4 the other kernels of the adjoint sweep
-constexpr int NSTAGE = 5;
+// 3 adjoint memory integral, 4 the other kernels of the adjoint sweep,
+// 5 the 1/N resummation sweeps
+constexpr int NSTAGE = 6;
 
 struct AdjState;  // kb_adjoint.hip
 
@@ -88,6 +89,9 @@ struct kb_ctx {
   // resummed-kernel triangle Ir, the (IF, Ir) ring, the bubble pass's
   // transforms (Fx, rx) for the chain pass, and one step state per row for
   // rebuilding the rows of a window that was written from outside a step
+  long PNP = 0;            // slot pairs per site block of those triangles (PCB-wide column blocks)
+  int2* pitems = nullptr;  // their sweeps' work items: (column block, 64-row chunk)
+  int npitems = 0;
   double* PF = nullptr;
   double* PR = nullptr;
   double* IR = nullptr;

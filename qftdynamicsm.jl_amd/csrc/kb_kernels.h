@@ -102,6 +102,7 @@ struct SigmaArgs {
   double* PF;
   double* PR;
   const double* Iring;
+  long PNP;  // slot pairs per site block of the bubble triangles
   double ivol;
 };
 
@@ -113,17 +114,25 @@ struct NinvArgs {
   const double* PR;
   double* IR;        // resummed-kernel triangle Ir(later, earlier)
   double* Iring;     // (IF, Ir)(t-1, t') per slot, self-energy layout
-  const int2* items;
+  const int2* items;  // (column block of PCB slots, row chunk) work items
   int nitems;
-  long NP;
+  long PNP;
   int N, V;
   double dt, lam;
   int passes;        // NINV_PASS_* bits
   const StepState* st;
 };
 enum NinvPasses { NINV_PASS_IR = 1, NINV_PASS_ALL = 7 };
-constexpr int NINV_THREADS = 256;
-constexpr int NINV_ROWS = 64;
+// tile of the resummation sweeps: threads per workgroup, slot rows per work
+// item (build-time; the defaults were picked by timing at 32^3, N_mem = 400)
+#ifndef KB_NINV_THREADS
+#define KB_NINV_THREADS 512
+#endif
+#ifndef KB_NINV_ROWS
+#define KB_NINV_ROWS 128
+#endif
+constexpr int NINV_THREADS = KB_NINV_THREADS;
+constexpr int NINV_ROWS = KB_NINV_ROWS;
 __global__ void ninv_resum_kernel(NinvArgs a);
 inline size_t ninv_lds(int N) { return sizeof(double) * 2 * (size_t)N * MB; }
 
@@ -184,7 +193,7 @@ __global__ void history_kernel(const double* H, const double* S, long NP, int N,
 __global__ void gather_pair(const double* H, long NP, int V, long pair, int comp, double sign, double* out);
 __global__ void scatter_pair(double* H, long NP, int V, long pair, int comp, double sign, const double* in);
 __global__ void gather_sigma(const double* S, int N, int V, int slot, int comp, double* out);
-__global__ void gather_tri(const double* P, long NP, int V, long pair, double sign, double* out);
+__global__ void gather_tri(const double* P, long PNP, int V, long pp, double sign, double* out);
 __global__ void init_rows_kernel(double* H, long NP, int V, int N, const double* F11, const double* F21,
                                  const double* F22, double dt);
 __global__ void init_particles_kernel(const double* kL2, const double* n, double m, double dt, int V, double* F11,

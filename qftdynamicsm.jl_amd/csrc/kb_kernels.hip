@@ -709,10 +709,10 @@ __global__ void scatter_pair(double* __restrict__ H, long NP, int V, long pair, 
   if (p < V) H[hoff(p / MB, pair, p % MB, NP) + comp] = in[p] * sign;
 }
 
-__global__ void gather_tri(const double* __restrict__ P, long NP, int V, long pair, double sign,
+__global__ void gather_tri(const double* __restrict__ P, long PNP, int V, long pp, double sign,
                            double* __restrict__ out) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < V) out[p] = P[poff(p / MB, pair, p % MB, NP)] * sign;
+  if (p < V) out[p] = P[poff(p / MB, pp, p % MB, PNP)] * sign;
 }
 
 __global__ void gather_sigma(const double* __restrict__ S, int N, int V, int slot, int comp,

@@ -46,9 +46,22 @@ __device__ __forceinline__ int rank_of(int s, int smin, int N) {
   return r < 0 ? r + N : r;
 }
 
-__device__ __forceinline__ double ld_nt(const double* p) { return __builtin_nontemporal_load(p); }
+typedef double nv_d2 __attribute__((ext_vector_type(2)));
+// the PCB values of one row of a column block (one 16-byte load per column
+// pair; p points at the row's first column for this lane), streamed
+__device__ __forceinline__ void ld_row(const double* p, double (&v)[PCB]) {
+#pragma unroll
+  for (int k = 0; k < PCB / 2; ++k) {
+    const nv_d2 a = __builtin_nontemporal_load(reinterpret_cast<const nv_d2*>(p + (size_t)k * 2 * MB));
+    v[2 * k] = a.x;
+    v[2 * k + 1] = a.y;
+  }
+}
 
-constexpr int NB_ROWS = 4;  // rows per batch, two batches in flight
+#ifndef KB_NINV_BATCH
+#define KB_NINV_BATCH 2
+#endif
+constexpr int NB_ROWS = KB_NINV_BATCH;  // rows per batch, two batches in flight
 
 // One sweep over the stored pairs (r >= c in slot space) of one triangle of
 // this site block.  With i the later and j the earlier time of the pair:
@@ -58,6 +71,21 @@ constexpr int NB_ROWS = 4;  // rows per batch, two batches in flight
 // contributions of their pairs are dropped by select, so values not yet
 // written for this window never enter a live sum.
 template <int MODE>
+__device__ __forceinline__ void pair_term(double v, bool ok, bool nrm, double cr, double cc, double& row,
+                                          double& col) {
+  if (MODE == SW_SYM) {
+    row += ok ? cc * v : 0.0;
+    col += ok ? cr * v : 0.0;
+  } else if (MODE == SW_COL) {
+    col += (ok && nrm) ? cr * v : 0.0;
+    row += (ok && !nrm) ? cc * v : 0.0;
+  } else {
+    row += (ok && nrm) ? cc * v : 0.0;
+    col += (ok && !nrm) ? cr * v : 0.0;
+  }
+}
+
+template <int MODE>
 __device__ void sweep(const double* __restrict__ Tb, const double* cv, double* acc, const int2* __restrict__ items,
                       int nitems, int* next_item, int N, int Rmax, int smin, int L, int excl, int lane, int worker,
                       int nworkers) {
@@ -65,15 +93,15 @@ __device__ void sweep(const double* __restrict__ Tb, const double* cv, double* a
   int item = worker;
   while (item < nitems) {
     const int2 jk = items[item];
-    const int c0 = jk.x * CB;
+    const int c0 = jk.x * PCB;
     const int r0 = c0 + jk.y * NINV_ROWS;
     const int r1 = min(r0 + NINV_ROWS, Rmax);
     if (c0 < Rmax && r0 < r1) {
-      const long cbase = colblock_offset(jk.x, N);
-      double cc[CB], col[CB];
-      int rc[CB];
+      const long cbase = pcolblock_offset(jk.x, N);
+      double cc[PCB], col[PCB];
+      int rc[PCB];
 #pragma unroll
-      for (int jc = 0; jc < CB; ++jc) {
+      for (int jc = 0; jc < PCB; ++jc) {
         const int c = c0 + jc;
         rc[jc] = c == excl ? N : rank_of(c, smin, N);
         const bool ok = c < Rmax && rc[jc] < L;
@@ -83,49 +111,37 @@ __device__ void sweep(const double* __restrict__ Tb, const double* cv, double* a
       }
       int r = r0;
       // rows that cross the diagonal of the column block
-      for (; r < r1 && r < c0 + CB; ++r) {
+      for (; r < r1 && r < c0 + PCB; ++r) {
         const int rr = r == excl ? N : rank_of(r, smin, N);
         const bool okr = rr < L;
         const double cr = okr ? cv[r * MB + lane] : 0.0;
         double row = 0.0;
-        const long pb = cbase + (long)(r - c0) * CB;
-        for (int jc = 0; jc < CB; ++jc) {
+        const long pb = cbase + (long)(r - c0) * PCB;
+        for (int jc = 0; jc < PCB; ++jc) {
           const int c = c0 + jc;
           if (c > r) break;
-          const double v = Tb[(size_t)(pb + jc) * MB];
+          const double v = Tb[prel(pb + jc)];
           const bool ok = okr && rc[jc] < L;
           if (c == r) {
             if (MODE == SW_SYM) row += ok ? cr * v : 0.0;
             continue;
           }
-          const bool nrm = rr > rc[jc];  // r is the later time
-          if (MODE == SW_SYM) {
-            row += ok ? cc[jc] * v : 0.0;
-            col[jc] += ok ? cr * v : 0.0;
-          } else if (MODE == SW_COL) {
-            col[jc] += (ok && nrm) ? cr * v : 0.0;
-            row += (ok && !nrm) ? cc[jc] * v : 0.0;
-          } else {
-            row += (ok && nrm) ? cc[jc] * v : 0.0;
-            col[jc] += (ok && !nrm) ? cr * v : 0.0;
-          }
+          pair_term<MODE>(v, ok, rr > rc[jc], cr, cc[jc], row, col[jc]);
         }
         atomicAdd(&acc[r * MB + lane], row);
       }
       // full rows, NB_ROWS per batch, two batches in flight; rows past r1 are
       // clamped to r1 - 1 (re-read, not used)
       if (r < r1) {
-        double X[NB_ROWS][CB], Y[NB_ROWS][CB];
-        auto load = [&](double (&b)[NB_ROWS][CB], int rs) {
+        double X[NB_ROWS][PCB], Y[NB_ROWS][PCB];
+        auto load = [&](double (&b)[NB_ROWS][PCB], int rs) {
 #pragma unroll
           for (int h = 0; h < NB_ROWS; ++h) {
             const int rr = min(rs + h, r1 - 1);
-            const long pb = cbase + (long)(rr - c0) * CB;
-#pragma unroll
-            for (int jc = 0; jc < CB; ++jc) b[h][jc] = ld_nt(Tb + (size_t)(pb + jc) * MB);
+            ld_row(Tb + prel(cbase + (long)(rr - c0) * PCB), b[h]);  // row start: a multiple of PCB
           }
         };
-        auto process = [&](const double (&b)[NB_ROWS][CB], int rs) {
+        auto process = [&](const double (&b)[NB_ROWS][PCB], int rs) {
 #pragma unroll
           for (int h = 0; h < NB_ROWS; ++h) {
             const int rw = rs + h;
@@ -135,21 +151,8 @@ __device__ void sweep(const double* __restrict__ Tb, const double* cv, double* a
               const double cr = okr ? cv[rw * MB + lane] : 0.0;
               double row = 0.0;
 #pragma unroll
-              for (int jc = 0; jc < CB; ++jc) {
-                const double v = b[h][jc];
-                const bool ok = okr && rc[jc] < L;
-                const bool nrm = rr > rc[jc];
-                if (MODE == SW_SYM) {
-                  row += ok ? cc[jc] * v : 0.0;
-                  col[jc] += ok ? cr * v : 0.0;
-                } else if (MODE == SW_COL) {
-                  col[jc] += (ok && nrm) ? cr * v : 0.0;
-                  row += (ok && !nrm) ? cc[jc] * v : 0.0;
-                } else {
-                  row += (ok && nrm) ? cc[jc] * v : 0.0;
-                  col[jc] += (ok && !nrm) ? cr * v : 0.0;
-                }
-              }
+              for (int jc = 0; jc < PCB; ++jc)
+                pair_term<MODE>(b[h][jc], okr && rc[jc] < L, rr > rc[jc], cr, cc[jc], row, col[jc]);
               atomicAdd(&acc[rw * MB + lane], row);
             }
           }
@@ -167,7 +170,7 @@ __device__ void sweep(const double* __restrict__ Tb, const double* cv, double* a
         }
       }
 #pragma unroll
-      for (int jc = 0; jc < CB; ++jc) {
+      for (int jc = 0; jc < PCB; ++jc) {
         const int c = c0 + jc;
         if (c < Rmax) atomicAdd(&acc[c * MB + lane], col[jc]);
       }
@@ -198,9 +201,9 @@ __global__ __launch_bounds__(NINV_THREADS) void ninv_resum_kernel(NinvArgs a) {
   const int sT = ss.s_tm1;  // slot of T = t - 1
   const double dt = a.dt;
   const double a3 = a.lam / 3.0, a6 = a.lam / 6.0;
-  const double* __restrict__ PFb = a.PF + poff(mb, 0, lane, a.NP);
-  const double* __restrict__ PRb = a.PR + poff(mb, 0, lane, a.NP);
-  double* __restrict__ IRb = a.IR + poff(mb, 0, lane, a.NP);
+  const double* __restrict__ PFb = a.PF + poff(mb, 0, lane, a.PNP);
+  const double* __restrict__ PRb = a.PR + poff(mb, 0, lane, a.PNP);
+  double* __restrict__ IRb = a.IR + poff(mb, 0, lane, a.PNP);
   double* __restrict__ Ib = a.Iring + soff(mb, 0, lane, N);
 
   // sweep 1: column sums of IR with Fr(T, i); row T of IR is not written
@@ -209,7 +212,7 @@ __global__ __launch_bounds__(NINV_THREADS) void ninv_resum_kernel(NinvArgs a) {
     const int rk = rank_of(x, smin, N);
     double v = 0.0;
     if (x < Rmax && rk < L && x != sT) {
-      const double pr = PRb[(size_t)pair_off(sT, x, N) * MB];
+      const double pr = PRb[prel(ppair_off(sT, x, N))];
       v = sT >= x ? pr : -pr;
     }
     cv[x * MB + lane] = v;
@@ -226,10 +229,10 @@ __global__ __launch_bounds__(NINV_THREADS) void ninv_resum_kernel(NinvArgs a) {
     if (x < Rmax && rk < L) {
       double ir = 0.0;
       if (x != sT) {
-        const long p = pair_off(sT, x, N);
-        const double pr = PRb[(size_t)p * MB];
+        const long p = ppair_off(sT, x, N);
+        const double pr = PRb[prel(p)];
         ir = a3 * ((sT >= x ? pr : -pr) - dt * acc[x * MB + lane]);
-        IRb[(size_t)p * MB] = ir;
+        IRb[prel(p)] = ir;
       }
       Ib[(size_t)x * MB * 2 + 1] = ir;
       w_ir = (rk == 0 ? 0.5 : 1.0) * ir;
@@ -246,7 +249,7 @@ __global__ __launch_bounds__(NINV_THREADS) void ninv_resum_kernel(NinvArgs a) {
   for (int x = worker; x < N; x += nworkers) {
     const int rk = rank_of(x, smin, N);
     double d = 0.0;
-    if (x < Rmax && rk < L) d = a6 * (PFb[(size_t)pair_off(sT, x, N) * MB] - dt * acc[x * MB + lane]);
+    if (x < Rmax && rk < L) d = a6 * (PFb[prel(ppair_off(sT, x, N))] - dt * acc[x * MB + lane]);
     cv[x * MB + lane] = d;
     acc[x * MB + lane] = 0.0;
   }
@@ -260,7 +263,7 @@ __global__ __launch_bounds__(NINV_THREADS) void ninv_resum_kernel(NinvArgs a) {
     const int rk = rank_of(x, smin, N);
     double dp = 0.0;
     if (x < Rmax && rk < L && rk > 0) {
-      const double pr = PRb[(size_t)pair_off(smin, x, N) * MB];
+      const double pr = PRb[prel(ppair_off(smin, x, N))];
       dp = cv[x * MB + lane] + half_c * dmin * (smin >= x ? pr : -pr);
     }
     cv[x * MB + lane] = dp;

@@ -252,7 +252,7 @@ __device__ __forceinline__ void emit(const SigmaArgs& a, int s, int slot, double
     }
     if constexpr (MODE == SG_BUBBLE) {
       const int s_tm1 = a.st->s_tm1;
-      const size_t o = poff(s / MB, pair_off(s_tm1, slot, a.N), s % MB, a.NP);
+      const size_t o = poff(s / MB, ppair_off(s_tm1, slot, a.N), s % MB, a.PNP);
       a.PF[o] = a.preF * z.x;
       a.PR[o] = (s_tm1 >= slot ? a.preR : -a.preR) * z.y;  // sign-folded like rho
     } else {

@@ -65,7 +65,9 @@ end
 
 pexpcode(::TwoPIScalarLambdaLO) = Cint(0)
 pexpcode(::TwoPIScalarLambdaNLOquantum) = Cint(1)
-pexpcode(p) = throw(ArgumentError("GPUfull implements LambdaLO and LambdaNLO (quantum), not $(typeof(p))"))
+pexpcode(::TwoPIScalarNinverseLO) = Cint(2)
+pexpcode(::TwoPIScalarNinverseNLO) = Cint(3)
+pexpcode(p) = throw(ArgumentError("GPUfull implements LambdaLO, LambdaNLO (quantum), NinverseLO and NinverseNLO, not $(typeof(p))"))
 
 # replaces the CPUfull branch of getTwoPISimData
 function getTwoPISimData(problem::QFTdynamicsProblem, ::TwoPIScalarGPUfull)
