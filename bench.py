@@ -351,7 +351,21 @@ def main():
         for _ in range(3):
             y.copy_(x)
         torch.cuda.synchronize()
-        print(json.dumps({"calibration": "copy", "bytes_read": x.numel() * 8, "bytes_written": y.numel() * 8}))
+        # streaming-read rate of a library reduction (fp64 sum over 17 GB), the
+        # practical ceiling the memory-integral kernel is compared with
+        z = torch.full((1 << 31,), 1.0, dtype=torch.float64, device="cuda")
+        for _ in range(2):
+            z.sum()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(10):
+            z.sum()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 10
+        print(json.dumps({"calibration": "copy", "bytes_read": x.numel() * 8, "bytes_written": y.numel() * 8,
+                          "sum_read_GBs": z.numel() * 8 / ms / 1e6}))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not supported "

@@ -221,3 +221,32 @@ def test_ninverse_unsupported_combinations_are_rejected():
     g.init_particles(np.zeros(64), 1.0)
     with pytest.raises(KBError):
         g.adj_record(max_steps=10)
+
+
+def test_ninverse_parameter_dict_run_matches_oracle():
+    """The reference's 1/N test problem (TwoPIScalar_Tests_NinverseNLO_
+    checkfastimplementation.jl:29-66: Mass 1, dt 0.01, lambda 10, N 8,
+    TopHatT1, RNone, full memory) run through the parameter-Dict driver; every
+    measurement's energies and n(p) against the oracle."""
+    from qftkb import TwoPIScalarfile, run
+    print("\n[case] 1/N parameter-Dict run 6^3, 12 steps", flush=True)
+    parameters = {"Mass": 1, "Nx": 6, "sdim": 3, "dt": 0.01, "Nsteps": 12, "NstepsinMemory": 0, "Nmeas": 0,
+                  "Mod": "phi4", "Lambda": 10, "ONgroup": 8, "init": "TopHatT1", "xi": 0.4, "eta": 5.0, "sig": 1.0,
+                  "T": 5, "n": 2, "Reno": "RNone", "Pexp": "NinverseNLO", "Num": "GPUfull", "Nchunks": 0,
+                  "tag": "testing", "ov": True}
+    sol = run(TwoPIScalarfile(), dict(parameters))
+    N = sol.problem.disc.NstepsinMemory
+    o = Oracle(6, 3, N, 0.01, 10.0, 8, 1.0, 1.0, pexp=NINV_NLO)
+    o.init_particles(tophat(o.kL2(), 3), 1.0)
+    ref = {2: o.measure(2)}
+    for t in range(3, 15):
+        o.evolve(t)
+        ref[t] = o.measure(t)
+    got = [m for m in sol.measurearray if m is not None]
+    assert len(got) == len(ref)
+    for m, t in zip(got, sorted(ref)):
+        mo = ref[t]
+        for k in ("E", "E_kin", "E_pot", "M2"):
+            assert abs(getattr(m, k) - mo[k]) <= RTOL * abs(mo[k]), (t, k)
+        assert abs(m.E_mem - mo["E_mem"]) <= RTOL * abs(mo["E"]), (t, "E_mem")
+        assert rel(m.n, mo["n"]) <= RTOL, (t, "n")
