@@ -3,6 +3,7 @@
 // simdata.indices, src/DefSimDataElements.jl:141-165) and kernel launches.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -13,6 +14,13 @@
 
 #include "../../include/qftkb.h"
 #include "kb_internal.h"
+
+#ifndef KB_MSPLIT_WGS
+#define KB_MSPLIT_WGS 512
+#endif
+#ifndef KB_MSPLIT_MAX
+#define KB_MSPLIT_MAX 8
+#endif
 
 using namespace qftkb;
 using namespace qftkb_host;
@@ -343,8 +351,17 @@ int enqueue_step(kb_ctx* c) {
   }
   {
     StageTimer st(c, 0);
-    hipError_t e = launch_memint(c->tile, c->NB, memint_lds(c->N), c->stream, c->margs_dev, c->wg0_dev, 1);
-    if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "memory-integral launch failed: %s", hipGetErrorString(e));
+    if (c->msplit > 1) {
+      hipLaunchKernelGGL((memint_partial<64, 2, 256>), dim3(c->NB * c->msplit), dim3(256), memint_lds(c->N),
+                         c->stream, c->margs_dev, c->msplit, c->mpart);
+      LAUNCHCHK(c);
+      hipLaunchKernelGGL(memint_finish, dim3(c->NB), dim3(MEMINT_THREADS), memint_lds(c->N), c->stream,
+                         c->margs_dev, c->msplit, c->mpart);
+      LAUNCHCHK(c);
+    } else {
+      hipError_t e = launch_memint(c->tile, c->NB, memint_lds(c->N), c->stream, c->margs_dev, c->wg0_dev, 1);
+      if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "memory-integral launch failed: %s", hipGetErrorString(e));
+    }
   }
   return KB_OK;
 }
@@ -386,7 +403,7 @@ void free_all(kb_ctx* c) {
                     (void*)c->modes, (void*)c->vbuf, (void*)c->tw, (void*)c->flag, (void*)c->items_r[0],
                     (void*)c->items_r[1], (void*)c->items_r[2], (void*)c->smap_dev, (void*)c->rmap_dev,
                     (void*)c->wgt_dev, (void*)c->PF, (void*)c->PR, (void*)c->IR, (void*)c->Iring, (void*)c->Xn,
-                    (void*)c->stfill_dev, (void*)c->pitems,
+                    (void*)c->stfill_dev, (void*)c->pitems, (void*)c->mpart,
                     (void*)c->st_dev, (void*)c->stptr_dev, (void*)c->N_dev, (void*)c->margs_dev, (void*)c->wg0_dev})
     if (ptr) (void)hipFree(ptr);
   adj_free(c);
@@ -557,6 +574,13 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
         hipMemcpy(c->wgt_dev, wgt.data(), sizeof(double) * c->V, hipMemcpyHostToDevice) != hipSuccess)
       return fail(KB_ERR_HIP);
   }
+  // orbit storage leaves fewer site blocks than CUs: split each block's
+  // memory integral over several workgroups (about KB_MSPLIT_WGS in all)
+  if (sym && c->NB < 256) {
+    c->msplit = std::min(KB_MSPLIT_MAX, (KB_MSPLIT_WGS + c->NB - 1) / c->NB);
+    if ((rc = dmalloc(c, (void**)&c->mpart, (size_t)c->NB * c->msplit * 2 * c->N * MB * sizeof(double))))
+      return fail(rc);
+  }
   if ((rc = dmalloc(c, (void**)&c->kL2, (size_t)c->Vpad * sizeof(double)))) return fail(rc);
   if ((rc = dmalloc(c, (void**)&c->m2, 8 * sizeof(double)))) return fail(rc);
   if ((rc = dmalloc(c, (void**)&c->partial, (size_t)6 * meas_grid(c) * sizeof(double)))) return fail(rc);
@@ -608,6 +632,11 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
     hipError_t e = hipSuccess;
     for (int v = 0; v < NMEMINT_VARIANTS && e == hipSuccess; ++v)
       e = hipFuncSetAttribute(kMemintVariants[v].fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute((const void*)memint_partial<64, 2, 256>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute((const void*)memint_finish, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) {
       seterr(c, KB_ERR_ARG, "memory-integral kernel: %zu bytes of LDS rejected: %s", lds, hipGetErrorString(e));
       return fail(KB_ERR_ARG);

@@ -64,6 +64,8 @@ struct kb_ctx {
   int2* items_r[3] = {};  // work items per row-chunk choice (MEMINT_ROW_CHOICES)
   int nitems_r[3] = {};
   int tile = 0;           // memory-integral tile shape (kMemintVariants index)
+  int msplit = 1;         // workgroups per site block of the memory integral (split form if > 1)
+  double* mpart = nullptr;  // their partial accumulators
   int fft_tile = 0;       // self-energy plane kernels' block size (kFFTThreads index)
   // device-resident step state and launch tables (graph-capturable steps)
   qftkb::StepState* st_dev = nullptr;

@@ -176,6 +176,12 @@ __global__ void hartree_step_kernel(const double* H, long NP, int V, int N, cons
 // one launch over all runs: workgroups [wg_start[r], wg_start[r+1]) belong to run r
 template <int ROWS, int B, int TPB>
 __global__ void memint_step(const MemintArgs* runs, const int* wg_start, int nruns);
+// split form for few site blocks: nsplit workgroups per site block write
+// partial accumulators (2 N MB doubles each) to part; memint_finish sums
+// them and runs the leapfrog (one run, grid = site blocks)
+template <int ROWS, int B, int TPB>
+__global__ void memint_partial(const MemintArgs* runs, int nsplit, double* part);
+__global__ void memint_finish(const MemintArgs* runs, int nsplit, const double* part);
 
 // memory-integral tile shapes: ROWS slot rows per work item, B rows per
 // batch (two batches in flight), threads per workgroup (one workgroup per

@@ -407,23 +407,114 @@ __device__ __forceinline__ void process_batch(const RowBatch<B>& b, int r, int r
   }
 }
 
-template <int ROWS, int B, int TPB>
-__global__ __launch_bounds__(TPB, 512 / TPB) void memint_step(const MemintArgs* __restrict__ runs,
-                                                                 const int* __restrict__ wg_start, int nruns) {
-  extern __shared__ double smem[];
-  // which run this workgroup belongs to (runs are few; wg_start is uniform)
-  int run = 0;
-  while (run + 1 < nruns && (int)blockIdx.x >= wg_start[run + 1]) ++run;
-  const MemintArgs a = runs[run];
-  const StepState ss = *a.st;
+// Sweep of work items [ibeg, iend) of one site block: memory-integral row
+// and column sums into the LDS accumulators (see above).  Workers start at
+// ibeg + worker and take further items from the LDS counter *next_item.
+template <int ROWS, int B>
+__device__ __forceinline__ void memint_sweep(const MemintArgs& a, const StepState& ss, int mb, double* accF,
+                                             double* accR, int* next_item, int ibeg, int iend) {
   const int N = a.N;
-  double* accF = smem;           // R_F per slot, lane
-  double* accR = accF + N * MB;  // R_r
-  __shared__ double red[TPB / 64][MB];
-  __shared__ double diagF[2][MB];
-  __shared__ int next_item;
+  const int tid = threadIdx.x;
+  const int lane = tid & (MB - 1);
+  const int worker = tid >> 3;
+  const int Rmax = ss.Rmax;
+  const int dead = ss.dead;
+  const int smin = ss.s_min;
+  const int L = ss.L;
+  const double* __restrict__ Hb = a.H + hoff(mb, 0, 0, a.NP) + lane * 2;
+  // self-energies of this block: (SigF, Sigr) per slot for this lane
+  const double2* __restrict__ Sb = reinterpret_cast<const double2*>(a.S + soff(mb, 0, lane, N));
+  // work items: (column block J, chunk of MI_ROWS rows), in storage order;
+  // the first item of each worker is static, later ones are taken from an
+  // LDS counter, so workers that finish early take more
+  int item = ibeg + worker;
+  while (item < iend) {
+    const int2 jk = a.items[item];
+    const int J = jk.x;
+    const int c0 = J * CB;
+    const int r0 = c0 + jk.y * ROWS;
+    const int r1 = min(r0 + ROWS, Rmax);
+    if (c0 < Rmax && r0 < r1) {
+      const long cbase = colblock_offset(J, N);
+      ColState cs;
+      cs.phi = 0;
+#pragma unroll
+      for (int jc = 0; jc < CB; ++jc) {
+        const int c = c0 + jc;
+        const bool ok = c < Rmax && c != dead;
+        const double2 fs = Sb[(size_t)min(c, N - 1) * MB];
+        double w, m;
+        slot_weights(c, smin, N, L, w, m);
+        cs.u[jc] = ok ? m * w * fs.y : 0.0;
+        cs.v[jc] = ok ? w * fs.x : 0.0;
+        cs.ms[jc] = ok ? m * fs.y : 0.0;
+        cs.cF[jc] = cs.cR[jc] = 0.0;
+        cs.phi |= (c < smin ? 1u : 0u) << jc;
+      }
+      int r = r0;
+      // rows crossing the diagonal of the block: columns c0 .. r only
+      for (; r < r1 && r < c0 + CB; ++r) {
+        const bool okr = r != dead;
+        const double2 fs = Sb[(size_t)r * MB];
+        double w, m;
+        slot_weights(r, smin, N, L, w, m);
+        const double ur = okr ? m * w * fs.y : 0.0;
+        const double vr = okr ? w * fs.x : 0.0;
+        const double msr = okr ? m * fs.y : 0.0;
+        const unsigned neg = r >= smin ? cs.phi : 0u;
+        const long pb = cbase + (long)(r - c0) * CB;
+        double rowF = 0.0, rowR = 0.0;
+        for (int jc = 0; jc < CB; ++jc) {
+          const int c = c0 + jc;
+          if (c > r) break;
+          const double2 e = *reinterpret_cast<const double2*>(Hb + (size_t)(pb + jc) * MB * 2);
+          if (c == r)
+            rowF -= ur * e.x;
+          else
+            pair_update(e.x, e.y, (neg >> jc) & 1u, cs.u[jc], cs.v[jc], cs.ms[jc], ur, vr, msr, rowF, rowR,
+                        cs.cF[jc], cs.cR[jc]);
+        }
+        atomicAdd(&accF[r * MB + lane], rowF);
+        atomicAdd(&accR[r * MB + lane], rowR);
+      }
+      // full rows: two batches of MI_B rows in flight
+      if (r < r1) {
+        const int rlast = r1 - 1;
+        RowBatch<B> X, Y;
+        load_batch(X, Hb, Sb, cbase, c0, r, rlast);
+        while (true) {
+          load_batch(Y, Hb, Sb, cbase, c0, r + B, rlast);
+          process_batch(X, r, r1, dead, smin, N, L, cs, accF, accR, lane);
+          r += B;
+          if (r >= r1) break;
+          load_batch(X, Hb, Sb, cbase, c0, r + B, rlast);
+          process_batch(Y, r, r1, dead, smin, N, L, cs, accF, accR, lane);
+          r += B;
+          if (r >= r1) break;
+        }
+      }
+#pragma unroll
+      for (int jc = 0; jc < CB; ++jc) {
+        const int c = c0 + jc;
+        if (c < Rmax && c != dead) {
+          atomicAdd(&accF[c * MB + lane], cs.cF[jc]);
+          atomicAdd(&accR[c * MB + lane], cs.cR[jc]);
+        }
+      }
+    }
+    // next item: one LDS atomic per worker
+    int nxt = 0;
+    if (lane == 0) nxt = atomicAdd(next_item, 1);
+    item = __shfl(nxt, tid & ~(MB - 1) & 63);
+  }
+}
 
-  const int mb = blockIdx.x - wg_start[run];
+// Leapfrog of row t and the diagonal from the accumulated memory integrals
+// of one site block; red holds blockDim.x / 64 rows of MB doubles.
+__device__ __forceinline__ void memint_leapfrog(const MemintArgs& a, const StepState& ss, int mb,
+                                                const double* accF, const double* accR, double (*red)[MB],
+                                                double (*diagF)[MB]) {
+  const int N = a.N;
   const int tid = threadIdx.x;
   const int lane = tid & (MB - 1);
   const int worker = tid >> 3;
@@ -432,105 +523,7 @@ __global__ __launch_bounds__(TPB, 512 / TPB) void memint_step(const MemintArgs* 
   const int dead = ss.dead;
   const int smin = ss.s_min;
   const int L = ss.L;
-
-  for (int i = tid; i < N * MB; i += blockDim.x) {
-    accF[i] = 0.0;
-    accR[i] = 0.0;
-  }
-  if (tid == 0) next_item = nworkers;
-  __syncthreads();
-
-  const double* __restrict__ Hb = a.H + hoff(mb, 0, 0, a.NP) + lane * 2;
-  // self-energies of this block: (SigF, Sigr) per slot for this lane
   const double2* __restrict__ Sb = reinterpret_cast<const double2*>(a.S + soff(mb, 0, lane, N));
-
-  if (a.integral) {
-    // work items: (column block J, chunk of MI_ROWS rows), in storage order;
-    // the first item of each worker is static, later ones are taken from an
-    // LDS counter, so workers that finish early take more
-    int item = worker;
-    while (item < a.nitems) {
-      const int2 jk = a.items[item];
-      const int J = jk.x;
-      const int c0 = J * CB;
-      const int r0 = c0 + jk.y * ROWS;
-      const int r1 = min(r0 + ROWS, Rmax);
-      if (c0 < Rmax && r0 < r1) {
-        const long cbase = colblock_offset(J, N);
-        ColState cs;
-        cs.phi = 0;
-#pragma unroll
-        for (int jc = 0; jc < CB; ++jc) {
-          const int c = c0 + jc;
-          const bool ok = c < Rmax && c != dead;
-          const double2 fs = Sb[(size_t)min(c, N - 1) * MB];
-          double w, m;
-          slot_weights(c, smin, N, L, w, m);
-          cs.u[jc] = ok ? m * w * fs.y : 0.0;
-          cs.v[jc] = ok ? w * fs.x : 0.0;
-          cs.ms[jc] = ok ? m * fs.y : 0.0;
-          cs.cF[jc] = cs.cR[jc] = 0.0;
-          cs.phi |= (c < smin ? 1u : 0u) << jc;
-        }
-        int r = r0;
-        // rows crossing the diagonal of the block: columns c0 .. r only
-        for (; r < r1 && r < c0 + CB; ++r) {
-          const bool okr = r != dead;
-          const double2 fs = Sb[(size_t)r * MB];
-          double w, m;
-          slot_weights(r, smin, N, L, w, m);
-          const double ur = okr ? m * w * fs.y : 0.0;
-          const double vr = okr ? w * fs.x : 0.0;
-          const double msr = okr ? m * fs.y : 0.0;
-          const unsigned neg = r >= smin ? cs.phi : 0u;
-          const long pb = cbase + (long)(r - c0) * CB;
-          double rowF = 0.0, rowR = 0.0;
-          for (int jc = 0; jc < CB; ++jc) {
-            const int c = c0 + jc;
-            if (c > r) break;
-            const double2 e = *reinterpret_cast<const double2*>(Hb + (size_t)(pb + jc) * MB * 2);
-            if (c == r)
-              rowF -= ur * e.x;
-            else
-              pair_update(e.x, e.y, (neg >> jc) & 1u, cs.u[jc], cs.v[jc], cs.ms[jc], ur, vr, msr, rowF, rowR,
-                          cs.cF[jc], cs.cR[jc]);
-          }
-          atomicAdd(&accF[r * MB + lane], rowF);
-          atomicAdd(&accR[r * MB + lane], rowR);
-        }
-        // full rows: two batches of MI_B rows in flight
-        if (r < r1) {
-          const int rlast = r1 - 1;
-          RowBatch<B> X, Y;
-          load_batch(X, Hb, Sb, cbase, c0, r, rlast);
-          while (true) {
-            load_batch(Y, Hb, Sb, cbase, c0, r + B, rlast);
-            process_batch(X, r, r1, dead, smin, N, L, cs, accF, accR, lane);
-            r += B;
-            if (r >= r1) break;
-            load_batch(X, Hb, Sb, cbase, c0, r + B, rlast);
-            process_batch(Y, r, r1, dead, smin, N, L, cs, accF, accR, lane);
-            r += B;
-            if (r >= r1) break;
-          }
-        }
-#pragma unroll
-        for (int jc = 0; jc < CB; ++jc) {
-          const int c = c0 + jc;
-          if (c < Rmax && c != dead) {
-            atomicAdd(&accF[c * MB + lane], cs.cF[jc]);
-            atomicAdd(&accR[c * MB + lane], cs.cR[jc]);
-          }
-        }
-      }
-      // next item: one LDS atomic per worker
-      int nxt = 0;
-      if (lane == 0) nxt = atomicAdd(&next_item, 1);
-      item = __shfl(nxt, tid & ~(MB - 1) & 63);
-    }
-  }
-  __syncthreads();
-
   // ---- leapfrog of row t (evolve_r!, evolve_F!: Evolution.jl:890-915) ----
   const int p = mb * MB + lane;
   const bool real_site = p < a.V;
@@ -597,6 +590,88 @@ __global__ __launch_bounds__(TPB, 512 / TPB) void memint_step(const MemintArgs* 
     }
   }
 }
+
+template <int ROWS, int B, int TPB>
+__global__ __launch_bounds__(TPB, 512 / TPB) void memint_step(const MemintArgs* __restrict__ runs,
+                                                                 const int* __restrict__ wg_start, int nruns) {
+  extern __shared__ double smem[];
+  // which run this workgroup belongs to (runs are few; wg_start is uniform)
+  int run = 0;
+  while (run + 1 < nruns && (int)blockIdx.x >= wg_start[run + 1]) ++run;
+  const MemintArgs a = runs[run];
+  const StepState ss = *a.st;
+  const int N = a.N;
+  double* accF = smem;           // R_F per slot, lane
+  double* accR = accF + N * MB;  // R_r
+  __shared__ double red[TPB / 64][MB];
+  __shared__ double diagF[2][MB];
+  __shared__ int next_item;
+  const int mb = blockIdx.x - wg_start[run];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < N * MB; i += blockDim.x) {
+    accF[i] = 0.0;
+    accR[i] = 0.0;
+  }
+  if (tid == 0) next_item = blockDim.x >> 3;
+  __syncthreads();
+  if (a.integral) memint_sweep<ROWS, B>(a, ss, mb, accF, accR, &next_item, 0, a.nitems);
+  __syncthreads();
+  memint_leapfrog(a, ss, mb, accF, accR, red, diagF);
+}
+
+// Split form for windows with few site blocks (cubic-orbit storage): each
+// site block's work items are cut into nsplit contiguous ranges, one
+// workgroup each; memint_partial writes its accumulators to part, and
+// memint_finish sums them and runs the leapfrog.
+template <int ROWS, int B, int TPB>
+__global__ __launch_bounds__(TPB, 512 / TPB) void memint_partial(const MemintArgs* __restrict__ runs, int nsplit,
+                                                                    double* __restrict__ part) {
+  extern __shared__ double smem[];
+  const MemintArgs a = runs[0];
+  const StepState ss = *a.st;
+  const int N = a.N;
+  double* accF = smem;
+  double* accR = accF + N * MB;
+  __shared__ int next_item;
+  const int mb = blockIdx.x / nsplit;
+  const int sp = blockIdx.x - mb * nsplit;
+  const int ibeg = (int)((long)a.nitems * sp / nsplit);
+  const int iend = (int)((long)a.nitems * (sp + 1) / nsplit);
+  const int tid = threadIdx.x;
+  for (int i = tid; i < N * MB; i += blockDim.x) {
+    accF[i] = 0.0;
+    accR[i] = 0.0;
+  }
+  if (tid == 0) next_item = ibeg + (blockDim.x >> 3);
+  __syncthreads();
+  if (a.integral) memint_sweep<ROWS, B>(a, ss, mb, accF, accR, &next_item, ibeg, iend);
+  __syncthreads();
+  double* out = part + (size_t)blockIdx.x * 2 * N * MB;
+  for (int i = tid; i < 2 * N * MB; i += blockDim.x) out[i] = smem[i];
+}
+
+__global__ __launch_bounds__(MEMINT_THREADS) void memint_finish(const MemintArgs* __restrict__ runs, int nsplit,
+                                                                const double* __restrict__ part) {
+  extern __shared__ double smem[];
+  const MemintArgs a = runs[0];
+  const StepState ss = *a.st;
+  const int N = a.N;
+  double* accF = smem;
+  double* accR = accF + N * MB;
+  __shared__ double red[MEMINT_THREADS / 64][MB];
+  __shared__ double diagF[2][MB];
+  const int mb = blockIdx.x;
+  const double* in = part + (size_t)mb * nsplit * 2 * N * MB;
+  for (int i = threadIdx.x; i < 2 * N * MB; i += blockDim.x) {
+    double v = 0.0;
+    for (int k = 0; k < nsplit; ++k) v += in[(size_t)k * 2 * N * MB + i];
+    smem[i] = v;
+  }
+  __syncthreads();
+  memint_leapfrog(a, ss, mb, accF, accR, red, diagF);
+}
+
+template __global__ void memint_partial<64, 2, 256>(const MemintArgs*, int, double*);
 
 // the tile shapes the autotuner chooses from; entry 0 is the default
 const MemintVariant kMemintVariants[NMEMINT_VARIANTS] = {

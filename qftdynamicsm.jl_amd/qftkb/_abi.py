@@ -157,7 +157,14 @@ def committed_tiles(Nx, sdim, NstepsinMemory, symmetric=False):
             table = json.load(f)
     except (OSError, ValueError):
         return None, None
-    e = table.get("choices", {}).get(tiles_key(Nx, sdim, NstepsinMemory, symmetric))
+    choices = table.get("choices", {})
+    e = choices.get(tiles_key(Nx, sdim, NstepsinMemory, symmetric))
+    only_fft = False
+    if e is None and symmetric:
+        # orbit storage transforms the same full lattice: reuse the full-storage
+        # self-energy choice (its memory integral runs in the split form)
+        e = choices.get(tiles_key(Nx, sdim, NstepsinMemory, False))
+        only_fft = True
     if e is None:
         return None, None
     L = load_library()
@@ -175,7 +182,7 @@ def committed_tiles(Nx, sdim, NstepsinMemory, symmetric=False):
             if n.value == ff.get("threads"):
                 f = i
             i += 1
-    return v, f
+    return (None if only_fft else v), f
 
 
 def _ptr(a):
