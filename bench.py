@@ -102,6 +102,49 @@ def cpu_baseline(w, budget_s=30.0):
                       f"32^3 N_mem={N} window, {len(times)} timed step(s) after fill{note}"}
 
 
+def ninv_transform_count(n):
+    """Full-lattice transforms of one 1/N self-energy stage in the reference's
+    CPUfull code (calcIrs!, calcIFs!, calcSelfEnergyattp!: Evolution.jl:
+    170-206, 354-436, 683-698) over a window of n times, by its loop bounds;
+    the oracle restates those loops and counts the same number."""
+    irs = 3 * (n - 1) + 2 * (n - 1) * (n - 2)
+    ifs = 0
+    for k in range(n):  # t' = tmin + k
+        first = max(0, k - 1)
+        beg = 1 if k == 0 else k
+        ifs += 2 + 8 * first + 4 * max(0, (n - 1) - beg) + 4 + (4 if k > 0 else 0) + 1
+    return irs + ifs + 4 * n
+
+
+def ninv_cpu_baseline(w, nsample=40):
+    """The oracle's restatement of the reference's CPUfull 1/N self-energy
+    stage (threaded over the time sums like the reference's chunks) timed
+    once on the same 32^3 lattice with an N_mem = nsample window, scaled to
+    N_mem by the exact transform count.  The memory integrals, a fraction of
+    a second per step at N_mem = 400, are left out."""
+    from oracle.oracle import Oracle
+
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    N = w["NstepsinMemory"]
+    o = Oracle(w["Nx"], w["sdim"], nsample, w["dt"], w["Lambda"], w["ONgroup"], w["Mass"], w["Mass"] ** 2,
+               pexp=3, nthreads=nthreads)
+    o.fill_free_window(nsample, thermal_n(o.kL2(), w["Mass"], w["T"]), w["Mass"])
+    o.expand()
+    c0 = o.ninv_transforms()
+    t0 = time.perf_counter()
+    o.ninv_selfenergies(nsample)
+    ts = time.perf_counter() - t0
+    done = o.ninv_transforms() - c0
+    o.close()
+    assert done == ninv_transform_count(nsample - 1), (done, ninv_transform_count(nsample - 1))
+    per = ts * ninv_transform_count(N - 1) / done
+    return {"value": 1.0 / per, "unit": "KB steps/s", "cores": nthreads, "kind": "port",
+            "sample": f"oracle 1/N self-energy stage (CPUfull calcIrs!/calcIFs!/calcSelfEnergyattp!), "
+                      f"32^3 lattice, N_mem={nsample} window: {done} lattice transforms in {ts:.2f} s, "
+                      f"scaled to the {ninv_transform_count(N - 1)} transforms of N_mem={N}; memory "
+                      f"integrals not included"}
+
+
 ENSEMBLE_SEED = 20240611
 ENSEMBLE_NX = (16, 16, 24, 24, 32, 32, 40, 40)
 
@@ -428,6 +471,7 @@ def main():
     traffic = None
     if pmc and pmc.get("Nx") == w["Nx"] and pmc.get("N") == N and not args.sym:
         traffic = pmc.get("traffic_bytes")
+    no_cpu = args.no_cpu
     if args.sym or args.ninverse:
         args.no_adjoint = args.no_cpu = True
     if args.ninverse:
@@ -473,9 +517,8 @@ def main():
                             "self_energy_stage_ms": ms_fft / max(n_fft, 1),
                             "fft_passes_ms": ms_fft / max(n_fft, 1) - avg_rs_s * 1e3,
                             "hartree_ms": ms_h / max(n_mi, 1)}
-        line["cpu_baseline"] = {"value": None, "note": "the oracle restates the reference's CPUfull 1/N code, "
-                                "which transforms every term of the time sums (N_mem^2 lattice FFTs per step); "
-                                "not timed at this size"}
+        if not no_cpu:
+            line["cpu_baseline"] = ninv_cpu_baseline(w)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_bench()
     if not args.no_cpu:

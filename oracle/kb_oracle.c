@@ -139,6 +139,7 @@ struct or_ctx {
   /* NinverseNLO: Irx, IFx per slot (CyclVector of rfftlattice,
    * SimDataStruct.jl:70-71), kept as the complex full-grid extension */
   cplx *Irx, *IFx;
+  long long ntransforms; /* full-lattice transforms of the 1/N stage so far */
   double *kL2, *omega2;
   double hartreemass2;
   int nthreads;
@@ -488,9 +489,17 @@ int or_selfenergies_direct(or_ctx *c, int tmone) {
  * with an odd part they do not even agree with each other (the fast form
  * takes real(Fx) real(rx), the simple one real(Fx rx)). */
 typedef struct {
-  cplx *a, *b, *g, *h, *pen, *tmp;
+  cplx *a, *b, *g, *h, *Y, *pen, *tmp;
   double *RHS, *R2, *R3;
 } ninv_ws;
+
+static int thread_id(void) {
+#ifdef _OPENMP
+  return omp_get_thread_num();
+#else
+  return 0;
+#endif
+}
 
 static void ninv_ws_alloc(or_ctx *c, ninv_ws *w) {
   const size_t V = (size_t)c->V;
@@ -498,6 +507,7 @@ static void ninv_ws_alloc(or_ctx *c, ninv_ws *w) {
   w->b = (cplx *)malloc(sizeof(cplx) * V);
   w->g = (cplx *)malloc(sizeof(cplx) * V);
   w->h = (cplx *)malloc(sizeof(cplx) * V);
+  w->Y = (cplx *)malloc(sizeof(cplx) * V);
   w->pen = (cplx *)malloc(sizeof(cplx) * c->Nx);
   w->tmp = (cplx *)malloc(sizeof(cplx) * c->Nx);
   w->RHS = (double *)malloc(sizeof(double) * V);
@@ -510,6 +520,7 @@ static void ninv_ws_free(ninv_ws *w) {
   free(w->b);
   free(w->g);
   free(w->h);
+  free(w->Y);
   free(w->pen);
   free(w->tmp);
   free(w->RHS);
@@ -519,6 +530,8 @@ static void ninv_ws_free(ninv_ws *w) {
 
 /* out = rfft(x * s) on the full grid */
 static void nfwd(or_ctx *c, ninv_ws *w, const double *x, double s, cplx *out) {
+#pragma omp atomic
+  c->ntransforms += 1;
   for (int i = 0; i < c->V; ++i) {
     out[i].re = x[i] * s;
     out[i].im = 0.0;
@@ -528,6 +541,8 @@ static void nfwd(or_ctx *c, ninv_ws *w, const double *x, double s, cplx *out) {
 
 /* out = brfft(Y) (real lattice); Y is preserved */
 static void nbwd(or_ctx *c, ninv_ws *w, const cplx *Y, double *out) {
+#pragma omp atomic
+  c->ntransforms += 1;
   for (int i = 0; i < c->V; ++i) w->h[i] = Y[i];
   dft3(c, w->h, w->pen, w->tmp, +1);
   for (int i = 0; i < c->V; ++i) out[i] = w->h[i].re;
@@ -537,7 +552,8 @@ static cplx *ircyc(or_ctx *c, cplx *base, int i) { return base + (size_t)(mod1(i
 
 /* calcIrs! CPUfull, Evolution.jl:170-206 (tp counts down: Ir(tp) needs
  * Ir(z) for z > tp) */
-static void ninv_Irs(or_ctx *c, ninv_ws *w, int tmone) {
+/* w: nt workspaces, one per thread */
+static void ninv_Irs(or_ctx *c, ninv_ws *w, int nt, int tmone) {
   const int V = c->V, N = c->N;
   const double ivol = 1.0 / (double)V;
   const double f1 = (c->lambda / 3.0) * ivol * ivol;
@@ -554,19 +570,26 @@ static void ninv_Irs(or_ctx *c, ninv_ws *w, int tmone) {
       I[i].re = s * f1 * w->a[i].re * w->b[i].re;
       I[i].im = 0.0;
     }
-    for (int i = 0; i < V; ++i) w->RHS[i] = 0.0;
-    for (int z = tp + 1; z <= tmone - 1; ++z) {
-      nfwd(c, w, symptr(c, c->F, z, tp), 1.0, w->a);
-      nfwd(c, w, symptr(c, c->r, z, tp), 1.0, w->b);
-      const double sz = thesign(z, tp, N);
-      for (int i = 0; i < V; ++i) {
-        w->g[i].re = w->a[i].re * w->b[i].re;
-        w->g[i].im = 0.0;
+#pragma omp parallel num_threads(nt)
+    {
+      ninv_ws *u = &w[thread_id()];
+      for (int i = 0; i < V; ++i) u->RHS[i] = 0.0;
+#pragma omp for schedule(static)
+      for (int z = tp + 1; z <= tmone - 1; ++z) {
+        nfwd(c, u, symptr(c, c->F, z, tp), 1.0, u->a);
+        nfwd(c, u, symptr(c, c->r, z, tp), 1.0, u->b);
+        const double sz = thesign(z, tp, N);
+        for (int i = 0; i < V; ++i) {
+          u->g[i].re = u->a[i].re * u->b[i].re;
+          u->g[i].im = 0.0;
+        }
+        nbwd(c, u, u->g, u->R2);
+        nbwd(c, u, ircyc(c, c->Irx, z), u->R3);
+        for (int i = 0; i < V; ++i) u->RHS[i] -= f3 * sz * u->R2[i] * u->R3[i];
       }
-      nbwd(c, w, w->g, w->R2);
-      nbwd(c, w, ircyc(c, c->Irx, z), w->R3);
-      for (int i = 0; i < V; ++i) w->RHS[i] -= f3 * sz * w->R2[i] * w->R3[i];
     }
+    for (int k = 1; k < nt; ++k)
+      for (int i = 0; i < V; ++i) w->RHS[i] += w[k].RHS[i];
     nfwd(c, w, w->RHS, 1.0, w->g);
     for (int i = 0; i < V; ++i) I[i].re += f2 * w->g[i].re;
   }
@@ -629,40 +652,49 @@ static void ninv_Fr(or_ctx *c, ninv_ws *w, int z, int tp, double sc, double s, c
 /* calcIFs! CPUfull, Evolution.jl:354-436 (tp counts up: IF(tp) needs IF(z)
  * for z < tp).  IFx[tp] += ivol * Fx is complex in the reference; its
  * imaginary part is kept. */
-static void ninv_IFs(or_ctx *c, ninv_ws *w, int tmone) {
+static void ninv_IFs(or_ctx *c, ninv_ws *w, int nt, int tmone) {
   const int V = c->V, N = c->N, tmin = c->tmin;
   const double ivol = 1.0 / (double)V;
   const double lam = c->lambda, dt = c->dt;
   const double f1 = lam / 6.0 * ivol * ivol, f2 = lam / 6.0 * dt, f3 = lam / 3.0 * dt;
-  cplx *Y = (cplx *)malloc(sizeof(cplx) * V);
   for (int tp = tmin; tp <= tmone; ++tp) {
     cplx *IF = ircyc(c, c->IFx, tp);
+    cplx *Y = w->Y;
     ninv_F2r2(c, w, tmone, tp, 1.0, Y);
     for (int i = 0; i < V; ++i) {
       IF[i].re = f1 * Y[i].re; /* real(factor1 * Fx) */
       IF[i].im = 0.0;
     }
-    for (int i = 0; i < V; ++i) w->RHS[i] = 0.0;
-    /* first integral, z = tmin+1 .. tp-1: Ir part (nr 1) and IF part (nr 2) */
-    for (int z = tmin + 1; z <= tp - 1; ++z) {
-      nbwd(c, w, ircyc(c, c->Irx, z), w->R3);
-      ninv_F2r2(c, w, z, tp, ivol, Y);
-      nbwd(c, w, Y, w->R2);
-      for (int i = 0; i < V; ++i) w->RHS[i] -= f2 * w->R3[i] * w->R2[i];
-      ninv_Fr(c, w, z, tp, ivol, 1.0, Y);
-      const double s = thesign(z, tp, N);
-      nbwd(c, w, Y, w->R3);
-      nbwd(c, w, ircyc(c, c->IFx, z), w->R2);
-      for (int i = 0; i < V; ++i) w->RHS[i] += f3 * s * w->R2[i] * w->R3[i];
-    }
-    /* second integral, z = max(tp, tmin+1) .. tmone-1: Ir part only */
     const int beg = tp == tmin ? tmin + 1 : tp;
-    for (int z = beg; z <= tmone - 1; ++z) {
-      ninv_F2r2(c, w, z, tp, ivol, Y);
-      nbwd(c, w, Y, w->R3);
-      nbwd(c, w, ircyc(c, c->Irx, z), w->R2);
-      for (int i = 0; i < V; ++i) w->RHS[i] -= f2 * w->R2[i] * w->R3[i];
+#pragma omp parallel num_threads(nt)
+    {
+      ninv_ws *u = &w[thread_id()];
+      cplx *Yu = u->Y;
+      for (int i = 0; i < V; ++i) u->RHS[i] = 0.0;
+      /* first integral, z = tmin+1 .. tp-1: Ir part (nr 1) and IF part (nr 2) */
+#pragma omp for schedule(static)
+      for (int z = tmin + 1; z <= tp - 1; ++z) {
+        nbwd(c, u, ircyc(c, c->Irx, z), u->R3);
+        ninv_F2r2(c, u, z, tp, ivol, Yu);
+        nbwd(c, u, Yu, u->R2);
+        for (int i = 0; i < V; ++i) u->RHS[i] -= f2 * u->R3[i] * u->R2[i];
+        ninv_Fr(c, u, z, tp, ivol, 1.0, Yu);
+        const double s = thesign(z, tp, N);
+        nbwd(c, u, Yu, u->R3);
+        nbwd(c, u, ircyc(c, c->IFx, z), u->R2);
+        for (int i = 0; i < V; ++i) u->RHS[i] += f3 * s * u->R2[i] * u->R3[i];
+      }
+      /* second integral, z = max(tp, tmin+1) .. tmone-1: Ir part only */
+#pragma omp for schedule(static)
+      for (int z = beg; z <= tmone - 1; ++z) {
+        ninv_F2r2(c, u, z, tp, ivol, Yu);
+        nbwd(c, u, Yu, u->R3);
+        nbwd(c, u, ircyc(c, c->Irx, z), u->R2);
+        for (int i = 0; i < V; ++i) u->RHS[i] -= f2 * u->R2[i] * u->R3[i];
+      }
     }
+    for (int k = 1; k < nt; ++k)
+      for (int i = 0; i < V; ++i) w->RHS[i] += w[k].RHS[i];
     /* border z = tmin, trapezoid weight 1/2 */
     ninv_F2r2(c, w, tmin, tp, 1.0, Y);
     nbwd(c, w, Y, w->R3);
@@ -681,7 +713,6 @@ static void ninv_IFs(or_ctx *c, ninv_ws *w, int tmone) {
       IF[i].im += ivol * w->g[i].im;
     }
   }
-  free(Y);
 }
 
 /* calcIFssimple!, Evolution.jl:599-624 */
@@ -762,33 +793,39 @@ static void ninv_sigma_tp(or_ctx *c, ninv_ws *w, int tmone, int tp) {
  * tp, then the self-energies for tp in [tmin, tmone] */
 static int ninv_selfenergies(or_ctx *c, int tmone, int simple) {
   if (!inwin(c, tmone)) return fail(c, "tmone %d outside window [%d,%d]", tmone, c->tmin, c->tmax);
-  ninv_ws w;
-  ninv_ws_alloc(c, &w);
+  const int nt = simple ? 1 : c->nthreads;
+  ninv_ws *w = (ninv_ws *)malloc(sizeof(ninv_ws) * nt);
+  for (int k = 0; k < nt; ++k) ninv_ws_alloc(c, &w[k]);
   if (simple) {
-    ninv_Irs_simple(c, &w, tmone);
-    ninv_IFs_simple(c, &w, tmone);
+    ninv_Irs_simple(c, w, tmone);
+    ninv_IFs_simple(c, w, tmone);
   } else {
-    ninv_Irs(c, &w, tmone);
-    ninv_IFs(c, &w, tmone);
+    ninv_Irs(c, w, nt, tmone);
+    ninv_IFs(c, w, nt, tmone);
   }
-  for (int tp = c->tmin; tp <= c->tmax - 1; ++tp) ninv_sigma_tp(c, &w, tmone, tp);
-  ninv_ws_free(&w);
+  const int lo = c->tmin, hi = c->tmax - 1;
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 1)
+  for (int tp = lo; tp <= hi; ++tp) ninv_sigma_tp(c, &w[thread_id()], tmone, tp);
+  for (int k = 0; k < nt; ++k) ninv_ws_free(&w[k]);
+  free(w);
   return 0;
 }
 
 int or_ninv_resum(or_ctx *c, int tmone, int simple) {
   if (c->pexp != 3) return fail(c, "resummation: NinverseNLO only");
   if (!inwin(c, tmone)) return fail(c, "tmone %d outside window [%d,%d]", tmone, c->tmin, c->tmax);
-  ninv_ws w;
-  ninv_ws_alloc(c, &w);
+  const int nt = simple ? 1 : c->nthreads;
+  ninv_ws *w = (ninv_ws *)malloc(sizeof(ninv_ws) * nt);
+  for (int k = 0; k < nt; ++k) ninv_ws_alloc(c, &w[k]);
   if (simple) {
-    ninv_Irs_simple(c, &w, tmone);
-    ninv_IFs_simple(c, &w, tmone);
+    ninv_Irs_simple(c, w, tmone);
+    ninv_IFs_simple(c, w, tmone);
   } else {
-    ninv_Irs(c, &w, tmone);
-    ninv_IFs(c, &w, tmone);
+    ninv_Irs(c, w, nt, tmone);
+    ninv_IFs(c, w, nt, tmone);
   }
-  ninv_ws_free(&w);
+  for (int k = 0; k < nt; ++k) ninv_ws_free(&w[k]);
+  free(w);
   return 0;
 }
 
@@ -796,6 +833,8 @@ int or_ninv_selfenergies(or_ctx *c, int tmone, int simple) {
   if (c->pexp != 3) return fail(c, "1/N self-energies: NinverseNLO only");
   return ninv_selfenergies(c, tmone, simple);
 }
+
+long long or_ninv_transforms(or_ctx *c) { return c->ntransforms; }
 
 /* Irx / IFx at the slot of time tp (which = 0 IFx, 1 Irx), as V complex
  * values (re, im interleaved) */

@@ -79,6 +79,9 @@ int or_ninv_selfenergies(or_ctx *c, int tmone, int simple);
  * values, (re, im) interleaved, full grid (the Hermitian extension of the
  * reference's rfft-layout array). */
 int or_get_ninv(or_ctx *c, int which, int tp, double *out);
+/* Full-lattice transforms the 1/N stage has run so far (for scaling a timed
+ * sample to another window length). */
+long long or_ninv_transforms(or_ctx *c);
 int or_set_hartree(or_ctx *c, double m2);
 double or_get_hartree(or_ctx *c);
 

@@ -59,6 +59,8 @@ def lib():
         L.or_ninv_resum.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.or_ninv_selfenergies.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.or_get_ninv.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+        L.or_ninv_transforms.restype = ctypes.c_longlong
+        L.or_ninv_transforms.argtypes = [ctypes.c_void_p]
         L.or_hartree.restype = ctypes.c_double
         L.or_hartree.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.or_set_hartree.argtypes = [ctypes.c_void_p, ctypes.c_double]
@@ -163,6 +165,9 @@ class Oracle:
         out = np.empty(2 * self.V)
         self._chk(self._L.or_get_ninv(self._c, 1 if which == "r" else 0, int(tp), _ptr(out)))
         return out.view(np.complex128).copy()
+
+    def ninv_transforms(self):
+        return int(self._L.or_ninv_transforms(self._c))
 
     def hartree(self, i, j):
         return self._L.or_hartree(self._c, int(i), int(j))
