@@ -476,6 +476,9 @@ def main():
         args.no_adjoint = args.no_cpu = True
     if args.ninverse:
         traffic = None
+        pmc_n = load_pmc(os.path.join(ROOT, "profiles", "ninv_pmc_32c_n400.json"))
+        if pmc_n and pmc_n.get("Nx") == w["Nx"] and pmc_n.get("N") == N:
+            traffic = pmc_n.get("traffic_bytes")
     line = {
         "metric": ("KB steps/s at 32^3 N_mem=400 FP64 (% HBM peak); adjoint/forward cost ratio"
                    if not (args.sym or args.ninverse) else
@@ -511,7 +514,7 @@ def main():
         Br = bytes_resum(N, Vs)
         ach = Br / avg_rs_s / 1e9
         line["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "ninv_resum_kernel",
+                            "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "kernel": "ninv_resum_kernel",
                             "algorithmic_bytes": Br, "avg_kernel_ms": avg_rs_s * 1e3,
                             "memint_ms": avg_mi_s * 1e3, "memint_frac": achieved / HBM_PEAK_GBS,
                             "self_energy_stage_ms": ms_fft / max(n_fft, 1),
