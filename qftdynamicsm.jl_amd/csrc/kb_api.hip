@@ -1226,6 +1226,8 @@ int ens_enqueue(kb_ens* e) {
     int rc;
     if (c->p.pexp == KB_PEXP_LAMBDA_NLO && (rc = launch_sigma(c)))
       return ens_err(e, rc, "member self-energy: %s", c->err.c_str());
+    if (c->p.pexp == KB_PEXP_NINVERSE_NLO && (rc = launch_ninv_stage(c)))
+      return ens_err(e, rc, "member 1/N self-energy: %s", c->err.c_str());
     if ((rc = launch_hartree_step(c))) return ens_err(e, rc, "member Hartree: %s", c->err.c_str());
   }
   // members keep the default tile shape (their work items are 64-row chunks)
@@ -1253,7 +1255,6 @@ int kb_ens_create(const kb_params* params, int nruns, kb_ens** out) {
   };
   for (int i = 0; i < nruns; ++i) {
     if (params[i].device != e->device) return fail(KB_ERR_ARG);
-    if (params[i].pexp == KB_PEXP_NINVERSE_NLO) return fail(KB_ERR_ARG);  // the 1/N stage runs per context
     kb_ctx* c = nullptr;
     int rc = kb_create(&params[i], &c);
     if (rc) return fail(rc);
@@ -1315,6 +1316,11 @@ int kb_ens_step(kb_ens* e, int t) {
       return ens_err(e, KB_ERR_WINDOW, "member %zu: t=%d but window is [%d,%d]", i, t, c->st.tmin, c->st.tmax);
   }
   for (auto* c : e->m) expand(c);
+  for (auto* c : e->m)
+    if (c->p.pexp == KB_PEXP_NINVERSE_NLO && !c->ninv_valid) {
+      const int rc = ninv_rebuild(c);  // eager, before the step's launches or graph
+      if (rc) return ens_err(e, rc, "member 1/N rebuild: %s", c->err.c_str());
+    }
   EventPair ev{nullptr, nullptr};
   if (e->timing) {
     if (hipEventCreate(&ev.a) != hipSuccess || hipEventCreate(&ev.b) != hipSuccess)

@@ -210,13 +210,10 @@ def test_ninverse_orbit_storage_equals_full_storage():
 
 
 def test_ninverse_unsupported_combinations_are_rejected():
-    from qftkb import KBContext, KBEnsemble
+    from qftkb import KBContext
     from qftkb._abi import KBError
     with pytest.raises(KBError):
         KBContext(72, 3, 4, 0.05, 1.0, 8, 1.0, 1.0, pexp=NINV_NLO)  # above the plane-kernel size
-    with pytest.raises(KBError):
-        KBEnsemble([dict(Nx=4, sdim=3, NstepsinMemory=5, dt=0.05, Lambda=1.0, ONgroup=8, Mass=1.0, Mass2=1.0,
-                         pexp=NINV_NLO)])
     g = KBContext(4, 3, 5, 0.05, 1.0, 8, 1.0, 1.0, pexp=NINV_NLO)
     g.init_particles(np.zeros(64), 1.0)
     with pytest.raises(KBError):
@@ -250,3 +247,32 @@ def test_ninverse_parameter_dict_run_matches_oracle():
             assert abs(getattr(m, k) - mo[k]) <= RTOL * abs(mo[k]), (t, k)
         assert abs(m.E_mem - mo["E_mem"]) <= RTOL * abs(mo["E"]), (t, "E_mem")
         assert rel(m.n, mo["n"]) <= RTOL, (t, "n")
+
+
+def test_ninverse_in_mixed_ensemble():
+    """An ensemble of runs in different expansions and sizes (1/N NLO, 1/N LO,
+    lambda NLO), graph-replayed, each member against its own oracle run."""
+    from qftkb import KBEnsemble
+    runs = [dict(Nx=6, sdim=3, NstepsinMemory=8, dt=0.05, Lambda=10.0, ONgroup=8, Mass=1.0, Mass2=1.0, pexp=NINV_NLO),
+            dict(Nx=4, sdim=3, NstepsinMemory=20, dt=0.05, Lambda=5.0, ONgroup=4, Mass=1.0, Mass2=1.0, pexp=NINV_NLO),
+            dict(Nx=8, sdim=2, NstepsinMemory=6, dt=0.05, Lambda=10.0, ONgroup=8, Mass=1.0, Mass2=1.0, pexp=NINV_LO),
+            dict(Nx=5, sdim=3, NstepsinMemory=7, dt=0.05, Lambda=1.0, ONgroup=1, Mass=1.0, Mass2=1.0, pexp=1)]
+    print("\n[case] mixed-expansion ensemble, 4 runs", flush=True)
+    e = KBEnsemble(runs)
+    e.use_graph(True)
+    oracles = []
+    for r, m in zip(runs, e.members):
+        o = Oracle(r["Nx"], r["sdim"], r["NstepsinMemory"], r["dt"], r["Lambda"], r["ONgroup"], r["Mass"],
+                   r["Mass2"], pexp=r["pexp"])
+        n = tophat(o.kL2(), r["sdim"])
+        o.init_particles(n, 1.0)
+        m.init_particles(n, 1.0)
+        oracles.append(o)
+    for t in range(3, 18):
+        e.step(t)
+        for o in oracles:
+            o.evolve(t)
+    e.synchronize()
+    for o, m in zip(oracles, e.members):
+        compare_meas(o, m, 17)
+        compare_window(o, m)
