@@ -407,8 +407,10 @@ def main():
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / 10
+        from qftkb._abi import read_bandwidth
         print(json.dumps({"calibration": "copy", "bytes_read": x.numel() * 8, "bytes_written": y.numel() * 8,
-                          "sum_read_GBs": z.numel() * 8 / ms / 1e6}))
+                          "sum_read_GBs": z.numel() * 8 / ms / 1e6,
+                          "streaming_read_GBs": read_bandwidth(0, 16 << 30)}))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not supported "

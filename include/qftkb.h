@@ -160,6 +160,11 @@ int kb_get_sigma(kb_ctx *ctx, int which, int tp, double *out);
  * Irx/IFx, SimDataStruct.jl:70-71), i ignored.  KB_ERR_STATE for other
  * expansions. */
 int kb_ninv_get(kb_ctx *ctx, int which, int i, int j, double *out);
+/* Benchmark support, no reference counterpart: the streaming-read rate of
+ * device `device` (GB/s) measured with non-temporal 16-byte loads over a
+ * buffer of `bytes` (at least 16 MiB), the practical ceiling of the
+ * bandwidth-bound kernels.  Allocates and frees the buffer. */
+int kb_read_bandwidth(int device, long long bytes, double *gbs);
 /* Scan the window for NaN/Inf; returns KB_ERR_NONFINITE if any. */
 int kb_check_finite(kb_ctx *ctx);
 

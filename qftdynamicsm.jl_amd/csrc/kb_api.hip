@@ -935,6 +935,36 @@ int kb_get_sigma(kb_ctx* c, int which, int tp, double* out) {
   return lat_download(c, out, c->vbuf);
 }
 
+int kb_read_bandwidth(int device, long long bytes, double* gbs) {
+  if (!gbs || bytes < (1LL << 24)) return KB_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return KB_ERR_HIP;
+  const size_t n2 = (size_t)bytes / 16;
+  const int blocks = 8192;
+  double *x = nullptr, *out = nullptr;
+  hipEvent_t a = nullptr, b = nullptr;
+  int rc = KB_OK;
+  if (hipMalloc((void**)&x, n2 * 16) != hipSuccess || hipMalloc((void**)&out, blocks * sizeof(double)) != hipSuccess ||
+      hipMemset(x, 0, n2 * 16) != hipSuccess || hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+    rc = KB_ERR_OOM;
+  } else {
+    for (int w = 0; w < 2; ++w) hipLaunchKernelGGL(read_probe_kernel, dim3(blocks), dim3(256), 0, 0, x, n2, out);
+    (void)hipEventRecord(a, 0);
+    for (int r = 0; r < 10; ++r) hipLaunchKernelGGL(read_probe_kernel, dim3(blocks), dim3(256), 0, 0, x, n2, out);
+    (void)hipEventRecord(b, 0);
+    float ms = 0.f;
+    if (hipGetLastError() != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+        hipEventElapsedTime(&ms, a, b) != hipSuccess)
+      rc = KB_ERR_HIP;
+    else
+      *gbs = (double)n2 * 16.0 / (ms / 10.0) / 1e6;
+  }
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+  if (x) (void)hipFree(x);
+  if (out) (void)hipFree(out);
+  return rc;
+}
+
 int kb_ninv_get(kb_ctx* c, int which, int i, int j, double* out) {
   if (!c || !out || which < 0 || which > 4) return KB_ERR_ARG;
   if (c->p.pexp != KB_PEXP_NINVERSE_NLO) return seterr(c, KB_ERR_STATE, "no 1/N resummation in this context");

@@ -200,6 +200,7 @@ __global__ void gather_pair(const double* H, long NP, int V, long pair, int comp
 __global__ void scatter_pair(double* H, long NP, int V, long pair, int comp, double sign, const double* in);
 __global__ void gather_sigma(const double* S, int N, int V, int slot, int comp, double* out);
 __global__ void gather_tri(const double* P, long PNP, int V, long pp, double sign, double* out);
+__global__ void read_probe_kernel(const double* x, size_t n2, double* out);
 __global__ void init_rows_kernel(double* H, long NP, int V, int N, const double* F11, const double* F21,
                                  const double* F22, double dt);
 __global__ void init_particles_kernel(const double* kL2, const double* n, double m, double dt, int V, double* F11,

@@ -784,6 +784,34 @@ __global__ void scatter_pair(double* __restrict__ H, long NP, int V, long pair, 
   if (p < V) H[hoff(p / MB, pair, p % MB, NP) + comp] = in[p] * sign;
 }
 
+// Streaming-read probe (benchmark support): each thread reads U 16-byte
+// vectors per iteration, non-temporally, and the block writes one sum.
+__global__ __launch_bounds__(256) void read_probe_kernel(const double* __restrict__ x, size_t n2,
+                                                          double* __restrict__ out) {
+  typedef double rp_d2 __attribute__((ext_vector_type(2)));
+  constexpr int U = 8;
+  const rp_d2* v = reinterpret_cast<const rp_d2*>(x);
+  double acc = 0.0;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + (U - 1) * stride < n2; i += U * stride) {
+    rp_d2 a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) a[u] = __builtin_nontemporal_load(v + i + u * stride);
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc += a[u].x + a[u].y;
+  }
+  for (; i < n2; i += stride) acc += v[i].x + v[i].y;
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sum = 0.0;
+    for (int k = 0; k < (int)blockDim.x; ++k) sum += red[k];
+    out[blockIdx.x] = sum;
+  }
+}
+
 __global__ void gather_tri(const double* __restrict__ P, long PNP, int V, long pp, double sign,
                            double* __restrict__ out) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;

@@ -35,7 +35,8 @@ PEXP_CODES = {"LambdaLO": KB_PEXP_LAMBDA_LO, "LambdaNLO": KB_PEXP_LAMBDA_NLO, "N
 EXPORTED = (
     "kb_create", "kb_destroy", "kb_last_error", "kb_info", "kb_get_kL2", "kb_set_kL2",
     "kb_init_particles", "kb_init_rows", "kb_step", "kb_window", "kb_hartree", "kb_measure",
-    "kb_measure_history", "kb_get_entry", "kb_set_entry", "kb_get_sigma", "kb_ninv_get", "kb_check_finite",
+    "kb_measure_history", "kb_get_entry", "kb_set_entry", "kb_get_sigma", "kb_ninv_get", "kb_read_bandwidth",
+    "kb_check_finite",
     "kb_fill_free_window", "kb_synchronize", "kb_set_stream", "kb_enable_timing", "kb_timing",
     "kb_reset_timing", "kb_use_graph", "kb_ens_create", "kb_ens_destroy", "kb_ens_last_error", "kb_ens_member",
     "kb_ens_step", "kb_ens_use_graph", "kb_ens_synchronize", "kb_ens_enable_timing", "kb_ens_timing",
@@ -103,6 +104,7 @@ def load_library(path: str = LIB_PATH):
     L.kb_set_entry.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
     L.kb_get_sigma.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp]
     L.kb_ninv_get.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+    L.kb_read_bandwidth.argtypes = [ctypes.c_int, ctypes.c_longlong, _dp]
     L.kb_check_finite.argtypes = [vp]
     L.kb_fill_free_window.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_double]
     L.kb_synchronize.argtypes = [vp]
@@ -146,6 +148,16 @@ TILES_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tiles.jso
 
 def tiles_key(Nx, sdim, NstepsinMemory, symmetric=False):
     return f"{int(Nx)}^{int(sdim)} N_mem={int(NstepsinMemory)}" + (" orbits" if symmetric else "")
+
+
+def read_bandwidth(device=0, nbytes=16 << 30):
+    """Streaming-read rate of the device in GB/s (kb_read_bandwidth)."""
+    L = load_library()
+    out = ctypes.c_double()
+    rc = L.kb_read_bandwidth(int(device), int(nbytes), ctypes.byref(out))
+    if rc != KB_OK:
+        raise KBError(rc, "kb_read_bandwidth failed")
+    return out.value
 
 
 def committed_tiles(Nx, sdim, NstepsinMemory, symmetric=False):
