@@ -150,7 +150,7 @@ ENSEMBLE_NX = (16, 16, 24, 24, 32, 32, 40, 40)
 
 
 def ensemble_runs(seed=ENSEMBLE_SEED, nmem=300):
-    """Config 3: lattices {16,16,24,24,32,32,40,40}^3, lambda ~ U[0.5, 2],
+    """The 8-run uneven ensemble: lattices {16,16,24,24,32,32,40,40}^3, lambda ~ U[0.5, 2],
     box occupations n0 ~ U[1, 4], Q ~ U[0.5, 1.5], drawn from one PCG64 seed."""
     rng = np.random.Generator(np.random.PCG64(seed))
     lam = rng.uniform(0.5, 2.0, len(ENSEMBLE_NX))
@@ -209,7 +209,7 @@ def ensemble_bench(args):
         t_seq += dt_run
         c.close()
     line = {
-        "metric": "ensemble KB steps/s (8 uneven runs, config 3)",
+        "metric": "ensemble KB steps/s (8 uneven runs, 16^3..40^3, N_mem=300)",
         "value": len(runs) * steps / t_ens, "unit": "member KB steps/s",
         "ensemble_steps_per_s": steps / t_ens,
         "sequential_member_steps_per_s": len(runs) * steps / t_seq,
@@ -217,7 +217,7 @@ def ensemble_bench(args):
         "ms_per_ensemble_step": 1e3 * t_ens / steps,
         "sequential_ms_per_step_by_run": dict(zip([f"{r['Nx']}^3 #{i}" for i, r in enumerate(runs)], per_run)),
         "steps": steps, "warmup": warm, "dtype": "f64",
-        "config": {"workload": "config 3 ensemble", "Nx": list(ENSEMBLE_NX), "NstepsinMemory": N,
+        "config": {"workload": "8-run uneven ensemble {16,16,24,24,32,32,40,40}^3", "Nx": list(ENSEMBLE_NX), "NstepsinMemory": N,
                    "seed": ENSEMBLE_SEED, "Lambda": [r["Lambda"] for r in runs], "box": occ},
     }
     print(json.dumps(line), flush=True)
@@ -323,7 +323,7 @@ def adjoint_bench(fd_bins=False, fd=True):
         out["fd_rel_err_bins"] = max(abs(x - y) for x, y in zip(out["dL_dn0_bins"], fdb)) / scale
     out["fd_s"] = time.perf_counter() - t0
     out["fd_cost_ratio"] = 2 * (1 + (a["nbins"] if fd_bins else 0)) * 1.0  # forward runs per gradient, FD
-    gold = load_pmc(os.path.join(ROOT, "tests", "golden", "cfg4_fd.json"))
+    gold = load_pmc(os.path.join(ROOT, "tests", "golden", "adjoint_24c_n200_fd.json"))
     if gold and "dL_dlambda_fd" in gold:
         out["oracle_fd_dL_dlambda"] = gold["dL_dlambda_fd"]
         out["oracle_fd_rel_err_lambda"] = abs(gr["lambda"] - gold["dL_dlambda_fd"]) / abs(gold["dL_dlambda_fd"])
@@ -372,7 +372,7 @@ def main():
     ap.add_argument("--nmem", type=int, default=WORKLOAD["NstepsinMemory"])
     ap.add_argument("--nx", type=int, default=WORKLOAD["Nx"])
     ap.add_argument("--ensemble", action="store_true",
-                    help="config 3: 8 uneven runs advanced as one ensemble vs the same runs one after another")
+                    help="8 uneven runs (16^3..40^3) advanced as one ensemble vs the same runs one after another")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost-ratio leg")
     ap.add_argument("--sym", action="store_true",
                     help="cubic-orbit storage (a separate metric: the stored window shrinks ~34x at 32^3)")

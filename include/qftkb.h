@@ -1,14 +1,15 @@
 /*
  * qftkb.h -- C ABI of libqftkb.so: the MI355X (gfx950) back-end for the
  * 2PI Kadanoff-Baym time step of the reference's src/TwoPIScalar model
- * (CPUfull storage semantics, LambdaLO / LambdaNLO quantum expansions).
+ * (CPUfull storage semantics; LambdaLO, LambdaNLO quantum, NinverseLO and
+ * NinverseNLO expansions).
  *
  * The reference selects its numerics back-end by Julia dispatch on the
  * "Num" parameter (TwoPIScalar_Modelfile.jl:104-108) and then calls
  * initSimData!, evolve!, getHartreeMass2 and createMeasurement on the
  * back-end's SimData type.  Each entry point below replaces one of those
- * methods for a new numerics type; the Julia binding a maintainer would add
- * is shown in INTEGRATION.md.
+ * methods for a new numerics type; the Julia binding a maintainer adds is
+ * qftdynamicsm.jl_amd/julia/TwoPIScalar_GPUfull.jl.
  *
  * Conventions
  *   - Every call returns an int status: KB_OK (0) or a KB_ERR_* code; the
@@ -216,7 +217,7 @@ int kb_autotune_fft(kb_ctx *ctx, int reps, int *best, double *ms);
  * the initial state, by hand-written reverse kernels of the whole KB step
  * (evolve!, TwoPIScalar_Evolution.jl:924-981).  The reference has no
  * derivative code; these entry points are what a ChainRulesCore rrule of
- * the back-end's evolve loop calls (INTEGRATION.md).  Mass2 is held fixed. */
+ * the back-end's evolve loop calls.  Mass2 is held fixed. */
 
 /* Start recording right after kb_init_particles / kb_init_rows (window
  * [1,2]).  Later kb_step calls run eagerly and keep what the reverse sweep

@@ -4,8 +4,9 @@
 # hot-path method calls libqftkb.so (include/qftkb.h) through ccall.
 #
 # Include it after TwoPIScalar_SimDataStruct.jl / _Initialization.jl /
-# _Evolution.jl / _Measurement.jl (see INTEGRATION.md for the three lines
-# that wire it into TwoPIScalar_Modelfile.jl).  Methods dispatch on the new
+# _Evolution.jl / _Measurement.jl; three more lines wire it into
+# TwoPIScalar_Modelfile.jl (the include, the "Num" => "GPUfull" switch at
+# :104-108, and the getTwoPISimData branch).  Methods dispatch on the new
 # types exactly like the CPUfull ones they stand beside:
 #   getTwoPISimData          (TwoPIScalar_SimDataStruct.jl:669-708)
 #   getTwoPIScalarTmpData    (:710-721)

@@ -3,11 +3,11 @@ workload: 24^3 lattice, N_mem = 200, 500 steps (t = 3 .. 502), dt = 0.1,
 lambda = 1, O(1), m = 1, thermal T = 2 initial occupations, loss
 L = sum_p exp(-|p|^2) n(p, 502).
 
-Writes tests/golden/cfg4_fd.json with L(lambda0 +- h), h = 1e-5, and the
+Writes tests/golden/adjoint_24c_n200_fd.json with L(lambda0 +- h), h = 1e-5, and the
 central difference; with --bins also the 16 radial |p| bins over
 [0, sqrt(12)] of dL/dn(p, 0) (each bin's occupations moved together).
 Slow (minutes per oracle run); run by hand:
-    python tests/golden/make_cfg4_fd.py [--bins] [--threads N]
+    python tests/golden/make_adjoint_fd.py [--bins] [--threads N]
 """
 import argparse
 import json
@@ -26,7 +26,7 @@ NX, SDIM, NMEM, DT, LAM, ON, M = 24, 3, 200, 0.1, 1.0, 1, 1.0
 T_FINAL = 502
 H = 1e-5
 NBINS = 16
-OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cfg4_fd.json")
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "adjoint_24c_n200_fd.json")
 
 
 def occupations(k2):

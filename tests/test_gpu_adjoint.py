@@ -189,11 +189,11 @@ def test_adjoint_vs_oracle_fd_at_16cubed():
 def test_differentiation_workload_vs_oracle_fd_fixture():
     """24^3, N_mem = 200, 500 steps, checkpoints every 50: the adjoint
     gradient against the oracle's central finite differences committed in
-    tests/golden/cfg4_fd.json (made by tests/golden/make_cfg4_fd.py)."""
+    tests/golden/adjoint_24c_n200_fd.json (made by tests/golden/make_adjoint_fd.py)."""
     import json
     import os
     from qftkb import KBContext
-    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "cfg4_fd.json")))
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "adjoint_24c_n200_fd.json")))
     print("\n[case] differentiation workload 24^3 N_mem=200 500 steps, C=50", flush=True)
     c = KBContext(gold["Nx"], gold["sdim"], gold["N_mem"], gold["dt"], gold["Lambda"], gold["ONgroup"],
                   gold["Mass"], gold["Mass"] ** 2)

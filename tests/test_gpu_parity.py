@@ -1,6 +1,6 @@
 """Parity of the HIP path (through the C ABI) with the oracle.
 
-Tolerance (north-star): F, rho and n(p,t) within 1e-10 relative error.  For
+Tolerance: F, rho and n(p,t) within 1e-10 relative error.  For
 lattice-valued quantities the relative error is norm-wise per lattice,
 max_p |gpu - oracle| / max_p |oracle|; scalars use |gpu - oracle| / |oracle|
 with the energy E as the scale for E_mem (a small difference of large
@@ -188,11 +188,11 @@ def test_free_theory_golden():
 
 
 def test_config1_box_16cubed_full_memory():
-    """16^3, lambda = 1, m = 1, box n(p) = 2 Theta(1-|p|),
+    """The 16^3 box problem: lambda = 1, m = 1, box n(p) = 2 Theta(1-|p|),
     dt = 0.1, 200 steps, full memory (N_mem = Nsteps + 2): n(p,t) and E at
     every step against the oracle."""
     from qftkb import lattice
-    print("\n[case] config 1: 16^3 box, N_mem=202, 200 steps", flush=True)
+    print("\n[case] 16^3 box, N_mem=202, 200 steps", flush=True)
     Nx, steps = 16, 200
     o, g = make_pair(Nx, 3, steps + 2)
     n = lattice.n_box(o.kL2(), 2.0, 1.0)
@@ -205,7 +205,7 @@ def test_config1_box_16cubed_full_memory():
         mo, mg = compare_meas(o, g, t)
         worst = max(worst, rel(mg["n"], mo["n"]))
     compare_window(o, g)
-    print(f"[case] config 1 worst per-site n rel err {worst:.2e}", flush=True)
+    print(f"[case] 16^3 box worst per-site n rel err {worst:.2e}", flush=True)
 
 
 def test_32cubed_identical_window_steps():

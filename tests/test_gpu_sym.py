@@ -76,7 +76,7 @@ def test_orbit_storage_equals_full_storage(args, pexp, T):
 
 
 def test_orbit_storage_config1_box_against_oracle():
-    """The 16^3 box state (config 1 shape, 60 steps) in orbit storage against
+    """The 16^3 box state (60 steps) in orbit storage against
     the oracle, n(p,t) and E at every 10th step."""
     from qftkb import KBContext
     from qftkb.lattice import n_box
