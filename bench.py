@@ -272,7 +272,7 @@ def adjoint_bench(fd_bins=False, fd=True):
                        f"L = sum_p exp(-|p|^2) n(p, t_f)",
            "forward_s": t_fwd, "loss": L}
     grads = {}
-    for C in (a["checkpoint_every"], 0):
+    for C in (a["checkpoint_every"], 0, -1):
         g = make(lam)
         g.init_particles(n0, M)
         g.adj_record(C, T)
@@ -283,7 +283,7 @@ def adjoint_bench(fd_bins=False, fd=True):
         t2 = time.perf_counter()
         info = g.adj_info()
         g.close()
-        key = f"checkpoint_every_{C}" if C else "tape_every_column"
+        key = f"checkpoint_every_{C}" if C > 0 else ("tape_every_column" if C == 0 else "packed_tape")
         out[key] = {"record_forward_s": t1 - t0, "reverse_s": t2 - t1, "gradient_s": t2 - t0,
                     "ratio": (t2 - t0) / t_fwd, "reverse_ratio": (t2 - t1) / t_fwd,
                     "recomputed_steps": info["recomputed_steps"], "adjoint_device_gb": info["device_bytes"] / 1e9}
@@ -327,6 +327,92 @@ def adjoint_bench(fd_bins=False, fd=True):
     if gold and "dL_dlambda_fd" in gold:
         out["oracle_fd_dL_dlambda"] = gold["dL_dlambda_fd"]
         out["oracle_fd_rel_err_lambda"] = abs(gr["lambda"] - gold["dL_dlambda_fd"]) / abs(gold["dL_dlambda_fd"])
+    return out
+
+
+LARGE_ADJ_WORKLOAD = dict(Nx=48, sdim=3, NstepsinMemory=300, dt=0.05, Lambda=0.5, ONgroup=1, Mass=1.0, n0=4.0,
+                          Q=1.0, steps=4000, h=1e-5, nbins=16)
+
+
+def large_adjoint_bench(fd=True):
+    """The largest single-GPU differentiation: 48^3, N_mem=300 (80.7 GB
+    window), dt=0.05, 4000 steps, box n0=4, Q=1, lambda=0.5; L = sum_p
+    exp(-|p|^2) n(p, t_f).  Forward steps/s and the memory integral's HBM
+    fraction; the gradient with the orbit-packed tape (the box state is
+    cubic-symmetric: the 3702 dropped columns take 52 GB instead of 1.96 TB),
+    its cost ratio, and GPU central differences of dL/dlambda."""
+    from qftkb import KBContext, lattice
+
+    a = LARGE_ADJ_WORKLOAD
+    Nx, N, dt, lam, ON, M = a["Nx"], a["NstepsinMemory"], a["dt"], a["Lambda"], a["ONgroup"], a["Mass"]
+    T = 2 + a["steps"]
+
+    def make(lmb):
+        return KBContext(Nx, a["sdim"], N, dt, lmb, ON, M, M * M)
+
+    def forward(c, graph, label):
+        c.use_graph(graph)
+        t0 = time.perf_counter()
+        for t in range(3, T + 1):
+            c.step(t)
+            if (t - 2) % 500 == 0:
+                c.synchronize()
+                print(f"[bench] {label}: step {t}/{T}, {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+                      flush=True)
+        c.synchronize()
+        return time.perf_counter() - t0
+
+    c = make(lam)
+    k2 = c.kL2()
+    n0 = lattice.n_box(k2, a["n0"], a["Q"])
+    wgt = np.exp(-k2)
+    c.init_particles(n0, M)
+    t_fwd = forward(c, True, "forward")
+    L = float((wgt * c.measure(T)["n"]).sum())
+    # memory-integral time on the full window (60 more steps, timed per kernel)
+    c.enable_timing(True)
+    c.reset_timing()
+    for t in range(T + 1, T + 61):
+        c.step(t)
+    c.synchronize()
+    ms_mi, n_mi = c.timing(0)
+    c.enable_timing(False)
+    c.close()
+    avg = ms_mi / max(n_mi, 1) / 1e3
+    B = bytes_memint(N, Nx ** 3)
+    out = {"workload": f"{Nx}^3 lattice, N_mem={N}, dt={dt}, {a['steps']} steps, box n0={a['n0']} Q={a['Q']}, "
+                       f"lambda={lam}, L = sum_p exp(-|p|^2) n(p, t_f)",
+           "forward_s": t_fwd, "forward_steps_per_s": a["steps"] / t_fwd, "loss": L,
+           "memint_ms": avg * 1e3, "memint_hbm_frac": B / avg / 1e9 / HBM_PEAK_GBS}
+    g = make(lam)
+    g.init_particles(n0, M)
+    g.adj_record(-1, T)
+    t_rec = forward(g, False, "recording forward")
+    t0 = time.perf_counter()
+    gr = g.adj_gradient(wgt)
+    t_rev = time.perf_counter() - t0
+    info = g.adj_info()
+    g.close()
+    print(f"[bench] packed-tape gradient: record {t_rec:.1f} s, reverse {t_rev:.1f} s", file=sys.stderr, flush=True)
+    out["packed_tape"] = {"record_forward_s": t_rec, "reverse_s": t_rev, "gradient_s": t_rec + t_rev,
+                          "ratio": (t_rec + t_rev) / t_fwd, "adjoint_device_gb": info["device_bytes"] / 1e9,
+                          "recomputed_steps": info["recomputed_steps"]}
+    out["ratio"] = out["packed_tape"]["ratio"]
+    out["dL_dlambda"] = gr["lambda"]
+    bins = radial_bins(k2, a["sdim"], a["nbins"])
+    out["dL_dn0_bins"] = [float(gr["n0"][bins == b].sum()) for b in range(a["nbins"])]
+    if fd:
+        h = a["h"]
+        Ls = []
+        for sgn in (1, -1):
+            f = make(lam + sgn * h)
+            f.init_particles(n0, M)
+            forward(f, True, f"finite difference {'+' if sgn > 0 else '-'}h")
+            Ls.append(float((wgt * f.measure(T)["n"]).sum()))
+            f.close()
+        fdv = (Ls[0] - Ls[1]) / (2 * h)
+        out["fd_dL_dlambda"] = fdv
+        out["fd_rel_err_lambda"] = abs(gr["lambda"] - fdv) / abs(fdv)
     return out
 
 
@@ -379,6 +465,8 @@ def main():
     ap.add_argument("--adjoint", action="store_true",
                     help="only the differentiation workload, with finite differences of all 16 radial bins")
     ap.add_argument("--no-fd", action="store_true", help="with --adjoint: skip the finite differences")
+    ap.add_argument("--adjoint-large", action="store_true",
+                    help="only the 48^3 N_mem=300 4000-step differentiation (packed tape; about 6 minutes)")
     ap.add_argument("--autotune", metavar="OUT_JSON",
                     help="pick the memory-integral tile shape for the standard workloads and write the choices")
     ap.add_argument("--ninverse", action="store_true",
@@ -421,6 +509,9 @@ def main():
         return
     if args.adjoint:
         print(json.dumps({"adjoint": adjoint_bench(fd_bins=True, fd=not args.no_fd)}), flush=True)
+        return
+    if args.adjoint_large:
+        print(json.dumps({"adjoint_large": large_adjoint_bench(fd=not args.no_fd)}), flush=True)
         return
     if args.autotune:
         print(json.dumps({"autotune": autotune_bench(args.autotune)}), flush=True)

@@ -226,7 +226,15 @@ int kb_autotune_fft(kb_ctx *ctx, int reps, int *best, double *ms);
  *                         (N_mem slot pairs, V sites) is kept on a tape;
  *   checkpoint_every = C: the window is kept after steps 2, 2+C, 2+2C, ...
  *                         and each C-step segment is recomputed (taping its
- *                         columns) before it is reversed.
+ *                         columns) before it is reversed;
+ *   checkpoint_every = -1: the tape of every dropped column, packed to one
+ *                         value per orbit of the lattice's cubic group
+ *                         (axis permutations, reflections n -> Nx - n).  The
+ *                         dynamics keep a cubic-symmetric state symmetric,
+ *                         so a site's column is its orbit's; kL2 and the
+ *                         initial rows must be orbit-constant to 1e-12
+ *                         (KB_ERR_ARG otherwise).  At 48^3 this shrinks the
+ *                         tape 38x.
  * Fails with KB_ERR_OOM if the tape or checkpoints do not fit. */
 int kb_adj_record(kb_ctx *ctx, int checkpoint_every, int max_steps);
 /* Reverse sweep from T = tmax down to the initial rows.  wgt[V] are the

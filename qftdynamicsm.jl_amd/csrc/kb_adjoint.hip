@@ -31,6 +31,7 @@
 // column's N slot pairs.  With checkpointing, only the window at every C-th
 // step is kept and each segment's steps are recomputed (taping) before it
 // is reversed.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -126,6 +127,26 @@ __global__ void tape_copy_kernel(double* __restrict__ H, double* __restrict__ ta
     st2(tp, ld2(h));
   else
     st2(h, ld2(tp));
+}
+
+// packed tape of a cubic-symmetric state: one (F, r) pair per (slot x,
+// orbit), read at the orbit's first site; layout [x][orbit][2]
+__global__ void tape_pack_kernel(const double* __restrict__ H, double* __restrict__ tape,
+                                 const int* __restrict__ orb_rep, int norb, long NP, int N, int s) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)norb * N) return;
+  const int x = (int)(idx / norb), o = (int)(idx % norb);
+  const int r = orb_rep[o];
+  st2(tape + (size_t)idx * 2, ld2(H + hoff(r / MB, pair_off(s, x, N), r % MB, NP)));
+}
+
+// the primal column back from a packed tape: every site takes its orbit's pair
+__global__ void tape_unpack_kernel(double* __restrict__ H, const double* __restrict__ tape,
+                                   const int* __restrict__ site_orb, int V, int norb, long NP, int N, int s) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)V * N) return;
+  const int x = (int)(idx / V), p = (int)(idx % V);
+  st2(H + hoff(p / MB, pair_off(s, x, N), p % MB, NP), ld2(tape + ((size_t)x * norb + site_orb[p]) * 2));
 }
 
 // clear the adjoint of the slot pairs (s, x) and, with a tape column, put
@@ -781,6 +802,11 @@ struct AdjState {
   double* tape = nullptr;
   int tape_cols = 0;
   size_t col_doubles = 0;  // one tape column
+  // packed tape (cubic-symmetric states): one value per orbit and slot
+  bool packed = false;
+  int norb = 0;
+  int* orb_rep = nullptr;   // orbit -> first site
+  int* site_orb = nullptr;  // site -> orbit
   std::vector<double*> ckpt;
   size_t hbytes = 0;
   size_t bytes = 0;
@@ -803,7 +829,7 @@ int amalloc(kb_ctx* c, AdjState* s, void** p, size_t bytes) {
 void free_state(AdjState* s) {
   if (!s) return;
   for (void* p : {(void*)s->Hbar, (void*)s->Sbar, (void*)s->X, (void*)s->P, (void*)s->LP, (void*)s->m2r,
-                  (void*)s->Mb, (void*)s->out, (void*)s->tape})
+                  (void*)s->Mb, (void*)s->out, (void*)s->tape, (void*)s->orb_rep, (void*)s->site_orb})
     if (p) (void)hipFree(p);
   for (int j = 0; j < AK; ++j) {
     if (s->Sring[j]) (void)hipFree(s->Sring[j]);
@@ -828,6 +854,13 @@ int save_column(kb_ctx* c, int t) {
   if (st.dead < 0) return KB_OK;
   const int k = tape_index(c, t);
   if (k < 0 || k >= s->tape_cols) return seterr(c, KB_ERR_STATE, "adjoint: tape column %d out of range", k);
+  if (s->packed) {
+    const long n = (long)s->norb * c->N;
+    hipLaunchKernelGGL(tape_pack_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->H,
+                       s->tape + (size_t)k * s->col_doubles, s->orb_rep, s->norb, c->NP, c->N, st.s_t);
+    LAUNCHCHK(c);
+    return KB_OK;
+  }
   const dim3 grid((c->N * MB + 255) / 256, c->NB);
   hipLaunchKernelGGL(tape_copy_kernel, grid, dim3(256), 0, c->stream, c->H, s->tape + (size_t)k * s->col_doubles,
                      c->NP, c->N, st.s_t, 1);
@@ -953,8 +986,15 @@ int reverse_step(kb_ctx* c, int t, int T, int npend, bool apply) {
     const double* col = nullptr;
     if (a.st.dead >= 0) col = s->tape + (size_t)tape_index(c, t) * s->col_doubles;
     const dim3 grid((N * MB + 255) / 256, c->NB);
-    hipLaunchKernelGGL(adj_slide_kernel, grid, dim3(256), 0, c->stream, c->H, s->Hbar, col, c->NP, N, a.st.s_t);
+    hipLaunchKernelGGL(adj_slide_kernel, grid, dim3(256), 0, c->stream, c->H, s->Hbar, s->packed ? nullptr : col,
+                       c->NP, N, a.st.s_t);
     LAUNCHCHK(c);
+    if (s->packed && col) {
+      const long n = (long)c->V * N;
+      hipLaunchKernelGGL(tape_unpack_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->H, col,
+                         s->site_orb, c->V, s->norb, c->NP, N, a.st.s_t);
+      LAUNCHCHK(c);
+    }
   }
   return KB_OK;
 }
@@ -1012,7 +1052,7 @@ void adj_free(kb_ctx* c) {
 extern "C" {
 
 int kb_adj_record(kb_ctx* c, int checkpoint_every, int max_steps) {
-  if (!c || checkpoint_every < 0 || max_steps < 2) return KB_ERR_ARG;
+  if (!c || checkpoint_every < -1 || max_steps < 2) return KB_ERR_ARG;
   if (!c->initialised || c->st.tmax != 2)
     return seterr(c, KB_ERR_STATE, "adjoint recording starts right after initialisation (window [1,2])");
   if (c->in_ensemble) return seterr(c, KB_ERR_STATE, "adjoint: not available for ensemble members");
@@ -1036,12 +1076,52 @@ int kb_adj_record(kb_ctx* c, int checkpoint_every, int max_steps) {
   drop_graph(c);
   AdjState* s = new (std::nothrow) AdjState();
   if (!s) return KB_ERR_OOM;
-  s->C = checkpoint_every;
+  s->C = checkpoint_every < 0 ? 0 : checkpoint_every;
+  s->packed = checkpoint_every < 0;
   s->max_steps = max_steps;
   const size_t hcount = (size_t)c->NB * (size_t)c->NP * MB * 2;
   s->hbytes = hcount * sizeof(double);
   const size_t ring = (size_t)c->NB * c->N * MB * 2;
   s->col_doubles = ring;
+  if (s->packed) {
+    // the packed tape keeps one value per orbit: the state must be carried
+    // exactly by its orbits (kL2 and the initial rows orbit-constant)
+    std::vector<int> orbit_of, rep_site;
+    cubic_orbits(c->p.Nx, c->p.sdim, orbit_of, rep_site);
+    std::vector<double> lat((size_t)c->V);
+    auto symmetric = [&](const std::vector<double>& v) {
+      double scale = 0.0, dev = 0.0;
+      for (int p = 0; p < c->V; ++p) {
+        scale = std::max(scale, std::fabs(v[p]));
+        dev = std::max(dev, std::fabs(v[p] - v[rep_site[orbit_of[p]]]));
+      }
+      return dev <= 1e-12 * scale;
+    };
+    bool ok = true;
+    if (hipMemcpy(lat.data(), c->kL2, sizeof(double) * c->V, hipMemcpyDeviceToHost) != hipSuccess) ok = false;
+    ok = ok && symmetric(lat);
+    const int rows[3][2] = {{1, 1}, {2, 1}, {2, 2}};
+    for (int k = 0; k < 3 && ok; ++k) {
+      hipLaunchKernelGGL(gather_pair, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->H, c->NP, c->V,
+                         pair_off((rows[k][0] - 1) % c->N, (rows[k][1] - 1) % c->N, c->N), 0, 1.0, c->vbuf);
+      ok = hipMemcpyAsync(lat.data(), c->vbuf, sizeof(double) * c->V, hipMemcpyDeviceToHost, c->stream) ==
+               hipSuccess &&
+           hipStreamSynchronize(c->stream) == hipSuccess && symmetric(lat);
+    }
+    if (!ok) {
+      delete s;
+      return seterr(c, KB_ERR_ARG, "adjoint: the packed tape needs a cubic-symmetric state (kL2 and initial rows)");
+    }
+    s->norb = (int)rep_site.size();
+    s->col_doubles = (size_t)s->norb * c->N * 2;
+    if (hipMalloc((void**)&s->orb_rep, sizeof(int) * s->norb) != hipSuccess ||
+        hipMalloc((void**)&s->site_orb, sizeof(int) * c->V) != hipSuccess ||
+        hipMemcpy(s->orb_rep, rep_site.data(), sizeof(int) * s->norb, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->site_orb, orbit_of.data(), sizeof(int) * c->V, hipMemcpyHostToDevice) != hipSuccess) {
+      free_state(s);
+      return seterr(c, KB_ERR_OOM, "adjoint: orbit maps");
+    }
+  }
   if (s->C > 0)
     s->tape_cols = s->C;
   else
@@ -1059,7 +1139,7 @@ int kb_adj_record(kb_ctx* c, int checkpoint_every, int max_steps) {
       (rc = amalloc(c, s, (void**)&s->P, 2 * (size_t)c->NB * 8)) ||
       (rc = amalloc(c, s, (void**)&s->LP, (size_t)c->NB * 8)) || (rc = amalloc(c, s, (void**)&s->m2r, 4 * 8)) ||
       (rc = amalloc(c, s, (void**)&s->Mb, 8)) || (rc = amalloc(c, s, (void**)&s->out, 4 * (size_t)c->V * 8)) ||
-      (rc = amalloc(c, s, (void**)&s->tape, (size_t)s->tape_cols * ring * 8)))
+      (rc = amalloc(c, s, (void**)&s->tape, (size_t)s->tape_cols * s->col_doubles * 8)))
     return fail(rc);
   if (s->C > 0) {
     const int nck = (max_steps - 2) / s->C + 1;  // windows at t = 2, 2 + C, ...

@@ -428,21 +428,7 @@ int dmalloc(kb_ctx* c, void** ptr, size_t bytes) {
 
 int meas_grid(const kb_ctx* c) { return (c->V + MEAS_THREADS - 1) / MEAS_THREADS; }
 
-}  // namespace
-#pragma GCC visibility pop
-
-extern "C" {
-
-const char* kb_last_error(const kb_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
-
-namespace {
-
-// Orbits of the cubic group of the lattice (axis permutations and
-// reflections n_i -> Nx - n_i) on the sites, numbered in order of first
-// appearance; the first site of an orbit is its representative.  kL2 and
-// the whole KB step are invariant under the group, so a cubic-symmetric
-// state is carried exactly by one value per orbit.
-static void cubic_orbits(int Nx, int sdim, std::vector<int>& orbit_of, std::vector<int>& rep_site) {
+void cubic_orbits(int Nx, int sdim, std::vector<int>& orbit_of, std::vector<int>& rep_site) {
   int Vl = 1;
   for (int d = 0; d < sdim; ++d) Vl *= Nx;
   orbit_of.assign(Vl, -1);
@@ -476,6 +462,20 @@ static void cubic_orbits(int Nx, int sdim, std::vector<int>& orbit_of, std::vect
   }
 }
 
+}  // namespace
+#pragma GCC visibility pop
+
+extern "C" {
+
+const char* kb_last_error(const kb_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+namespace {
+
+// Orbits of the cubic group of the lattice (axis permutations and
+// reflections n_i -> Nx - n_i) on the sites, numbered in order of first
+// appearance; the first site of an orbit is its representative.  kL2 and
+// the whole KB step are invariant under the group, so a cubic-symmetric
+// state is carried exactly by one value per orbit.
 static int create_ctx(const kb_params* p, bool sym, kb_ctx** out);
 
 }  // namespace

@@ -141,6 +141,9 @@ double hartree_prefactor(const kb_ctx* c);
 inline bool has_sigma(const kb_ctx* c) {
   return c->p.pexp == KB_PEXP_LAMBDA_NLO || c->p.pexp == KB_PEXP_NINVERSE_NLO;
 }
+// orbits of the lattice's cubic group (axis permutations, reflections
+// n -> Nx - n): lattice site -> orbit, and orbit -> its first site
+void cubic_orbits(int Nx, int sdim, std::vector<int>& orbit_of, std::vector<int>& rep_site);
 // the kernels of one forward step (the window in c->st_dev is advanced first)
 int enqueue_step(kb_ctx* c);
 size_t memint_lds(int N);

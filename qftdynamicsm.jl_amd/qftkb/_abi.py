@@ -432,7 +432,9 @@ class KBContext:
     def adj_record(self, checkpoint_every=0, max_steps=None):
         """Record the coming kb_step calls for the reverse sweep (call right
         after initialisation).  checkpoint_every = 0 keeps a tape of every
-        dropped column; C > 0 keeps the window every C steps and recomputes."""
+        dropped column; C > 0 keeps the window every C steps and recomputes;
+        -1 keeps the tape packed to one value per cubic orbit (the state must
+        be cubic-symmetric)."""
         if max_steps is None:
             raise ValueError("max_steps is required")
         self._chk(self._L.kb_adj_record(self._h, int(checkpoint_every), int(max_steps)))

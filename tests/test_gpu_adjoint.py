@@ -146,6 +146,42 @@ def test_checkpointed_sweep_equals_taped_sweep(C):
     c1.close()
 
 
+@pytest.mark.parametrize("args,T", [
+    pytest.param((6, 3, 8, 0.1, 1.4, 1, 1.0, 1.0, 1), 40, id="6^3-sliding"),
+    pytest.param((5, 3, 7, 0.1, 1.2, 1, 1.0, 1.0, 1), 25, id="5^3-ragged-blocks"),
+    pytest.param((8, 2, 6, 0.08, 1.7, 4, 0.9, 0.81, 1), 30, id="8^2-O(4)"),
+])
+def test_packed_tape_equals_plain_tape(args, T):
+    """checkpoint_every = -1 keeps one tape value per cubic orbit; for a
+    cubic-symmetric state the gradient equals the plain tape's."""
+    print(f"\n[case] packed tape {args} T={T}", flush=True)
+    n0, m, wgt = setup(args)
+    c0, L0, g0 = gpu_gradient(args, n0, m, T, wgt, C=0)
+    c1, L1, g1 = gpu_gradient(args, n0, m, T, wgt, C=-1)
+    assert L1 == pytest.approx(L0, rel=1e-13)
+    assert g1["lambda"] == pytest.approx(g0["lambda"], rel=1e-11)
+    assert rel(g1["n0"], g0["n0"]) <= 1e-11
+    assert rel(g1["F0"], g0["F0"]) <= 1e-11
+    assert c1.adj_info()["device_bytes"] < c0.adj_info()["device_bytes"]
+    c0.close()
+    c1.close()
+
+
+def test_packed_tape_rejects_a_state_without_cubic_symmetry():
+    from qftkb import KBContext
+    from qftkb._abi import KBError
+    args = (6, 3, 8, 0.1, 1.4, 1, 1.0, 1.0)
+    n0, m, _ = setup(args + (1,))
+    n0 = n0.copy()
+    n0[1] += 0.1  # site (1, 0, 0) only
+    c = KBContext(*args)
+    c.init_particles(n0, m)
+    with pytest.raises(KBError):
+        c.adj_record(-1, 20)
+    c.adj_record(0, 20)  # the plain tape still records
+    c.close()
+
+
 def test_second_forward_and_gradient_repeat():
     """After a gradient the context is back at the initial state; the same
     forward run and sweep give the same numbers."""
