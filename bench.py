@@ -598,7 +598,10 @@ def main():
                      "kernel": "memint_step", "algorithmic_bytes": B,
                      "avg_kernel_ms": avg_mi_s * 1e3, "sigma_stage_ms": ms_fft / max(n_fft, 1),
                      "tiles_rows_batch_threads_fftthreads": tile_shape,
-                     "hartree_ms": ms_h / max(n_mi, 1)},
+                     "hartree_ms": ms_h / max(n_mi, 1),
+                     # the whole step: memory-integral bytes plus a fused self-energy's
+                     # 4 (N-1) V doubles, over the wall time per step
+                     "step_achieved": (B + 4 * (N - 1) * Vs * 8) / (elapsed / args.steps) / 1e9},
         "cpu_baseline": None,
         "adjoint": None,
     }
