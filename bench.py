@@ -528,7 +528,8 @@ def main():
     ctx = KBContext(w["Nx"], w["sdim"], N, w["dt"], w["Lambda"], w["ONgroup"], w["Mass"], w["Mass"] ** 2,
                     symmetric=args.sym, pexp=pexp)
     Vs = ctx.storage_info()[0]  # stored sites: V, or the cubic orbits
-    tile_shape = list(ctx.tile_shapes()[ctx.tiles()]) + [ctx.fft_shapes()[ctx.fft_tiles()]]
+    ft = ctx.fft_tiles()
+    tile_shape = list(ctx.tile_shapes()[ctx.tiles()]) + ["orbit product" if ft < 0 else ctx.fft_shapes()[ft]]
     n = thermal_n(ctx.kL2(), w["Mass"], w["T"])
     ctx.fill_free_window(N, n, w["Mass"])
     print(f"[bench] window filled: {w['Nx']}^3 N_mem={N}, device bytes {ctx.device_bytes() / 1e9:.2f} GB",

@@ -21,6 +21,11 @@
 #ifndef KB_MSPLIT_MAX
 #define KB_MSPLIT_MAX 8
 #endif
+// largest orbit count whose self-energy runs as orbit products (the matrix
+// takes orbits^2 doubles: 134 MB at the limit)
+#ifndef KB_ORBIT_PRODUCT_MAX
+#define KB_ORBIT_PRODUCT_MAX 4096
+#endif
 
 using namespace qftkb;
 using namespace qftkb_host;
@@ -150,6 +155,27 @@ int launch_sigma(kb_ctx* c, double2* X, double* S_out) {
   a.preR = ivol * ivol * ivol * (-(lam * lam) * (ON + 2.0) / (6.0 * ON * ON));
   a.X = X;
   const int L = rows;
+  if (c->fft_tile == KB_FFT_ORBIT_PRODUCT) {
+    if (!c->Mo || X || S_out) return seterr(c, KB_ERR_STATE, "self-energy: the orbit product serves the forward step only");
+    OrbitArgs o{};
+    o.Mo = c->Mo;
+    o.H = c->H;
+    o.Y = c->Yo;
+    o.G = c->G;
+    o.S = c->S;
+    o.st = c->st_dev;
+    o.NP = c->NP;
+    o.N = c->N;
+    o.V = c->V;
+    o.Mp = c->Mp;
+    o.Kp = c->Kp;
+    o.ldg = c->ldg;
+    o.preF = a.preF;
+    o.preR = a.preR;
+    hipError_t e = launch_orbit_sigma(o, c->stream);
+    if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "orbit self-energy launch failed: %s", hipGetErrorString(e));
+    return KB_OK;
+  }
   if (sigma_planes_supported(c->p.Nx)) {
     hipError_t e = launch_sigma_planes(a, L, c->stream, c->fft_tile);
     if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "self-energy launch failed: %s", hipGetErrorString(e));
@@ -403,7 +429,7 @@ void free_all(kb_ctx* c) {
                     (void*)c->modes, (void*)c->vbuf, (void*)c->tw, (void*)c->flag, (void*)c->items_r[0],
                     (void*)c->items_r[1], (void*)c->items_r[2], (void*)c->smap_dev, (void*)c->rmap_dev,
                     (void*)c->wgt_dev, (void*)c->PF, (void*)c->PR, (void*)c->IR, (void*)c->Iring, (void*)c->Xn,
-                    (void*)c->stfill_dev, (void*)c->pitems, (void*)c->mpart,
+                    (void*)c->stfill_dev, (void*)c->pitems, (void*)c->mpart, (void*)c->Mo, (void*)c->Yo, (void*)c->G,
                     (void*)c->st_dev, (void*)c->stptr_dev, (void*)c->N_dev, (void*)c->margs_dev, (void*)c->wg0_dev})
     if (ptr) (void)hipFree(ptr);
   adj_free(c);
@@ -573,6 +599,43 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
         hipMemcpy(c->rmap_dev, rmap.data(), sizeof(int) * c->Vl, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->wgt_dev, wgt.data(), sizeof(double) * c->V, hipMemcpyHostToDevice) != hipSuccess)
       return fail(KB_ERR_HIP);
+  }
+  // orbit storage, LambdaNLO: the self-energy stage as products with the
+  // orbit cosine matrix Mo[o][b] = sum_{x in orbit b} cos(2 pi k_o . x / Nx),
+  // k_o the representative of orbit o (zero-padded to the product's tiles)
+  if (sym && p->pexp == KB_PEXP_LAMBDA_NLO && c->V <= KB_ORBIT_PRODUCT_MAX) {
+    c->Mp = (c->V + ORBIT_TILE - 1) / ORBIT_TILE * ORBIT_TILE;
+    c->Kp = (c->V + ORBIT_K - 1) / ORBIT_K * ORBIT_K;
+    std::vector<double> mo((size_t)c->Mp * c->Kp, 0.0), ctab(p->Nx);
+    for (int j = 0; j < p->Nx; ++j)
+      ctab[j] = (double)cosl(2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)p->Nx);
+    std::vector<int> xc((size_t)c->Vl * p->sdim);
+    for (int x = 0; x < c->Vl; ++x)
+      for (int d = 0, rem = x; d < p->sdim; ++d, rem /= p->Nx) xc[(size_t)x * p->sdim + d] = rem % p->Nx;
+    for (int o = 0; o < c->V; ++o) {
+      const int* ko = &xc[(size_t)c->rep_site[o] * p->sdim];
+      double* row = &mo[(size_t)o * c->Kp];
+      for (int x = 0; x < c->Vl; ++x) {
+        int m = 0;
+        for (int d = 0; d < p->sdim; ++d) m += ko[d] * xc[(size_t)x * p->sdim + d];
+        row[c->orbit_of[x]] += ctab[m % p->Nx];
+      }
+    }
+    // Y and G: Kp rows x whole column tiles, zero outside the live block
+    c->ldg = (2 * (c->N - 1) + ORBIT_TILE - 1) / ORBIT_TILE * ORBIT_TILE;
+    const size_t ybytes = (size_t)c->Kp * c->ldg * sizeof(double);
+    if ((rc = dmalloc(c, (void**)&c->Mo, mo.size() * sizeof(double))) ||
+        (rc = dmalloc(c, (void**)&c->Yo, ybytes)) || (rc = dmalloc(c, (void**)&c->G, ybytes)))
+      return fail(rc);
+    if (hipMemset(c->Yo, 0, ybytes) != hipSuccess || hipMemset(c->G, 0, ybytes) != hipSuccess)
+      return fail(KB_ERR_HIP);
+    if (hipMemcpy(c->Mo, mo.data(), mo.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(KB_ERR_HIP);
+    if (orbit_prepare() != hipSuccess) {
+      seterr(c, KB_ERR_ARG, "orbit product kernels: LDS attribute rejected");
+      return fail(KB_ERR_ARG);
+    }
+    c->fft_tile = KB_FFT_ORBIT_PRODUCT;
   }
   // orbit storage leaves fewer site blocks than CUs: split each block's
   // memory integral over several workgroups (about KB_MSPLIT_WGS in all)
@@ -1122,7 +1185,9 @@ int kb_fft_shape(int variant, int* threads) {
 }
 
 int kb_set_fft_tiles(kb_ctx* c, int variant) {
-  if (!c || variant < 0 || variant >= NFFT_VARIANTS) return KB_ERR_ARG;
+  if (!c || (variant != KB_FFT_ORBIT_PRODUCT && (variant < 0 || variant >= NFFT_VARIANTS)) ||
+      (variant == KB_FFT_ORBIT_PRODUCT && !c->Mo))
+    return KB_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->fft_tile = variant;
   drop_graph(c);
@@ -1142,12 +1207,14 @@ int kb_autotune_fft(kb_ctx* c, int reps, int* best, double* ms) {
   if (c->in_ensemble || c->adj) return seterr(c, KB_ERR_STATE, "autotune: not for ensemble members or while recording");
   if (c->p.pexp != KB_PEXP_LAMBDA_NLO || !sigma_planes_supported(c->p.Nx))
     return seterr(c, KB_ERR_ARG, "autotune: no plane self-energy stage for this context");
-  std::vector<double> t(NFFT_VARIANTS, 0.0);
+  // the plane kernels' block sizes, then (orbit storage) the orbit product
+  const int nv = NFFT_VARIANTS + (c->Mo ? 1 : 0);
+  std::vector<double> t(nv, 0.0);
   hipEvent_t a = take_event(c), b = take_event(c);
   if (!a || !b) return seterr(c, KB_ERR_HIP, "autotune: no events");
   int rc = KB_OK;
-  for (int v = 0; v < NFFT_VARIANTS && rc == KB_OK; ++v) {
-    c->fft_tile = v;
+  for (int v = 0; v < nv && rc == KB_OK; ++v) {
+    c->fft_tile = v < NFFT_VARIANTS ? v : KB_FFT_ORBIT_PRODUCT;
     // recomputes Sigma(t-1, .) of the last step: the same values
     for (int k = 0; k <= reps && rc == KB_OK; ++k) {
       if (k == 1 && hipEventRecord(a, c->stream) != hipSuccess) rc = seterr(c, KB_ERR_HIP, "autotune: event");
@@ -1164,12 +1231,13 @@ int kb_autotune_fft(kb_ctx* c, int reps, int* best, double* ms) {
   c->freeEv.push_back(b);
   if (rc) return rc;
   int bv = 0;
-  for (int v = 1; v < NFFT_VARIANTS; ++v)
+  for (int v = 1; v < nv; ++v)
     if (t[v] < t[bv]) bv = v;
+  if (bv == NFFT_VARIANTS) bv = KB_FFT_ORBIT_PRODUCT;
   if ((rc = kb_set_fft_tiles(c, bv))) return rc;
   if (best) *best = bv;
   if (ms)
-    for (int v = 0; v < NFFT_VARIANTS; ++v) ms[v] = t[v];
+    for (int v = 0; v < nv; ++v) ms[v] = t[v];
   return KB_OK;
 }
 

@@ -66,7 +66,13 @@ struct kb_ctx {
   int tile = 0;           // memory-integral tile shape (kMemintVariants index)
   int msplit = 1;         // workgroups per site block of the memory integral (split form if > 1)
   double* mpart = nullptr;  // their partial accumulators
-  int fft_tile = 0;       // self-energy plane kernels' block size (kFFTThreads index)
+  int fft_tile = 0;       // self-energy plane kernels' block size (kFFTThreads index),
+                          // or KB_FFT_ORBIT_PRODUCT
+  // orbit storage, LambdaNLO: the orbit cosine matrix and the sunset products
+  double* Mo = nullptr;
+  double* Yo = nullptr;  // row t-1 packed per orbit
+  double* G = nullptr;
+  int Mp = 0, Kp = 0, ldg = 0;
   // device-resident step state and launch tables (graph-capturable steps)
   qftkb::StepState* st_dev = nullptr;
   qftkb::StepState** stptr_dev = nullptr;  // {st_dev}

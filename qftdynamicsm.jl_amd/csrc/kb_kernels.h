@@ -220,6 +220,25 @@ hipError_t launch_sigma_planes_adjoint(const SigmaArgs& a, int rows, hipStream_t
 hipError_t launch_bubble_planes(const SigmaArgs& a, int rows, hipStream_t st);
 hipError_t launch_chain_planes(const SigmaArgs& a, int rows, hipStream_t st);
 
+// self-energy stage of a cubic-orbit-stored window as two FP64 matrix
+// products with the orbit cosine matrix Mo (kb_orbit.hip)
+struct OrbitArgs {
+  const double* Mo;  // [Mp][Kp], zero-padded: Mo[o][b] = sum_{x in orbit b} cos(k_o . x)
+  const double* H;
+  double* Y;         // [Kp][ldg] (F, rho)(t-1, t') packed from the history, columns 2q + comp
+  double* G;         // [Kp][ldg] sunset products of (t-1, t'), same layout
+  double* S;
+  const StepState* st;
+  long NP;
+  int N, V, Mp, Kp, ldg;  // ldg: 2 (N - 1) rounded up to whole tiles; rows V.. of Y and G are zero
+  double preF, preR;
+};
+constexpr int ORBIT_TILE = 64;  // output tile of the orbit products (rows of Mo padded to it)
+constexpr int ORBIT_K = 64;     // k step (columns of Mo padded to it)
+size_t orbit_lds();
+hipError_t orbit_prepare();  // LDS attribute of the product kernels
+hipError_t launch_orbit_sigma(const OrbitArgs& a, hipStream_t st);
+
 constexpr int MEMINT_THREADS = 256;  // default workgroup of the memory-integral kernels
 constexpr int MEMINT_ROWS = 64;  // rows per work item of the adjoint kernels
 constexpr int MEAS_THREADS = 256;

@@ -28,6 +28,8 @@ KB_PEXP_LAMBDA_LO = 0
 KB_PEXP_LAMBDA_NLO = 1
 KB_PEXP_NINVERSE_LO = 2
 KB_PEXP_NINVERSE_NLO = 3
+# self-energy form of orbit-storage LambdaNLO contexts: orbit cosine products
+FFT_ORBIT_PRODUCT = -1
 PEXP_CODES = {"LambdaLO": KB_PEXP_LAMBDA_LO, "LambdaNLO": KB_PEXP_LAMBDA_NLO, "NinverseLO": KB_PEXP_NINVERSE_LO,
               "NinverseNLO": KB_PEXP_NINVERSE_NLO}
 
@@ -236,6 +238,8 @@ class KBContext:
         self.params = p
         self.V = int(Nx) ** int(sdim)
         self.symmetric = bool(symmetric)
+        # orbit storage, LambdaNLO: created on the orbit product
+        self.orbit_product = self.symmetric and self.fft_tiles() == FFT_ORBIT_PRODUCT
         if _owner is None and tiles is not None:
             if tiles == "auto" and symmetric:
                 v, f = committed_tiles(Nx, sdim, NstepsinMemory, symmetric=True)
@@ -245,7 +249,9 @@ class KBContext:
                 v, f = int(tiles), None
             if v is not None:
                 self.set_tiles(v)
-            if f is not None:
+            # an orbit-storage context on the orbit product keeps it: the stored
+            # choice is for the plane kernels of full storage
+            if f is not None and not self.orbit_product:
                 self.set_fft_tiles(f)
 
     def close(self):
@@ -412,6 +418,8 @@ class KBContext:
         return out
 
     def set_fft_tiles(self, variant):
+        """A plane-kernel block size (index into fft_shapes()), or
+        FFT_ORBIT_PRODUCT in an orbit-storage LambdaNLO context."""
         self._chk(self._L.kb_set_fft_tiles(self._h, int(variant)))
 
     def fft_tiles(self):
@@ -421,12 +429,17 @@ class KBContext:
 
     def autotune_fft(self, reps=3):
         """Re-time the last step's self-energy stage for every plane-kernel
-        block size and keep the fastest.  Returns {"variant", "threads", "ms"}."""
+        block size (and, in orbit storage, the orbit product, timed last) and
+        keep the fastest.  Returns {"variant", "threads", "ms"}; threads is
+        None when the orbit product wins."""
         shapes = self.fft_shapes()
-        ms = np.zeros(len(shapes))
+        ms = np.zeros(len(shapes) + 1)
         best = ctypes.c_int()
         self._chk(self._L.kb_autotune_fft(self._h, int(reps), ctypes.byref(best), _ptr(ms)))
-        return {"variant": best.value, "threads": shapes[best.value], "ms": ms.tolist()}
+        if not self.orbit_product:
+            ms = ms[:len(shapes)]
+        b = best.value
+        return {"variant": b, "threads": None if b == FFT_ORBIT_PRODUCT else shapes[b], "ms": ms.tolist()}
 
     # ---- adjoint ------------------------------------------------------
     def adj_record(self, checkpoint_every=0, max_steps=None):
@@ -494,7 +507,7 @@ class KBEnsemble:
             # members share the default memory-integral shape (one launch over
             # all of them); the self-energy stage is per member
             _, f = committed_tiles(r["Nx"], r["sdim"], r["NstepsinMemory"])
-            if f is not None:
+            if f is not None and not member.orbit_product:
                 member.set_fft_tiles(f)
             self.members.append(member)
 

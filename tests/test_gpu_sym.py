@@ -95,3 +95,70 @@ def test_orbit_storage_config1_box_against_oracle():
             assert abs(mg["E"] - mo["E"]) <= 1e-10 * abs(mo["E"])
             assert rel(mg["n"], mo["n"]) <= 1e-10
     g.close()
+
+
+def _thermal_run(ctx, N, steps):
+    n = thermal(ctx.kL2())
+    ctx.fill_free_window(N, n, 1.0)
+    for t in range(N + 1, N + 1 + steps):
+        ctx.step(t)
+    return N + steps
+
+
+@pytest.mark.parametrize("args,filled", [
+    pytest.param((16, 3, 20, 0.1, 1.0, 1, 1.0, 1.0), False, id="16^3-N20"),
+    pytest.param((12, 2, 9, 0.1, 1.5, 2, 1.0, 1.0), False, id="12^2-O(2)"),
+    pytest.param((32, 3, 400, 0.1, 1.0, 1, 1.0, 1.0), True, id="32^3-N400-filled"),
+])
+def test_orbit_product_equals_plane_kernels(args, filled):
+    """Orbit storage, LambdaNLO: the self-energy as products with the orbit
+    cosine matrix (the default) and as plane FFTs of the expanded lattice are
+    the same sums in another order: one trajectory to 1e-12."""
+    from qftkb import KBContext
+    from qftkb._abi import FFT_ORBIT_PRODUCT
+    print(f"\n[case] orbit product vs plane kernels {args}", flush=True)
+    a = KBContext(*args, symmetric=True)
+    b = KBContext(*args, symmetric=True, tiles=None)
+    assert a.orbit_product and a.fft_tiles() == FFT_ORBIT_PRODUCT
+    b.set_fft_tiles(0)
+    assert b.fft_tiles() == 0
+    if filled:
+        T = _thermal_run(a, args[2], 4)
+        assert _thermal_run(b, args[2], 4) == T
+    else:
+        n = thermal(a.kL2())
+        a.init_particles(n, 1.0)
+        b.init_particles(n, 1.0)
+        T = 3 * args[2]
+        for t in range(3, T + 1):
+            a.step(t)
+            b.step(t)
+    tmin, tmax = a.window()
+    assert tmax == T
+    for tp in range(tmin, tmax + 1):
+        for w in "Fr":
+            assert rel(a.get(w, tmax, tp), b.get(w, tmax, tp)) <= 1e-12, (w, tp)
+    for w in ("F", "r"):
+        sa = np.array([a.sigma(w, tp) for tp in range(tmin, tmax)])
+        sb = np.array([b.sigma(w, tp) for tp in range(tmin, tmax)])
+        assert np.max(np.abs(sa - sb)) <= 1e-12 * np.max(np.abs(sb)), w
+    ma, mb = a.measure(T), b.measure(T)
+    assert abs(ma["E"] - mb["E"]) <= 1e-12 * abs(mb["E"])
+    # the autotuner times the plane shapes and the product (last entry)
+    r = a.autotune_fft(reps=2)
+    assert len(r["ms"]) == len(a.fft_shapes()) + 1 and all(x > 0 for x in r["ms"])
+    assert r["variant"] == FFT_ORBIT_PRODUCT or 0 <= r["variant"] < len(a.fft_shapes())
+    a.close()
+    b.close()
+
+
+def test_orbit_product_only_for_orbit_storage_lambda_nlo():
+    from qftkb import KBContext, KBError
+    from qftkb._abi import FFT_ORBIT_PRODUCT
+    args = (8, 3, 10, 0.1, 1.0, 1, 1.0, 1.0)
+    for kw in ({}, {"symmetric": True, "pexp": 0}, {"symmetric": True, "pexp": 3}):
+        c = KBContext(*args, **kw)
+        assert not c.orbit_product
+        with pytest.raises(KBError):
+            c.set_fft_tiles(FFT_ORBIT_PRODUCT)
+        c.close()
