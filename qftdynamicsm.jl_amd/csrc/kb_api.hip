@@ -15,9 +15,6 @@
 #include "../../include/qftkb.h"
 #include "kb_internal.h"
 
-#ifndef KB_MSPLIT_WGS
-#define KB_MSPLIT_WGS 512
-#endif
 #ifndef KB_MSPLIT_MAX
 #define KB_MSPLIT_MAX 8
 #endif
@@ -381,7 +378,7 @@ int enqueue_step(kb_ctx* c) {
       hipLaunchKernelGGL((memint_partial<64, 2, 256>), dim3(c->NB * c->msplit), dim3(256), memint_lds(c->N),
                          c->stream, c->margs_dev, c->msplit, c->mpart);
       LAUNCHCHK(c);
-      hipLaunchKernelGGL(memint_finish, dim3(c->NB), dim3(MEMINT_THREADS), memint_lds(c->N), c->stream,
+      hipLaunchKernelGGL(memint_finish, dim3(c->NB), dim3(MEMINT_FINISH_THREADS), memint_lds(c->N), c->stream,
                          c->margs_dev, c->msplit, c->mpart);
       LAUNCHCHK(c);
     } else {
@@ -637,13 +634,6 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
     }
     c->fft_tile = KB_FFT_ORBIT_PRODUCT;
   }
-  // orbit storage leaves fewer site blocks than CUs: split each block's
-  // memory integral over several workgroups (about KB_MSPLIT_WGS in all)
-  if (sym && c->NB < 256) {
-    c->msplit = std::min(KB_MSPLIT_MAX, (KB_MSPLIT_WGS + c->NB - 1) / c->NB);
-    if ((rc = dmalloc(c, (void**)&c->mpart, (size_t)c->NB * c->msplit * 2 * c->N * MB * sizeof(double))))
-      return fail(rc);
-  }
   if ((rc = dmalloc(c, (void**)&c->kL2, (size_t)c->Vpad * sizeof(double)))) return fail(rc);
   if ((rc = dmalloc(c, (void**)&c->m2, 8 * sizeof(double)))) return fail(rc);
   if ((rc = dmalloc(c, (void**)&c->partial, (size_t)6 * meas_grid(c) * sizeof(double)))) return fail(rc);
@@ -704,6 +694,23 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
       seterr(c, KB_ERR_ARG, "memory-integral kernel: %zu bytes of LDS rejected: %s", lds, hipGetErrorString(e));
       return fail(KB_ERR_ARG);
     }
+  }
+  // orbit storage leaves fewer site blocks than workgroup slots: split each
+  // block's memory integral over msplit workgroups, as many as fit the
+  // device's slots for this window's LDS size in one round (a grid that
+  // overfills some CUs by one workgroup waits for those CUs)
+  if (sym) {
+    int per_cu = 0;
+    hipDeviceProp_t prop;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)memint_partial<64, 2, 256>, 256,
+                                                     memint_lds(c->N)) != hipSuccess ||
+        hipGetDeviceProperties(&prop, p->device) != hipSuccess)
+      return fail(KB_ERR_HIP);
+    const int slots = per_cu * prop.multiProcessorCount;
+    c->msplit = std::max(1, std::min(KB_MSPLIT_MAX, slots / c->NB));
+    if (c->msplit > 1 &&
+        (rc = dmalloc(c, (void**)&c->mpart, (size_t)c->NB * c->msplit * 2 * c->N * MB * sizeof(double))))
+      return fail(rc);
   }
   if (hipMemsetAsync(c->H, 0, hcount * sizeof(double), c->stream) != hipSuccess) return fail(KB_ERR_HIP);
   if (hipMemsetAsync(c->S, 0, (size_t)c->NB * c->N * MB * 2 * sizeof(double), c->stream) != hipSuccess)

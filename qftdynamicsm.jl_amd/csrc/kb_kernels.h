@@ -240,6 +240,7 @@ hipError_t orbit_prepare();  // LDS attribute of the product kernels
 hipError_t launch_orbit_sigma(const OrbitArgs& a, hipStream_t st);
 
 constexpr int MEMINT_THREADS = 256;  // default workgroup of the memory-integral kernels
+constexpr int MEMINT_FINISH_THREADS = 1024;
 constexpr int MEMINT_ROWS = 64;  // rows per work item of the adjoint kernels
 constexpr int MEAS_THREADS = 256;
 

@@ -650,15 +650,16 @@ __global__ __launch_bounds__(TPB, 512 / TPB) void memint_partial(const MemintArg
   for (int i = tid; i < 2 * N * MB; i += blockDim.x) out[i] = smem[i];
 }
 
-__global__ __launch_bounds__(MEMINT_THREADS) void memint_finish(const MemintArgs* __restrict__ runs, int nsplit,
-                                                                const double* __restrict__ part) {
+// (1024 threads: the sum over the splits' partials is latency-bound)
+__global__ __launch_bounds__(MEMINT_FINISH_THREADS) void memint_finish(const MemintArgs* __restrict__ runs,
+                                                                       int nsplit, const double* __restrict__ part) {
   extern __shared__ double smem[];
   const MemintArgs a = runs[0];
   const StepState ss = *a.st;
   const int N = a.N;
   double* accF = smem;
   double* accR = accF + N * MB;
-  __shared__ double red[MEMINT_THREADS / 64][MB];
+  __shared__ double red[MEMINT_FINISH_THREADS / 64][MB];
   __shared__ double diagF[2][MB];
   const int mb = blockIdx.x;
   const double* in = part + (size_t)mb * nsplit * 2 * N * MB;
