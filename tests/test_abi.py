@@ -26,6 +26,21 @@ def test_library_exports_every_declared_symbol():
     assert sorted(qftkb.EXPORTED) == decl
 
 
+def test_library_exports_nothing_else():
+    """Kernel stubs, helpers and template instantiations stay out of the
+    dynamic symbol table (linker version script)."""
+    import shutil
+    import subprocess
+    import qftkb
+    nm = shutil.which("nm")
+    if nm is None:
+        pytest.skip("nm not available")
+    out = subprocess.run([nm, "-D", "--defined-only", qftkb.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    names = sorted(line.split()[-1] for line in out.splitlines() if line.strip())
+    assert names == declared_symbols(), [n for n in names if n not in declared_symbols()][:10]
+
+
 def test_create_rejects_bad_parameters_without_device():
     """Argument validation happens before any HIP call."""
     from qftkb._abi import KB_ERR_ARG, kb_params, load_library
