@@ -70,6 +70,10 @@ CASES = [
     (7, 2, 30, 12),
     (12, 1, 5, 25),
     (1, 3, 6, 20),
+    # prime Nx (the plane kernels' direct-DFT stage) at the minimum window N_mem = 3
+    (11, 2, 3, 12),
+    (11, 3, 3, 8),
+    (13, 1, 3, 15),
 ]
 
 

@@ -108,6 +108,7 @@ def _thermal_run(ctx, N, steps):
 @pytest.mark.parametrize("args,filled", [
     pytest.param((16, 3, 20, 0.1, 1.0, 1, 1.0, 1.0), False, id="16^3-N20"),
     pytest.param((12, 2, 9, 0.1, 1.5, 2, 1.0, 1.0), False, id="12^2-O(2)"),
+    pytest.param((11, 3, 6, 0.1, 1.0, 1, 1.0, 1.0), False, id="11^3-odd"),
     pytest.param((32, 3, 400, 0.1, 1.0, 1, 1.0, 1.0), True, id="32^3-N400-filled"),
 ])
 def test_orbit_product_equals_plane_kernels(args, filled):
