@@ -163,3 +163,33 @@ def test_orbit_product_only_for_orbit_storage_lambda_nlo():
         with pytest.raises(KBError):
             c.set_fft_tiles(FFT_ORBIT_PRODUCT)
         c.close()
+
+
+def test_orbit_storage_graph_replay_equals_eager():
+    """Orbit storage (orbit product, split memory integral) replayed from a
+    captured hipGraph gives the eager trajectory: the same kernels in the same
+    order, so equal to rounding, checked after many replays while the window
+    fills and slides, with eager measurements between the replays."""
+    from qftkb import KBContext
+    args = (16, 3, 12, 0.1, 1.0, 1, 1.0, 1.0)
+    print("\n[case] orbit storage, graph replay vs eager", flush=True)
+    a = KBContext(*args, symmetric=True)
+    b = KBContext(*args, symmetric=True)
+    assert a.orbit_product and b.orbit_product
+    b.use_graph(True)
+    n = thermal(a.kL2())
+    a.init_particles(n, 1.0)
+    b.init_particles(n, 1.0)
+    for t in range(3, 41):
+        a.step(t)
+        b.step(t)
+        if t % 4 == 0 or t == 40:
+            tmin, tmax = a.window()
+            assert b.window() == (tmin, tmax)
+            for tp in range(tmin, tmax + 1):
+                for w in "Fr":
+                    assert rel(b.get(w, tmax, tp), a.get(w, tmax, tp)) <= 1e-14, (t, w, tp)
+            ma, mb = a.measure(t), b.measure(t)
+            assert abs(ma["E"] - mb["E"]) <= 1e-14 * abs(ma["E"]), t
+    a.close()
+    b.close()
