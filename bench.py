@@ -571,7 +571,7 @@ def main():
     if args.ninverse:
         traffic = None
         pmc_n = load_pmc(os.path.join(ROOT, "profiles", "ninv_pmc_32c_n400.json"))
-        if pmc_n and pmc_n.get("Nx") == w["Nx"] and pmc_n.get("N") == N:
+        if pmc_n and pmc_n.get("Nx") == w["Nx"] and pmc_n.get("N") == N and not args.sym:
             traffic = pmc_n.get("traffic_bytes")
     line = {
         "metric": ("KB steps/s at 32^3 N_mem=400 FP64 (% HBM peak); adjoint/forward cost ratio"
