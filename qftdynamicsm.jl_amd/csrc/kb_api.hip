@@ -126,6 +126,32 @@ int launch_hartree(kb_ctx* c, int i, int j) {
   return KB_OK;
 }
 
+// the orbit products' arguments for the step in st
+OrbitArgs orbit_args(kb_ctx* c, const StepState* st, double preF, double preR) {
+  OrbitArgs o{};
+  o.Mo = c->Mo;
+  o.H = c->H;
+  o.Y = c->Yo;
+  o.G = c->G;
+  o.S = c->S;
+  o.st = st;
+  o.NP = c->NP;
+  o.N = c->N;
+  o.V = c->V;
+  o.Mp = c->Mp;
+  o.Kp = c->Kp;
+  o.ldg = c->ldg;
+  o.preF = preF;
+  o.preR = preR;
+  o.X = c->Xo;
+  o.PF = c->PF;
+  o.PR = c->PR;
+  o.Iring = c->Iring;
+  o.PNP = c->PNP;
+  o.ivol = 1.0 / (double)c->Vl;
+  return o;
+}
+
 int launch_sigma(kb_ctx* c, double2* X, double* S_out) {
   const int rows = c->N - 1;  // grid rows; rows q >= L exit at once
   const double ivol = 1.0 / (double)c->Vl;
@@ -154,22 +180,8 @@ int launch_sigma(kb_ctx* c, double2* X, double* S_out) {
   const int L = rows;
   if (c->fft_tile == KB_FFT_ORBIT_PRODUCT) {
     if (!c->Mo || X || S_out) return seterr(c, KB_ERR_STATE, "self-energy: the orbit product serves the forward step only");
-    OrbitArgs o{};
-    o.Mo = c->Mo;
-    o.H = c->H;
-    o.Y = c->Yo;
-    o.G = c->G;
-    o.S = c->S;
-    o.st = c->st_dev;
-    o.NP = c->NP;
-    o.N = c->N;
-    o.V = c->V;
-    o.Mp = c->Mp;
-    o.Kp = c->Kp;
-    o.ldg = c->ldg;
-    o.preF = a.preF;
-    o.preR = a.preR;
-    hipError_t e = launch_orbit_sigma(o, c->stream);
+    OrbitArgs o = orbit_args(c, c->st_dev, a.preF, a.preR);
+    hipError_t e = launch_orbit(o, ORBIT_SUNSET, c->stream);
     if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "orbit self-energy launch failed: %s", hipGetErrorString(e));
     return KB_OK;
   }
@@ -236,8 +248,15 @@ SigmaArgs ninv_sigma_args(kb_ctx* c, const StepState* st) {
 }
 
 int launch_bubble(kb_ctx* c, const StepState* st, double2* X) {
-  SigmaArgs a = ninv_sigma_args(c, st);
   const double ivol = 1.0 / (double)c->Vl;
+  if (c->fft_tile == KB_FFT_ORBIT_PRODUCT) {
+    OrbitArgs o = orbit_args(c, st, ivol * ivol, ivol * ivol);
+    if (!X) o.X = nullptr;  // rows rebuilt outside a step keep no transforms
+    hipError_t e = launch_orbit(o, ORBIT_BUBBLE, c->stream);
+    if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "orbit bubble launch failed: %s", hipGetErrorString(e));
+    return KB_OK;
+  }
+  SigmaArgs a = ninv_sigma_args(c, st);
   a.preF = a.preR = ivol * ivol;
   a.X = X;
   hipError_t e = launch_bubble_planes(a, c->N - 1, c->stream);
@@ -266,9 +285,15 @@ int launch_resum(kb_ctx* c, const StepState* st, int passes) {
 }
 
 int launch_chain(kb_ctx* c) {
-  SigmaArgs a = ninv_sigma_args(c, c->st_dev);
   const double ivol = 1.0 / (double)c->Vl;
-  a.preF = a.preR = -c->p.Lambda / (3.0 * (double)c->p.ONgroup) * ivol;
+  const double pre = -c->p.Lambda / (3.0 * (double)c->p.ONgroup) * ivol;
+  if (c->fft_tile == KB_FFT_ORBIT_PRODUCT) {
+    hipError_t e = launch_orbit(orbit_args(c, c->st_dev, pre, pre), ORBIT_CHAIN, c->stream);
+    if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "orbit chain launch failed: %s", hipGetErrorString(e));
+    return KB_OK;
+  }
+  SigmaArgs a = ninv_sigma_args(c, c->st_dev);
+  a.preF = a.preR = pre;
   a.X = c->Xn;
   hipError_t e = launch_chain_planes(a, c->N - 1, c->stream);
   if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "chain launch failed: %s", hipGetErrorString(e));
@@ -426,7 +451,7 @@ void free_all(kb_ctx* c) {
                     (void*)c->modes, (void*)c->vbuf, (void*)c->tw, (void*)c->flag, (void*)c->items_r[0],
                     (void*)c->items_r[1], (void*)c->items_r[2], (void*)c->smap_dev, (void*)c->rmap_dev,
                     (void*)c->wgt_dev, (void*)c->PF, (void*)c->PR, (void*)c->IR, (void*)c->Iring, (void*)c->Xn,
-                    (void*)c->stfill_dev, (void*)c->pitems, (void*)c->mpart, (void*)c->Mo, (void*)c->Yo, (void*)c->G,
+                    (void*)c->stfill_dev, (void*)c->pitems, (void*)c->mpart, (void*)c->Mo, (void*)c->Yo, (void*)c->Xo, (void*)c->G,
                     (void*)c->st_dev, (void*)c->stptr_dev, (void*)c->N_dev, (void*)c->margs_dev, (void*)c->wg0_dev})
     if (ptr) (void)hipFree(ptr);
   adj_free(c);
@@ -600,7 +625,7 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
   // orbit storage, LambdaNLO: the self-energy stage as products with the
   // orbit cosine matrix Mo[o][b] = sum_{x in orbit b} cos(2 pi k_o . x / Nx),
   // k_o the representative of orbit o (zero-padded to the product's tiles)
-  if (sym && p->pexp == KB_PEXP_LAMBDA_NLO && c->V <= KB_ORBIT_PRODUCT_MAX) {
+  if (sym && (p->pexp == KB_PEXP_LAMBDA_NLO || p->pexp == KB_PEXP_NINVERSE_NLO) && c->V <= KB_ORBIT_PRODUCT_MAX) {
     c->Mp = (c->V + ORBIT_TILE - 1) / ORBIT_TILE * ORBIT_TILE;
     c->Kp = (c->V + ORBIT_K - 1) / ORBIT_K * ORBIT_K;
     std::vector<double> mo((size_t)c->Mp * c->Kp, 0.0), ctab(p->Nx);
@@ -626,6 +651,10 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
       return fail(rc);
     if (hipMemset(c->Yo, 0, ybytes) != hipSuccess || hipMemset(c->G, 0, ybytes) != hipSuccess)
       return fail(KB_ERR_HIP);
+    if (p->pexp == KB_PEXP_NINVERSE_NLO) {
+      if ((rc = dmalloc(c, (void**)&c->Xo, ybytes))) return fail(rc);
+      if (hipMemset(c->Xo, 0, ybytes) != hipSuccess) return fail(KB_ERR_HIP);
+    }
     if (hipMemcpy(c->Mo, mo.data(), mo.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
       return fail(KB_ERR_HIP);
     if (orbit_prepare() != hipSuccess) {

@@ -71,6 +71,7 @@ struct kb_ctx {
   // orbit storage, LambdaNLO: the orbit cosine matrix and the sunset products
   double* Mo = nullptr;
   double* Yo = nullptr;  // row t-1 packed per orbit
+  double* Xo = nullptr;  // 1/N: (Fx, rx) of the bubble pass, kept for the chain pass
   double* G = nullptr;
   int Mp = 0, Kp = 0, ldg = 0;
   // device-resident step state and launch tables (graph-capturable steps)

@@ -225,19 +225,30 @@ hipError_t launch_chain_planes(const SigmaArgs& a, int rows, hipStream_t st);
 struct OrbitArgs {
   const double* Mo;  // [Mp][Kp], zero-padded: Mo[o][b] = sum_{x in orbit b} cos(k_o . x)
   const double* H;
-  double* Y;         // [Kp][ldg] (F, rho)(t-1, t') packed from the history, columns 2q + comp
-  double* G;         // [Kp][ldg] sunset products of (t-1, t'), same layout
+  double* Y;         // [Kp][ldg] inputs of (t-1, t') packed per orbit, columns 2q + comp
+  double* G;         // [Kp][ldg] pointwise products of (t-1, t'), same layout
   double* S;
   const StepState* st;
   long NP;
   int N, V, Mp, Kp, ldg;  // ldg: 2 (N - 1) rounded up to whole tiles; rows V.. of Y and G are zero
   double preF, preR;
+  // 1/N expansion: (Fx, rx) of the bubble pass kept for the chain pass
+  // (Y's layout; not kept when NULL), the bubble triangles, the (IF, Ir) ring
+  double* X;
+  double* PF;
+  double* PR;
+  const double* Iring;
+  long PNP;
+  double ivol;
 };
+// what the two products compute: the sunset (Sigma into the ring), the 1/N
+// bubbles of row t-1 (into the bubble triangles), the 1/N chain (Sigma)
+enum OrbitStage { ORBIT_SUNSET = 0, ORBIT_BUBBLE = 1, ORBIT_CHAIN = 2 };
 constexpr int ORBIT_TILE = 64;  // output tile of the orbit products (rows of Mo padded to it)
 constexpr int ORBIT_K = 64;     // k step (columns of Mo padded to it)
 size_t orbit_lds();
 hipError_t orbit_prepare();  // LDS attribute of the product kernels
-hipError_t launch_orbit_sigma(const OrbitArgs& a, hipStream_t st);
+hipError_t launch_orbit(const OrbitArgs& a, OrbitStage stage, hipStream_t st);
 
 constexpr int MEMINT_THREADS = 256;  // default workgroup of the memory-integral kernels
 constexpr int MEMINT_FINISH_THREADS = 1024;

@@ -17,13 +17,20 @@
 // orbit_gather packs row t-1 of the history (F and the sign-folded rho of the
 // pairs (t-1, t')) into Y[orbit][2 q + comp]: the pairs of one row lie spread
 // over the whole triangle, so they are read once here rather than by every
-// tile of the product.  orbit_gemm<STAGE> computes one 64 x 64 tile of Mo . Y
+// tile of the product.  orbit_gemm<MODE> computes one 64 x 64 tile of Mo . Y
 // with v_mfma_f64_16x16x4f64 (8 waves, 32 x 16 per wave), staging 64 x 64
 // tiles of Mo and of Y in double-buffered LDS; the next k step's tiles are
-// loaded into registers while the current one is multiplied.  STAGE 0
-// multiplies Y, and its epilogue pairs the two columns of each t' across
-// neighbouring lanes and stores the sunset products G in Y's layout; STAGE 1
-// multiplies G and writes Sigma into the ring.
+// loaded into registers while the current one is multiplied.  The first
+// product multiplies Y, and its epilogue pairs the two columns of each t'
+// across neighbouring lanes and stores the pointwise products G in Y's
+// layout; the second multiplies G into the stage's destination.
+//
+// The 1/N expansion's two transform passes (TwoPIScalar_Evolution.jl:
+// 144-735, the bubble and chain passes of the plane kernels) take the same
+// form: the bubble pass stores (Fx^2 - rx^2/4, Fx rx) and keeps (Fx, rx) in
+// X, then writes row t-1 of the bubble triangles; the chain pass gathers
+// (IF, Ir) from its ring, forms (Fx IFx - rx Irx/4, Fx Irx + rx IFx) with
+// (IFx, Irx) = C (IF, Ir) / V, and writes Sigma.
 //
 // MFMA operand layout (checked on gfx950): lane l holds A[l % 16][l / 16]
 // and B[l / 16][l % 16]; accumulator value v of lane l is D[4 v + l / 16][l % 16].
@@ -55,7 +62,9 @@ constexpr int GB = 64, GQ = 32;
 
 }  // namespace
 
-// blockIdx.x: 64 orbits, blockIdx.y: 32 t'
+// blockIdx.x: 64 orbits, blockIdx.y: 32 t'.  RING: (IF, Ir)(t-1, t') from
+// the 1/N ring (self-energy layout) instead of (F, rho) from the history.
+template <bool RING>
 __global__ __launch_bounds__(OTHREADS) void orbit_gather(OrbitArgs a) {
   __shared__ double2 t[GQ][GB + 1];
   const StepState ss = *a.st;
@@ -72,8 +81,12 @@ __global__ __launch_bounds__(OTHREADS) void orbit_gather(OrbitArgs a) {
     double2 v = make_double2(0.0, 0.0);
     if (b < a.V && q < ss.L) {
       const int slot = (ss.tmin + q - 1) % a.N;
-      v = *reinterpret_cast<const double2*>(a.H + hoff(b / MB, pair_off(ss.s_tm1, slot, a.N), b % MB, a.NP));
-      if (ss.s_tm1 < slot) v.y = -v.y;
+      if (RING) {
+        v = *reinterpret_cast<const double2*>(a.Iring + soff(b / MB, slot, b % MB, a.N));
+      } else {
+        v = *reinterpret_cast<const double2*>(a.H + hoff(b / MB, pair_off(ss.s_tm1, slot, a.N), b % MB, a.NP));
+        if (ss.s_tm1 < slot) v.y = -v.y;
+      }
     }
     t[qq][bb] = v;
   }
@@ -88,7 +101,11 @@ __global__ __launch_bounds__(OTHREADS) void orbit_gather(OrbitArgs a) {
   }
 }
 
-template <int STAGE>
+// MODE: first products (from Y) of the sunset, bubble and chain passes,
+// then the second products (from G) into the ring or the bubble triangles
+enum OrbitMode { OM_SUNSET, OM_BUBBLE, OM_CHAIN, OM_TO_RING, OM_TO_BUBBLES };
+
+template <int MODE>
 __global__ __launch_bounds__(OTHREADS) void orbit_gemm(OrbitArgs a) {
   extern __shared__ double olds[];
   const StepState ss = *a.st;
@@ -99,7 +116,8 @@ __global__ __launch_bounds__(OTHREADS) void orbit_gemm(OrbitArgs a) {
   const int row0 = blockIdx.x * OT;  // orbits (output rows)
   const int col0 = blockIdx.y * OT;  // columns 2 q + comp
   if (col0 >= 2 * L) return;         // whole tile past the live rows
-  const double* Bsrc = STAGE == 0 ? a.Y : a.G;
+  constexpr bool FIRST = MODE == OM_SUNSET || MODE == OM_BUBBLE || MODE == OM_CHAIN;
+  const double* Bsrc = FIRST ? a.Y : a.G;
   // 8 waves, 2 per SIMD (one multiplies while the other waits for LDS):
   // wave w owns rows wr .. wr + 31 and columns wc .. wc + 15 of the tile
   const int wr = (wave >> 2) * 32, wc = (wave & 3) * 16;
@@ -181,15 +199,45 @@ __global__ __launch_bounds__(OTHREADS) void orbit_gemm(OrbitArgs a) {
         const int q = col >> 1, comp = col & 1;
         const double mine = acc[i][v];
         const double other = __shfl_xor(mine, 1);
-        if (STAGE == 0) {
+        const bool live = o < a.V && q < L;
+        const size_t gi = (size_t)o * a.ldg + col;
+        if constexpr (MODE == OM_SUNSET) {
           const double Fx = comp ? other : mine, rx = comp ? mine : other;
           const double FF = Fx * Fx, rr = rx * rx;
           const double g = comp ? rx * (FF - (1.0 / 12.0) * rr) : Fx * (FF - 0.75 * rr);
-          if (o < a.V && q < L) a.G[(size_t)o * a.ldg + col] = g;
-        } else {
-          if (o < a.V && q < L) {
+          if (live) a.G[gi] = g;
+        } else if constexpr (MODE == OM_BUBBLE) {
+          // F2kr2 and Fr integrands: Fx^2 - rx^2 / 4, Fx rx
+          const double Fx = comp ? other : mine, rx = comp ? mine : other;
+          const double g = comp ? Fx * rx : Fx * Fx - 0.25 * (rx * rx);
+          if (live) {
+            a.G[gi] = g;
+            if (a.X) a.X[gi] = mine;
+          }
+        } else if constexpr (MODE == OM_CHAIN) {
+          // (Fx IFx - rx Irx / 4, Fx Irx + rx IFx) with (IFx, Irx) = C (IF, Ir) / V
+          const double IFx = a.ivol * (comp ? other : mine), Irx = a.ivol * (comp ? mine : other);
+          double Fx = 0.0, rx = 0.0;
+          if (live) {
+            const size_t xi = (size_t)o * a.ldg + 2 * q;
+            Fx = a.X[xi];
+            rx = a.X[xi + 1];
+          }
+          const double g = comp ? Fx * Irx + rx * IFx : Fx * IFx - 0.25 * (rx * Irx);
+          if (live) a.G[gi] = g;
+        } else if constexpr (MODE == OM_TO_RING) {
+          if (live) {
             const int slot = (ss.tmin + q - 1) % a.N;
             a.S[soff(o / MB, slot, o % MB, a.N) + comp] = (comp ? a.preR : a.preF) * mine;
+          }
+        } else {  // OM_TO_BUBBLES: row t-1 of the bubble triangles, Fr sign-folded like rho
+          if (live) {
+            const int slot = (ss.tmin + q - 1) % a.N;
+            const size_t pi = poff(o / MB, ppair_off(ss.s_tm1, slot, a.N), o % MB, a.PNP);
+            if (comp)
+              a.PR[pi] = (ss.s_tm1 >= slot ? a.preR : -a.preR) * mine;
+            else
+              a.PF[pi] = a.preF * mine;
           }
         }
       }
@@ -198,22 +246,38 @@ __global__ __launch_bounds__(OTHREADS) void orbit_gemm(OrbitArgs a) {
 size_t orbit_lds() { return sizeof(double) * 2 * (size_t)BUFSZ; }
 
 hipError_t orbit_prepare() {
-  hipError_t e = hipFuncSetAttribute((const void*)orbit_gemm<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)orbit_lds());
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute((const void*)orbit_gemm<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)orbit_lds());
+  const void* fns[] = {(const void*)orbit_gemm<OM_SUNSET>, (const void*)orbit_gemm<OM_BUBBLE>,
+                       (const void*)orbit_gemm<OM_CHAIN>, (const void*)orbit_gemm<OM_TO_RING>,
+                       (const void*)orbit_gemm<OM_TO_BUBBLES>};
+  hipError_t e = hipSuccess;
+  for (const void* f : fns)
+    if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)orbit_lds());
   return e;
 }
 
-hipError_t launch_orbit_sigma(const OrbitArgs& a, hipStream_t st) {
+// gather of the stage's inputs, the first product with the pointwise step,
+// the second product into its destination
+hipError_t launch_orbit(const OrbitArgs& a, OrbitStage stage, hipStream_t st) {
   const int rows = a.N - 1;  // live t' (rows q >= L exit at once)
-  hipLaunchKernelGGL(orbit_gather, dim3((a.V + GB - 1) / GB, (rows + GQ - 1) / GQ), dim3(OTHREADS), 0, st, a);
+  const dim3 ggrid((a.V + GB - 1) / GB, (rows + GQ - 1) / GQ);
+  if (stage == ORBIT_CHAIN)
+    hipLaunchKernelGGL(orbit_gather<true>, ggrid, dim3(OTHREADS), 0, st, a);
+  else
+    hipLaunchKernelGGL(orbit_gather<false>, ggrid, dim3(OTHREADS), 0, st, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const dim3 grid(a.Mp / OT, (2 * rows + OT - 1) / OT);
-  hipLaunchKernelGGL(orbit_gemm<0>, grid, dim3(OTHREADS), orbit_lds(), st, a);
+  const size_t lds = orbit_lds();
+  switch (stage) {
+    case ORBIT_SUNSET: hipLaunchKernelGGL(orbit_gemm<OM_SUNSET>, grid, dim3(OTHREADS), lds, st, a); break;
+    case ORBIT_BUBBLE: hipLaunchKernelGGL(orbit_gemm<OM_BUBBLE>, grid, dim3(OTHREADS), lds, st, a); break;
+    default: hipLaunchKernelGGL(orbit_gemm<OM_CHAIN>, grid, dim3(OTHREADS), lds, st, a); break;
+  }
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(orbit_gemm<1>, grid, dim3(OTHREADS), orbit_lds(), st, a);
+  if (stage == ORBIT_BUBBLE)
+    hipLaunchKernelGGL(orbit_gemm<OM_TO_BUBBLES>, grid, dim3(OTHREADS), lds, st, a);
+  else
+    hipLaunchKernelGGL(orbit_gemm<OM_TO_RING>, grid, dim3(OTHREADS), lds, st, a);
   return hipGetLastError();
 }
 

@@ -209,6 +209,50 @@ def test_ninverse_orbit_storage_equals_full_storage():
         assert np.abs(sym.sigma(w, tp) - b).max() <= 1e-12 * scale, (w, tp)
 
 
+@pytest.mark.parametrize("Nx,Nmem,fill", [(8, 9, False), (12, 10, True), (7, 6, False)])
+def test_ninverse_orbit_product_equals_plane_kernels(Nx, Nmem, fill):
+    """Orbit storage, NinverseNLO: the bubble and chain passes as products
+    with the orbit cosine matrix (the default) and as plane FFTs over the
+    expanded lattice give one trajectory to 1e-12; with a filled window the
+    bubble rows are first rebuilt one row at a time through the same path.
+    The product context is graph-replayed."""
+    from qftkb import KBContext
+    from qftkb._abi import FFT_ORBIT_PRODUCT
+    print(f"\n[case] 1/N orbit product vs plane kernels Nx={Nx} Nmem={Nmem} fill={fill}", flush=True)
+    args = (Nx, 3, Nmem, 0.05, 10.0, 8, 1.0, 1.0)
+    a = KBContext(*args, pexp=NINV_NLO, symmetric=True)
+    b = KBContext(*args, pexp=NINV_NLO, symmetric=True, tiles=None)
+    assert a.orbit_product and a.fft_tiles() == FFT_ORBIT_PRODUCT
+    b.set_fft_tiles(0)
+    a.use_graph(True)
+    n = tophat(a.kL2(), 3)
+    if fill:
+        a.fill_free_window(Nmem, n, 1.0)
+        b.fill_free_window(Nmem, n, 1.0)
+        t0 = Nmem + 1
+    else:
+        a.init_particles(n, 1.0)
+        b.init_particles(n, 1.0)
+        t0 = 3
+    for t in range(t0, t0 + 2 * Nmem):
+        a.step(t)
+        b.step(t)
+    tmin, tmax = b.window()
+    assert a.window() == (tmin, tmax)
+    for tp in range(tmin, tmax + 1):
+        for w in "Fr":
+            assert rel(a.get(w, tmax, tp), b.get(w, tmax, tp)) <= 1e-12, (w, tp)
+    ref = {(w, tp): b.sigma(w, tp) for tp in range(tmin, tmax) for w in "Fr"}
+    scale = max(np.abs(v).max() for v in ref.values())
+    for (w, tp), v in ref.items():
+        assert np.abs(a.sigma(w, tp) - v).max() <= 1e-12 * scale, (w, tp)
+    for which in ("F2kr2", "Fr", "Ir"):
+        x, y = a.ninv(which, tmax - 1, tmin + 1), b.ninv(which, tmax - 1, tmin + 1)
+        assert rel(x, y) <= 1e-12, which
+    a.close()
+    b.close()
+
+
 def test_ninverse_unsupported_combinations_are_rejected():
     from qftkb import KBContext
     from qftkb._abi import KBError

@@ -153,15 +153,25 @@ def test_orbit_product_equals_plane_kernels(args, filled):
     b.close()
 
 
-def test_orbit_product_only_for_orbit_storage_lambda_nlo():
+def test_orbit_product_only_for_orbit_storage_transform_stages():
+    """The orbit product exists in orbit-storage contexts with a transform
+    stage (LambdaNLO sunset, NinverseNLO bubble and chain), not in full
+    storage or LambdaLO."""
     from qftkb import KBContext, KBError
     from qftkb._abi import FFT_ORBIT_PRODUCT
     args = (8, 3, 10, 0.1, 1.0, 1, 1.0, 1.0)
-    for kw in ({}, {"symmetric": True, "pexp": 0}, {"symmetric": True, "pexp": 3}):
+    for kw in ({}, {"pexp": 3}, {"symmetric": True, "pexp": 0}):
         c = KBContext(*args, **kw)
         assert not c.orbit_product
         with pytest.raises(KBError):
             c.set_fft_tiles(FFT_ORBIT_PRODUCT)
+        c.close()
+    for pexp in (1, 3):
+        c = KBContext(*args, symmetric=True, pexp=pexp)
+        assert c.orbit_product
+        c.set_fft_tiles(0)
+        c.set_fft_tiles(FFT_ORBIT_PRODUCT)
+        assert c.fft_tiles() == FFT_ORBIT_PRODUCT
         c.close()
 
 
