@@ -206,9 +206,10 @@ int kb_get_tiles(const kb_ctx *ctx, int *variant);
 int kb_autotune(kb_ctx *ctx, int reps, int *best, double *ms);
 /* The same for the self-energy stage's plane kernels (threads per
  * workgroup); re-running the stage recomputes the same self-energies.
- * Orbit-storage LambdaNLO contexts (kb_create_sym, up to 4096 orbits) have
- * one more form, KB_FFT_ORBIT_PRODUCT, their default: both transforms of the
- * sunset as FP64 matrix products with the orbit cosine matrix.  There
+ * Orbit-storage LambdaNLO and NinverseNLO contexts (kb_create_sym, up to
+ * 4096 orbits) have one more form, KB_FFT_ORBIT_PRODUCT, their default: the
+ * transforms of the sunset (or of the 1/N bubble and chain passes) as FP64
+ * matrix products with the orbit cosine matrix.  In LambdaNLO contexts
  * kb_autotune_fft times it as well, writes its average after the plane
  * shapes' (ms needs kb_fft_shape's count + 1 entries) and *best is
  * KB_FFT_ORBIT_PRODUCT when it is the fastest. */
