@@ -573,11 +573,14 @@ def main():
         pmc_n = load_pmc(os.path.join(ROOT, "profiles", "ninv_pmc_32c_n400.json"))
         if pmc_n and pmc_n.get("Nx") == w["Nx"] and pmc_n.get("N") == N and not args.sym:
             traffic = pmc_n.get("traffic_bytes")
+    size = f"{w['Nx']}^3 N_mem={N}"
+    headline = (w["Nx"], N) == (WORKLOAD["Nx"], WORKLOAD["NstepsinMemory"])
     line = {
         "metric": ("KB steps/s at 32^3 N_mem=400 FP64 (% HBM peak); adjoint/forward cost ratio"
-                   if not (args.sym or args.ninverse) else
-                   "KB steps/s at 32^3 N_mem=400 FP64, 1/N expansion at NLO (separate metric)" if args.ninverse else
-                   "KB steps/s at 32^3 N_mem=400 FP64 with cubic-orbit storage (separate metric)"),
+                   if headline and not (args.sym or args.ninverse) else
+                   f"KB steps/s at {size} FP64, 1/N expansion at NLO (separate metric)" if args.ninverse else
+                   f"KB steps/s at {size} FP64 with cubic-orbit storage (separate metric)" if args.sym else
+                   f"KB steps/s at {size} FP64 (separate workload)"),
         "value": steps_per_s,
         "unit": "KB steps/s",
         "n_gpus": 1,
