@@ -203,3 +203,26 @@ def test_orbit_storage_graph_replay_equals_eager():
             assert abs(ma["E"] - mb["E"]) <= 1e-14 * abs(ma["E"]), t
     a.close()
     b.close()
+
+
+def test_orbit_storage_above_product_limit_uses_plane_kernels():
+    """Past 4096 orbits (56^3 has 4495) orbit storage keeps the plane-FFT
+    self-energy; it still equals full storage."""
+    from qftkb import KBContext
+    args = (56, 3, 4, 0.1, 1.0, 1, 1.0, 1.0)
+    print("\n[case] 56^3 orbit storage without the orbit product", flush=True)
+    full = KBContext(*args)
+    red = KBContext(*args, symmetric=True)
+    assert red.storage_info()[0] == 4495 and not red.orbit_product
+    n = thermal(full.kL2())
+    full.init_particles(n, 1.0)
+    red.init_particles(n, 1.0)
+    for t in range(3, 9):
+        full.step(t)
+        red.step(t)
+    tmin, tmax = full.window()
+    for tp in range(tmin, tmax + 1):
+        for w in "Fr":
+            assert rel(red.get(w, tmax, tp), full.get(w, tmax, tp)) <= 1e-12, (w, tp)
+    full.close()
+    red.close()
