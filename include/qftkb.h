@@ -176,6 +176,23 @@ int kb_check_finite(kb_ctx *ctx);
  * w = sqrt(kL2 + m^2), and sets the window indices.  n[V] may be NULL (0). */
 int kb_fill_free_window(kb_ctx *ctx, int tmax, const double *n, double mass);
 
+/* Checkpoint and restart of a run, no counterpart in the reference (its
+ * CPUfull SimData lives in process memory): a snapshot of the window, the
+ * self-energy ring, kL2, the Hartree mass and the window indices, in the
+ * device layout behind a versioned header.  kb_window_bytes gives its size
+ * (about the device memory of the window: 42.4 GB at 32^3, N_mem = 400);
+ * kb_save_window writes it to a caller buffer of that size (synchronising);
+ * kb_load_window restores it into a context created with the same
+ * parameters (Nx, sdim, N_mem, N, dt, lambda, masses, expansion, storage;
+ * the device may differ), after which kb_step continues the run where the
+ * snapshot was taken.  KB_ERR_ARG for a buffer of the wrong size, another
+ * format or other parameters; KB_ERR_STATE while recording for the adjoint
+ * or for ensemble members.  NinverseNLO rebuilds its bubble and Ir rows at
+ * the next step. */
+int kb_window_bytes(const kb_ctx *ctx, unsigned long long *bytes);
+int kb_save_window(kb_ctx *ctx, void *buf, unsigned long long bytes);
+int kb_load_window(kb_ctx *ctx, const void *buf, unsigned long long bytes);
+
 /* Stream and timing. */
 int kb_synchronize(kb_ctx *ctx);
 /* Launch on a caller-provided hipStream_t (NULL = library's own stream). */

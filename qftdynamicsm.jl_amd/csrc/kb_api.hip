@@ -1098,6 +1098,111 @@ int kb_check_finite(kb_ctx* c) {
   return KB_OK;
 }
 
+// ---- checkpoint and restart -------------------------------------------------
+namespace {
+
+struct WinHeader {
+  char magic[8];  // "QFTKBWIN"
+  int version;
+  int sym;
+  kb_params p;
+  int V, N;
+  long long NP;
+  StepState st;
+  double initmass;
+  int init_particles;
+  int pad;
+  unsigned long long hcount, scount, kcount;  // doubles of H, S and kL2 that follow (then 8 of m2)
+};
+constexpr int WIN_VERSION = 1;
+constexpr int WIN_M2 = 8;
+
+void win_counts(const kb_ctx* c, unsigned long long& h, unsigned long long& s, unsigned long long& k) {
+  h = (unsigned long long)c->NB * (unsigned long long)c->NP * MB * 2;
+  s = (unsigned long long)c->NB * c->N * MB * 2;
+  k = (unsigned long long)c->Vpad;
+}
+
+unsigned long long win_bytes(const kb_ctx* c) {
+  unsigned long long h, s, k;
+  win_counts(c, h, s, k);
+  return sizeof(WinHeader) + sizeof(double) * (h + s + k + WIN_M2);
+}
+
+bool same_params(const kb_params& a, const kb_params& b) {
+  return a.Nx == b.Nx && a.sdim == b.sdim && a.NstepsinMemory == b.NstepsinMemory && a.ONgroup == b.ONgroup &&
+         a.dt == b.dt && a.Lambda == b.Lambda && a.Mass == b.Mass && a.Mass2 == b.Mass2 && a.pexp == b.pexp;
+}
+
+}  // namespace
+
+int kb_window_bytes(const kb_ctx* c, unsigned long long* bytes) {
+  if (!c || !bytes) return KB_ERR_ARG;
+  *bytes = win_bytes(c);
+  return KB_OK;
+}
+
+int kb_save_window(kb_ctx* c, void* buf, unsigned long long bytes) {
+  if (!c || !buf) return KB_ERR_ARG;
+  if (bytes != win_bytes(c)) return seterr(c, KB_ERR_ARG, "save: buffer of %llu bytes, snapshot is %llu", bytes, win_bytes(c));
+  if (!c->initialised) return seterr(c, KB_ERR_STATE, "save: context not initialised");
+  if (c->adj || c->in_ensemble) return seterr(c, KB_ERR_STATE, "save: not while recording or for ensemble members");
+  WinHeader h{};
+  std::memcpy(h.magic, "QFTKBWIN", 8);
+  h.version = WIN_VERSION;
+  h.sym = c->sym ? 1 : 0;
+  h.p = c->p;
+  h.V = c->V;
+  h.N = c->N;
+  h.NP = c->NP;
+  h.st = c->st;
+  h.initmass = c->initmass;
+  h.init_particles = c->init_particles ? 1 : 0;
+  win_counts(c, h.hcount, h.scount, h.kcount);
+  char* out = static_cast<char*>(buf);
+  std::memcpy(out, &h, sizeof(h));
+  double* d = reinterpret_cast<double*>(out + sizeof(h));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(d, c->H, sizeof(double) * h.hcount, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(d + h.hcount, c->S, sizeof(double) * h.scount, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(d + h.hcount + h.scount, c->kL2, sizeof(double) * h.kcount, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(d + h.hcount + h.scount + h.kcount, c->m2, sizeof(double) * WIN_M2, hipMemcpyDeviceToHost));
+  return KB_OK;
+}
+
+int kb_load_window(kb_ctx* c, const void* buf, unsigned long long bytes) {
+  if (!c || !buf) return KB_ERR_ARG;
+  if (c->adj || c->in_ensemble) return seterr(c, KB_ERR_STATE, "load: not while recording or for ensemble members");
+  if (bytes != win_bytes(c)) return seterr(c, KB_ERR_ARG, "load: snapshot of %llu bytes, this context takes %llu", bytes, win_bytes(c));
+  WinHeader h;
+  std::memcpy(&h, buf, sizeof(h));
+  unsigned long long hc, sc, kc;
+  win_counts(c, hc, sc, kc);
+  if (std::memcmp(h.magic, "QFTKBWIN", 8) != 0 || h.version != WIN_VERSION)
+    return seterr(c, KB_ERR_ARG, "load: not a window snapshot of this format");
+  if (!same_params(h.p, c->p) || h.sym != (c->sym ? 1 : 0) || h.V != c->V || h.N != c->N || h.NP != c->NP ||
+      h.hcount != hc || h.scount != sc || h.kcount != kc)
+    return seterr(c, KB_ERR_ARG, "load: snapshot taken with other parameters or storage");
+  if (h.st.tmax < 2 || h.st.tmax - h.st.tmin + 1 > c->N)
+    return seterr(c, KB_ERR_ARG, "load: snapshot window [%d,%d] is not valid", h.st.tmin, h.st.tmax);
+  const double* d = reinterpret_cast<const double*>(static_cast<const char*>(buf) + sizeof(h));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(c->H, d, sizeof(double) * hc, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->S, d + hc, sizeof(double) * sc, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->kL2, d + hc + sc, sizeof(double) * kc, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->m2, d + hc + sc + kc, sizeof(double) * WIN_M2, hipMemcpyHostToDevice));
+  c->st = h.st;
+  int rc = upload_state(c);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->initmass = h.initmass;
+  c->init_particles = h.init_particles != 0;
+  c->initialised = true;
+  c->ninv_valid = false;
+  drop_graph(c);
+  return KB_OK;
+}
+
 int kb_fill_free_window(kb_ctx* c, int tmax, const double* n, double mass) {
   if (!c) return KB_ERR_ARG;
   if (tmax < c->N) return seterr(c, KB_ERR_ARG, "fill: tmax=%d < NstepsinMemory=%d", tmax, c->N);

@@ -39,7 +39,7 @@ EXPORTED = (
     "kb_init_particles", "kb_init_rows", "kb_step", "kb_window", "kb_hartree", "kb_measure",
     "kb_measure_history", "kb_get_entry", "kb_set_entry", "kb_get_sigma", "kb_ninv_get", "kb_read_bandwidth",
     "kb_check_finite",
-    "kb_fill_free_window", "kb_synchronize", "kb_set_stream", "kb_enable_timing", "kb_timing",
+    "kb_fill_free_window", "kb_window_bytes", "kb_save_window", "kb_load_window", "kb_synchronize", "kb_set_stream", "kb_enable_timing", "kb_timing",
     "kb_reset_timing", "kb_use_graph", "kb_ens_create", "kb_ens_destroy", "kb_ens_last_error", "kb_ens_member",
     "kb_ens_step", "kb_ens_use_graph", "kb_ens_synchronize", "kb_ens_enable_timing", "kb_ens_timing",
     "kb_adj_record", "kb_adj_gradient", "kb_adj_info", "kb_adj_stop", "kb_adj_debug_stop", "kb_adj_get_entry",
@@ -109,6 +109,9 @@ def load_library(path: str = LIB_PATH):
     L.kb_read_bandwidth.argtypes = [ctypes.c_int, ctypes.c_longlong, _dp]
     L.kb_check_finite.argtypes = [vp]
     L.kb_fill_free_window.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_double]
+    L.kb_window_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_ulonglong)]
+    L.kb_save_window.argtypes = [vp, vp, ctypes.c_ulonglong]
+    L.kb_load_window.argtypes = [vp, vp, ctypes.c_ulonglong]
     L.kb_synchronize.argtypes = [vp]
     L.kb_set_stream.argtypes = [vp, vp]
     L.kb_enable_timing.argtypes = [vp, ctypes.c_int]
@@ -296,6 +299,20 @@ class KBContext:
     def init_rows(self, F11, F21, F22):
         a = [_f64(x, self.V) for x in (F11, F21, F22)]
         self._chk(self._L.kb_init_rows(self._h, *[_ptr(x) for x in a]))
+
+    def save_window(self):
+        """Snapshot of the run (window, self-energy ring, kL2, Hartree mass,
+        window indices) as a uint8 array, for load_window of a context with
+        the same parameters: checkpoint and restart."""
+        n = ctypes.c_ulonglong()
+        self._chk(self._L.kb_window_bytes(self._h, ctypes.byref(n)))
+        buf = np.empty(n.value, dtype=np.uint8)
+        self._chk(self._L.kb_save_window(self._h, buf.ctypes.data_as(ctypes.c_void_p), n.value))
+        return buf
+
+    def load_window(self, buf):
+        buf = np.ascontiguousarray(buf, dtype=np.uint8)
+        self._chk(self._L.kb_load_window(self._h, buf.ctypes.data_as(ctypes.c_void_p), buf.nbytes))
 
     def fill_free_window(self, tmax, n=None, mass=1.0):
         nn = None if n is None else _f64(n, self.V)
