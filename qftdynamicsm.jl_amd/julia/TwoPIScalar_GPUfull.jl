@@ -136,6 +136,23 @@ function getHartreeMass2(model::TwoPIScalarPhi4, pexp::TwoPIScalarPertExpansion,
     return simdata.hartreemass2[1]
 end
 
+# checkpoint and restart (kb_save_window / kb_load_window): a snapshot of the
+# run as bytes, restored into SimData built from the same parameters; evolve!
+# then continues at simdata.indices[2] + 1
+function saveWindow(simdata::TwoPIScalarSimDataGPUfull)
+    n = Ref{Culonglong}(0)
+    kbcheck(simdata.ctx, ccall((:kb_window_bytes, libqftkb), Cint, (Ptr{Cvoid}, Ref{Culonglong}), simdata.ctx, n))
+    buf = Vector{UInt8}(undef, n[])
+    kbcheck(simdata.ctx, ccall((:kb_save_window, libqftkb), Cint, (Ptr{Cvoid}, Ptr{UInt8}, Culonglong),
+                               simdata.ctx, buf, n[]))
+    return buf
+end
+
+function loadWindow!(simdata::TwoPIScalarSimDataGPUfull, buf::Vector{UInt8})
+    kbcheck(simdata.ctx, ccall((:kb_load_window, libqftkb), Cint, (Ptr{Cvoid}, Ptr{UInt8}, Culonglong),
+                               simdata.ctx, buf, length(buf)))
+end
+
 function evolve!(thesolution::QFTdynamicsSolutionTwoPIScalar, tmpdata::TwoPIScalarTmpDataGPUfull, t)
     simdata = thesolution.simdata
     kbcheck(simdata.ctx, ccall((:kb_step, libqftkb), Cint, (Ptr{Cvoid}, Cint), simdata.ctx, t))
