@@ -614,7 +614,10 @@ def main():
         Br = bytes_resum(N, Vs)
         ach = Br / avg_rs_s / 1e9
         line["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "kernel": "ninv_resum_kernel",
+                            "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                            # orbit storage splits the sweeps (3 partial + 3 finish kernels)
+                            "kernel": "ninv_split_sweep<1..3> + ninv_split_finish<1..3>" if args.sym
+                            else "ninv_resum_kernel",
                             "algorithmic_bytes": Br, "avg_kernel_ms": avg_rs_s * 1e3,
                             "memint_ms": avg_mi_s * 1e3, "memint_frac": achieved / HBM_PEAK_GBS,
                             "self_energy_stage_ms": ms_fft / max(n_fft, 1),

@@ -279,7 +279,22 @@ int launch_resum(kb_ctx* c, const StepState* st, int passes) {
   a.lam = c->p.Lambda;
   a.passes = passes;
   a.st = st;
-  hipLaunchKernelGGL(ninv_resum_kernel, dim3(c->NB), dim3(NINV_THREADS), ninv_lds(c->N), c->stream, a);
+  const size_t lds = ninv_lds(c->N);
+  if (c->nsplit > 1) {
+    const int ns = c->nsplit;
+    const dim3 g(c->NB * ns), gf(c->NB);
+    hipLaunchKernelGGL(ninv_split_sweep<1>, g, dim3(NINV_THREADS), lds, c->stream, a, ns, c->npart, c->ncv);
+    hipLaunchKernelGGL(ninv_split_finish<1>, gf, dim3(NINV_FINISH_THREADS), lds, c->stream, a, ns, c->npart, c->ncv);
+    LAUNCHCHK(c);
+    if (passes == NINV_PASS_IR) return KB_OK;
+    hipLaunchKernelGGL(ninv_split_sweep<2>, g, dim3(NINV_THREADS), lds, c->stream, a, ns, c->npart, c->ncv);
+    hipLaunchKernelGGL(ninv_split_finish<2>, gf, dim3(NINV_FINISH_THREADS), lds, c->stream, a, ns, c->npart, c->ncv);
+    hipLaunchKernelGGL(ninv_split_sweep<3>, g, dim3(NINV_THREADS), lds, c->stream, a, ns, c->npart, c->ncv);
+    hipLaunchKernelGGL(ninv_split_finish<3>, gf, dim3(NINV_FINISH_THREADS), lds, c->stream, a, ns, c->npart, c->ncv);
+    LAUNCHCHK(c);
+    return KB_OK;
+  }
+  hipLaunchKernelGGL(ninv_resum_kernel, dim3(c->NB), dim3(NINV_THREADS), lds, c->stream, a);
   LAUNCHCHK(c);
   return KB_OK;
 }
@@ -451,7 +466,7 @@ void free_all(kb_ctx* c) {
                     (void*)c->modes, (void*)c->vbuf, (void*)c->tw, (void*)c->flag, (void*)c->items_r[0],
                     (void*)c->items_r[1], (void*)c->items_r[2], (void*)c->smap_dev, (void*)c->rmap_dev,
                     (void*)c->wgt_dev, (void*)c->PF, (void*)c->PR, (void*)c->IR, (void*)c->Iring, (void*)c->Xn,
-                    (void*)c->stfill_dev, (void*)c->pitems, (void*)c->mpart, (void*)c->Mo, (void*)c->Yo, (void*)c->Xo, (void*)c->G,
+                    (void*)c->stfill_dev, (void*)c->pitems, (void*)c->mpart, (void*)c->npart, (void*)c->ncv, (void*)c->Mo, (void*)c->Yo, (void*)c->Xo, (void*)c->G,
                     (void*)c->st_dev, (void*)c->stptr_dev, (void*)c->N_dev, (void*)c->margs_dev, (void*)c->wg0_dev})
     if (ptr) (void)hipFree(ptr);
   adj_free(c);
@@ -749,10 +764,28 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
     for (double* ptr : {c->PF, c->PR, c->IR})
       if (hipMemsetAsync(ptr, 0, pcount * sizeof(double), c->stream) != hipSuccess) return fail(KB_ERR_HIP);
     if (hipMemsetAsync(c->Iring, 0, rcount * sizeof(double), c->stream) != hipSuccess) return fail(KB_ERR_HIP);
-    if (hipFuncSetAttribute((const void*)ninv_resum_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)ninv_lds(640)) != hipSuccess) {
-      seterr(c, KB_ERR_ARG, "resummation kernel: LDS attribute rejected");
-      return fail(KB_ERR_ARG);
+    const void* rfns[] = {(const void*)ninv_resum_kernel,      (const void*)ninv_split_sweep<1>,
+                          (const void*)ninv_split_sweep<2>,    (const void*)ninv_split_sweep<3>,
+                          (const void*)ninv_split_finish<1>,   (const void*)ninv_split_finish<2>,
+                          (const void*)ninv_split_finish<3>};
+    for (const void* f : rfns)
+      if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ninv_lds(640)) != hipSuccess) {
+        seterr(c, KB_ERR_ARG, "resummation kernels: LDS attribute rejected");
+        return fail(KB_ERR_ARG);
+      }
+    // orbit storage: split the sweeps like the memory integral
+    if (sym) {
+      int per_cu = 0;
+      hipDeviceProp_t prop;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)ninv_split_sweep<1>, NINV_THREADS,
+                                                       ninv_lds(c->N)) != hipSuccess ||
+          hipGetDeviceProperties(&prop, p->device) != hipSuccess)
+        return fail(KB_ERR_HIP);
+      c->nsplit = std::max(1, std::min(KB_MSPLIT_MAX, per_cu * prop.multiProcessorCount / c->NB));
+      if (c->nsplit > 1 &&
+          ((rc = dmalloc(c, (void**)&c->npart, (size_t)c->NB * c->nsplit * c->N * MB * sizeof(double))) ||
+           (rc = dmalloc(c, (void**)&c->ncv, (size_t)c->NB * c->N * MB * sizeof(double)))))
+        return fail(rc);
     }
   }
   // twiddles exp(-2 pi i k / Nx) and the default k^2 table

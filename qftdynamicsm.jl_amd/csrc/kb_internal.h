@@ -66,6 +66,9 @@ struct kb_ctx {
   int tile = 0;           // memory-integral tile shape (kMemintVariants index)
   int msplit = 1;         // workgroups per site block of the memory integral (split form if > 1)
   double* mpart = nullptr;  // their partial accumulators
+  int nsplit = 1;           // the same for the 1/N resummation sweeps
+  double* npart = nullptr;  // their partial accumulators
+  double* ncv = nullptr;    // coefficient vectors passed between the split sweeps
   int fft_tile = 0;       // self-energy plane kernels' block size (kFFTThreads index),
                           // or KB_FFT_ORBIT_PRODUCT
   // orbit storage, LambdaNLO: the orbit cosine matrix and the sunset products

@@ -134,6 +134,15 @@ enum NinvPasses { NINV_PASS_IR = 1, NINV_PASS_ALL = 7 };
 constexpr int NINV_THREADS = KB_NINV_THREADS;
 constexpr int NINV_ROWS = KB_NINV_ROWS;
 __global__ void ninv_resum_kernel(NinvArgs a);
+// split form for few site blocks (cubic-orbit storage): per sweep, nsplit
+// workgroups per site block write partial accumulators (N MB doubles each)
+// to part; the sweep's finish kernel sums them, runs the phase that follows
+// and passes the next coefficient vector through cvg ([site block][slot][lane])
+template <int SWEEP>
+__global__ void ninv_split_sweep(NinvArgs a, int nsplit, double* part, const double* cvg);
+template <int SWEEP>
+__global__ void ninv_split_finish(NinvArgs a, int nsplit, const double* part, double* cvg);
+constexpr int NINV_FINISH_THREADS = 1024;
 inline size_t ninv_lds(int N) { return sizeof(double) * 2 * (size_t)N * MB; }
 
 // Per-run arguments of memint_step; step-dependent values come from *st.

@@ -87,11 +87,11 @@ __device__ __forceinline__ void pair_term(double v, bool ok, bool nrm, double cr
 
 template <int MODE>
 __device__ void sweep(const double* __restrict__ Tb, const double* cv, double* acc, const int2* __restrict__ items,
-                      int nitems, int* next_item, int N, int Rmax, int smin, int L, int excl, int lane, int worker,
-                      int nworkers) {
+                      int ibeg, int iend, int* next_item, int N, int Rmax, int smin, int L, int excl, int lane,
+                      int worker, int nworkers) {
   const int tid = threadIdx.x;
-  int item = worker;
-  while (item < nitems) {
+  int item = ibeg + worker;
+  while (item < iend) {
     const int2 jk = items[item];
     const int c0 = jk.x * PCB;
     const int r0 = c0 + jk.y * NINV_ROWS;
@@ -181,6 +181,111 @@ __device__ void sweep(const double* __restrict__ Tb, const double* cv, double* a
   }
 }
 
+// One site block's view of the step: slots, coefficients and the rows of
+// its triangles and ring for this lane.
+struct NinvBlock {
+  int N, Rmax, smin, L, sT, lane, worker, nworkers;
+  double dt, a3, a6;
+  const double* PFb;
+  const double* PRb;
+  double* IRb;
+  double* Ib;
+};
+
+__device__ __forceinline__ NinvBlock ninv_block(const NinvArgs& a, const StepState& ss, int mb) {
+  NinvBlock b;
+  b.N = a.N;
+  b.Rmax = ss.Rmax;
+  b.smin = ss.s_min;
+  b.L = ss.L;
+  b.sT = ss.s_tm1;  // slot of T = t - 1
+  b.lane = threadIdx.x & (MB - 1);
+  b.worker = threadIdx.x >> 3;
+  b.nworkers = blockDim.x >> 3;
+  b.dt = a.dt;
+  b.a3 = a.lam / 3.0;
+  b.a6 = a.lam / 6.0;
+  b.PFb = a.PF + poff(mb, 0, b.lane, a.PNP);
+  b.PRb = a.PR + poff(mb, 0, b.lane, a.PNP);
+  b.IRb = a.IR + poff(mb, 0, b.lane, a.PNP);
+  b.Ib = a.Iring + soff(mb, 0, b.lane, a.N);
+  return b;
+}
+
+// before sweep 1: cv = Fr(T, x) (physical sign) for live x != T, acc = 0;
+// row T of IR is not written yet and takes no part
+__device__ __forceinline__ void phase_fr(const NinvBlock& b, double* cv, double* acc) {
+  for (int x = b.worker; x < b.N; x += b.nworkers) {
+    const int rk = rank_of(x, b.smin, b.N);
+    double v = 0.0;
+    if (x < b.Rmax && rk < b.L && x != b.sT) {
+      const double pr = b.PRb[prel(ppair_off(b.sT, x, b.N))];
+      v = b.sT >= x ? pr : -pr;
+    }
+    cv[x * MB + b.lane] = v;
+    acc[x * MB + b.lane] = 0.0;
+  }
+}
+
+// after sweep 1: Ir(T, t') from the column sums -- row T of IR, the ring's
+// Ir -- and cv = w Ir for sweep 2; acc = 0
+__device__ __forceinline__ void phase_ir(const NinvBlock& b, double* cv, double* acc) {
+  for (int x = b.worker; x < b.N; x += b.nworkers) {
+    const int rk = rank_of(x, b.smin, b.N);
+    double w_ir = 0.0;
+    if (x < b.Rmax && rk < b.L) {
+      double ir = 0.0;
+      if (x != b.sT) {
+        const long p = ppair_off(b.sT, x, b.N);
+        const double pr = b.PRb[prel(p)];
+        ir = b.a3 * ((b.sT >= x ? pr : -pr) - b.dt * acc[x * MB + b.lane]);
+        b.IRb[prel(p)] = ir;
+      }
+      b.Ib[(size_t)x * MB * 2 + 1] = ir;
+      w_ir = (rk == 0 ? 0.5 : 1.0) * ir;
+    }
+    cv[x * MB + b.lane] = w_ir;
+    acc[x * MB + b.lane] = 0.0;
+  }
+}
+
+// after sweep 2: d = lambda/6 [F2kr2(T, t') - dt sum_z w_z Ir(T,z) F2kr2(z,t')]
+// into cv; acc = 0
+__device__ __forceinline__ void phase_d(const NinvBlock& b, double* cv, double* acc) {
+  for (int x = b.worker; x < b.N; x += b.nworkers) {
+    const int rk = rank_of(x, b.smin, b.N);
+    double d = 0.0;
+    if (x < b.Rmax && rk < b.L) d = b.a6 * (b.PFb[prel(ppair_off(b.sT, x, b.N))] - b.dt * acc[x * MB + b.lane]);
+    cv[x * MB + b.lane] = d;
+    acc[x * MB + b.lane] = 0.0;
+  }
+}
+
+// d'(t') = d(t') + c/2 d(tmin) Fr(tmin, t') for t' > tmin; 0 at tmin
+__device__ __forceinline__ void phase_dprime(const NinvBlock& b, double* cv, double dmin) {
+  const double half_c = 0.5 * b.a3 * b.dt;
+  for (int x = b.worker; x < b.N; x += b.nworkers) {
+    const int rk = rank_of(x, b.smin, b.N);
+    double dp = 0.0;
+    if (x < b.Rmax && rk < b.L && rk > 0) {
+      const double pr = b.PRb[prel(ppair_off(b.smin, x, b.N))];
+      dp = cv[x * MB + b.lane] + half_c * dmin * (b.smin >= x ? pr : -pr);
+    }
+    cv[x * MB + b.lane] = dp;
+  }
+}
+
+// after sweep 3: IF(T, t') = d'(t') - dt sum_{tmin<z<t'} d'(z) Ir(t', z)
+// into the ring; IF(T, tmin) = d(tmin) (written by the caller if skip_min)
+__device__ __forceinline__ void phase_if(const NinvBlock& b, const double* cv, const double* acc, double dmin,
+                                         bool skip_min) {
+  for (int x = b.worker; x < b.N; x += b.nworkers) {
+    const int rk = rank_of(x, b.smin, b.N);
+    if (x < b.Rmax && rk < b.L && !(skip_min && rk == 0))
+      b.Ib[(size_t)x * MB * 2] = rk == 0 ? dmin : cv[x * MB + b.lane] - b.dt * acc[x * MB + b.lane];
+  }
+}
+
 }  // namespace
 
 __global__ __launch_bounds__(NINV_THREADS) void ninv_resum_kernel(NinvArgs a) {
@@ -190,92 +295,126 @@ __global__ __launch_bounds__(NINV_THREADS) void ninv_resum_kernel(NinvArgs a) {
   double* acc = smem;          // accumulators per slot, lane
   double* cv = acc + N * MB;   // coefficient vector of the sweep
   __shared__ int next_item;
-  const int mb = blockIdx.x;
+  const NinvBlock b = ninv_block(a, ss, blockIdx.x);
   const int tid = threadIdx.x;
-  const int lane = tid & (MB - 1);
-  const int worker = tid >> 3;
-  const int nworkers = blockDim.x >> 3;
-  const int Rmax = ss.Rmax;
-  const int smin = ss.s_min;
-  const int L = ss.L;
-  const int sT = ss.s_tm1;  // slot of T = t - 1
-  const double dt = a.dt;
-  const double a3 = a.lam / 3.0, a6 = a.lam / 6.0;
-  const double* __restrict__ PFb = a.PF + poff(mb, 0, lane, a.PNP);
-  const double* __restrict__ PRb = a.PR + poff(mb, 0, lane, a.PNP);
-  double* __restrict__ IRb = a.IR + poff(mb, 0, lane, a.PNP);
-  double* __restrict__ Ib = a.Iring + soff(mb, 0, lane, N);
 
-  // sweep 1: column sums of IR with Fr(T, i); row T of IR is not written
-  // yet and takes no part
-  for (int x = worker; x < N; x += nworkers) {
-    const int rk = rank_of(x, smin, N);
-    double v = 0.0;
-    if (x < Rmax && rk < L && x != sT) {
-      const double pr = PRb[prel(ppair_off(sT, x, N))];
-      v = sT >= x ? pr : -pr;
-    }
-    cv[x * MB + lane] = v;
-    acc[x * MB + lane] = 0.0;
-  }
-  if (tid == 0) next_item = nworkers;
+  // sweep 1: column sums of IR with Fr(T, i)
+  phase_fr(b, cv, acc);
+  if (tid == 0) next_item = b.nworkers;
   __syncthreads();
-  sweep<SW_COL>(IRb, cv, acc, a.items, a.nitems, &next_item, N, Rmax, smin, L, sT, lane, worker, nworkers);
+  sweep<SW_COL>(b.IRb, cv, acc, a.items, 0, a.nitems, &next_item, N, b.Rmax, b.smin, b.L, b.sT, b.lane, b.worker,
+                b.nworkers);
   __syncthreads();
-  // Ir(T, t'): row T of IR, the ring's Ir, and w Ir for sweep 2
-  for (int x = worker; x < N; x += nworkers) {
-    const int rk = rank_of(x, smin, N);
-    double w_ir = 0.0;
-    if (x < Rmax && rk < L) {
-      double ir = 0.0;
-      if (x != sT) {
-        const long p = ppair_off(sT, x, N);
-        const double pr = PRb[prel(p)];
-        ir = a3 * ((sT >= x ? pr : -pr) - dt * acc[x * MB + lane]);
-        IRb[prel(p)] = ir;
-      }
-      Ib[(size_t)x * MB * 2 + 1] = ir;
-      w_ir = (rk == 0 ? 0.5 : 1.0) * ir;
-    }
-    cv[x * MB + lane] = w_ir;
-    acc[x * MB + lane] = 0.0;
-  }
+  phase_ir(b, cv, acc);
   if (a.passes == NINV_PASS_IR) return;
-  if (tid == 0) next_item = nworkers;
+  if (tid == 0) next_item = b.nworkers;
   __syncthreads();
-  // sweep 2: d = lambda/6 [F2kr2(T, t') - dt sum_z w_z Ir(T,z) F2kr2(z,t')]
-  sweep<SW_SYM>(PFb, cv, acc, a.items, a.nitems, &next_item, N, Rmax, smin, L, -1, lane, worker, nworkers);
+  // sweep 2: symmetric product with F2kr2
+  sweep<SW_SYM>(b.PFb, cv, acc, a.items, 0, a.nitems, &next_item, N, b.Rmax, b.smin, b.L, -1, b.lane, b.worker,
+                b.nworkers);
   __syncthreads();
-  for (int x = worker; x < N; x += nworkers) {
-    const int rk = rank_of(x, smin, N);
-    double d = 0.0;
-    if (x < Rmax && rk < L) d = a6 * (PFb[prel(ppair_off(sT, x, N))] - dt * acc[x * MB + lane]);
-    cv[x * MB + lane] = d;
-    acc[x * MB + lane] = 0.0;
-  }
+  phase_d(b, cv, acc);
   __syncthreads();
-  const double dmin = cv[smin * MB + lane];  // d(tmin) = IF(T, tmin)
-  if (tid == 0) next_item = nworkers;
+  const double dmin = cv[b.smin * MB + b.lane];  // d(tmin) = IF(T, tmin)
+  if (tid == 0) next_item = b.nworkers;
   __syncthreads();
-  // d'(t') = d(t') + c/2 d(tmin) Fr(tmin, t') for t' > tmin; 0 at tmin
-  const double half_c = 0.5 * a3 * dt;
-  for (int x = worker; x < N; x += nworkers) {
-    const int rk = rank_of(x, smin, N);
-    double dp = 0.0;
-    if (x < Rmax && rk < L && rk > 0) {
-      const double pr = PRb[prel(ppair_off(smin, x, N))];
-      dp = cv[x * MB + lane] + half_c * dmin * (smin >= x ? pr : -pr);
-    }
-    cv[x * MB + lane] = dp;
-  }
+  phase_dprime(b, cv, dmin);
   __syncthreads();
-  // sweep 3: IF(T, t') = d'(t') - dt sum_{tmin<z<t'} d'(z) Ir(t', z)
-  sweep<SW_ROW>(IRb, cv, acc, a.items, a.nitems, &next_item, N, Rmax, smin, L, -1, lane, worker, nworkers);
+  // sweep 3: row sums of IR with d'
+  sweep<SW_ROW>(b.IRb, cv, acc, a.items, 0, a.nitems, &next_item, N, b.Rmax, b.smin, b.L, -1, b.lane, b.worker,
+                b.nworkers);
   __syncthreads();
-  for (int x = worker; x < N; x += nworkers) {
-    const int rk = rank_of(x, smin, N);
-    if (x < Rmax && rk < L) Ib[(size_t)x * MB * 2] = rk == 0 ? dmin : cv[x * MB + lane] - dt * acc[x * MB + lane];
-  }
+  phase_if(b, cv, acc, dmin, false);
 }
+
+// Split form for windows with few site blocks (cubic-orbit storage): each
+// sweep's work items of a site block are cut into nsplit ranges, one
+// workgroup each, which write their accumulators to part; the finish kernel
+// of the sweep sums them and runs the phase that follows, passing the next
+// coefficient vector through cvg [site block][slot][lane].
+template <int SWEEP>
+__global__ __launch_bounds__(NINV_THREADS) void ninv_split_sweep(NinvArgs a, int nsplit, double* __restrict__ part,
+                                                                 const double* __restrict__ cvg) {
+  extern __shared__ double smem[];
+  const StepState ss = *a.st;
+  const int N = a.N;
+  double* acc = smem;
+  double* cv = acc + N * MB;
+  __shared__ int next_item;
+  const int mb = blockIdx.x / nsplit;
+  const int sp = blockIdx.x - mb * nsplit;
+  const int ibeg = (int)((long)a.nitems * sp / nsplit);
+  const int iend = (int)((long)a.nitems * (sp + 1) / nsplit);
+  const NinvBlock b = ninv_block(a, ss, mb);
+  const int tid = threadIdx.x;
+  if (SWEEP == 1) {
+    phase_fr(b, cv, acc);
+  } else {
+    for (int x = b.worker; x < N; x += b.nworkers) {
+      cv[x * MB + b.lane] = cvg[((size_t)mb * N + x) * MB + b.lane];
+      acc[x * MB + b.lane] = 0.0;
+    }
+  }
+  if (tid == 0) next_item = ibeg + b.nworkers;
+  __syncthreads();
+  if (SWEEP == 1)
+    sweep<SW_COL>(b.IRb, cv, acc, a.items, ibeg, iend, &next_item, N, b.Rmax, b.smin, b.L, b.sT, b.lane, b.worker,
+                  b.nworkers);
+  else if (SWEEP == 2)
+    sweep<SW_SYM>(b.PFb, cv, acc, a.items, ibeg, iend, &next_item, N, b.Rmax, b.smin, b.L, -1, b.lane, b.worker,
+                  b.nworkers);
+  else
+    sweep<SW_ROW>(b.IRb, cv, acc, a.items, ibeg, iend, &next_item, N, b.Rmax, b.smin, b.L, -1, b.lane, b.worker,
+                  b.nworkers);
+  __syncthreads();
+  double* out = part + (size_t)blockIdx.x * N * MB;
+  for (int i = tid; i < N * MB; i += blockDim.x) out[i] = acc[i];
+}
+
+template <int SWEEP>
+__global__ __launch_bounds__(NINV_FINISH_THREADS) void ninv_split_finish(NinvArgs a, int nsplit,
+                                                                         const double* __restrict__ part,
+                                                                         double* __restrict__ cvg) {
+  extern __shared__ double smem[];
+  const StepState ss = *a.st;
+  const int N = a.N;
+  double* acc = smem;
+  double* cv = acc + N * MB;
+  const int mb = blockIdx.x;
+  const NinvBlock b = ninv_block(a, ss, mb);
+  const double* in = part + (size_t)mb * nsplit * N * MB;
+  for (int i = threadIdx.x; i < N * MB; i += blockDim.x) {
+    double v = 0.0;
+    for (int k = 0; k < nsplit; ++k) v += in[(size_t)k * N * MB + i];
+    acc[i] = v;
+  }
+  if (SWEEP == 3) {
+    for (int i = threadIdx.x; i < N * MB; i += blockDim.x) cv[i] = cvg[(size_t)mb * N * MB + i];
+  }
+  __syncthreads();
+  if (SWEEP == 1) {
+    phase_ir(b, cv, acc);
+  } else if (SWEEP == 2) {
+    phase_d(b, cv, acc);
+    __syncthreads();
+    const double dmin = cv[b.smin * MB + b.lane];
+    __syncthreads();
+    phase_dprime(b, cv, dmin);
+    // IF(T, tmin) = d(tmin), before d' replaces d
+    if (b.worker == 0 && b.smin < b.Rmax && b.L > 0) b.Ib[(size_t)b.smin * MB * 2] = dmin;
+  } else {
+    phase_if(b, cv, acc, 0.0, true);
+    return;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < N * MB; i += blockDim.x) cvg[(size_t)mb * N * MB + i] = cv[i];
+}
+
+template __global__ void ninv_split_sweep<1>(NinvArgs, int, double*, const double*);
+template __global__ void ninv_split_sweep<2>(NinvArgs, int, double*, const double*);
+template __global__ void ninv_split_sweep<3>(NinvArgs, int, double*, const double*);
+template __global__ void ninv_split_finish<1>(NinvArgs, int, const double*, double*);
+template __global__ void ninv_split_finish<2>(NinvArgs, int, const double*, double*);
+template __global__ void ninv_split_finish<3>(NinvArgs, int, const double*, double*);
 
 }  // namespace qftkb
