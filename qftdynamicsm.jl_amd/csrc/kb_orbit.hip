@@ -193,54 +193,54 @@ __global__ __launch_bounds__(OTHREADS) void orbit_gemm(OrbitArgs a) {
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int o = row0 + wr + 16 * i + 4 * v + (lane >> 4);
-        const int col = col0 + wc + (lane & 15);
-        const int q = col >> 1, comp = col & 1;
-        const double mine = acc[i][v];
-        const double other = __shfl_xor(mine, 1);
-        const bool live = o < a.V && q < L;
-        const size_t gi = (size_t)o * a.ldg + col;
-        if constexpr (MODE == OM_SUNSET) {
-          const double Fx = comp ? other : mine, rx = comp ? mine : other;
-          const double FF = Fx * Fx, rr = rx * rx;
-          const double g = comp ? rx * (FF - (1.0 / 12.0) * rr) : Fx * (FF - 0.75 * rr);
-          if (live) a.G[gi] = g;
-        } else if constexpr (MODE == OM_BUBBLE) {
-          // F2kr2 and Fr integrands: Fx^2 - rx^2 / 4, Fx rx
-          const double Fx = comp ? other : mine, rx = comp ? mine : other;
-          const double g = comp ? Fx * rx : Fx * Fx - 0.25 * (rx * rx);
-          if (live) {
-            a.G[gi] = g;
-            if (a.X) a.X[gi] = mine;
-          }
-        } else if constexpr (MODE == OM_CHAIN) {
-          // (Fx IFx - rx Irx / 4, Fx Irx + rx IFx) with (IFx, Irx) = C (IF, Ir) / V
-          const double IFx = a.ivol * (comp ? other : mine), Irx = a.ivol * (comp ? mine : other);
-          double Fx = 0.0, rx = 0.0;
-          if (live) {
-            const size_t xi = (size_t)o * a.ldg + 2 * q;
-            Fx = a.X[xi];
-            rx = a.X[xi + 1];
-          }
-          const double g = comp ? Fx * Irx + rx * IFx : Fx * IFx - 0.25 * (rx * Irx);
-          if (live) a.G[gi] = g;
-        } else if constexpr (MODE == OM_TO_RING) {
-          if (live) {
-            const int slot = (ss.tmin + q - 1) % a.N;
-            a.S[soff(o / MB, slot, o % MB, a.N) + comp] = (comp ? a.preR : a.preF) * mine;
-          }
-        } else {  // OM_TO_BUBBLES: row t-1 of the bubble triangles, Fr sign-folded like rho
-          if (live) {
-            const int slot = (ss.tmin + q - 1) % a.N;
-            const size_t pi = poff(o / MB, ppair_off(ss.s_tm1, slot, a.N), o % MB, a.PNP);
-            if (comp)
-              a.PR[pi] = (ss.s_tm1 >= slot ? a.preR : -a.preR) * mine;
-            else
-              a.PF[pi] = a.preF * mine;
-          }
+    for (int v = 0; v < 4; ++v) {
+      const int o = row0 + wr + 16 * i + 4 * v + (lane >> 4);
+      const int col = col0 + wc + (lane & 15);
+      const int q = col >> 1, comp = col & 1;
+      const double mine = acc[i][v];
+      const double other = __shfl_xor(mine, 1);
+      const bool live = o < a.V && q < L;
+      const size_t gi = (size_t)o * a.ldg + col;
+      if constexpr (MODE == OM_SUNSET) {
+        const double Fx = comp ? other : mine, rx = comp ? mine : other;
+        const double FF = Fx * Fx, rr = rx * rx;
+        const double g = comp ? rx * (FF - (1.0 / 12.0) * rr) : Fx * (FF - 0.75 * rr);
+        if (live) a.G[gi] = g;
+      } else if constexpr (MODE == OM_BUBBLE) {
+        // F2kr2 and Fr integrands: Fx^2 - rx^2 / 4, Fx rx
+        const double Fx = comp ? other : mine, rx = comp ? mine : other;
+        const double g = comp ? Fx * rx : Fx * Fx - 0.25 * (rx * rx);
+        if (live) {
+          a.G[gi] = g;
+          if (a.X) a.X[gi] = mine;
+        }
+      } else if constexpr (MODE == OM_CHAIN) {
+        // (Fx IFx - rx Irx / 4, Fx Irx + rx IFx) with (IFx, Irx) = C (IF, Ir) / V
+        const double IFx = a.ivol * (comp ? other : mine), Irx = a.ivol * (comp ? mine : other);
+        double Fx = 0.0, rx = 0.0;
+        if (live) {
+          const size_t xi = (size_t)o * a.ldg + 2 * q;
+          Fx = a.X[xi];
+          rx = a.X[xi + 1];
+        }
+        const double g = comp ? Fx * Irx + rx * IFx : Fx * IFx - 0.25 * (rx * Irx);
+        if (live) a.G[gi] = g;
+      } else if constexpr (MODE == OM_TO_RING) {
+        if (live) {
+          const int slot = (ss.tmin + q - 1) % a.N;
+          a.S[soff(o / MB, slot, o % MB, a.N) + comp] = (comp ? a.preR : a.preF) * mine;
+        }
+      } else {  // OM_TO_BUBBLES: row t-1 of the bubble triangles, Fr sign-folded like rho
+        if (live) {
+          const int slot = (ss.tmin + q - 1) % a.N;
+          const size_t pi = poff(o / MB, ppair_off(ss.s_tm1, slot, a.N), o % MB, a.PNP);
+          if (comp)
+            a.PR[pi] = (ss.s_tm1 >= slot ? a.preR : -a.preR) * mine;
+          else
+            a.PF[pi] = a.preF * mine;
         }
       }
+    }
 }
 
 size_t orbit_lds() { return sizeof(double) * 2 * (size_t)BUFSZ; }
