@@ -28,6 +28,7 @@ WORKLOAD = dict(Nx=32, sdim=3, NstepsinMemory=400, dt=0.1, Lambda=1.0, ONgroup=1
 ADJ_WORKLOAD = dict(Nx=24, sdim=3, NstepsinMemory=200, dt=0.1, Lambda=1.0, ONgroup=1, Mass=1.0, T=2.0,
                     steps=500, checkpoint_every=50, nbins=16, h=1e-5)
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
 
 
 def thermal_n(kL2, Mass, T):
@@ -529,6 +530,7 @@ def main():
                     symmetric=args.sym, pexp=pexp)
     Vs = ctx.storage_info()[0]  # stored sites: V, or the cubic orbits
     ft = ctx.fft_tiles()
+    ctx_orbit_product = ft < 0
     tile_shape = list(ctx.tile_shapes()[ctx.tiles()]) + ["orbit product" if ft < 0 else ctx.fft_shapes()[ft]]
     n = thermal_n(ctx.kL2(), w["Mass"], w["T"])
     ctx.fill_free_window(N, n, w["Mass"])
@@ -609,6 +611,16 @@ def main():
         "cpu_baseline": None,
         "adjoint": None,
     }
+    if args.sym and not args.ninverse and ctx_orbit_product:
+        # the self-energy stage as orbit products: 2 products of 2 V^2 (2L) useful flops each,
+        # over the stage's HIP-event time (which includes the 5 us gather)
+        stage_s = ms_fft / max(n_fft, 1) / 1e3
+        flops = 2 * 2 * Vs * Vs * 2 * (N - 1)
+        line["roofline_self_energy"] = {"bound": "mfma", "achieved": flops / stage_s / 1e12,
+                                        "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                        "frac": flops / stage_s / 1e12 / FP64_MFMA_PEAK_TFLOPS,
+                                        "flops": flops, "stage_ms": stage_s * 1e3,
+                                        "kernels": "orbit_gather + orbit_gemm<0> + orbit_gemm<3>"}
     if args.ninverse:
         avg_rs_s = ms_rs / max(n_rs, 1) / 1e3
         Br = bytes_resum(N, Vs)
