@@ -10,7 +10,7 @@ problem / solution API, running the KB step on MI355X through libqftkb.so.
         evolve(solution, tmpdata, t)
         ...
 """
-from ._abi import KBContext, KBEnsemble, KBError, load_library, LIB_PATH, EXPORTED  # noqa: F401
+from ._abi import KBContext, KBEnsemble, KBError, load_library, LIB_PATH, EXPORTED, FFT_ORBIT_PRODUCT  # noqa: F401
 from . import lattice, output  # noqa: F401
 from .problem import (  # noqa: F401
     TwoPIScalarfile, QFTdynamicsProblem, QFTdynamicsSolution, TwoPIScalarProblem, SimSetup,
