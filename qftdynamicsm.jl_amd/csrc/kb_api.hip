@@ -565,9 +565,9 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
       p->NstepsinMemory > 640 || p->ONgroup < 1 || !(p->dt > 0) || p->pexp < KB_PEXP_LAMBDA_LO ||
       p->pexp > KB_PEXP_NINVERSE_NLO)
     return KB_ERR_ARG;
-  // the 1/N stage runs on the plane kernels only
-  if (p->pexp == KB_PEXP_NINVERSE_NLO && !sigma_planes_supported(p->Nx)) return KB_ERR_ARG;
-  if (sym && !sigma_planes_supported(p->Nx)) return KB_ERR_ARG;  // orbit storage uses the plane kernels
+  // the 1/N stage runs on the plane kernels (in orbit storage also as orbit
+  // products: checked below, once the orbits are known)
+  if (p->pexp == KB_PEXP_NINVERSE_NLO && !sym && !sigma_planes_supported(p->Nx)) return KB_ERR_ARG;
   kb_ctx* c = new (std::nothrow) kb_ctx();
   if (!c) return KB_ERR_OOM;
   c->p = *p;
@@ -597,6 +597,12 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
     delete c;
     return code;
   };
+  // orbit storage gathers and scatters through the site maps in the plane
+  // kernels only (Nx <= 64); past them, a transform stage needs the orbit
+  // products (LambdaLO has none)
+  if (sym && !sigma_planes_supported(p->Nx) && (p->pexp == KB_PEXP_LAMBDA_NLO || p->pexp == KB_PEXP_NINVERSE_NLO) &&
+      c->V > KB_ORBIT_PRODUCT_MAX)
+    return fail(KB_ERR_ARG);
   if (hipSetDevice(p->device) != hipSuccess) return fail(KB_ERR_HIP);
   if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) return fail(KB_ERR_HIP);
   c->stream = c->own;
@@ -1362,6 +1368,8 @@ int kb_set_fft_tiles(kb_ctx* c, int variant) {
   if (!c || (variant != KB_FFT_ORBIT_PRODUCT && (variant < 0 || variant >= NFFT_VARIANTS)) ||
       (variant == KB_FFT_ORBIT_PRODUCT && !c->Mo))
     return KB_ERR_ARG;
+  // orbit storage past the plane kernels' sizes has only the orbit product
+  if (variant != KB_FFT_ORBIT_PRODUCT && c->sym && !sigma_planes_supported(c->p.Nx)) return KB_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->fft_tile = variant;
   drop_graph(c);

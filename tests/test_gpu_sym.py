@@ -226,3 +226,61 @@ def test_orbit_storage_above_product_limit_uses_plane_kernels():
             assert rel(red.get(w, tmax, tp), full.get(w, tmax, tp)) <= 1e-12, (w, tp)
     full.close()
     red.close()
+
+
+@pytest.mark.parametrize("args,pexp,T", [
+    pytest.param((72, 2, 6, 0.1, 1.0, 1, 1.0, 1.0), 1, 20, id="72^2-NLO"),
+    pytest.param((72, 3, 4, 0.1, 1.0, 1, 1.0, 1.0), 0, 8, id="72^3-LO"),
+])
+def test_orbit_storage_past_the_plane_kernels(args, pexp, T):
+    """Nx > 64: orbit storage runs LambdaNLO on the orbit product alone (the
+    plane kernels stop at 64; full storage uses its pencil passes there) and
+    LambdaLO without transforms; both equal full storage."""
+    from qftkb import KBContext, KBError
+    print(f"\n[case] orbit storage past the plane kernels {args} pexp={pexp}", flush=True)
+    full = KBContext(*args, pexp=pexp)
+    red = KBContext(*args, pexp=pexp, symmetric=True)
+    assert red.orbit_product == (pexp == 1)
+    if pexp == 1:
+        with pytest.raises(KBError):
+            red.set_fft_tiles(0)
+    n = thermal(full.kL2())
+    full.init_particles(n, 1.0)
+    red.init_particles(n, 1.0)
+    for t in range(3, T + 1):
+        full.step(t)
+        red.step(t)
+    tmin, tmax = full.window()
+    for tp in range(tmin, tmax + 1):
+        for w in "Fr":
+            assert rel(red.get(w, tmax, tp), full.get(w, tmax, tp)) <= 1e-12, (w, tp)
+    mf, mr = full.measure(T), red.measure(T)
+    assert abs(mr["E"] - mf["E"]) <= 1e-12 * abs(mf["E"])
+    full.close()
+    red.close()
+
+
+def test_orbit_storage_ninverse_past_the_plane_kernels_against_oracle():
+    """NinverseNLO at 72^2 exists only in orbit storage (full storage needs the
+    plane kernels, Nx <= 64): trajectory against the oracle."""
+    from qftkb import KBContext
+    args = (72, 2, 5, 0.05, 10.0, 8, 1.0, 1.0)
+    print("\n[case] 1/N orbit storage 72^2 vs oracle", flush=True)
+    o = Oracle(*args, pexp=3)
+    g = KBContext(*args, pexp=3, symmetric=True)
+    assert g.orbit_product
+    n = thermal(o.kL2())
+    o.init_particles(n, 1.0)
+    g.init_particles(n, 1.0)
+    for t in range(3, 13):
+        o.evolve(t)
+        g.step(t)
+    tmin, tmax = o.window()
+    for tp in range(tmin, tmax + 1):
+        for w in "Fr":
+            assert rel(g.get(w, tmax, tp), o.get(w, tmax, tp)) <= 1e-10, (w, tp)
+    scale = max(np.abs(o.sigma(w, tp)).max() for w in "Fr" for tp in range(tmin, tmax))
+    for tp in range(tmin, tmax):
+        for w in "Fr":
+            assert np.abs(g.sigma(w, tp) - o.sigma(w, tp)).max() <= 1e-10 * scale, (w, tp)
+    g.close()
