@@ -90,11 +90,14 @@ int kb_info(const kb_ctx *ctx, long long *V, long long *device_bytes);
  * the lattice's cubic group (axis permutations and reflections
  * n_i -> Nx - n_i), about 48x fewer stored values in 3D (969 orbits for
  * the 32768 sites of 32^3).  Exact for cubic-symmetric states (the step
- * maps them to cubic-symmetric states); the self-energy transforms still
- * run on the whole lattice.  Every entry point keeps exchanging whole
+ * maps them to cubic-symmetric states).  The self-energy transforms run as
+ * FP64 matrix products with the orbit cosine matrix (up to 4096 orbits,
+ * KB_FFT_ORBIT_PRODUCT below) or as plane FFTs over the whole lattice
+ * (Nx <= 64).  Every entry point keeps exchanging whole
  * lattices of V sites: inputs are read at each orbit's representative site
  * (the caller supplies symmetric data), outputs are expanded over the
- * orbits.  Nx <= 64; no adjoint, no ensembles.  The reference's CPUred
+ * orbits.  KB_ERR_ARG for a transform stage with neither form available
+ * (Nx > 64 and more than 4096 orbits); no adjoint, no ensembles.  The reference's CPUred
  * storage (TwoPIScalar_SimDataStruct.jl:204-260) groups sites by FFTWhelper
  * levels instead; this is the exact-symmetry version of that idea. */
 int kb_create_sym(const kb_params *p, kb_ctx **out);
