@@ -415,9 +415,17 @@ int enqueue_step(kb_ctx* c) {
   {
     StageTimer st(c, 0);
     if (c->msplit > 1) {
-      hipLaunchKernelGGL((memint_partial<64, 2, 256>), dim3(c->NB * c->msplit), dim3(256), memint_lds(c->N),
-                         c->stream, c->margs_dev, c->msplit, c->mpart);
-      LAUNCHCHK(c);
+      // the split kernel's item height must be the shape's: its item table is
+      // the one in margs_dev
+      const int rows = kMemintVariants[c->tile].rows;
+      const void* fn = rows == 32    ? (const void*)memint_partial<32, 2, 256>
+                       : rows == 128 ? (const void*)memint_partial<128, 2, 256>
+                                     : (const void*)memint_partial<64, 2, 256>;
+      const MemintArgs* margs = c->margs_dev;
+      int ns = c->msplit;
+      double* part = c->mpart;
+      void* kargs[] = {(void*)&margs, (void*)&ns, (void*)&part};
+      HIPCHK(c, hipLaunchKernel(fn, dim3(c->NB * c->msplit), dim3(256), kargs, memint_lds(c->N), c->stream));
       hipLaunchKernelGGL(memint_finish, dim3(c->NB), dim3(MEMINT_FINISH_THREADS), memint_lds(c->N), c->stream,
                          c->margs_dev, c->msplit, c->mpart);
       LAUNCHCHK(c);
@@ -735,9 +743,10 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
     hipError_t e = hipSuccess;
     for (int v = 0; v < NMEMINT_VARIANTS && e == hipSuccess; ++v)
       e = hipFuncSetAttribute(kMemintVariants[v].fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e == hipSuccess)
-      e = hipFuncSetAttribute((const void*)memint_partial<64, 2, 256>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);
+    const void* partials[] = {(const void*)memint_partial<32, 2, 256>, (const void*)memint_partial<64, 2, 256>,
+                              (const void*)memint_partial<128, 2, 256>};
+    for (const void* f : partials)
+      if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e == hipSuccess)
       e = hipFuncSetAttribute((const void*)memint_finish, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) {

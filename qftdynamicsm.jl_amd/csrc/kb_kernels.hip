@@ -672,7 +672,11 @@ __global__ __launch_bounds__(MEMINT_FINISH_THREADS) void memint_finish(const Mem
   memint_leapfrog(a, ss, mb, accF, accR, red, diagF);
 }
 
+// one instantiation per item height of the tile shapes (the item table in
+// MemintArgs follows the context's shape)
+template __global__ void memint_partial<32, 2, 256>(const MemintArgs*, int, double*);
 template __global__ void memint_partial<64, 2, 256>(const MemintArgs*, int, double*);
+template __global__ void memint_partial<128, 2, 256>(const MemintArgs*, int, double*);
 
 // the tile shapes the autotuner chooses from; entry 0 is the default
 const MemintVariant kMemintVariants[NMEMINT_VARIANTS] = {

@@ -284,3 +284,31 @@ def test_orbit_storage_ninverse_past_the_plane_kernels_against_oracle():
         for w in "Fr":
             assert np.abs(g.sigma(w, tp) - o.sigma(w, tp)).max() <= 1e-10 * scale, (w, tp)
     g.close()
+
+
+def test_orbit_storage_every_tile_height():
+    """The split memory integral follows the context's tile shape: its item
+    height (32, 64 or 128 slot rows) sets both the item table and the split
+    kernel.  Each height gives the default trajectory."""
+    from qftkb import KBContext
+    args = (16, 3, 20, 0.1, 1.0, 1, 1.0, 1.0)
+    ref = KBContext(*args, symmetric=True, tiles=None)
+    shapes = ref.tile_shapes()
+    n = thermal(ref.kL2())
+    ref.init_particles(n, 1.0)
+    for t in range(3, 31):
+        ref.step(t)
+    tmin, tmax = ref.window()
+    for rows in (32, 64, 128):
+        v = next(i for i, s in enumerate(shapes) if s[0] == rows)
+        print(f"\n[case] orbit storage, tile {shapes[v]}", flush=True)
+        g = KBContext(*args, symmetric=True, tiles=None)
+        g.set_tiles(v)
+        g.init_particles(n, 1.0)
+        for t in range(3, 31):
+            g.step(t)
+        for tp in range(tmin, tmax + 1):
+            for w in "Fr":
+                assert rel(g.get(w, tmax, tp), ref.get(w, tmax, tp)) <= 1e-12, (rows, w, tp)
+        g.close()
+    ref.close()
