@@ -395,6 +395,25 @@ int launch_hartree_step(kb_ctx* c) {
   return KB_OK;
 }
 
+// Split memory integral (orbit storage): partial sweeps over item ranges,
+// then the sums and the leapfrog.  The split kernel's item height must be
+// the shape's: its item table is the one in margs_dev.
+int launch_memint_split(kb_ctx* c) {
+  const int rows = kMemintVariants[c->tile].rows;
+  const void* fn = rows == 32    ? (const void*)memint_partial<32, 2, 256>
+                   : rows == 128 ? (const void*)memint_partial<128, 2, 256>
+                                 : (const void*)memint_partial<64, 2, 256>;
+  const MemintArgs* margs = c->margs_dev;
+  int ns = c->msplit;
+  double* part = c->mpart;
+  void* kargs[] = {(void*)&margs, (void*)&ns, (void*)&part};
+  HIPCHK(c, hipLaunchKernel(fn, dim3(c->NB * c->msplit), dim3(256), kargs, memint_lds(c->N), c->stream));
+  hipLaunchKernelGGL(memint_finish, dim3(c->NB), dim3(MEMINT_FINISH_THREADS), memint_lds(c->N), c->stream,
+                     c->margs_dev, c->msplit, c->mpart);
+  LAUNCHCHK(c);
+  return KB_OK;
+}
+
 // The kernels of one KB step (evolve!, Evolution.jl:924-981); every
 // argument is step-invariant, the window comes from st_dev.
 int enqueue_step(kb_ctx* c) {
@@ -415,20 +434,7 @@ int enqueue_step(kb_ctx* c) {
   {
     StageTimer st(c, 0);
     if (c->msplit > 1) {
-      // the split kernel's item height must be the shape's: its item table is
-      // the one in margs_dev
-      const int rows = kMemintVariants[c->tile].rows;
-      const void* fn = rows == 32    ? (const void*)memint_partial<32, 2, 256>
-                       : rows == 128 ? (const void*)memint_partial<128, 2, 256>
-                                     : (const void*)memint_partial<64, 2, 256>;
-      const MemintArgs* margs = c->margs_dev;
-      int ns = c->msplit;
-      double* part = c->mpart;
-      void* kargs[] = {(void*)&margs, (void*)&ns, (void*)&part};
-      HIPCHK(c, hipLaunchKernel(fn, dim3(c->NB * c->msplit), dim3(256), kargs, memint_lds(c->N), c->stream));
-      hipLaunchKernelGGL(memint_finish, dim3(c->NB), dim3(MEMINT_FINISH_THREADS), memint_lds(c->N), c->stream,
-                         c->margs_dev, c->msplit, c->mpart);
-      LAUNCHCHK(c);
+      if ((rc = launch_memint_split(c))) return rc;
     } else {
       hipError_t e = launch_memint(c->tile, c->NB, memint_lds(c->N), c->stream, c->margs_dev, c->wg0_dev, 1);
       if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "memory-integral launch failed: %s", hipGetErrorString(e));
@@ -1341,10 +1347,13 @@ int kb_autotune(kb_ctx* c, int reps, int* best, double* ms) {
   int rc = KB_OK;
   for (int v = 0; v < NMEMINT_VARIANTS && rc == KB_OK; ++v) {
     if ((rc = kb_set_tiles(c, v))) break;
-    // re-evaluates row t of the last step from the same inputs
+    // re-evaluates row t of the last step from the same inputs, in the form
+    // the steps use (split in orbit storage)
     for (int k = 0; k <= reps && rc == KB_OK; ++k) {
       if (k == 1 && hipEventRecord(a, c->stream) != hipSuccess) rc = seterr(c, KB_ERR_HIP, "autotune: event");
-      if (launch_memint(v, c->NB, memint_lds(c->N), c->stream, c->margs_dev, c->wg0_dev, 1) != hipSuccess)
+      if (c->msplit > 1)
+        rc = launch_memint_split(c);
+      else if (launch_memint(v, c->NB, memint_lds(c->N), c->stream, c->margs_dev, c->wg0_dev, 1) != hipSuccess)
         rc = seterr(c, KB_ERR_HIP, "autotune: launch failed");
     }
     if (rc) break;

@@ -311,4 +311,16 @@ def test_orbit_storage_every_tile_height():
             for w in "Fr":
                 assert rel(g.get(w, tmax, tp), ref.get(w, tmax, tp)) <= 1e-12, (rows, w, tp)
         g.close()
+    # the autotuner re-times the split form mid-run and the run continues
+    g = KBContext(*args, symmetric=True, tiles=None)
+    g.init_particles(n, 1.0)
+    for t in range(3, 31):
+        g.step(t)
+        if t == 12:
+            r = g.autotune(reps=2)
+            assert all(x > 0 for x in r["ms"]) and g.tiles() == r["variant"]
+    for tp in range(tmin, tmax + 1):
+        for w in "Fr":
+            assert rel(g.get(w, tmax, tp), ref.get(w, tmax, tp)) <= 1e-12, ("autotune", w, tp)
+    g.close()
     ref.close()
