@@ -8,7 +8,8 @@
 # TwoPIScalar_Modelfile.jl (the include, the "Num" => "GPUfull" switch at
 # :104-108, and the getTwoPISimData branch).  Methods dispatch on the new
 # types exactly like the CPUfull ones they stand beside:
-#   getTwoPISimData          (TwoPIScalar_SimDataStruct.jl:669-708)
+#   getTwoPISimData          (TwoPIScalar_SimDataStruct.jl:669-708; also
+#                             "Num" => "GPUorbits" for cubic-orbit storage)
 #   getTwoPIScalarTmpData    (:710-721)
 #   initialize!              (TwoPIScalar_Initialization.jl:289-325)
 #   getHartreeMass2          (:207-218)
@@ -17,9 +18,15 @@
 ################################################################################
 const libqftkb = get(ENV, "QFTKB_LIB", joinpath(@__DIR__, "..", "qftkb", "libqftkb.so"))
 
-export TwoPIScalarGPUfull
+export TwoPIScalarGPUfull, TwoPIScalarGPUorbits
 struct TwoPIScalarGPUfull <: TwoPIScalarNumerics
     nchunks::Int      # kept for parameter compatibility; unused on the GPU
+end
+# one stored value per orbit of the lattice's cubic group (kb_create_sym):
+# exact for the cubic-symmetric states every initialisation produces, the
+# counterpart of CPUred ("Num" => "GPUorbits")
+struct TwoPIScalarGPUorbits <: TwoPIScalarNumerics
+    nchunks::Int
 end
 
 # mirrors kb_params in include/qftkb.h (field order and C types)
@@ -71,12 +78,17 @@ pexpcode(::TwoPIScalarNinverseNLO) = Cint(3)
 pexpcode(p) = throw(ArgumentError("GPUfull implements LambdaLO, LambdaNLO (quantum), NinverseLO and NinverseNLO, not $(typeof(p))"))
 
 # replaces the CPUfull branch of getTwoPISimData
-function getTwoPISimData(problem::QFTdynamicsProblem, ::TwoPIScalarGPUfull)
+getTwoPISimData(problem::QFTdynamicsProblem, ::TwoPIScalarGPUfull) = gpusimdata(problem, :kb_create)
+getTwoPISimData(problem::QFTdynamicsProblem, ::TwoPIScalarGPUorbits) = gpusimdata(problem, :kb_create_sym)
+
+function gpusimdata(problem::QFTdynamicsProblem, create::Symbol)
     d, m = problem.disc, problem.model
     p = Ref(KBParams(d.Nx, d.sdim, problem.simsetup.NstepsinMemory, m.ONgroup, d.dt, m.Lambda, d.Mass,
                      m.Mass2, pexpcode(problem.pexp), 0))
     h = Ref{Ptr{Cvoid}}(C_NULL)
-    rc = ccall((:kb_create, libqftkb), Cint, (Ref{KBParams}, Ref{Ptr{Cvoid}}), p, h)
+    rc = create === :kb_create_sym ?
+        ccall((:kb_create_sym, libqftkb), Cint, (Ref{KBParams}, Ref{Ptr{Cvoid}}), p, h) :
+        ccall((:kb_create, libqftkb), Cint, (Ref{KBParams}, Ref{Ptr{Cvoid}}), p, h)
     rc == 0 || throw(ErrorException("kb_create failed with code $rc"))
     kL2 = zeros(d.Nx^d.sdim)
     for element in d.fftwhelper, idx in element.ind

@@ -6,7 +6,11 @@ The reference builds a QFTdynamicsProblem from a parameter Dict
 (:133-143), then runs initialize!, evolve! and measure!
 (TwoPIScalar_Initialization.jl:289-325, TwoPIScalar_Evolution.jl:924-981,
 TwoPIScalar_Measurement.jl:223-228).  The same names and parameter keys are
-kept here; the numerics string for this back-end is "GPUfull".
+kept here; the numerics strings for this back-end are "GPUfull" (every
+site stored, the drop-in for CPUfull) and "GPUorbits" (one value per orbit
+of the lattice's cubic group, exact for the cubic-symmetric states that every
+initialisation here produces; the counterpart of the reference's reduced
+CPUred storage).
 
 Left out on purpose: the run-directory side effects of
 createfilestructandSimSetup (reading etc/paths.txt, purging and creating
@@ -152,6 +156,9 @@ class TwoPIScalarProblem:
     simsetup: SimSetup
 
 
+# the device back-ends: every site stored, or one value per cubic orbit
+NUMERICS = ("GPUfull", "GPUorbits")
+
 _INIT_KEYS = {"Thermal": ("T",), "Pnr": ("n",), "Gauss": ("xi", "eta", "sig"), "Box": ("n0", "Q"),
               "TopHatT1": (), "TopHatT2": (), "TopHatT3": ()}
 
@@ -159,8 +166,8 @@ _INIT_KEYS = {"Thermal": ("T",), "Pnr": ("n",), "Gauss": ("xi", "eta", "sig"), "
 def _make_problem(modelfile, parameters: dict) -> TwoPIScalarProblem:
     """QFTdynamicsProblem(::TwoPIScalarfile, parameters), Modelfile.jl:46-124."""
     check_parameters(parameters)
-    if parameters["Num"] != "GPUfull":
-        raise ValueError(f"Num={parameters['Num']!r}: this package implements the 'GPUfull' numerics")
+    if parameters["Num"] not in NUMERICS:
+        raise ValueError(f"Num={parameters['Num']!r}: this package implements the numerics {NUMERICS}")
     simsetup = createSimSetup(modelfile, parameters)
     Nx, sdim = parameters["Nx"], parameters["sdim"]
     V = Nx ** sdim
@@ -240,7 +247,7 @@ class TwoPIScalarSimDataGPU:
         d, m = problem.disc, problem.model
         pexp = PEXP_CODES[problem.pexp]
         self.ctx = KBContext(d.Nx, d.sdim, d.NstepsinMemory, d.dt, m.Lambda, m.ONgroup, d.Mass, m.Mass2,
-                             pexp=pexp, device=device)
+                             pexp=pexp, device=device, symmetric=problem.num == "GPUorbits")
         self.ctx.set_kL2(d.kL2)
         self.NstepsinMemory = d.NstepsinMemory
         self.Nx, self.sdim = d.Nx, d.sdim

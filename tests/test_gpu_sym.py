@@ -324,3 +324,25 @@ def test_orbit_storage_every_tile_height():
             assert rel(g.get(w, tmax, tp), ref.get(w, tmax, tp)) <= 1e-12, ("autotune", w, tp)
     g.close()
     ref.close()
+
+
+@pytest.mark.parametrize("init", ["Thermal", "Box", "TopHatT1", "Gauss"])
+def test_parameter_dict_run_with_orbit_numerics(init):
+    """"Num" => "GPUorbits" runs the reference-shaped driver loop on orbit
+    storage; every initial state of the reference is cubic-symmetric, so the
+    measurements equal those of "GPUfull" to rounding."""
+    from qftkb import TwoPIScalarfile, run
+    print(f"\n[case] parameter-Dict run, GPUorbits vs GPUfull, init={init}", flush=True)
+    base = {"Mass": 1.0, "Nx": 8, "sdim": 3, "dt": 0.1, "Nsteps": 30, "NstepsinMemory": 12, "Nmeas": 10,
+            "Mod": "phi4", "Lambda": 1.0, "ONgroup": 1, "init": init, "T": 2.0, "n0": 2.0, "Q": 1.0,
+            "xi": 0.4, "eta": 5.0, "sig": 1.0, "Reno": "RNone", "Pexp": "LambdaNLO", "Nchunks": 0, "tag": "",
+            "ov": True}
+    full = run(TwoPIScalarfile(), dict(base, Num="GPUfull"))
+    orb = run(TwoPIScalarfile(), dict(base, Num="GPUorbits"))
+    assert orb.simdata.ctx.storage_info()[0] < full.simdata.ctx.storage_info()[0]
+    for mf, mo in zip(full.measurearray, orb.measurearray):
+        assert mf is not None and mo is not None
+        for k in ("E", "E_kin", "E_pot", "E_mem"):
+            a, b = getattr(mo, k), getattr(mf, k)
+            assert abs(a - b) <= 1e-12 * max(abs(b), abs(mf.E)), (k, a, b)
+        np.testing.assert_allclose(mo.n, mf.n, rtol=1e-11, atol=1e-13)

@@ -68,6 +68,11 @@ def test_problem_factory_and_errors():
     assert prob.model.Mass2 == 1.0 and prob.simsetup.NstepsinMemory == 302
     with pytest.raises(ValueError):
         _make_problem(TwoPIScalarfile(), base_params(Num="CPUfull"))
+    # the two device numerics: every site, or one value per cubic orbit
+    for num in ("GPUfull", "GPUorbits"):
+        assert _make_problem(TwoPIScalarfile(), base_params(Num=num)).num == num
+    with pytest.raises(ValueError):
+        _make_problem(TwoPIScalarfile(), base_params(Num="CPUred"))
     with pytest.raises(NotImplementedError):
         _make_problem(TwoPIScalarfile(), base_params(init="Quench"))
     with pytest.raises(ValueError):
