@@ -97,7 +97,8 @@ int kb_info(const kb_ctx *ctx, long long *V, long long *device_bytes);
  * lattices of V sites: inputs are read at each orbit's representative site
  * (the caller supplies symmetric data), outputs are expanded over the
  * orbits.  KB_ERR_ARG for a transform stage with neither form available
- * (Nx > 64 and more than 4096 orbits); no adjoint, no ensembles.  The reference's CPUred
+ * (Nx > 64 and more than 4096 orbits); no adjoint.  Ensembles take such
+ * members through kb_ens_create_sym.  The reference's CPUred
  * storage (TwoPIScalar_SimDataStruct.jl:204-260) groups sites by FFTWhelper
  * levels instead; this is the exact-symmetry version of that idea. */
 int kb_create_sym(const kb_params *p, kb_ctx **out);
@@ -298,6 +299,12 @@ typedef struct kb_ens kb_ens;
 
 /* nruns parameter sets; all members live on params[0].device. */
 int kb_ens_create(const kb_params *params, int nruns, kb_ens **out);
+/* The same with a storage per member: symmetric[i] != 0 creates member i in
+ * cubic-orbit storage (as kb_create_sym), 0 stores every lattice site;
+ * symmetric NULL: every member in orbit storage.  The memory-integral launch
+ * covers each member's stored sites; a member's self-energy stage is the one
+ * kb_create_sym picks for it. */
+int kb_ens_create_sym(const kb_params *params, const int *symmetric, int nruns, kb_ens **out);
 int kb_ens_destroy(kb_ens *ens);
 const char *kb_ens_last_error(const kb_ens *ens);
 /* Member i as a kb_ctx for initialisation, measurement and read-out

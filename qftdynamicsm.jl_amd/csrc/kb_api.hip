@@ -1541,6 +1541,11 @@ extern "C" {
 const char* kb_ens_last_error(const kb_ens* e) { return e ? e->err.c_str() : "null ensemble"; }
 
 int kb_ens_create(const kb_params* params, int nruns, kb_ens** out) {
+  std::vector<int> full(nruns > 0 ? nruns : 0, 0);
+  return kb_ens_create_sym(params, full.data(), nruns, out);
+}
+
+int kb_ens_create_sym(const kb_params* params, const int* symmetric, int nruns, kb_ens** out) {
   if (!params || nruns < 1 || nruns > 64 || !out) return KB_ERR_ARG;
   *out = nullptr;
   kb_ens* e = new (std::nothrow) kb_ens();
@@ -1554,7 +1559,9 @@ int kb_ens_create(const kb_params* params, int nruns, kb_ens** out) {
   for (int i = 0; i < nruns; ++i) {
     if (params[i].device != e->device) return fail(KB_ERR_ARG);
     kb_ctx* c = nullptr;
-    int rc = kb_create(&params[i], &c);
+    // an orbit-storage member keeps its own storage: the memory-integral
+    // arguments carry its stored sites, its self-energy stage is its own
+    int rc = create_ctx(&params[i], !symmetric || symmetric[i] != 0, &c);
     if (rc) return fail(rc);
     e->m.push_back(c);
   }

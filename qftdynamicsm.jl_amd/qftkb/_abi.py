@@ -40,7 +40,7 @@ EXPORTED = (
     "kb_measure_history", "kb_get_entry", "kb_set_entry", "kb_get_sigma", "kb_ninv_get", "kb_read_bandwidth",
     "kb_check_finite",
     "kb_fill_free_window", "kb_window_bytes", "kb_save_window", "kb_load_window", "kb_synchronize", "kb_set_stream", "kb_enable_timing", "kb_timing",
-    "kb_reset_timing", "kb_use_graph", "kb_ens_create", "kb_ens_destroy", "kb_ens_last_error", "kb_ens_member",
+    "kb_reset_timing", "kb_use_graph", "kb_ens_create", "kb_ens_create_sym", "kb_ens_destroy", "kb_ens_last_error", "kb_ens_member",
     "kb_ens_step", "kb_ens_use_graph", "kb_ens_synchronize", "kb_ens_enable_timing", "kb_ens_timing",
     "kb_adj_record", "kb_adj_gradient", "kb_adj_info", "kb_adj_stop", "kb_adj_debug_stop", "kb_adj_get_entry",
     "kb_tile_shape", "kb_set_tiles", "kb_get_tiles", "kb_autotune",
@@ -135,6 +135,8 @@ def load_library(path: str = LIB_PATH):
     L.kb_adj_debug_stop.argtypes = [vp, ctypes.c_int]
     L.kb_adj_get_entry.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
     L.kb_ens_create.argtypes = [ctypes.POINTER(kb_params), ctypes.c_int, ctypes.POINTER(vp)]
+    L.kb_ens_create_sym.argtypes = [ctypes.POINTER(kb_params), ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                    ctypes.POINTER(vp)]
     L.kb_ens_destroy.argtypes = [vp]
     L.kb_ens_last_error.restype = ctypes.c_char_p
     L.kb_ens_last_error.argtypes = [vp]
@@ -506,24 +508,29 @@ class KBContext:
 
 class KBEnsemble:
     """Several KB runs advanced together (a kb_ens).  members[i] is a
-    KBContext view of run i for initialisation and read-out."""
+    KBContext view of run i for initialisation and read-out.  A run whose
+    dict has symmetric=True is stored in cubic orbits (as KBContext's
+    symmetric=True)."""
 
     def __init__(self, runs, device=0):
         self._L = load_library()
+        runs = [dict(r) for r in runs]
+        sym = [bool(r.pop("symmetric", False)) for r in runs]
         arr = (kb_params * len(runs))(*[make_params(**dict(r, device=device)) for r in runs])
         h = ctypes.c_void_p()
-        rc = self._L.kb_ens_create(arr, len(runs), ctypes.byref(h))
+        flags = (ctypes.c_int * len(runs))(*[1 if x else 0 for x in sym])
+        rc = self._L.kb_ens_create_sym(arr, flags, len(runs), ctypes.byref(h))
         if rc != KB_OK:
-            raise KBError(rc, "kb_ens_create failed")
+            raise KBError(rc, "kb_ens_create_sym failed (invalid parameters, no HIP device, or out of memory)")
         self._h = h
         self.members = []
         for i, r in enumerate(runs):
             m = ctypes.c_void_p()
             self._chk(self._L.kb_ens_member(self._h, i, ctypes.byref(m)))
-            member = KBContext(_handle=m, _owner=self, device=device, **r)
+            member = KBContext(_handle=m, _owner=self, device=device, symmetric=sym[i], **r)
             # members share the default memory-integral shape (one launch over
             # all of them); the self-energy stage is per member
-            _, f = committed_tiles(r["Nx"], r["sdim"], r["NstepsinMemory"])
+            _, f = committed_tiles(r["Nx"], r["sdim"], r["NstepsinMemory"], symmetric=sym[i])
             if f is not None and not member.orbit_product:
                 member.set_fft_tiles(f)
             self.members.append(member)

@@ -109,3 +109,87 @@ def test_ensemble_members_match_oracle(graph):
     with pytest.raises(KBError):
         ens.members[0].step(26)
     ens.close()
+
+
+SYM_RUNS = [  # orbit-storage members beside a full-storage one; one 1/N member
+    dict(Nx=6, sdim=3, NstepsinMemory=9, dt=0.1, Lambda=1.6, ONgroup=1, Mass=1.0, Mass2=1.0, symmetric=True),
+    dict(Nx=8, sdim=3, NstepsinMemory=6, dt=0.05, Lambda=1.1, ONgroup=4, Mass=0.8, Mass2=0.64),
+    dict(Nx=7, sdim=2, NstepsinMemory=7, dt=0.1, Lambda=2.0, ONgroup=2, Mass=1.0, Mass2=1.0, symmetric=True),
+    dict(Nx=8, sdim=3, NstepsinMemory=9, dt=0.05, Lambda=2.0, ONgroup=4, Mass=1.0, Mass2=1.0, pexp=3,
+         symmetric=True),
+]
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_orbit_storage_members(graph):
+    """Members in cubic-orbit storage step with the others: the LambdaNLO
+    members follow the oracle, and every member follows a standalone context
+    of its own storage (whose memory integral runs in the split form: the
+    same sums in another order)."""
+    from qftkb import KBContext, KBEnsemble
+    print(f"\n[case] ensemble with orbit-storage members, graph={graph}", flush=True)
+    ens = KBEnsemble(SYM_RUNS)
+    ens.use_graph(graph)
+    alone, oracles = [], []
+    for r, m in zip(SYM_RUNS, ens.members):
+        assert m.symmetric == r.get("symmetric", False)
+        c = KBContext(**r)
+        n = thermal(c.kL2(), m=r["Mass"])
+        m.init_particles(n, r["Mass"])
+        c.init_particles(n, r["Mass"])
+        alone.append(c)
+        o = None
+        if r.get("pexp", 2) != 3:
+            o = Oracle(r["Nx"], r["sdim"], r["NstepsinMemory"], r["dt"], r["Lambda"], r["ONgroup"], r["Mass"],
+                       r["Mass2"])
+            o.init_particles(n, r["Mass"])
+        oracles.append(o)
+    for t in range(3, 24):
+        ens.step(t)
+        for c, o in zip(alone, oracles):
+            c.step(t)
+            if o is not None:
+                o.evolve(t)
+        if t % 5 == 0:
+            for c, o, m in zip(alone, oracles, ens.members):
+                mg, mc = m.measure(t), c.measure(t)
+                assert abs(mg["E"] - mc["E"]) <= 1e-12 * abs(mc["E"])
+                if o is not None:
+                    mo = o.measure(t)
+                    assert abs(mg["E"] - mo["E"]) <= RTOL * abs(mo["E"])
+                    assert rel(mg["n"], mo["n"]) <= RTOL
+    for c, o, m in zip(alone, oracles, ens.members):
+        check_window(c, m, tol=1e-12)
+        if o is not None:
+            check_window(o, m)
+        c.close()
+    ens.close()
+
+
+def test_orbit_storage_ensemble_at_32cubed():
+    """Two 32^3 orbit-storage runs and a 24^3 one at N_mem = 40 from filled
+    windows: each member matches its standalone context."""
+    from qftkb import KBContext, KBEnsemble, lattice
+    print("\n[case] orbit-storage ensemble 32^3, 32^3, 24^3 N_mem=40", flush=True)
+    runs = [dict(Nx=nx, sdim=3, NstepsinMemory=40, dt=0.1, Lambda=lam, ONgroup=1, Mass=1.0, Mass2=1.0,
+                 symmetric=True) for nx, lam in ((32, 1.0), (32, 1.7), (24, 0.6))]
+    ens = KBEnsemble(runs)
+    ens.use_graph(True)
+    alone = []
+    for r, m, (n0, Q) in zip(runs, ens.members, ((2.0, 1.0), (3.0, 0.7), (1.5, 1.3))):
+        c = KBContext(**r)
+        for x in (m, c):
+            x.fill_free_window(40, lattice.n_box(x.kL2(), n0, Q), 1.0)
+        alone.append(c)
+    for t in range(41, 49):
+        ens.step(t)
+        for c in alone:
+            c.step(t)
+    for c, m in zip(alone, ens.members):
+        for tp in range(9, 49):
+            for w in "Fr":
+                assert rel(m.get(w, 48, tp), c.get(w, 48, tp)) <= 1e-12, (w, tp)
+        mg, mc = m.measure(48), c.measure(48)
+        assert abs(mg["E"] - mc["E"]) <= 1e-12 * abs(mc["E"])
+        c.close()
+    ens.close()
