@@ -193,7 +193,7 @@ def ensemble_bench(args):
     # the same runs one after another, each a plain context (graph on too)
     t_seq = 0.0
     per_run = []
-    for r, o in zip(runs, occ):
+    for r, o in zip(runs, occ) if not args.no_sequential else ():
         c = KBContext(**r)
         c.fill_free_window(N, lattice.n_box(c.kL2(), o["n0"], o["Q"]), 1.0)
         c.use_graph(True)
@@ -216,8 +216,8 @@ def ensemble_bench(args):
                   (" with cubic-orbit storage (separate metric)" if args.sym else ""),
         "value": len(runs) * steps / t_ens, "unit": "member KB steps/s",
         "ensemble_steps_per_s": steps / t_ens,
-        "sequential_member_steps_per_s": len(runs) * steps / t_seq,
-        "speedup_vs_sequential": t_seq / t_ens,
+        "sequential_member_steps_per_s": len(runs) * steps / t_seq if t_seq else None,
+        "speedup_vs_sequential": t_seq / t_ens if t_seq else None,
         "ms_per_ensemble_step": 1e3 * t_ens / steps,
         "sequential_ms_per_step_by_run": dict(zip([f"{r['Nx']}^3 #{i}" for i, r in enumerate(runs)], per_run)),
         "steps": steps, "warmup": warm, "dtype": "f64",
@@ -464,6 +464,8 @@ def main():
     ap.add_argument("--nx", type=int, default=WORKLOAD["Nx"])
     ap.add_argument("--ensemble", action="store_true",
                     help="8 uneven runs (16^3..40^3) advanced as one ensemble vs the same runs one after another")
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="with --ensemble: skip stepping the runs one after another")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost-ratio leg")
     ap.add_argument("--sym", action="store_true",
                     help="cubic-orbit storage (a separate metric: the stored window shrinks ~34x at 32^3)")

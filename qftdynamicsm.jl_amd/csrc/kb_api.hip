@@ -1461,6 +1461,13 @@ struct kb_ens {
   std::vector<kb_ctx*> m;
   int device = 0;
   hipStream_t stream = nullptr;
+  // members' self-energy stages run concurrently on up to 4 lanes: lane 0
+  // is stream itself, lanes 1.. are side streams forked from and joined back
+  // into it (4 lanes on 4 hardware queues; member i on lane[i])
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> join;
+  std::vector<int> lane;
+  hipEvent_t fork = nullptr;
   StepState** stptr_dev = nullptr;
   int* N_dev = nullptr;
   MemintArgs* margs_dev = nullptr;
@@ -1510,7 +1517,45 @@ void ens_free(kb_ens* e) {
     (void)hipEventDestroy(ev.a);
     (void)hipEventDestroy(ev.b);
   }
+  for (auto ev : e->join)
+    if (ev) (void)hipEventDestroy(ev);
+  if (e->fork) (void)hipEventDestroy(e->fork);
+  for (size_t j = 1; j < e->side.size(); ++j)
+    if (e->side[j]) (void)hipStreamDestroy(e->side[j]);
   if (e->stream) (void)hipStreamDestroy(e->stream);
+}
+
+// the members' stages on the side streams: each member to the least-loaded
+// stream, largest first; the load of a member is the flops of its transforms
+int ens_make_lanes(kb_ens* e) {
+  const int nr = (int)e->m.size();
+  const int ns = std::min(nr, 4);
+  e->lane.assign(nr, 0);
+  if (ns < 2) return KB_OK;
+  e->side.assign(ns, nullptr);
+  e->join.assign(ns, nullptr);
+  e->side[0] = e->stream;
+  for (int j = 1; j < ns; ++j)
+    if (hipStreamCreateWithFlags(&e->side[j], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e->join[j], hipEventDisableTiming) != hipSuccess)
+      return KB_ERR_HIP;
+  if (hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess) return KB_ERR_HIP;
+  std::vector<double> work(nr);
+  std::vector<int> order(nr);
+  for (int i = 0; i < nr; ++i) {
+    const kb_ctx* c = e->m[i];
+    const double sites = c->Mo ? (double)c->Mp * c->Mp : 10.0 * c->p.Nx * std::pow((double)c->p.Nx, c->p.sdim);
+    work[i] = sites * c->N;
+    order[i] = i;
+  }
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+  std::vector<double> load(ns, 0.0);
+  for (int i : order) {
+    const int j = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    e->lane[i] = j;
+    load[j] += work[i];
+  }
+  return KB_OK;
 }
 
 // one ensemble step: advance all windows, each member's self-energy and
@@ -1520,14 +1565,33 @@ int ens_enqueue(kb_ens* e) {
   hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64 * ((nr + 63) / 64)), 0, e->stream, e->stptr_dev, e->N_dev, nr);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return ens_err(e, KB_ERR_HIP, "advance launch failed: %s", hipGetErrorString(err));
-  for (auto* c : e->m) {
-    int rc;
-    if (c->p.pexp == KB_PEXP_LAMBDA_NLO && (rc = launch_sigma(c)))
-      return ens_err(e, rc, "member self-energy: %s", c->err.c_str());
-    if (c->p.pexp == KB_PEXP_NINVERSE_NLO && (rc = launch_ninv_stage(c)))
-      return ens_err(e, rc, "member 1/N self-energy: %s", c->err.c_str());
-    if ((rc = launch_hartree_step(c))) return ens_err(e, rc, "member Hartree: %s", c->err.c_str());
+  const bool forked = !e->side.empty();
+  if (forked) {
+    err = hipEventRecord(e->fork, e->stream);
+    for (size_t j = 1; j < e->side.size() && err == hipSuccess; ++j) err = hipStreamWaitEvent(e->side[j], e->fork, 0);
+    if (err != hipSuccess) return ens_err(e, KB_ERR_HIP, "fork: %s", hipGetErrorString(err));
   }
+  int rc = KB_OK;
+  for (size_t i = 0; i < e->m.size() && rc == KB_OK; ++i) {
+    kb_ctx* c = e->m[i];
+    if (forked) c->stream = e->side[e->lane[i]];
+    if (c->p.pexp == KB_PEXP_LAMBDA_NLO && (rc = launch_sigma(c)))
+      rc = ens_err(e, rc, "member self-energy: %s", c->err.c_str());
+    else if (c->p.pexp == KB_PEXP_NINVERSE_NLO && (rc = launch_ninv_stage(c)))
+      rc = ens_err(e, rc, "member 1/N self-energy: %s", c->err.c_str());
+    else if ((rc = launch_hartree_step(c)))
+      rc = ens_err(e, rc, "member Hartree: %s", c->err.c_str());
+  }
+  for (auto* c : e->m) c->stream = e->stream;
+  if (forked) {
+    // joined even after a failed launch, so that a capture ends cleanly
+    for (size_t j = 1; j < e->side.size(); ++j) {
+      hipError_t je = hipEventRecord(e->join[j], e->side[j]);
+      if (je == hipSuccess) je = hipStreamWaitEvent(e->stream, e->join[j], 0);
+      if (je != hipSuccess && rc == KB_OK) rc = ens_err(e, KB_ERR_HIP, "join: %s", hipGetErrorString(je));
+    }
+  }
+  if (rc) return rc;
   // members keep the default tile shape (their work items are 64-row chunks)
   err = launch_memint(0, e->total_wg, memint_lds(e->maxN), e->stream, e->margs_dev, e->wg_dev, nr);
   if (err != hipSuccess) return ens_err(e, KB_ERR_HIP, "memint launch failed: %s", hipGetErrorString(err));
@@ -1567,6 +1631,7 @@ int kb_ens_create_sym(const kb_params* params, const int* symmetric, int nruns, 
   }
   if (hipSetDevice(e->device) != hipSuccess) return fail(KB_ERR_HIP);
   if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) return fail(KB_ERR_HIP);
+  if (ens_make_lanes(e) != KB_OK) return fail(KB_ERR_HIP);
   std::vector<StepState*> sp;
   std::vector<int> Ns, wg;
   std::vector<MemintArgs> ma;
