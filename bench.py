@@ -179,6 +179,11 @@ def ensemble_bench(args):
     for _ in range(warm):
         ens.step(t)
         t += 1
+    # the pooled memory integral's shape, timed on the last step (then one
+    # step to capture the graph again)
+    tuned = ens.autotune(3)
+    ens.step(t)
+    t += 1
     ens.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -220,6 +225,8 @@ def ensemble_bench(args):
         "speedup_vs_sequential": t_seq / t_ens if t_seq else None,
         "ms_per_ensemble_step": 1e3 * t_ens / steps,
         "sequential_ms_per_step_by_run": dict(zip([f"{r['Nx']}^3 #{i}" for i, r in enumerate(runs)], per_run)),
+        "ensemble_tiles": {k: tuned[k] for k in ("rows", "batch", "threads")},
+        "ensemble_tile_ms": [round(x, 4) for x in tuned["ms"]],
         "steps": steps, "warmup": warm, "dtype": "f64",
         "config": {"workload": "8-run uneven ensemble {16,16,24,24,32,32,40,40}^3", "Nx": list(ENSEMBLE_NX), "NstepsinMemory": N,
                    "seed": ENSEMBLE_SEED, "Lambda": [r["Lambda"] for r in runs], "box": occ,

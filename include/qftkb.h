@@ -320,6 +320,17 @@ int kb_ens_synchronize(kb_ens *ens);
 /* HIP-event time of whole ensemble steps since the last reset. */
 int kb_ens_enable_timing(kb_ens *ens, int on);
 int kb_ens_timing(kb_ens *ens, double *total_ms, long long *count);
+/* Shape of the pooled memory-integral launch (a kb_tile_shape variant; 0
+ * at creation), one for all members.  kb_ens_autotune re-times the last
+ * ensemble step's memory integral in every shape (reps launches each; each
+ * recomputes row t of every member from the same inputs, so row t may change
+ * in the last bits) and keeps the fastest; ms, if not NULL, receives one
+ * average per shape.  KB_ERR_STATE before the first kb_ens_step or after a
+ * member was re-initialised.  No counterpart in the reference (launch
+ * configuration). */
+int kb_ens_set_tiles(kb_ens *ens, int variant);
+int kb_ens_get_tiles(const kb_ens *ens, int *variant);
+int kb_ens_autotune(kb_ens *ens, int reps, int *best, double *ms);
 
 #ifdef __cplusplus
 }

@@ -1474,6 +1474,8 @@ struct kb_ens {
   int* wg_dev = nullptr;
   int total_wg = 0;
   int maxN = 0;
+  int tile = 0;     // shape of the pooled memory-integral launch (kMemintVariants)
+  int last_t = 0;   // t of the last ensemble step (0: none yet)
   bool use_graph = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -1592,9 +1594,26 @@ int ens_enqueue(kb_ens* e) {
     }
   }
   if (rc) return rc;
-  // members keep the default tile shape (their work items are 64-row chunks)
-  err = launch_memint(0, e->total_wg, memint_lds(e->maxN), e->stream, e->margs_dev, e->wg_dev, nr);
+  // one shape for all members (their item tables follow its row height)
+  err = launch_memint(e->tile, e->total_wg, memint_lds(e->maxN), e->stream, e->margs_dev, e->wg_dev, nr);
   if (err != hipSuccess) return ens_err(e, KB_ERR_HIP, "memint launch failed: %s", hipGetErrorString(err));
+  return KB_OK;
+}
+
+// the pooled launch's shape: every member's work items are cut to its row
+// height (a member's tile selects its item table)
+int ens_set_tile(kb_ens* e, int v) {
+  hipError_t err = hipStreamSynchronize(e->stream);
+  if (err != hipSuccess) return ens_err(e, KB_ERR_HIP, "tiles: %s", hipGetErrorString(err));
+  std::vector<MemintArgs> ma;
+  for (auto* c : e->m) {
+    c->tile = v;
+    ma.push_back(memint_args(c));
+  }
+  err = hipMemcpy(e->margs_dev, ma.data(), sizeof(MemintArgs) * ma.size(), hipMemcpyHostToDevice);
+  if (err != hipSuccess) return ens_err(e, KB_ERR_HIP, "tiles: %s", hipGetErrorString(err));
+  e->tile = v;
+  ens_drop_graph(e);
   return KB_OK;
 }
 
@@ -1603,6 +1622,64 @@ int ens_enqueue(kb_ens* e) {
 extern "C" {
 
 const char* kb_ens_last_error(const kb_ens* e) { return e ? e->err.c_str() : "null ensemble"; }
+
+int kb_ens_set_tiles(kb_ens* e, int variant) {
+  if (!e || variant < 0 || variant >= NMEMINT_VARIANTS) return KB_ERR_ARG;
+  (void)hipSetDevice(e->device);
+  return ens_set_tile(e, variant);
+}
+
+int kb_ens_get_tiles(const kb_ens* e, int* variant) {
+  if (!e || !variant) return KB_ERR_ARG;
+  *variant = e->tile;
+  return KB_OK;
+}
+
+int kb_ens_autotune(kb_ens* e, int reps, int* best, double* ms) {
+  if (!e || reps < 1) return KB_ERR_ARG;
+  if (!e->last_t) return ens_err(e, KB_ERR_STATE, "autotune: take an ensemble step first (its integral is re-timed)");
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    const kb_ctx* c = e->m[i];
+    if (!c->initialised || c->st.tmax != e->last_t)
+      return ens_err(e, KB_ERR_STATE, "autotune: member %zu has changed since the last ensemble step", i);
+  }
+  (void)hipSetDevice(e->device);
+  const int nr = (int)e->m.size();
+  std::vector<double> t(NMEMINT_VARIANTS, 0.0);
+  hipEvent_t a = nullptr, b = nullptr;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+    if (a) (void)hipEventDestroy(a);
+    return ens_err(e, KB_ERR_HIP, "autotune: no events");
+  }
+  int rc = KB_OK;
+  for (int v = 0; v < NMEMINT_VARIANTS && rc == KB_OK; ++v) {
+    if ((rc = ens_set_tile(e, v))) break;
+    // re-evaluates row t of the last step of every member from the same inputs
+    for (int k = 0; k <= reps && rc == KB_OK; ++k) {
+      if (k == 1 && hipEventRecord(a, e->stream) != hipSuccess) rc = ens_err(e, KB_ERR_HIP, "autotune: event");
+      if (rc == KB_OK &&
+          launch_memint(v, e->total_wg, memint_lds(e->maxN), e->stream, e->margs_dev, e->wg_dev, nr) != hipSuccess)
+        rc = ens_err(e, KB_ERR_HIP, "autotune: launch failed");
+    }
+    if (rc) break;
+    float e_ms = 0.f;
+    if (hipEventRecord(b, e->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+        hipEventElapsedTime(&e_ms, a, b) != hipSuccess)
+      rc = ens_err(e, KB_ERR_HIP, "autotune: timing failed");
+    t[v] = e_ms / reps;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (rc) return rc;
+  int bv = 0;
+  for (int v = 1; v < NMEMINT_VARIANTS; ++v)
+    if (t[v] < t[bv]) bv = v;
+  if ((rc = ens_set_tile(e, bv))) return rc;
+  if (best) *best = bv;
+  if (ms)
+    for (int v = 0; v < NMEMINT_VARIANTS; ++v) ms[v] = t[v];
+  return KB_OK;
+}
 
 int kb_ens_create(const kb_params* params, int nruns, kb_ens** out) {
   std::vector<int> full(nruns > 0 ? nruns : 0, 0);
@@ -1724,6 +1801,7 @@ int kb_ens_step(kb_ens* e, int t) {
     (void)hipEventRecord(ev.b, e->stream);
     e->pending.push_back(ev);
   }
+  e->last_t = t;
   return KB_OK;
 }
 

@@ -40,7 +40,7 @@ EXPORTED = (
     "kb_measure_history", "kb_get_entry", "kb_set_entry", "kb_get_sigma", "kb_ninv_get", "kb_read_bandwidth",
     "kb_check_finite",
     "kb_fill_free_window", "kb_window_bytes", "kb_save_window", "kb_load_window", "kb_synchronize", "kb_set_stream", "kb_enable_timing", "kb_timing",
-    "kb_reset_timing", "kb_use_graph", "kb_ens_create", "kb_ens_create_sym", "kb_ens_destroy", "kb_ens_last_error", "kb_ens_member",
+    "kb_reset_timing", "kb_use_graph", "kb_ens_create", "kb_ens_create_sym", "kb_ens_set_tiles", "kb_ens_get_tiles", "kb_ens_autotune", "kb_ens_destroy", "kb_ens_last_error", "kb_ens_member",
     "kb_ens_step", "kb_ens_use_graph", "kb_ens_synchronize", "kb_ens_enable_timing", "kb_ens_timing",
     "kb_adj_record", "kb_adj_gradient", "kb_adj_info", "kb_adj_stop", "kb_adj_debug_stop", "kb_adj_get_entry",
     "kb_tile_shape", "kb_set_tiles", "kb_get_tiles", "kb_autotune",
@@ -137,6 +137,9 @@ def load_library(path: str = LIB_PATH):
     L.kb_ens_create.argtypes = [ctypes.POINTER(kb_params), ctypes.c_int, ctypes.POINTER(vp)]
     L.kb_ens_create_sym.argtypes = [ctypes.POINTER(kb_params), ctypes.POINTER(ctypes.c_int), ctypes.c_int,
                                     ctypes.POINTER(vp)]
+    L.kb_ens_set_tiles.argtypes = [vp, ctypes.c_int]
+    L.kb_ens_get_tiles.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    L.kb_ens_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]
     L.kb_ens_destroy.argtypes = [vp]
     L.kb_ens_last_error.restype = ctypes.c_char_p
     L.kb_ens_last_error.argtypes = [vp]
@@ -544,6 +547,26 @@ class KBEnsemble:
 
     def use_graph(self, on=True):
         self._chk(self._L.kb_ens_use_graph(self._h, 1 if on else 0))
+
+    def set_tiles(self, variant):
+        """Shape of the pooled memory-integral launch (a tile_shape variant)."""
+        self._chk(self._L.kb_ens_set_tiles(self._h, int(variant)))
+
+    def tiles(self):
+        v = ctypes.c_int()
+        self._chk(self._L.kb_ens_get_tiles(self._h, ctypes.byref(v)))
+        return v.value
+
+    def autotune(self, reps=3):
+        """Re-time the last ensemble step's pooled memory integral in every
+        tile shape and keep the fastest.  Returns {"variant", "rows", "batch",
+        "threads", "ms": [...]} like KBContext.autotune."""
+        shapes = self.members[0].tile_shapes()
+        ms = np.zeros(len(shapes))
+        best = ctypes.c_int()
+        self._chk(self._L.kb_ens_autotune(self._h, int(reps), ctypes.byref(best), _ptr(ms)))
+        r, b, n = shapes[best.value]
+        return {"variant": best.value, "rows": r, "batch": b, "threads": n, "ms": ms.tolist()}
 
     def synchronize(self):
         self._chk(self._L.kb_ens_synchronize(self._h))

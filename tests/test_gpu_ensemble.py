@@ -193,3 +193,42 @@ def test_orbit_storage_ensemble_at_32cubed():
         assert abs(mg["E"] - mc["E"]) <= 1e-12 * abs(mc["E"])
         c.close()
     ens.close()
+
+
+def test_ensemble_tiles_and_autotune():
+    """The pooled memory integral in every tile shape (members' item tables
+    follow the shape), and the autotuner, keep the members on the oracle."""
+    from qftkb import KBEnsemble, KBError
+    print("\n[case] ensemble tile shapes + autotune", flush=True)
+    ens = KBEnsemble(RUNS[:3] + SYM_RUNS[:1])
+    oracles = []
+    for r, m in zip(RUNS[:3] + SYM_RUNS[:1], ens.members):
+        o = Oracle(r["Nx"], r["sdim"], r["NstepsinMemory"], r["dt"], r["Lambda"], r["ONgroup"], r["Mass"],
+                   r["Mass2"])
+        n = thermal(o.kL2(), m=r["Mass"])
+        o.init_particles(n, r["Mass"])
+        m.init_particles(n, r["Mass"])
+        oracles.append(o)
+    with pytest.raises(KBError):
+        ens.autotune(1)  # no step yet
+    nshapes = len(ens.members[0].tile_shapes())
+    t = 3
+    for v in list(range(nshapes)) + [None]:
+        if v is None:
+            best = ens.autotune(2)
+            assert ens.tiles() == best["variant"] and len(best["ms"]) == nshapes
+            assert all(x > 0 for x in best["ms"])
+        else:
+            ens.set_tiles(v)
+            assert ens.tiles() == v
+        ens.use_graph(v is None or v % 2 == 1)
+        for _ in range(3):
+            ens.step(t)
+            for o in oracles:
+                o.evolve(t)
+            t += 1
+    for o, m in zip(oracles, ens.members):
+        check_window(o, m)
+        mo, mg = o.measure(t - 1), m.measure(t - 1)
+        assert abs(mg["E"] - mo["E"]) <= RTOL * abs(mo["E"])
+    ens.close()
