@@ -292,8 +292,9 @@ int kb_adj_get_entry(kb_ctx *ctx, int which, int i, int j, double *out);
  * Many independent KB runs (uneven lattices, couplings, memory windows)
  * advanced together: every member keeps its own device state; one ensemble
  * step launches each member's self-energy and Hartree kernels (on up to four
- * streams at once, joined before the next launch) and ONE memory-integral
- * launch over the site blocks of all members.  The
+ * streams at once) and ONE memory-integral launch over the site blocks of
+ * all members (an ensemble whose members are all in orbit storage launches
+ * it per stream instead, right after that stream's stages).  The
  * reference runs such ensembles as separate jobs (G_SimulationArgs.jl,
  * submit*.sh); there is no ensemble type to mirror. */
 typedef struct kb_ens kb_ens;

@@ -1470,10 +1470,20 @@ struct kb_ens {
   hipEvent_t fork = nullptr;
   StepState** stptr_dev = nullptr;
   int* N_dev = nullptr;
+  // memory-integral argument tables, runs ordered by lane: lane j owns runs
+  // [lrun[j], lrun[j+1]) and lwg[j] workgroups; wg_dev holds each run's
+  // first workgroup over all lanes (one pooled launch), lwg_dev within its
+  // lane (one launch per lane)
   MemintArgs* margs_dev = nullptr;
   int* wg_dev = nullptr;
+  int* lwg_dev = nullptr;
+  std::vector<int> lrun, lwg, lmaxN;
   int total_wg = 0;
   int maxN = 0;
+  // a memory integral per lane after the lane's stages (every member in
+  // orbit storage: its stage is matrix products that the other lanes'
+  // integrals overlap), else one pooled launch after all stages
+  bool lane_integrals = false;
   int tile = 0;     // shape of the pooled memory-integral launch (kMemintVariants)
   int last_t = 0;   // t of the last ensemble step (0: none yet)
   bool use_graph = false;
@@ -1513,7 +1523,7 @@ void ens_free(kb_ens* e) {
       kb_destroy(c);
     }
   }
-  for (void* p : {(void*)e->stptr_dev, (void*)e->N_dev, (void*)e->margs_dev, (void*)e->wg_dev})
+  for (void* p : {(void*)e->stptr_dev, (void*)e->N_dev, (void*)e->margs_dev, (void*)e->wg_dev, (void*)e->lwg_dev})
     if (p) (void)hipFree(p);
   for (auto& ev : e->pending) {
     (void)hipEventDestroy(ev.a);
@@ -1573,16 +1583,29 @@ int ens_enqueue(kb_ens* e) {
     for (size_t j = 1; j < e->side.size() && err == hipSuccess; ++j) err = hipStreamWaitEvent(e->side[j], e->fork, 0);
     if (err != hipSuccess) return ens_err(e, KB_ERR_HIP, "fork: %s", hipGetErrorString(err));
   }
+  // per lane: its members' self-energy and Hartree kernels, then (with
+  // lane_integrals) the memory integral over its members
   int rc = KB_OK;
-  for (size_t i = 0; i < e->m.size() && rc == KB_OK; ++i) {
-    kb_ctx* c = e->m[i];
-    if (forked) c->stream = e->side[e->lane[i]];
-    if (c->p.pexp == KB_PEXP_LAMBDA_NLO && (rc = launch_sigma(c)))
-      rc = ens_err(e, rc, "member self-energy: %s", c->err.c_str());
-    else if (c->p.pexp == KB_PEXP_NINVERSE_NLO && (rc = launch_ninv_stage(c)))
-      rc = ens_err(e, rc, "member 1/N self-energy: %s", c->err.c_str());
-    else if ((rc = launch_hartree_step(c)))
-      rc = ens_err(e, rc, "member Hartree: %s", c->err.c_str());
+  const int nl = (int)e->lwg.size();
+  for (int j = 0; j < nl && rc == KB_OK; ++j) {
+    hipStream_t sj = forked ? e->side[j] : e->stream;
+    for (size_t i = 0; i < e->m.size() && rc == KB_OK; ++i) {
+      if (e->lane[i] != j) continue;
+      kb_ctx* c = e->m[i];
+      c->stream = sj;
+      if (c->p.pexp == KB_PEXP_LAMBDA_NLO && (rc = launch_sigma(c)))
+        rc = ens_err(e, rc, "member self-energy: %s", c->err.c_str());
+      else if (c->p.pexp == KB_PEXP_NINVERSE_NLO && (rc = launch_ninv_stage(c)))
+        rc = ens_err(e, rc, "member 1/N self-energy: %s", c->err.c_str());
+      else if ((rc = launch_hartree_step(c)))
+        rc = ens_err(e, rc, "member Hartree: %s", c->err.c_str());
+    }
+    if (rc == KB_OK && e->lane_integrals && e->lwg[j] > 0) {
+      // one shape for all members (their item tables follow its row height)
+      err = launch_memint(e->tile, e->lwg[j], memint_lds(e->lmaxN[j]), sj, e->margs_dev + e->lrun[j],
+                          e->lwg_dev + e->lrun[j], e->lrun[j + 1] - e->lrun[j]);
+      if (err != hipSuccess) rc = ens_err(e, KB_ERR_HIP, "memint launch failed: %s", hipGetErrorString(err));
+    }
   }
   for (auto* c : e->m) c->stream = e->stream;
   if (forked) {
@@ -1594,24 +1617,60 @@ int ens_enqueue(kb_ens* e) {
     }
   }
   if (rc) return rc;
-  // one shape for all members (their item tables follow its row height)
-  err = launch_memint(e->tile, e->total_wg, memint_lds(e->maxN), e->stream, e->margs_dev, e->wg_dev, nr);
-  if (err != hipSuccess) return ens_err(e, KB_ERR_HIP, "memint launch failed: %s", hipGetErrorString(err));
+  if (!e->lane_integrals) {
+    err = launch_memint(e->tile, e->total_wg, memint_lds(e->maxN), e->stream, e->margs_dev, e->wg_dev, nr);
+    if (err != hipSuccess) return ens_err(e, KB_ERR_HIP, "memint launch failed: %s", hipGetErrorString(err));
+  }
   return KB_OK;
 }
 
-// the pooled launch's shape: every member's work items are cut to its row
-// height (a member's tile selects its item table)
+// the memory-integral argument tables in lane order; a member's tile
+// selects its item table
+int ens_upload_tables(kb_ens* e) {
+  const int nl = std::max(1, (int)e->side.size());
+  std::vector<MemintArgs> ma;
+  std::vector<int> wg, lwg;
+  e->lrun.assign(nl + 1, 0);
+  e->lwg.assign(nl, 0);
+  e->lmaxN.assign(nl, 0);
+  e->maxN = 0;
+  int total = 0;
+  for (int j = 0; j < nl; ++j) {
+    e->lrun[j] = (int)ma.size();
+    for (size_t i = 0; i < e->m.size(); ++i) {
+      const kb_ctx* c = e->m[i];
+      if (e->lane[i] != j) continue;
+      ma.push_back(memint_args(c));
+      wg.push_back(total);
+      lwg.push_back(e->lwg[j]);
+      total += c->NB;
+      e->lwg[j] += c->NB;
+      e->lmaxN[j] = std::max(e->lmaxN[j], c->N);
+      e->maxN = std::max(e->maxN, c->N);
+    }
+  }
+  e->lrun[nl] = (int)ma.size();
+  e->total_wg = total;
+  const size_t n = ma.size();
+  if (!e->margs_dev && (hipMalloc((void**)&e->margs_dev, sizeof(MemintArgs) * n) != hipSuccess ||
+                        hipMalloc((void**)&e->wg_dev, sizeof(int) * n) != hipSuccess ||
+                        hipMalloc((void**)&e->lwg_dev, sizeof(int) * n) != hipSuccess))
+    return ens_err(e, KB_ERR_OOM, "argument tables: out of device memory");
+  if (hipMemcpy(e->margs_dev, ma.data(), sizeof(MemintArgs) * n, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->wg_dev, wg.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->lwg_dev, lwg.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess)
+    return ens_err(e, KB_ERR_HIP, "argument tables: copy failed");
+  return KB_OK;
+}
+
+// the memory-integral shape, one for all members: every member's work items
+// are cut to its row height
 int ens_set_tile(kb_ens* e, int v) {
   hipError_t err = hipStreamSynchronize(e->stream);
   if (err != hipSuccess) return ens_err(e, KB_ERR_HIP, "tiles: %s", hipGetErrorString(err));
-  std::vector<MemintArgs> ma;
-  for (auto* c : e->m) {
-    c->tile = v;
-    ma.push_back(memint_args(c));
-  }
-  err = hipMemcpy(e->margs_dev, ma.data(), sizeof(MemintArgs) * ma.size(), hipMemcpyHostToDevice);
-  if (err != hipSuccess) return ens_err(e, KB_ERR_HIP, "tiles: %s", hipGetErrorString(err));
+  for (auto* c : e->m) c->tile = v;
+  int rc = ens_upload_tables(e);
+  if (rc) return rc;
   e->tile = v;
   ens_drop_graph(e);
   return KB_OK;
@@ -1710,31 +1769,23 @@ int kb_ens_create_sym(const kb_params* params, const int* symmetric, int nruns, 
   if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) return fail(KB_ERR_HIP);
   if (ens_make_lanes(e) != KB_OK) return fail(KB_ERR_HIP);
   std::vector<StepState*> sp;
-  std::vector<int> Ns, wg;
-  std::vector<MemintArgs> ma;
-  int total = 0;
+  std::vector<int> Ns;
   for (auto* c : e->m) {
     (void)hipStreamSynchronize(c->stream);
     c->stream = e->stream;
     c->in_ensemble = true;
     sp.push_back(c->st_dev);
     Ns.push_back(c->N);
-    wg.push_back(total);
-    total += c->NB;
-    ma.push_back(memint_args(c));
-    if (c->N > e->maxN) e->maxN = c->N;
   }
-  e->total_wg = total;
   if (hipMalloc((void**)&e->stptr_dev, sizeof(StepState*) * nruns) != hipSuccess ||
-      hipMalloc((void**)&e->N_dev, sizeof(int) * nruns) != hipSuccess ||
-      hipMalloc((void**)&e->margs_dev, sizeof(MemintArgs) * nruns) != hipSuccess ||
-      hipMalloc((void**)&e->wg_dev, sizeof(int) * nruns) != hipSuccess)
+      hipMalloc((void**)&e->N_dev, sizeof(int) * nruns) != hipSuccess)
     return fail(KB_ERR_OOM);
   if (hipMemcpy(e->stptr_dev, sp.data(), sizeof(StepState*) * nruns, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(e->N_dev, Ns.data(), sizeof(int) * nruns, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(e->margs_dev, ma.data(), sizeof(MemintArgs) * nruns, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(e->wg_dev, wg.data(), sizeof(int) * nruns, hipMemcpyHostToDevice) != hipSuccess)
+      hipMemcpy(e->N_dev, Ns.data(), sizeof(int) * nruns, hipMemcpyHostToDevice) != hipSuccess)
     return fail(KB_ERR_HIP);
+  if (int rc = ens_upload_tables(e)) return fail(rc);
+  e->lane_integrals = e->side.size() > 1;
+  for (auto* c : e->m) e->lane_integrals = e->lane_integrals && c->sym;
   *out = e;
   return KB_OK;
 }
