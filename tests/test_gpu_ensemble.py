@@ -232,3 +232,42 @@ def test_ensemble_tiles_and_autotune():
         mo, mg = o.measure(t - 1), m.measure(t - 1)
         assert abs(mg["E"] - mo["E"]) <= RTOL * abs(mo["E"])
     ens.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_all_orbit_ensemble_with_1n_members(graph):
+    """Every member in orbit storage (each lane runs its own memory integral
+    after its stages), two of them in the 1/N expansion (split resummation
+    sweeps on a side stream): each member follows a standalone context."""
+    from qftkb import KBContext, KBEnsemble
+    print(f"\n[case] all-orbit ensemble with 1/N members, graph={graph}", flush=True)
+    runs = [
+        dict(Nx=8, sdim=3, NstepsinMemory=9, dt=0.05, Lambda=2.0, ONgroup=4, Mass=1.0, Mass2=1.0, pexp=3,
+             symmetric=True),
+        dict(Nx=6, sdim=3, NstepsinMemory=9, dt=0.1, Lambda=1.6, ONgroup=1, Mass=1.0, Mass2=1.0, symmetric=True),
+        dict(Nx=9, sdim=2, NstepsinMemory=7, dt=0.05, Lambda=1.5, ONgroup=3, Mass=1.0, Mass2=1.0, pexp=3,
+             symmetric=True),
+        dict(Nx=10, sdim=3, NstepsinMemory=5, dt=0.1, Lambda=0.9, ONgroup=1, Mass=1.0, Mass2=1.0, symmetric=True),
+        dict(Nx=4, sdim=3, NstepsinMemory=6, dt=0.1, Lambda=1.2, ONgroup=2, Mass=1.0, Mass2=1.0, symmetric=True),
+    ]
+    ens = KBEnsemble(runs)
+    ens.use_graph(graph)
+    alone = []
+    for r, m in zip(runs, ens.members):
+        c = KBContext(**r)
+        n = thermal(c.kL2(), m=r["Mass"])
+        m.init_particles(n, r["Mass"])
+        c.init_particles(n, r["Mass"])
+        alone.append(c)
+    for t in range(3, 22):
+        ens.step(t)
+        for c in alone:
+            c.step(t)
+        if t % 6 == 0:
+            for c, m in zip(alone, ens.members):
+                mg, mc = m.measure(t), c.measure(t)
+                assert abs(mg["E"] - mc["E"]) <= 1e-12 * abs(mc["E"])
+    for c, m in zip(alone, ens.members):
+        check_window(c, m, tol=1e-12)
+        c.close()
+    ens.close()
