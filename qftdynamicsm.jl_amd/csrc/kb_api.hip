@@ -396,18 +396,15 @@ int launch_hartree_step(kb_ctx* c) {
 }
 
 // Split memory integral (orbit storage): partial sweeps over item ranges,
-// then the sums and the leapfrog.  The split kernel's item height must be
-// the shape's: its item table is the one in margs_dev.
+// then the sums and the leapfrog, in the context's tile shape (the item
+// table in margs_dev has that shape's row height).
 int launch_memint_split(kb_ctx* c) {
-  const int rows = kMemintVariants[c->tile].rows;
-  const void* fn = rows == 32    ? (const void*)memint_partial<32, 2, 256>
-                   : rows == 128 ? (const void*)memint_partial<128, 2, 256>
-                                 : (const void*)memint_partial<64, 2, 256>;
+  const MemintVariant& v = kMemintVariants[c->tile];
   const MemintArgs* margs = c->margs_dev;
   int ns = c->msplit;
   double* part = c->mpart;
   void* kargs[] = {(void*)&margs, (void*)&ns, (void*)&part};
-  HIPCHK(c, hipLaunchKernel(fn, dim3(c->NB * c->msplit), dim3(256), kargs, memint_lds(c->N), c->stream));
+  HIPCHK(c, hipLaunchKernel(v.partial, dim3(c->NB * c->msplit), dim3(v.threads), kargs, memint_lds(c->N), c->stream));
   hipLaunchKernelGGL(memint_finish, dim3(c->NB), dim3(MEMINT_FINISH_THREADS), memint_lds(c->N), c->stream,
                      c->margs_dev, c->msplit, c->mpart);
   LAUNCHCHK(c);
@@ -747,12 +744,11 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
     // largest supported window, not this context's
     const size_t lds = memint_lds(640);
     hipError_t e = hipSuccess;
-    for (int v = 0; v < NMEMINT_VARIANTS && e == hipSuccess; ++v)
+    for (int v = 0; v < NMEMINT_VARIANTS && e == hipSuccess; ++v) {
       e = hipFuncSetAttribute(kMemintVariants[v].fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    const void* partials[] = {(const void*)memint_partial<32, 2, 256>, (const void*)memint_partial<64, 2, 256>,
-                              (const void*)memint_partial<128, 2, 256>};
-    for (const void* f : partials)
-      if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e == hipSuccess)
+        e = hipFuncSetAttribute(kMemintVariants[v].partial, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    }
     if (e == hipSuccess)
       e = hipFuncSetAttribute((const void*)memint_finish, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) {

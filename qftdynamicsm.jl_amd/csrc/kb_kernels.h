@@ -193,11 +193,12 @@ __global__ void memint_partial(const MemintArgs* runs, int nsplit, double* part)
 __global__ void memint_finish(const MemintArgs* runs, int nsplit, const double* part);
 
 // memory-integral tile shapes: ROWS slot rows per work item, B rows per
-// batch (two batches in flight), threads per workgroup (one workgroup per
-// site block); fn is the kernel to launch
+// batch (two batches in flight), threads per workgroup; fn is the kernel
+// with one workgroup per site block, partial the split form's partial sweep
 struct MemintVariant {
   int rows, batch, threads;
   const void* fn;
+  const void* partial;
 };
 constexpr int NMEMINT_VARIANTS = 9;
 extern const MemintVariant kMemintVariants[NMEMINT_VARIANTS];

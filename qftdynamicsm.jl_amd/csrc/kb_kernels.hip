@@ -672,20 +672,17 @@ __global__ __launch_bounds__(MEMINT_FINISH_THREADS) void memint_finish(const Mem
   memint_leapfrog(a, ss, mb, accF, accR, red, diagF);
 }
 
-// one instantiation per item height of the tile shapes (the item table in
-// MemintArgs follows the context's shape)
-template __global__ void memint_partial<32, 2, 256>(const MemintArgs*, int, double*);
-template __global__ void memint_partial<64, 2, 256>(const MemintArgs*, int, double*);
-template __global__ void memint_partial<128, 2, 256>(const MemintArgs*, int, double*);
-
-// the tile shapes the autotuner chooses from; entry 0 is the default
+// the tile shapes the autotuner chooses from, as the whole-block kernel and
+// the split form's partial kernel (the item table in MemintArgs follows the
+// context's shape); entry 0 is the default
+#define KB_MEMINT_SHAPE(R, B, T) \
+  { R, B, T, (const void*)memint_step<R, B, T>, (const void*)memint_partial<R, B, T> }
 const MemintVariant kMemintVariants[NMEMINT_VARIANTS] = {
-    {64, 2, 256, (const void*)memint_step<64, 2, 256>},   {32, 4, 256, (const void*)memint_step<32, 4, 256>},
-    {128, 4, 256, (const void*)memint_step<128, 4, 256>}, {64, 4, 256, (const void*)memint_step<64, 4, 256>},
-    {128, 2, 256, (const void*)memint_step<128, 2, 256>}, {32, 2, 256, (const void*)memint_step<32, 2, 256>},
-    {64, 4, 512, (const void*)memint_step<64, 4, 512>},   {128, 2, 512, (const void*)memint_step<128, 2, 512>},
-    {32, 4, 512, (const void*)memint_step<32, 4, 512>},
+    KB_MEMINT_SHAPE(64, 2, 256), KB_MEMINT_SHAPE(32, 4, 256),  KB_MEMINT_SHAPE(128, 4, 256),
+    KB_MEMINT_SHAPE(64, 4, 256), KB_MEMINT_SHAPE(128, 2, 256), KB_MEMINT_SHAPE(32, 2, 256),
+    KB_MEMINT_SHAPE(64, 4, 512), KB_MEMINT_SHAPE(128, 2, 512), KB_MEMINT_SHAPE(32, 4, 512),
 };
+#undef KB_MEMINT_SHAPE
 
 // ---------------------------------------------------------------------------
 // Measurement (createMeasurement / getE_kin / getE_pot / getE_mem)
