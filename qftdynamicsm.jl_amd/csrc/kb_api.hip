@@ -1566,8 +1566,10 @@ int ens_make_lanes(kb_ens* e) {
   return KB_OK;
 }
 
-// one ensemble step: advance all windows, each member's self-energy and
-// Hartree kernels, then one memory-integral launch over all members
+// one ensemble step: advance all windows; each member's self-energy and
+// Hartree kernels on its lane; the memory integrals of all members in one
+// launch after the lanes have joined, or (lane_integrals) one per lane
+// after that lane's stages
 int ens_enqueue(kb_ens* e) {
   const int nr = (int)e->m.size();
   hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64 * ((nr + 63) / 64)), 0, e->stream, e->stptr_dev, e->N_dev, nr);
