@@ -20,6 +20,11 @@ mistake:
     the C oracle restates the CPUfull back-end (a transform pair per term).
 
 pexp: 0 LambdaLO, 1 LambdaNLO, 2 NinverseLO, 3 NinverseNLO.
+
+Negative controls: `plant` names one deliberate mistake in the memory
+integrals or the memory energy (see PLANTS), so that tests can show that a
+check fails on a wrong restatement.  It is None (no mistake) unless a test
+sets it.
 """
 from __future__ import annotations
 
@@ -37,7 +42,22 @@ def kL2_table(Nx: int, sdim: int) -> np.ndarray:
     return (k1[None, None, :] + k1[None, :, None] + k1[:, None, None]).reshape(-1)
 
 
+# deliberate mistakes a test may plant (each one term of getRHS_F!,
+# getRHS_r! or getE_mem changed)
+PLANTS = (
+    "RF_sigr_sign",   # R_F: +w Sigr(z) F(z,t') instead of -w Sigr(z) F(z,t')
+    "RF_sigf_sign",   # R_F: -w SigF(z) rho(z,t') instead of +
+    "RF_tmin_weight",  # R_F: weight 1 instead of 1/2 at tmin
+    "Rr_sign",        # R_rho: + instead of -
+    "Rr_upper",       # R_rho: the sum stops at t-3 instead of t-2
+    "Emem_sign",      # E_mem with the opposite sign
+    "Emem_tmin_weight",  # E_mem: dt/4 instead of dt/8 at tmin
+)
+
+
 class NumpyKB:
+    plant = None
+
     def __init__(self, Nx, sdim, Nmem, dt, lam, ON, Mass, Mass2, pexp=1):
         self.Nx, self.sdim, self.N = Nx, sdim, Nmem
         self.V = Nx ** sdim
@@ -186,14 +206,17 @@ class NumpyKB:
             RF = np.zeros(self.V)
             Rr = np.zeros(self.V)
             if self.nlo:
-                for z in range(tp + 1, t - 1):
+                pl = self.plant
+                for z in range(tp + 1, t - 2 if pl == "Rr_upper" else t - 1):
                     Rr -= self.Sigr[z] * self.getr(z, tp)
                 for z in range(tmin, t - 1):
-                    w = 0.5 if z == tmin else 1.0
-                    RF -= w * self.Sigr[z] * self.getF(z, tp)
+                    w = 0.5 if z == tmin and pl != "RF_tmin_weight" else 1.0
+                    RF -= w * self.Sigr[z] * self.getF(z, tp) * (-1.0 if pl == "RF_sigr_sign" else 1.0)
                 for z in range(tmin, tp):
-                    w = 0.5 if z == tmin else 1.0
-                    RF += w * self.SigF[z] * self.getr(z, tp)
+                    w = 0.5 if z == tmin and pl != "RF_tmin_weight" else 1.0
+                    RF += w * self.SigF[z] * self.getr(z, tp) * (-1.0 if pl == "RF_sigf_sign" else 1.0)
+                if pl == "Rr_sign":
+                    Rr = -Rr
             if tp == t - 1:
                 newr[tp] = np.full(self.V, self.dt)
             else:
@@ -205,12 +228,13 @@ class NumpyKB:
         self.setr(t, t, np.zeros(self.V))
         RF = np.zeros(self.V)
         if self.nlo:
+            pl = self.plant
             for z in range(tmin, t - 1):
-                w = 0.5 if z == tmin else 1.0
-                RF -= w * self.Sigr[z] * self.getF(z, t)
+                w = 0.5 if z == tmin and pl != "RF_tmin_weight" else 1.0
+                RF -= w * self.Sigr[z] * self.getF(z, t) * (-1.0 if pl == "RF_sigr_sign" else 1.0)
             for z in range(tmin, t):
-                w = 0.5 if z == tmin else 1.0
-                RF += w * self.SigF[z] * self.getr(z, t)
+                w = 0.5 if z == tmin and pl != "RF_tmin_weight" else 1.0
+                RF += w * self.SigF[z] * self.getr(z, t) * (-1.0 if pl == "RF_sigf_sign" else 1.0)
         self.setF(t, t, (2 - Om) * self.getF(t - 1, t) - self.getF(t - 2, t) + dt3 * RF)
 
     def measure(self, t):
@@ -228,10 +252,13 @@ class NumpyKB:
         Emem = 0.0
         if t != 2 and self.nlo:
             tmin = self.tmin
-            Emem = 0.125 * dt * (self.Sigr[tmin] * self.getF(tmin, t - 1)
-                                 - self.SigF[tmin] * self.getr(tmin, t - 1)).sum()
+            w0 = 0.25 if self.plant == "Emem_tmin_weight" else 0.125
+            Emem = w0 * dt * (self.Sigr[tmin] * self.getF(tmin, t - 1)
+                              - self.SigF[tmin] * self.getr(tmin, t - 1)).sum()
             for tp in range(tmin + 1, t - 1):
                 Emem += 0.25 * dt * (self.Sigr[tp] * self.getF(tp, t - 1)
                                      - self.SigF[tp] * self.getr(tp, t - 1)).sum()
+            if self.plant == "Emem_sign":
+                Emem = -Emem
         return dict(time=(t - 1) * dt * self.Mass, E=Ekin + Epot + Emem, E_kin=Ekin, E_pot=Epot,
                     E_mem=Emem, M2=self.hartreemass2, n=n, ddF=ddF, F=F, omega=om, c=c)

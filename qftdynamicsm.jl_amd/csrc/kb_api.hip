@@ -1233,15 +1233,18 @@ int kb_load_window(kb_ctx* c, const void* buf, unsigned long long bytes) {
   if (!same_params(h.p, c->p) || h.sym != (c->sym ? 1 : 0) || h.V != c->V || h.N != c->N || h.NP != c->NP ||
       h.hcount != hc || h.scount != sc || h.kcount != kc)
     return seterr(c, KB_ERR_ARG, "load: snapshot taken with other parameters or storage");
-  if (h.st.tmax < 2 || h.st.tmax - h.st.tmin + 1 > c->N)
-    return seterr(c, KB_ERR_ARG, "load: snapshot window [%d,%d] is not valid", h.st.tmin, h.st.tmax);
+  // the window after step tmax is [max(1, tmax - N + 1), tmax]; any other
+  // header would make the next step sum slots outside the window
+  if (h.st.tmax < 2 || h.st.tmin != (h.st.tmax > c->N ? h.st.tmax - c->N + 1 : 1))
+    return seterr(c, KB_ERR_ARG, "load: snapshot window [%d,%d] is not a window of N_mem = %d", h.st.tmin,
+                  h.st.tmax, c->N);
   const double* d = reinterpret_cast<const double*>(static_cast<const char*>(buf) + sizeof(h));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemcpy(c->H, d, sizeof(double) * hc, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->S, d + hc, sizeof(double) * sc, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->kL2, d + hc + sc, sizeof(double) * kc, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->m2, d + hc + sc + kc, sizeof(double) * WIN_M2, hipMemcpyHostToDevice));
-  c->st = h.st;
+  c->st = state_of_step(h.st.tmax, c->N);
   int rc = upload_state(c);
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -51,8 +51,9 @@ def test_graph_replay_matches_oracle():
     check_window(o, g)
 
 
-def test_graph_and_eager_agree_bitwise_at_32cubed():
-    """At the benchmark lattice: the same 6 steps eager and from a graph."""
+def test_graph_and_eager_agree_at_32cubed():
+    """At the benchmark lattice: the same 6 steps eager and from a graph agree
+    to rounding (the memory integral's accumulation order is not fixed)."""
     from qftkb import KBContext
     print("\n[case] graph vs eager 32^3 N_mem=16", flush=True)
     ctxs = []

@@ -156,6 +156,110 @@ def test_energy_drift_small_lo():
     assert abs(E[-1] - E[len(E) // 2]) <= 1e-2 * abs(E[0])
 
 
+# ---------------------------------------------------------------------------
+# Physics pins of the NLO memory integrals (getRHS_F!, getRHS_r!,
+# TwoPIScalar_Evolution.jl:766-782, 846-854) and of the memory energy
+# (getE_mem, TwoPIScalar_Measurement.jl:161-174).  The 2PI dynamics conserve
+# the total energy E = E_kin + E_pot + E_mem; with full memory the measured
+# drift must shrink as dt does.  At lambda = 10 the memory terms dominate the
+# drift, so a wrong sign or a dropped term in R_F or E_mem leaves an O(1)
+# drift that does not shrink.  R_rho barely moves the energy; what fixes its
+# sign is causality: the retarded (spectral) function of an interacting
+# theory is damped, so |rho(t, t')| decays in t - t'; the opposite sign
+# anti-damps it.  The end weights of the quadratures (1/2 at tmin, the upper
+# limits) are O(dt) choices that no continuum property fixes; they are read
+# from the reference lines cited above.
+ECONS_DTS = (0.1, 0.05, 0.025)
+
+
+def energy_drift(ctx_factory, dt, tphys=2.0):
+    """max_t |E(t) - E(2)| / |E(2)| of a full-memory thermal run up to
+    physical time tphys."""
+    steps = int(round(tphys / dt))
+    o = ctx_factory(steps + 2, dt)
+    o.init_particles(thermal(o.kL2() if callable(o.kL2) else o.kL2), 1.0)
+    E0 = o.measure(2)["E"]
+    worst = 0.0
+    for t in range(3, steps + 3):
+        o.evolve(t)
+        worst = max(worst, abs(o.measure(t)["E"] - E0) / abs(E0))
+    return worst
+
+
+def energy_converges(d):
+    """The pass criterion of the energy test on drifts at dt = 0.1, 0.05,
+    0.025: second order at first, at least 2.4x per halving after, and below
+    0.8% at dt = 0.025 (correct code: 3.0-3.1x, 2.5-2.6x, 5e-3 to 6e-3)."""
+    return d[0] / d[1] >= 2.8 and d[1] / d[2] >= 2.4 and d[2] <= 8e-3
+
+
+ECONS_CASES = [(4, 3, 1), (6, 2, 4)]  # Nx, sdim, O(N)
+
+
+@pytest.mark.parametrize("Nx,sdim,ON", ECONS_CASES)
+def test_energy_conservation_nlo(Nx, sdim, ON):
+    lam = 10.0
+    fac = lambda N, dt: Oracle(Nx, sdim, N, dt, lam, ON, 1.0, 1.0, nthreads=2)  # noqa: E731
+    d = [energy_drift(fac, dt) for dt in ECONS_DTS]
+    assert energy_converges(d), f"energy drift {d}"
+    # the independent numpy restatement conserves it the same way
+    twin = lambda N, dt: NumpyKB(Nx, sdim, N, dt, lam, ON, 1.0, 1.0)  # noqa: E731
+    dn = [energy_drift(twin, dt) for dt in ECONS_DTS]
+    assert np.allclose(dn, d, rtol=1e-8), (dn, d)
+
+
+@pytest.mark.parametrize("plant", ["RF_sigr_sign", "RF_sigf_sign", "Emem_sign"])
+def test_energy_conservation_fails_on_planted_mistakes(plant):
+    """Negative control: one wrong sign in R_F or E_mem, planted in the numpy
+    restatement, must fail the criterion of the energy test."""
+    def fac(N, dt):
+        o = NumpyKB(4, 3, N, dt, 10.0, 1, 1.0, 1.0)
+        o.plant = plant
+        return o
+    d = [energy_drift(fac, dt) for dt in ECONS_DTS]
+    assert not energy_converges(d), f"planted {plant} passed: {d}"
+
+
+def rho_envelope(ctx, tend, tp=2, parts=4):
+    """max_p |rho(t, tp)| over `parts` consecutive ranges of t in [tp, tend]."""
+    get = (lambda a, b: ctx.get("r", a, b)) if hasattr(ctx, "get") else ctx.getr
+    r = np.array([get(t, tp) for t in range(tp, tend + 1)])
+    q = len(r) // parts
+    return [float(np.max(np.abs(r[i * q:(i + 1) * q]))) for i in range(parts)]
+
+
+def spectral_damped(env):
+    return all(env[i + 1] < env[i] for i in range(len(env) - 1)) and env[-1] <= 0.85 * env[0]
+
+
+def _damping_run(o, steps):
+    o.init_particles(thermal(o.kL2() if callable(o.kL2) else o.kL2), 1.0)
+    for t in range(3, steps + 3):
+        o.evolve(t)
+    return rho_envelope(o, steps + 2)
+
+
+@pytest.mark.parametrize("Nx,sdim,ON", ECONS_CASES)
+def test_spectral_function_is_damped(Nx, sdim, ON):
+    """lambda = 10, full memory, t up to 15: the envelope of rho(t, 2) falls
+    over each quarter of the run (correct code: 0.63 -> 0.43 on 4^3, 0.69 ->
+    0.54 for O(4) on 6^2)."""
+    steps = 150
+    env = _damping_run(Oracle(Nx, sdim, steps + 2, 0.1, 10.0, ON, 1.0, 1.0, nthreads=2), steps)
+    assert spectral_damped(env), env
+
+
+def test_spectral_damping_fails_on_planted_sign():
+    """Negative control: R_rho with the opposite sign anti-damps rho (its
+    envelope grows, 0.60 -> 0.88)."""
+    steps = 100
+    o = NumpyKB(4, 3, steps + 2, 0.1, 10.0, 1, 1.0, 1.0)
+    o.plant = "Rr_sign"
+    env = _damping_run(o, steps)
+    assert not spectral_damped(env), env
+    assert env[-1] > env[0]
+
+
 def test_invariants_and_cubic_symmetry():
     Nx = 6
     o = Oracle(Nx, 3, 5, 0.1, 1.5, 1, 1.0, 1.0, nthreads=2)
