@@ -83,7 +83,9 @@ int kb_create(const kb_params *p, kb_ctx **out);
 int kb_destroy(kb_ctx *ctx);
 const char *kb_last_error(const kb_ctx *ctx);
 
-/* Number of lattice sites V and bytes of device memory held. */
+/* Number of lattice sites V and bytes of device memory held.  (kb_last_error
+ * with a NULL context returns the calling thread's last message of a
+ * context-free call, kb_sunset below.) */
 int kb_info(const kb_ctx *ctx, long long *V, long long *device_bytes);
 
 /* Cubic-orbit storage: the same KB step with one stored value per orbit of
@@ -287,6 +289,28 @@ int kb_adj_stop(kb_ctx *ctx);
  * re-initialised afterwards. */
 int kb_adj_debug_stop(kb_ctx *ctx, int t_stop);
 int kb_adj_get_entry(kb_ctx *ctx, int which, int i, int j, double *out);
+
+/* ---- self-energies of caller lattices ----------------------------------------
+ * calcSelfEnergyattp! (TwoPIScalar_Evolution.jl:25-31, LambdaNLOquantum) on
+ * lattices that the caller holds in device memory, without a context: for
+ * each of nrows rows q, Sigr(q) and SigF(q) of the pair F(q), rho(q) (rho
+ * physical), by the plane kernels of the KB step with the same twiddles, so
+ * a row gives the same bits as kb_step's self-energy stage.  Rows are V =
+ * Nx^sdim doubles at q*V, column-major sites; F, rho, SigF and Sigr are
+ * device pointers on p->device.  Only Nx, sdim, ONgroup, Lambda, pexp
+ * (KB_PEXP_LAMBDA_NLO) and device of *p are read; Nx <= 64, 1 <= nrows <=
+ * 65535.  The outputs may not overlap the inputs or each other.
+ *
+ * The workspace is caller-owned device memory of kb_sunset_workspace_bytes
+ * (4 KiB of twiddles, plus nrows * V complex values in 3D), 256-byte
+ * aligned.  Kernels are enqueued on `stream` (a hipStream_t; NULL = the
+ * device's null stream) and the call returns without waiting.  Errors are
+ * reported by the return code; kb_last_error(NULL) gives the calling
+ * thread's last message of a context-free call.  Calls that share a
+ * workspace must not overlap. */
+int kb_sunset_workspace_bytes(const kb_params *p, int nrows, unsigned long long *bytes);
+int kb_sunset(const kb_params *p, int nrows, const double *F, const double *rho, double *SigF,
+              double *Sigr, void *workspace, unsigned long long workspace_bytes, void *stream);
 
 /* ---- ensembles ------------------------------------------------------------
  * Many independent KB runs (uneven lattices, couplings, memory windows)

@@ -54,6 +54,13 @@ void plan_fft(FFTPlan& pl, int n) {
     while (m % f == 0) { pl.fac[pl.nf++] = f; m /= f; }
 }
 
+void fft_twiddles(int n, double2* out) {
+  for (int k = 0; k < n; ++k) {
+    long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
+    out[k] = make_double2((double)cosl(a), (double)sinl(a));
+  }
+}
+
 hipEvent_t take_event(kb_ctx* c) {
   if (!c->freeEv.empty()) {
     hipEvent_t e = c->freeEv.back();
@@ -541,7 +548,7 @@ void cubic_orbits(int Nx, int sdim, std::vector<int>& orbit_of, std::vector<int>
 
 extern "C" {
 
-const char* kb_last_error(const kb_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* kb_last_error(const kb_ctx* ctx) { return ctx ? ctx->err.c_str() : free_call_error(); }
 
 namespace {
 
@@ -807,10 +814,7 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
   }
   // twiddles exp(-2 pi i k / Nx) and the default k^2 table
   std::vector<double2> tw(p->Nx);
-  for (int k = 0; k < p->Nx; ++k) {
-    long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)p->Nx;
-    tw[k] = make_double2((double)cosl(a), (double)sinl(a));
-  }
+  fft_twiddles(p->Nx, tw.data());
   std::vector<double> k2(c->Vpad, 0.0);
   for (int i = 0; i < c->V; ++i) {
     int rem = sym ? c->rep_site[i] : i;

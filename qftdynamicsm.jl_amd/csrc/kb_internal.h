@@ -133,6 +133,12 @@ struct kb_ctx {
 namespace qftkb_host {
 
 int seterr(kb_ctx* c, int code, const char* fmt, ...);
+// FFT factorisation and the twiddles exp(-2 pi i k / n), k < n (long double
+// on the host), shared by contexts and the context-free entry points
+void plan_fft(qftkb::FFTPlan& pl, int n);
+void fft_twiddles(int n, double2* out);
+// message of the calling thread's last failed context-free call
+const char* free_call_error();
 int upload(kb_ctx* c, double* dst, const double* src, size_t n);
 int download(kb_ctx* c, double* dst, const double* src, size_t n);
 // host lattices (Nx^sdim sites) to / from stored sites: identity in full

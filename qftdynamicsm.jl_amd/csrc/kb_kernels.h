@@ -76,7 +76,9 @@ __host__ __device__ inline StepState state_of_step(int t, int N) {
 }
 
 // modes of the plane kernels (kb_sigma.hip)
-enum SigmaMode { SG_SUNSET = 0, SG_SUNSET_ADJ = 1, SG_BUBBLE = 2, SG_CHAIN = 3 };
+// SG_PLAIN is the sunset of caller-supplied lattices: rows q < rows of the
+// physical F and rho in Fin/Rin (q * V + site), Sigma into SigFout/Sigrout
+enum SigmaMode { SG_SUNSET = 0, SG_SUNSET_ADJ = 1, SG_BUBBLE = 2, SG_CHAIN = 3, SG_PLAIN = 4 };
 
 struct SigmaArgs {
   const double* H;
@@ -104,6 +106,12 @@ struct SigmaArgs {
   const double* Iring;
   long PNP;  // slot pairs per site block of the bubble triangles
   double ivol;
+  // SG_PLAIN: input and output lattices, and their number
+  const double* Fin;
+  const double* Rin;
+  double* SigFout;
+  double* Sigrout;
+  int rows;
 };
 
 // The resummation sweeps of the 1/N expansion (kb_ninverse.hip): one
@@ -227,6 +235,8 @@ constexpr int NFFT_VARIANTS = 3;
 extern const int kFFTThreads[NFFT_VARIANTS];
 hipError_t launch_sigma_planes(const SigmaArgs& a, int rows, hipStream_t st, int variant = 0);
 hipError_t launch_sigma_planes_adjoint(const SigmaArgs& a, int rows, hipStream_t st);
+// the sunset of a.rows caller lattices (SG_PLAIN), default block size
+hipError_t launch_sigma_planes_plain(const SigmaArgs& a, hipStream_t st);
 hipError_t launch_bubble_planes(const SigmaArgs& a, int rows, hipStream_t st);
 hipError_t launch_chain_planes(const SigmaArgs& a, int rows, hipStream_t st);
 

@@ -239,8 +239,11 @@ __device__ __forceinline__ double2 pointwise(const SigmaArgs& a, double2 z, size
 // output of the stage for row q (time t'), site s: Sigma into the ring, the
 // adjoint added to (F_bar, rho_bar)(t-1, t'), or the bubbles of (t-1, t')
 template <int MODE>
-__device__ __forceinline__ void emit(const SigmaArgs& a, int s, int slot, double2 z) {
-  if constexpr (MODE == SG_SUNSET_ADJ) {
+__device__ __forceinline__ void emit(const SigmaArgs& a, int s, int slot, size_t rowoff, double2 z) {
+  if constexpr (MODE == SG_PLAIN) {
+    a.SigFout[rowoff + s] = a.preF * z.x;
+    a.Sigrout[rowoff + s] = a.preR * z.y;
+  } else if constexpr (MODE == SG_SUNSET_ADJ) {
     const int s_tm1 = a.st->s_tm1;
     double* o = a.Hbar + hoff(s / MB, pair_off(s_tm1, slot, a.N), s % MB, a.NP);
     o[0] += z.x;
@@ -263,6 +266,25 @@ __device__ __forceinline__ void emit(const SigmaArgs& a, int s, int slot, double
   }
 }
 
+// rows of the stage: the live times t' = tmin + q, q < L, of the window, or
+// the caller's lattices
+template <int MODE>
+__device__ __forceinline__ int stage_rows(const SigmaArgs& a) {
+  if constexpr (MODE == SG_PLAIN)
+    return a.rows;
+  else
+    return a.st->L;
+}
+
+// ring slot of row q (unused for caller lattices)
+template <int MODE>
+__device__ __forceinline__ int row_slot(const SigmaArgs& a, int q) {
+  if constexpr (MODE == SG_PLAIN)
+    return 0;
+  else
+    return (a.st->tmin + q - 1) % a.N;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -276,14 +298,10 @@ __global__ __launch_bounds__(TPB) void sigma_plane_fwd(SigmaArgs a) {
   double2* A = lds;
   double2* B = lds + ny * ls;
   const int q = blockIdx.y;
-  if (q >= a.st->L) return;  // rows t' = tmin + q, q < L, are live
+  if (q >= stage_rows<MODE>(a)) return;
   const int iz = blockIdx.x;
   const int izn = (nz - iz) % nz;
-  const int tp = a.st->tmin + q;
-  const int s_tm1 = a.st->s_tm1;
-  const int slot = (tp - 1) % a.N;
-  const long pr = pair_off(s_tm1, slot, a.N);
-  const double sg = s_tm1 >= slot ? 1.0 : -1.0;
+  const int slot = row_slot<MODE>(a, q);
   const size_t rowoff = (size_t)q * a.V;
   const int nplane = n * ny;
 
@@ -293,7 +311,11 @@ __global__ __launch_bounds__(TPB) void sigma_plane_fwd(SigmaArgs a) {
     const int iy = e / n, ix = e - iy * n;
     const int s1 = ix + n * (iy + ny * iz);
     const int s2 = (n - ix) % n + n * ((ny - iy) % ny + ny * izn);
-    if constexpr (MODE == SG_SUNSET_ADJ || MODE == SG_CHAIN) {
+    if constexpr (MODE == SG_PLAIN) {
+      const double* F = a.Fin + rowoff;
+      const double* R = a.Rin + rowoff;
+      A[iy * ls + ix] = make_double2(0.5 * (F[s1] + F[s2]), 0.5 * (R[s1] + R[s2]));
+    } else if constexpr (MODE == SG_SUNSET_ADJ || MODE == SG_CHAIN) {
       // a ring in self-energy layout: (SigF_bar, Sigr_bar) or (IF, Ir)
       const double* ring = MODE == SG_CHAIN ? a.Iring : a.Sbar;
       const int q1 = a.smap ? a.smap[s1] : s1, q2 = a.smap ? a.smap[s2] : s2;
@@ -301,6 +323,9 @@ __global__ __launch_bounds__(TPB) void sigma_plane_fwd(SigmaArgs a) {
       const double* h2 = ring + soff(q2 / MB, slot, q2 % MB, a.N);
       A[iy * ls + ix] = make_double2(0.5 * (h1[0] + h2[0]), 0.5 * (h1[1] + h2[1]));
     } else {
+      const int s_tm1 = a.st->s_tm1;
+      const long pr = pair_off(s_tm1, slot, a.N);
+      const double sg = s_tm1 >= slot ? 1.0 : -1.0;
       const int q1 = a.smap ? a.smap[s1] : s1, q2 = a.smap ? a.smap[s2] : s2;
       const double* h1 = a.H + hoff(q1 / MB, pr, q1 % MB, a.NP);
       const double* h2 = a.H + hoff(q2 / MB, pr, q2 % MB, a.NP);
@@ -328,7 +353,7 @@ __global__ __launch_bounds__(TPB) void sigma_plane_fwd(SigmaArgs a) {
     res = fft_lines<NT>(res, oth, a.plan, ny, ls, 1, a.tw, true);
     for (int e = threadIdx.x; e < nplane; e += blockDim.x) {
       const int iy = e / n, ix = e - iy * n;
-      emit<MODE>(a, ix + n * iy, slot, res[iy * ls + ix]);
+      emit<MODE>(a, ix + n * iy, slot, rowoff, res[iy * ls + ix]);
     }
     return;
   }
@@ -352,7 +377,7 @@ __global__ __launch_bounds__(TPB) void sigma_zpass(SigmaArgs a) {
   double2* A = lds;
   double2* B = lds + n * ls;
   const int q = blockIdx.y;
-  if (q >= a.st->L) return;
+  if (q >= stage_rows<MODE>(a)) return;
   const int iy = blockIdx.x;
   const size_t rowoff = (size_t)q * a.V;
   for (int e = threadIdx.x; e < n * nz; e += blockDim.x) {
@@ -384,10 +409,9 @@ __global__ __launch_bounds__(TPB) void sigma_plane_inv(SigmaArgs a) {
   double2* A = lds;
   double2* B = lds + ny * ls;
   const int q = blockIdx.y;
-  if (q >= a.st->L) return;
+  if (q >= stage_rows<MODE>(a)) return;
   const int iz = blockIdx.x;
-  const int tp = a.st->tmin + q;
-  const int slot = (tp - 1) % a.N;
+  const int slot = row_slot<MODE>(a, q);
   const size_t rowoff = (size_t)q * a.V;
   const int nplane = n * ny;
   for (int e = threadIdx.x; e < nplane; e += blockDim.x) {
@@ -400,7 +424,7 @@ __global__ __launch_bounds__(TPB) void sigma_plane_inv(SigmaArgs a) {
   res = fft_lines<NT>(res, oth, a.plan, ny, ls, 1, a.tw, true);
   for (int e = threadIdx.x; e < nplane; e += blockDim.x) {
     const int iy = e / n, ix = e - iy * n;
-    emit<MODE>(a, ix + n * (iy + ny * iz), slot, res[iy * ls + ix]);
+    emit<MODE>(a, ix + n * (iy + ny * iz), slot, rowoff, res[iy * ls + ix]);
   }
 }
 
@@ -444,6 +468,7 @@ static hipError_t prepare_t(size_t lds) {
   if (e == hipSuccess) e = prepare_one<NT, SG_SUNSET, 512>(lds);
   if (e == hipSuccess) e = prepare_one<NT, SG_BUBBLE, 256>(lds);
   if (e == hipSuccess) e = prepare_one<NT, SG_CHAIN, 256>(lds);
+  if (e == hipSuccess) e = prepare_one<NT, SG_PLAIN, 256>(lds);
   if (e != hipSuccess) return e;
   return prepare_one<NT, SG_SUNSET_ADJ, 256>(lds);
 }
@@ -492,6 +517,10 @@ hipError_t launch_sigma_planes(const SigmaArgs& a, int L, hipStream_t st, int va
 
 hipError_t launch_sigma_planes_adjoint(const SigmaArgs& a, int L, hipStream_t st) {
   return launch_planes_n<SG_SUNSET_ADJ, 256>(a, L, st);
+}
+
+hipError_t launch_sigma_planes_plain(const SigmaArgs& a, hipStream_t st) {
+  return launch_planes_n<SG_PLAIN, 256>(a, a.rows, st);
 }
 
 hipError_t launch_bubble_planes(const SigmaArgs& a, int L, hipStream_t st) {

@@ -45,7 +45,7 @@ EXPORTED = (
     "kb_adj_record", "kb_adj_gradient", "kb_adj_info", "kb_adj_stop", "kb_adj_debug_stop", "kb_adj_get_entry",
     "kb_tile_shape", "kb_set_tiles", "kb_get_tiles", "kb_autotune",
     "kb_fft_shape", "kb_set_fft_tiles", "kb_get_fft_tiles", "kb_autotune_fft",
-    "kb_create_sym", "kb_storage_info",
+    "kb_create_sym", "kb_storage_info", "kb_sunset_workspace_bytes", "kb_sunset",
 )
 
 
@@ -149,6 +149,9 @@ def load_library(path: str = LIB_PATH):
     L.kb_ens_synchronize.argtypes = [vp]
     L.kb_ens_enable_timing.argtypes = [vp, ctypes.c_int]
     L.kb_ens_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)]
+    L.kb_sunset_workspace_bytes.argtypes = [ctypes.POINTER(kb_params), ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_ulonglong)]
+    L.kb_sunset.argtypes = [ctypes.POINTER(kb_params), ctypes.c_int, vp, vp, vp, vp, vp, ctypes.c_ulonglong, vp]
     _lib = L
     return L
 
@@ -224,6 +227,33 @@ def _f64(a, V):
 def make_params(Nx, sdim, NstepsinMemory, dt, Lambda, ONgroup, Mass, Mass2, pexp=KB_PEXP_LAMBDA_NLO, device=0):
     return kb_params(int(Nx), int(sdim), int(NstepsinMemory), int(ONgroup), float(dt), float(Lambda),
                      float(Mass), float(Mass2), int(pexp), int(device))
+
+
+def _free_chk(rc):
+    """Raise for a failed context-free call (message: kb_last_error(NULL))."""
+    if rc != KB_OK:
+        raise KBError(rc, load_library().kb_last_error(None).decode())
+
+
+def sunset_workspace_bytes(Nx, sdim, nrows):
+    """Device workspace bytes kb_sunset needs for nrows lattices."""
+    L = load_library()
+    n = ctypes.c_ulonglong()
+    p = make_params(Nx, sdim, 3, 0.1, 1.0, 1, 1.0, 1.0)
+    _free_chk(L.kb_sunset_workspace_bytes(ctypes.byref(p), int(nrows), ctypes.byref(n)))
+    return n.value
+
+
+def sunset(Nx, sdim, Lambda, ONgroup, nrows, F, rho, SigF, Sigr, workspace, workspace_bytes, stream=None,
+           device=0):
+    """calcSelfEnergyattp! (TwoPIScalar_Evolution.jl:25-31) on nrows device
+    lattices: F, rho, SigF, Sigr and workspace are device addresses (ints),
+    rows of Nx^sdim doubles.  Enqueued on `stream` (a hipStream_t address,
+    None = the null stream); does not wait."""
+    L = load_library()
+    p = make_params(Nx, sdim, 3, 0.1, Lambda, ONgroup, 1.0, 1.0, device=device)
+    _free_chk(L.kb_sunset(ctypes.byref(p), int(nrows), F, rho, SigF, Sigr, workspace, int(workspace_bytes),
+                          stream))
 
 
 class KBContext:
