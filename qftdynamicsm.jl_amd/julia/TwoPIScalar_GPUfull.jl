@@ -11,7 +11,9 @@
 #   getTwoPISimData          (TwoPIScalar_SimDataStruct.jl:669-708; also
 #                             "Num" => "GPUorbits" for cubic-orbit storage)
 #   getTwoPIScalarTmpData    (:710-721)
-#   initialize!              (TwoPIScalar_Initialization.jl:289-325)
+#   initialize!              (TwoPIScalar_Initialization.jl:289-325; the
+#                             reference's method takes CPU TmpData only)
+#   initSimData!             (:8-29, Gauss :95-112)
 #   getHartreeMass2          (:207-218)
 #   evolve!                  (TwoPIScalar_Evolution.jl:924-981)
 #   createMeasurement        (TwoPIScalar_Measurement.jl:31-99)
@@ -81,9 +83,11 @@ pexpcode(p) = throw(ArgumentError("GPUfull implements LambdaLO, LambdaNLO (quant
 getTwoPISimData(problem::QFTdynamicsProblem, ::TwoPIScalarGPUfull) = gpusimdata(problem, :kb_create)
 getTwoPISimData(problem::QFTdynamicsProblem, ::TwoPIScalarGPUorbits) = gpusimdata(problem, :kb_create_sym)
 
-function gpusimdata(problem::QFTdynamicsProblem, create::Symbol)
+# Lambda: the coupling of the run (the problem's by default; the rrule in
+# TwoPIScalar_GPUrrule.jl evaluates at others)
+function gpusimdata(problem::QFTdynamicsProblem, create::Symbol; Lambda::Float64 = problem.model.Lambda)
     d, m = problem.disc, problem.model
-    p = Ref(KBParams(d.Nx, d.sdim, problem.simsetup.NstepsinMemory, m.ONgroup, d.dt, m.Lambda, d.Mass,
+    p = Ref(KBParams(d.Nx, d.sdim, problem.simsetup.NstepsinMemory, m.ONgroup, d.dt, Lambda, d.Mass,
                      m.Mass2, pexpcode(problem.pexp), 0))
     h = Ref{Ptr{Cvoid}}(C_NULL)
     rc = create === :kb_create_sym ?
@@ -105,7 +109,8 @@ getTwoPIScalarTmpData(simdata::TwoPIScalarSimDataGPUfull, problem::QFTdynamicsPr
 function Base.getproperty(s::TwoPIScalarSimDataGPUfull, f::Symbol)
     if f === :indices
         a, b = Ref{Cint}(0), Ref{Cint}(0)
-        ccall((:kb_window, libqftkb), Cint, (Ptr{Cvoid}, Ref{Cint}, Ref{Cint}), getfield(s, :ctx), a, b)
+        kbcheck(getfield(s, :ctx), ccall((:kb_window, libqftkb), Cint, (Ptr{Cvoid}, Ref{Cint}, Ref{Cint}),
+                                         getfield(s, :ctx), a, b))
         return [Int(a[]), Int(b[])]
     elseif f === :hartreemass2
         m = Ref{Cdouble}(0)
@@ -141,7 +146,30 @@ function initSimData!(initmass::Float64, simdata::TwoPIScalarSimDataGPUfull, tmp
                                simdata.ctx, F11, F21, F22))
 end
 
-# initialize! itself is unchanged: it dispatches to the initSimData! above.
+# initialize! (TwoPIScalar_Initialization.jl:289-325).  The reference defines
+# it for TwoPIScalarTmpDataCPU only, so the GPU numerics need their own
+# method.  The initial mass follows the reference's choice per model and
+# initialisation (:293-322): the discretisation mass for the tachyonic model
+# and for the Gauss, Quench, TopHatT1, Thermal and Box initialisations, the LO
+# gap mass (getGapMass, :250-266) for every other one.
+function initmassGPU(model::TwoPIScalarPhi4, pexp::TwoPIScalarPertExpansion, disc::TwoPIScalarDiscretization,
+                     init::TwoPIScalarInitialization)
+    if model isa Phi4Tachyonic || init isa TwoPIScalarGauss || init isa TwoPIScalarQuench ||
+       init isa TwoPIScalarTopHatT1 || init isa TwoPIScalarThermal || init isa TwoPIScalarBox
+        return disc.Mass
+    end
+    return getGapMass(model, pexp, disc, init)
+end
+
+function initialize!(thesolution::QFTdynamicsSolutionTwoPIScalar, tmpdata::TwoPIScalarTmpDataGPUfull)
+    @unpack problem, simdata, measurearray = thesolution
+    @unpack model, pexp, disc, init, reno, num, simsetup = problem
+    if isnewsimulation(simsetup) == true
+        initSimData!(initmassGPU(model, pexp, disc, init), simdata, tmpdata, model, pexp, disc, init)
+    end
+    simsetup.lastEvolStep = 2
+    measure!(thesolution, 2)
+end
 
 function getHartreeMass2(model::TwoPIScalarPhi4, pexp::TwoPIScalarPertExpansion, disc::TwoPIScalarDiscretization,
                          simdata::TwoPIScalarSimDataGPUfull, idxone::Int64, idxtwo::Int64)
