@@ -192,6 +192,7 @@ int launch_sigma(kb_ctx* c, double2* X, double* S_out) {
     if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "orbit self-energy launch failed: %s", hipGetErrorString(e));
     return KB_OK;
   }
+  if (plane_workspace_missing(c)) return seterr(c, KB_ERR_STATE, "self-energy: plane workspace not allocated");
   if (sigma_planes_supported(c->p.Nx)) {
     hipError_t e = launch_sigma_planes(a, L, c->stream, c->fft_tile);
     if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "self-energy launch failed: %s", hipGetErrorString(e));
@@ -254,18 +255,19 @@ SigmaArgs ninv_sigma_args(kb_ctx* c, const StepState* st) {
   return a;
 }
 
-int launch_bubble(kb_ctx* c, const StepState* st, double2* X) {
+int launch_bubble(kb_ctx* c, const StepState* st, bool keep) {
   const double ivol = 1.0 / (double)c->Vl;
   if (c->fft_tile == KB_FFT_ORBIT_PRODUCT) {
     OrbitArgs o = orbit_args(c, st, ivol * ivol, ivol * ivol);
-    if (!X) o.X = nullptr;  // rows rebuilt outside a step keep no transforms
+    if (!keep) o.X = nullptr;  // rows rebuilt outside a step keep no transforms
     hipError_t e = launch_orbit(o, ORBIT_BUBBLE, c->stream);
     if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "orbit bubble launch failed: %s", hipGetErrorString(e));
     return KB_OK;
   }
+  if (plane_workspace_missing(c)) return seterr(c, KB_ERR_STATE, "bubble pass: plane workspace not allocated");
   SigmaArgs a = ninv_sigma_args(c, st);
   a.preF = a.preR = ivol * ivol;
-  a.X = X;
+  a.X = keep ? c->Xn : nullptr;
   hipError_t e = launch_bubble_planes(a, c->N - 1, c->stream);
   if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "bubble launch failed: %s", hipGetErrorString(e));
   return KB_OK;
@@ -314,6 +316,7 @@ int launch_chain(kb_ctx* c) {
     if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "orbit chain launch failed: %s", hipGetErrorString(e));
     return KB_OK;
   }
+  if (plane_workspace_missing(c)) return seterr(c, KB_ERR_STATE, "chain pass: plane workspace not allocated");
   SigmaArgs a = ninv_sigma_args(c, c->st_dev);
   a.preF = a.preR = pre;
   a.X = c->Xn;
@@ -325,7 +328,7 @@ int launch_chain(kb_ctx* c) {
 // calcSelfEnergies! NinverseNLO (Evolution.jl:144-154) for the step in st_dev
 int launch_ninv_stage(kb_ctx* c) {
   int rc;
-  if ((rc = launch_bubble(c, c->st_dev, c->Xn))) return rc;
+  if ((rc = launch_bubble(c, c->st_dev, true))) return rc;
   {
     StageTimer st(c, 5);
     if ((rc = launch_resum(c, c->st_dev, NINV_PASS_ALL))) return rc;
@@ -356,7 +359,7 @@ int ninv_rebuild(kb_ctx* c) {
                              c->stream));
     int rc;
     for (size_t k = 0; k < fs.size(); ++k) {
-      if ((rc = launch_bubble(c, c->stfill_dev + k, nullptr))) return rc;
+      if ((rc = launch_bubble(c, c->stfill_dev + k, false))) return rc;
       if ((rc = launch_resum(c, c->stfill_dev + k, NINV_PASS_IR))) return rc;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -509,6 +512,28 @@ int dmalloc(kb_ctx* c, void** ptr, size_t bytes) {
 
 int meas_grid(const kb_ctx* c) { return (c->V + MEAS_THREADS - 1) / MEAS_THREADS; }
 
+// The plane kernels' workspace: W holds one complex lattice per row between
+// the passes (3D lattices, and the pencil passes past Nx = 64); Xn keeps the
+// 1/N bubble pass's transforms for its chain pass.  Allocated when a plane
+// form is first selected: LambdaLO and contexts on the orbit products never
+// hold them.
+int ensure_plane_workspace(kb_ctx* c) {
+  if (!has_sigma(c)) return KB_OK;
+  int rc;
+  const size_t rows = (size_t)c->N * c->Vl * sizeof(double2);
+  if ((c->p.sdim == 3 || !sigma_planes_supported(c->p.Nx)) && !c->W &&
+      (rc = dmalloc(c, (void**)&c->W, rows)))
+    return rc;
+  if (c->p.pexp == KB_PEXP_NINVERSE_NLO && !c->Xn && (rc = dmalloc(c, (void**)&c->Xn, rows))) return rc;
+  return KB_OK;
+}
+
+// a plane-kernel stage needs its workspace (a guard, not a path callers reach)
+bool plane_workspace_missing(const kb_ctx* c) {
+  return ((c->p.sdim == 3 || !sigma_planes_supported(c->p.Nx)) && !c->W) ||
+         (c->p.pexp == KB_PEXP_NINVERSE_NLO && !c->Xn);
+}
+
 void cubic_orbits(int Nx, int sdim, std::vector<int>& orbit_of, std::vector<int>& rep_site) {
   int Vl = 1;
   for (int d = 0; d < sdim; ++d) Vl *= Nx;
@@ -627,7 +652,6 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
   const size_t hcount = (size_t)c->NB * (size_t)c->NP * MB * 2;
   if ((rc = dmalloc(c, (void**)&c->H, hcount * sizeof(double)))) return fail(rc);
   if ((rc = dmalloc(c, (void**)&c->S, (size_t)c->NB * c->N * MB * 2 * sizeof(double)))) return fail(rc);
-  if ((rc = dmalloc(c, (void**)&c->W, (size_t)c->N * c->Vl * sizeof(double2)))) return fail(rc);
   c->PNP = pslot_pairs(c->N);
   const size_t pcount = (size_t)c->NB * (size_t)c->PNP * MB;  // one value per pair and site
   const size_t rcount = (size_t)c->NB * c->N * MB * 2;
@@ -636,7 +660,6 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
         (rc = dmalloc(c, (void**)&c->PR, pcount * sizeof(double))) ||
         (rc = dmalloc(c, (void**)&c->IR, pcount * sizeof(double))) ||
         (rc = dmalloc(c, (void**)&c->Iring, rcount * sizeof(double))) ||
-        (rc = dmalloc(c, (void**)&c->Xn, (size_t)c->N * c->Vl * sizeof(double2))) ||
         (rc = dmalloc(c, (void**)&c->stfill_dev, (size_t)c->N * sizeof(StepState))))
       return fail(rc);
     std::vector<int2> items;
@@ -702,6 +725,7 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
     }
     c->fft_tile = KB_FFT_ORBIT_PRODUCT;
   }
+  if (c->fft_tile != KB_FFT_ORBIT_PRODUCT && (rc = ensure_plane_workspace(c))) return fail(rc);
   if ((rc = dmalloc(c, (void**)&c->kL2, (size_t)c->Vpad * sizeof(double)))) return fail(rc);
   if ((rc = dmalloc(c, (void**)&c->m2, 8 * sizeof(double)))) return fail(rc);
   if ((rc = dmalloc(c, (void**)&c->partial, (size_t)6 * meas_grid(c) * sizeof(double)))) return fail(rc);
@@ -1392,6 +1416,10 @@ int kb_set_fft_tiles(kb_ctx* c, int variant) {
   // orbit storage past the plane kernels' sizes has only the orbit product
   if (variant != KB_FFT_ORBIT_PRODUCT && c->sym && !sigma_planes_supported(c->p.Nx)) return KB_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (variant != KB_FFT_ORBIT_PRODUCT) {
+    int rc = ensure_plane_workspace(c);
+    if (rc) return rc;
+  }
   c->fft_tile = variant;
   drop_graph(c);
   return KB_OK;
@@ -1412,6 +1440,7 @@ int kb_autotune_fft(kb_ctx* c, int reps, int* best, double* ms) {
     return seterr(c, KB_ERR_ARG, "autotune: no plane self-energy stage for this context");
   // the plane kernels' block sizes, then (orbit storage) the orbit product
   const int nv = NFFT_VARIANTS + (c->Mo ? 1 : 0);
+  if (int rw = ensure_plane_workspace(c)) return rw;
   std::vector<double> t(nv, 0.0);
   hipEvent_t a = take_event(c), b = take_event(c);
   if (!a || !b) return seterr(c, KB_ERR_HIP, "autotune: no events");

@@ -151,6 +151,10 @@ void drop_graph(kb_ctx* c);
 // self-energy stage of the step in c->st_dev into S_out (default: the
 // context's ring); X, if set, receives the transforms (Fx, rx)
 int launch_sigma(kb_ctx* c, double2* X = nullptr, double* S_out = nullptr);
+// the plane kernels' workspace (W, and Xn for the 1/N passes), allocated
+// when a plane form is first selected; and whether it is missing
+int ensure_plane_workspace(kb_ctx* c);
+bool plane_workspace_missing(const kb_ctx* c);
 // Hartreeprefactor of the context's expansion (Initialization.jl:245-247)
 double hartree_prefactor(const kb_ctx* c);
 // expansions with a self-energy (TwoPIScalarNLO, Problemdefinition.jl:48)

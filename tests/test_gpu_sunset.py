@@ -2,7 +2,7 @@
 TwoPIScalar_Evolution.jl:25-31) of lattices held in caller device memory,
 through the C ABI, against the oracle on random non-symmetric rows and
 against the self-energy stage of a KB step on the same rows.  Device memory
-comes from torch (plumbing only); tolerance 1e-10 against the oracle,
+comes from tests/hipmem.py (plumbing only); tolerance 1e-10 against the oracle,
 1e-14 against the step (the same kernels and twiddles)."""
 import numpy as np
 import pytest
@@ -20,21 +20,17 @@ def rel(a, b):
 
 def device_sunset(Nx, sdim, lam, ON, F_rows, r_rows):
     """Run kb_sunset on host rows (nrows x V); returns (SigF, Sigr) on the host."""
-    import torch
+    from hipmem import DeviceArray
     from qftkb._abi import sunset, sunset_workspace_bytes
-    nrows = F_rows.shape[0]
-    dev = torch.device("cuda", 0)
-    F = torch.from_numpy(np.ascontiguousarray(F_rows)).to(dev)
-    R = torch.from_numpy(np.ascontiguousarray(r_rows)).to(dev)
-    SF = torch.empty_like(F)
-    SR = torch.empty_like(F)
+    nrows, V = F_rows.shape
+    F = DeviceArray.from_host(F_rows)
+    R = DeviceArray.from_host(r_rows)
+    SF, SR = DeviceArray(nrows * V), DeviceArray(nrows * V)
     nb = sunset_workspace_bytes(Nx, sdim, nrows)
-    ws = torch.empty(nb, dtype=torch.uint8, device=dev)
-    assert ws.data_ptr() % 256 == 0
-    torch.cuda.synchronize()
-    sunset(Nx, sdim, lam, ON, nrows, F.data_ptr(), R.data_ptr(), SF.data_ptr(), SR.data_ptr(), ws.data_ptr(), nb)
-    torch.cuda.synchronize()
-    return SF.cpu().numpy(), SR.cpu().numpy()
+    ws = DeviceArray(nb, np.uint8)
+    assert ws.ptr % 256 == 0
+    sunset(Nx, sdim, lam, ON, nrows, F.ptr, R.ptr, SF.ptr, SR.ptr, ws.ptr, nb)
+    return SF.host().reshape(nrows, V), SR.host().reshape(nrows, V)
 
 
 def oracle_rows(Nx, sdim, lam, ON, seed, N=6):
@@ -95,25 +91,21 @@ def test_sunset_equals_the_step_stage(Nx, sdim, N):
 
 
 def test_sunset_rejects_bad_arguments():
-    import torch
+    from hipmem import DeviceArray, synchronize
     from qftkb import KBError
     from qftkb._abi import sunset, sunset_workspace_bytes
     with pytest.raises(KBError):
         sunset_workspace_bytes(65, 3, 4)  # past the plane kernels
     with pytest.raises(KBError):
         sunset_workspace_bytes(8, 3, 0)
-    dev = torch.device("cuda", 0)
     V, n = 512, 3
-    F = torch.zeros(n * V, dtype=torch.float64, device=dev)
-    R = torch.zeros_like(F)
-    S = torch.zeros_like(F)
+    F, R, S, S2 = (DeviceArray(n * V) for _ in range(4))
     nb = sunset_workspace_bytes(8, 3, n)
-    ws = torch.empty(nb, dtype=torch.uint8, device=dev)
+    ws = DeviceArray(nb, np.uint8)
     with pytest.raises(KBError) as e:
-        sunset(8, 3, 1.0, 1, n, F.data_ptr(), R.data_ptr(), S.data_ptr(), S.data_ptr(), ws.data_ptr(), nb)
+        sunset(8, 3, 1.0, 1, n, F.ptr, R.ptr, S.ptr, S.ptr, ws.ptr, nb)
     assert "overlap" in str(e.value)
-    S2 = torch.zeros_like(F)
     with pytest.raises(KBError) as e:
-        sunset(8, 3, 1.0, 1, n, F.data_ptr(), R.data_ptr(), S.data_ptr(), S2.data_ptr(), ws.data_ptr(), nb - 8)
+        sunset(8, 3, 1.0, 1, n, F.ptr, R.ptr, S.ptr, S2.ptr, ws.ptr, nb - 8)
     assert "workspace" in str(e.value)
-    torch.cuda.synchronize()
+    synchronize()

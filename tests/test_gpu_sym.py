@@ -286,10 +286,11 @@ def test_orbit_storage_ninverse_past_the_plane_kernels_against_oracle():
     g.close()
 
 
-def test_orbit_storage_every_tile_height():
-    """The split memory integral follows the context's tile shape: its item
-    height (32, 64 or 128 slot rows) sets both the item table and the split
-    kernel.  Each height gives the default trajectory."""
+def test_orbit_storage_every_tile_shape():
+    """The split memory integral runs the context's tile shape: its item
+    height (32, 64 or 128 slot rows) sets the item table, and its rows per
+    batch and threads the split kernel.  Every shape the autotuner can pick
+    gives the default trajectory."""
     from qftkb import KBContext
     args = (16, 3, 20, 0.1, 1.0, 1, 1.0, 1.0)
     ref = KBContext(*args, symmetric=True, tiles=None)
@@ -299,8 +300,8 @@ def test_orbit_storage_every_tile_height():
     for t in range(3, 31):
         ref.step(t)
     tmin, tmax = ref.window()
-    for rows in (32, 64, 128):
-        v = next(i for i, s in enumerate(shapes) if s[0] == rows)
+    assert len(shapes) == 9
+    for v in range(len(shapes)):
         print(f"\n[case] orbit storage, tile {shapes[v]}", flush=True)
         g = KBContext(*args, symmetric=True, tiles=None)
         g.set_tiles(v)
@@ -309,7 +310,7 @@ def test_orbit_storage_every_tile_height():
             g.step(t)
         for tp in range(tmin, tmax + 1):
             for w in "Fr":
-                assert rel(g.get(w, tmax, tp), ref.get(w, tmax, tp)) <= 1e-12, (rows, w, tp)
+                assert rel(g.get(w, tmax, tp), ref.get(w, tmax, tp)) <= 1e-12, (shapes[v], w, tp)
         g.close()
     # the autotuner re-times the split form mid-run and the run continues
     g = KBContext(*args, symmetric=True, tiles=None)
@@ -346,3 +347,41 @@ def test_parameter_dict_run_with_orbit_numerics(init):
             a, b = getattr(mo, k), getattr(mf, k)
             assert abs(a - b) <= 1e-12 * max(abs(b), abs(mf.E)), (k, a, b)
         np.testing.assert_allclose(mo.n, mf.n, rtol=1e-11, atol=1e-13)
+
+
+def test_plane_workspace_only_where_a_plane_stage_runs():
+    """The plane kernels' workspace (one complex lattice per slot, plus the
+    1/N transforms) is held only by contexts that run a plane stage: not by
+    LambdaLO, not by orbit-storage contexts on the orbit products, until a
+    plane form is selected."""
+    from qftkb import KBContext
+    Nx, N = 16, 12
+    Vl = Nx ** 3
+    W = N * Vl * 16
+    lo_full = KBContext(Nx, 3, N, 0.1, 1.0, 1, 1.0, 1.0, pexp=0)
+    nlo_full = KBContext(Nx, 3, N, 0.1, 1.0, 1, 1.0, 1.0)
+    assert nlo_full.device_bytes() - lo_full.device_bytes() == W
+    sym = KBContext(Nx, 3, N, 0.1, 1.0, 1, 1.0, 1.0, symmetric=True)
+    b0 = sym.device_bytes()
+    sym.set_fft_tiles(0)  # the plane kernels over the expanded lattice
+    assert sym.device_bytes() - b0 == W
+    sym.set_fft_tiles(-1)
+    sym.set_fft_tiles(1)
+    assert sym.device_bytes() - b0 == W  # allocated once
+    ninv = KBContext(Nx, 3, N, 0.05, 2.0, 4, 1.0, 1.0, pexp=3, symmetric=True)
+    c0 = ninv.device_bytes()
+    ninv.set_fft_tiles(0)
+    assert ninv.device_bytes() - c0 == 2 * W  # W and the kept transforms
+    # the plane form still runs after the late allocation: a step against the
+    # orbit-product form of the same state
+    k2 = sym.kL2()
+    n = 1.0 / (np.exp(np.sqrt(k2 + 1.0) / 2.0) - 1.0)
+    ref = KBContext(Nx, 3, N, 0.1, 1.0, 1, 1.0, 1.0, symmetric=True)
+    for c in (sym, ref):
+        c.init_particles(n, 1.0)
+        for t in range(3, 8):
+            c.step(t)
+    a, b = sym.get("F", 7, 5), ref.get("F", 7, 5)
+    assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
+    for c in (lo_full, nlo_full, sym, ninv, ref):
+        c.close()
