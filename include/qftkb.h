@@ -311,6 +311,19 @@ int kb_adj_get_entry(kb_ctx *ctx, int which, int i, int j, double *out);
 int kb_sunset_workspace_bytes(const kb_params *p, int nrows, unsigned long long *bytes);
 int kb_sunset(const kb_params *p, int nrows, const double *F, const double *rho, double *SigF,
               double *Sigr, void *workspace, unsigned long long workspace_bytes, void *stream);
+/* kb_sunset with no workspace argument and no allocation: no counterpart in
+ * the reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It makes no hipMalloc/hipFree or other allocating call.  The workspace is
+ * a 64 MiB device array inside the library, one per device, held until the
+ * process ends: KB_ERR_ARG when kb_sunset_workspace_bytes of the request
+ * exceeds 64 MiB (more than 15 rows of 64^3, 37 of 48^3, 127 of 32^3; 1D and
+ * 2D lattices need only the 4 KiB of twiddles).
+ * Every call on one device shares that array, so calls that overlap in time
+ * on one device (other streams, other threads) race; the results equal
+ * kb_sunset's. */
+int kb_sunset_noalloc(const kb_params *p, int nrows, const double *F, const double *rho,
+                      double *SigF, double *Sigr, void *stream);
 
 /* ---- ensembles ------------------------------------------------------------
  * Many independent KB runs (uneven lattices, couplings, memory windows)

@@ -8,6 +8,9 @@
 // Device memory: the caller's input and output rows, plus a workspace that
 // holds the twiddle table (the first WS_TWIDDLE_BYTES) and, for 3D lattices,
 // one complex lattice per row for the transforms between the plane passes.
+// kb_sunset takes the workspace from the caller; kb_sunset_noalloc uses the
+// file-scope device array `scratch` below (SCRATCH_BYTES on every device the
+// library runs on, held until the process ends), so it allocates nothing.
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
@@ -41,6 +44,14 @@ constexpr int NX_MAX = 64;                 // the plane kernels' largest lattice
 constexpr size_t WS_TWIDDLE_BYTES = 4096;  // >= NX_MAX complex values, keeps the rows 256-byte aligned
 constexpr int MAX_DEVICES = 64;
 constexpr int MAX_ROWS = 65535;            // grid rows of one launch
+
+// kb_sunset_noalloc's workspace.  In host code `scratch` names a host-side
+// stand-in, not a device address: only the address that hipGetSymbolAddress
+// returns for the current device (kept per device in g_scratch) is used.
+// Nothing in it needs clearing: every call writes the twiddles and the rows
+// before reading them.  Calls on one device share it and must not overlap.
+constexpr size_t SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) char scratch[SCRATCH_BYTES];
 
 // the host twiddle table, passed by value to the kernel that stores it
 struct Twiddles {
@@ -99,13 +110,20 @@ struct DeviceScope {
   }
 };
 
-// once per device: the dynamic-LDS limits of the plane kernels
+// once per device, with that device current: the dynamic-LDS limits of the
+// plane kernels and the device address of `scratch`
 std::mutex g_setup_mu;
 bool g_ready[MAX_DEVICES] = {};
+char* g_scratch[MAX_DEVICES] = {};
 
 int device_setup(int dev) {
   std::lock_guard<std::mutex> lk(g_setup_mu);
   if (g_ready[dev]) return KB_OK;
+  void* ptr = nullptr;
+  hipError_t se = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (se != hipSuccess || !ptr)
+    return fail(KB_ERR_HIP, "sunset: no device address for the scratch array: %s", hipGetErrorString(se));
+  g_scratch[dev] = static_cast<char*>(ptr);
   // every compiled length and the runtime-length instantiation
   static const int lengths[] = {8, 16, 24, 32, 40, 48, 64, 1};
   for (int n : lengths) {
@@ -181,4 +199,25 @@ int kb_sunset(const kb_params* p, int nrows, const double* F, const double* rho,
   if (ds.e != hipSuccess) return fail(KB_ERR_HIP, "sunset: device %d: %s", p->device, hipGetErrorString(ds.e));
   if ((rc = device_setup(p->device))) return rc;
   return run_sunset(p, nrows, F, rho, SigF, Sigr, static_cast<char*>(workspace), static_cast<hipStream_t>(stream));
+}
+
+int kb_sunset_noalloc(const kb_params* p, int nrows, const double* F, const double* rho, double* SigF,
+                      double* Sigr, void* stream) {
+  int rc = check_params(p, nrows);
+  if (rc) return rc;
+  const size_t lat = (size_t)nrows * p->Nx * (p->sdim >= 2 ? p->Nx : 1) * (p->sdim >= 3 ? p->Nx : 1);
+  if ((rc = check_buffers(F, rho, SigF, Sigr, lat * sizeof(double)))) return rc;
+  const size_t need = workspace_bytes(p, nrows);
+  if (need > SCRATCH_BYTES)
+    return fail(KB_ERR_ARG, "sunset_noalloc: %d rows of %d^%d need %zu bytes of scratch, the limit is %zu", nrows,
+                p->Nx, p->sdim, need, SCRATCH_BYTES);
+  DeviceScope ds(p->device);
+  if (ds.e != hipSuccess) return fail(KB_ERR_HIP, "sunset: device %d: %s", p->device, hipGetErrorString(ds.e));
+  if ((rc = device_setup(p->device))) return rc;
+  char* ws = g_scratch[p->device];
+  const size_t lb = lat * sizeof(double);
+  if (overlap(ws, need, F, lb) || overlap(ws, need, rho, lb) || overlap(ws, need, SigF, lb) ||
+      overlap(ws, need, Sigr, lb))
+    return fail(KB_ERR_ARG, "sunset_noalloc: lattices overlap the library's scratch");
+  return run_sunset(p, nrows, F, rho, SigF, Sigr, ws, static_cast<hipStream_t>(stream));
 }

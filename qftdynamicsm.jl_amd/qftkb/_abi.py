@@ -45,7 +45,7 @@ EXPORTED = (
     "kb_adj_record", "kb_adj_gradient", "kb_adj_info", "kb_adj_stop", "kb_adj_debug_stop", "kb_adj_get_entry",
     "kb_tile_shape", "kb_set_tiles", "kb_get_tiles", "kb_autotune",
     "kb_fft_shape", "kb_set_fft_tiles", "kb_get_fft_tiles", "kb_autotune_fft",
-    "kb_create_sym", "kb_storage_info", "kb_sunset_workspace_bytes", "kb_sunset",
+    "kb_create_sym", "kb_storage_info", "kb_sunset_workspace_bytes", "kb_sunset", "kb_sunset_noalloc",
 )
 
 
@@ -152,6 +152,7 @@ def load_library(path: str = LIB_PATH):
     L.kb_sunset_workspace_bytes.argtypes = [ctypes.POINTER(kb_params), ctypes.c_int,
                                             ctypes.POINTER(ctypes.c_ulonglong)]
     L.kb_sunset.argtypes = [ctypes.POINTER(kb_params), ctypes.c_int, vp, vp, vp, vp, vp, ctypes.c_ulonglong, vp]
+    L.kb_sunset_noalloc.argtypes = [ctypes.POINTER(kb_params), ctypes.c_int, vp, vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -254,6 +255,14 @@ def sunset(Nx, sdim, Lambda, ONgroup, nrows, F, rho, SigF, Sigr, workspace, work
     p = make_params(Nx, sdim, 3, 0.1, Lambda, ONgroup, 1.0, 1.0, device=device)
     _free_chk(L.kb_sunset(ctypes.byref(p), int(nrows), F, rho, SigF, Sigr, workspace, int(workspace_bytes),
                           stream))
+
+
+def sunset_noalloc(Nx, sdim, Lambda, ONgroup, nrows, F, rho, SigF, Sigr, stream=None, device=0):
+    """sunset() with the library's 64 MiB per-device scratch as workspace: no
+    allocation; calls on one device must not overlap."""
+    L = load_library()
+    p = make_params(Nx, sdim, 3, 0.1, Lambda, ONgroup, 1.0, 1.0, device=device)
+    _free_chk(L.kb_sunset_noalloc(ctypes.byref(p), int(nrows), F, rho, SigF, Sigr, stream))
 
 
 class KBContext:
