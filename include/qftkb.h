@@ -1,8 +1,8 @@
 /*
  * qftkb.h -- C ABI of libqftkb.so: the MI355X (gfx950) back-end for the
  * 2PI Kadanoff-Baym time step of the reference's src/TwoPIScalar model
- * (CPUfull storage semantics; LambdaLO, LambdaNLO quantum, NinverseLO and
- * NinverseNLO expansions).
+ * (CPUfull storage semantics; LambdaLO, LambdaNLO quantum and classical,
+ * NinverseLO and NinverseNLO expansions).
  *
  * The reference selects its numerics back-end by Julia dispatch on the
  * "Num" parameter (TwoPIScalar_Modelfile.jl:104-108) and then calls
@@ -48,6 +48,11 @@ extern "C" {
 #define KB_PEXP_LAMBDA_NLO 1 /* TwoPIScalarLambdaNLOquantum (:36)            */
 #define KB_PEXP_NINVERSE_LO 2  /* TwoPIScalarNinverseLO (:44)             */
 #define KB_PEXP_NINVERSE_NLO 3 /* TwoPIScalarNinverseNLO (:45)            */
+/* TwoPIScalarLambdaNLOclassical (:39): the sunset without its rho^2 terms,
+ * Sigr ~ C[rx Fx^2], SigF ~ C[Fx^3] (TwoPIScalar_Evolution.jl:107-120; the
+ * reference defines it for its reduced CPUred storage, here it runs on full
+ * and cubic-orbit storage alike) */
+#define KB_PEXP_LAMBDA_NLO_CLASSICAL 4
 
 typedef struct kb_ctx kb_ctx;
 
@@ -298,8 +303,9 @@ int kb_adj_get_entry(kb_ctx *ctx, int which, int i, int j, double *out);
  * a row gives the same bits as kb_step's self-energy stage.  Rows are V =
  * Nx^sdim doubles at q*V, column-major sites; F, rho, SigF and Sigr are
  * device pointers on p->device.  Only Nx, sdim, ONgroup, Lambda, pexp
- * (KB_PEXP_LAMBDA_NLO) and device of *p are read; Nx <= 64, 1 <= nrows <=
- * 65535.  The outputs may not overlap the inputs or each other.
+ * (KB_PEXP_LAMBDA_NLO, or KB_PEXP_LAMBDA_NLO_CLASSICAL for the classical
+ * sunset) and device of *p are read; Nx <= 64, 1 <= nrows <= 65535.  The
+ * outputs may not overlap the inputs or each other.
  *
  * The workspace is caller-owned device memory of kb_sunset_workspace_bytes
  * (4 KiB of twiddles, plus nrows * V complex values in 3D), 256-byte

@@ -19,7 +19,9 @@ mistake:
     a per-momentum recursion with no transform inside the time sums, where
     the C oracle restates the CPUfull back-end (a transform pair per term).
 
-pexp: 0 LambdaLO, 1 LambdaNLO, 2 NinverseLO, 3 NinverseNLO.
+pexp: 0 LambdaLO, 1 LambdaNLO, 2 NinverseLO, 3 NinverseNLO, 4 LambdaNLO
+classical (the sunset without its rho^2 terms, TwoPIScalar_Evolution.jl:
+107-120).
 
 Negative controls: `plant` names one deliberate mistake in the memory
 integrals or the memory energy (see PLANTS), so that tests can show that a
@@ -110,7 +112,7 @@ class NumpyKB:
 
     @property
     def nlo(self):
-        return self.pexp in (1, 3)
+        return self.pexp in (1, 3, 4)
 
     def hartree(self, a, b):
         return self.Mass2 + self.H() / self.V * self.getF(a, b).sum()
@@ -137,8 +139,12 @@ class NumpyKB:
         pr = ivol ** 3 * (-lam ** 2 * (ON + 2) / (6 * ON ** 2))
         pF = ivol ** 3 * (-lam ** 2 * (ON + 2) / (18 * ON ** 2))
         s = self.shape[::-1]
-        gr = np.fft.irfftn(rx * (Fx ** 2 - rx ** 2 / 12.0), s=s, axes=ax) * self.V
-        gF = np.fft.irfftn(Fx * (Fx ** 2 - 0.75 * rx ** 2), s=s, axes=ax) * self.V
+        if self.pexp == 4:
+            gr = np.fft.irfftn(rx * Fx ** 2, s=s, axes=ax) * self.V
+            gF = np.fft.irfftn(Fx ** 3, s=s, axes=ax) * self.V
+        else:
+            gr = np.fft.irfftn(rx * (Fx ** 2 - rx ** 2 / 12.0), s=s, axes=ax) * self.V
+            gF = np.fft.irfftn(Fx * (Fx ** 2 - 0.75 * rx ** 2), s=s, axes=ax) * self.V
         return pF * gF.reshape(-1), pr * gr.reshape(-1)
 
     def _cos(self, x):
@@ -193,7 +199,7 @@ class NumpyKB:
         assert t == self.tmax + 1
         self.expand()
         tmin = self.tmin
-        if self.pexp == 1:
+        if self.pexp in (1, 4):
             for tp in range(tmin, t):
                 self.SigF[tp], self.Sigr[tp] = self._sunset(self.getF(t - 1, tp), self.getr(t - 1, tp))
         elif self.pexp == 3:

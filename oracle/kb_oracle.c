@@ -210,7 +210,7 @@ static void reset_omega2(or_ctx *c) {
 or_ctx *or_create(int Nx, int sdim, int Nmem, double dt, double lambda,
                   int ONgroup, double Mass, double Mass2, int pexp,
                   int nthreads) {
-  if (Nx < 1 || sdim < 1 || sdim > 3 || Nmem < 3 || ONgroup < 1 || pexp < 0 || pexp > 3) return NULL;
+  if (Nx < 1 || sdim < 1 || sdim > 3 || Nmem < 3 || ONgroup < 1 || pexp < 0 || pexp > 4) return NULL;
   or_ctx *c = (or_ctx *)calloc(1, sizeof(or_ctx));
   c->Nx = Nx;
   c->sdim = sdim;
@@ -361,10 +361,12 @@ static void selfenergy_tp(or_ctx *c, int tmone, int tp, cplx *a, cplx *b,
   }
   dft3(c, a, pen, tmp, -1);
   dft3(c, b, pen, tmp, -1);
+  /* LambdaNLOclassical drops the rho^2 terms (Evolution.jl:107-120) */
+  const double qr = c->pexp == 4 ? 0.0 : 1.0 / 12.0, qF = c->pexp == 4 ? 0.0 : 0.75;
   for (int i = 0; i < V; ++i) {
     double Fx = a[i].re, rx = b[i].re;
-    double gr = rx * (Fx * Fx - (1.0 / 12.0) * (rx * rx));
-    double gF = Fx * (Fx * Fx - 0.75 * (rx * rx));
+    double gr = rx * (Fx * Fx - qr * (rx * rx));
+    double gF = Fx * (Fx * Fx - qF * (rx * rx));
     a[i].re = gF;
     a[i].im = 0.0;
     b[i].re = gr;
@@ -412,7 +414,8 @@ int or_selfenergies(or_ctx *c, int tmone) {
  * the role calcSelfEnergiessimple! (Evolution.jl:123-139) plays in the
  * reference's own fast-vs-simple test. */
 int or_selfenergies_direct(or_ctx *c, int tmone) {
-  if (c->pexp != 1) return fail(c, "direct self-energy: LambdaNLO only");
+  if (c->pexp != 1 && c->pexp != 4) return fail(c, "direct self-energy: LambdaNLO only");
+  const double qr = c->pexp == 4 ? 0.0 : 1.0 / 12.0, qF = c->pexp == 4 ? 0.0 : 0.75;
   const int V = c->V, Nx = c->Nx, sd = c->sdim;
   const double ivol = 1.0 / (double)V;
   const double lam = c->lambda, ON = (double)c->ON;
@@ -461,8 +464,8 @@ int or_selfenergies_direct(or_ctx *c, int tmone) {
           FFF += Fe[q] * Fe[k] * Fe[l];
           Frr += Fe[q] * re[k] * re[l];
         }
-      Sr[p] = pre_r * (double)V * (rFF - (1.0 / 12.0) * rrr);
-      SF[p] = pre_F * (double)V * (FFF - 0.75 * Frr);
+      Sr[p] = pre_r * (double)V * (rFF - qr * rrr);
+      SF[p] = pre_F * (double)V * (FFF - qF * Frr);
     }
   }
   free(Fe);

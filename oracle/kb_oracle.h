@@ -31,7 +31,10 @@ typedef struct or_ctx or_ctx;
  *       1 = LambdaNLO quantum (Evolution.jl:25-31),
  *       2 = NinverseLO (no self-energy, Hartree prefactor lambda/6:
  *           Evolution.jl:10-11, Initialization.jl:246),
- *       3 = NinverseNLO (resummed bubble chain, Evolution.jl:144-735). */
+ *       3 = NinverseNLO (resummed bubble chain, Evolution.jl:144-735),
+ *       4 = LambdaNLO classical (the sunset without its rho^2 terms,
+ *           Evolution.jl:107-120; the reference defines it for CPUred
+ *           storage, the same formulas on the full lattice). */
 or_ctx *or_create(int Nx, int sdim, int Nmem, double dt, double lambda,
                   int ONgroup, double Mass, double Mass2, int pexp,
                   int nthreads);

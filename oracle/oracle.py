@@ -87,7 +87,7 @@ class Oracle:
     """One reference-equivalent CPUfull simulation state.
 
     pexp: 0 = LambdaLO, 1 = LambdaNLO (quantum sunset), 2 = NinverseLO,
-    3 = NinverseNLO.
+    3 = NinverseNLO, 4 = LambdaNLO classical sunset.
     """
 
     def __init__(self, Nx, sdim, Nmem, dt, lam, ONgroup, Mass, Mass2, pexp=1, nthreads=None):

@@ -891,7 +891,7 @@ AdjArgs adj_args(kb_ctx* c, int t, int T) {
   a.h_ivol = h / (double)c->V;
   a.hl_ivol = c->p.Lambda * h / (double)c->V;
   a.two_over_lambda = c->p.Lambda != 0.0 ? 2.0 / c->p.Lambda : 0.0;
-  a.integral = c->p.pexp == KB_PEXP_LAMBDA_NLO;
+  a.integral = sunset_expansion(c->p.pexp);
   a.first = t == T;
   a.st = state_of_step(t, c->N);
   return a;
@@ -919,6 +919,7 @@ int launch_sigma_adjoint(kb_ctx* c) {
   a.X = s->X;
   a.Sbar = s->Sbar;
   a.Hbar = s->Hbar;
+  a.classical = c->p.pexp == KB_PEXP_LAMBDA_NLO_CLASSICAL;
   hipError_t e = launch_sigma_planes_adjoint(a, c->N - 1, c->stream);
   if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "adjoint self-energy launch failed: %s", hipGetErrorString(e));
   return KB_OK;
@@ -944,7 +945,7 @@ void set_pending(kb_ctx* c, AdjArgs& a, int k0, int n) {
 int reverse_step(kb_ctx* c, int t, int T, int npend, bool apply) {
   AdjState* s = c->adj;
   const int N = c->N;
-  const bool nlo = c->p.pexp == KB_PEXP_LAMBDA_NLO;
+  const bool nlo = sunset_expansion(c->p.pexp);
   AdjArgs a = adj_args(c, t, T);
   int rc;
   {
@@ -1057,9 +1058,9 @@ int kb_adj_record(kb_ctx* c, int checkpoint_every, int max_steps) {
     return seterr(c, KB_ERR_STATE, "adjoint recording starts right after initialisation (window [1,2])");
   if (c->in_ensemble) return seterr(c, KB_ERR_STATE, "adjoint: not available for ensemble members");
   if (c->sym) return seterr(c, KB_ERR_STATE, "adjoint: not available with cubic-orbit storage");
-  if (c->p.pexp != KB_PEXP_LAMBDA_LO && c->p.pexp != KB_PEXP_LAMBDA_NLO)
+  if (c->p.pexp != KB_PEXP_LAMBDA_LO && !sunset_expansion(c->p.pexp))
     return seterr(c, KB_ERR_ARG, "adjoint: lambda expansions only");
-  if (c->p.pexp == KB_PEXP_LAMBDA_NLO && !sigma_planes_supported(c->p.Nx))
+  if (sunset_expansion(c->p.pexp) && !sigma_planes_supported(c->p.Nx))
     return seterr(c, KB_ERR_ARG, "adjoint: Nx=%d above the plane self-energy limit", c->p.Nx);
   {
     // per function, shared by every context: the largest supported window

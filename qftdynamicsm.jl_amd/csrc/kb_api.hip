@@ -136,6 +136,7 @@ int launch_hartree(kb_ctx* c, int i, int j) {
 // the orbit products' arguments for the step in st
 OrbitArgs orbit_args(kb_ctx* c, const StepState* st, double preF, double preR) {
   OrbitArgs o{};
+  o.classical = c->p.pexp == KB_PEXP_LAMBDA_NLO_CLASSICAL;
   o.Mo = c->Mo;
   o.H = c->H;
   o.Y = c->Yo;
@@ -184,6 +185,7 @@ int launch_sigma(kb_ctx* c, double2* X, double* S_out) {
   a.preF = ivol * ivol * ivol * (-(lam * lam) * (ON + 2.0) / (18.0 * ON * ON));
   a.preR = ivol * ivol * ivol * (-(lam * lam) * (ON + 2.0) / (6.0 * ON * ON));
   a.X = X;
+  a.classical = c->p.pexp == KB_PEXP_LAMBDA_NLO_CLASSICAL;
   const int L = rows;
   if (c->fft_tile == KB_FFT_ORBIT_PRODUCT) {
     if (!c->Mo || X || S_out) return seterr(c, KB_ERR_STATE, "self-energy: the orbit product serves the forward step only");
@@ -427,7 +429,7 @@ int enqueue_step(kb_ctx* c) {
   int rc;
   hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, c->stream, c->stptr_dev, c->N_dev, 1);
   LAUNCHCHK(c);
-  if (c->p.pexp == KB_PEXP_LAMBDA_NLO) {
+  if (sunset_expansion(c->p.pexp)) {
     StageTimer st(c, 1);
     if ((rc = launch_sigma(c))) return rc;
   } else if (c->p.pexp == KB_PEXP_NINVERSE_NLO) {
@@ -606,7 +608,7 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
   *out = nullptr;
   if (p->sdim < 1 || p->sdim > 3 || p->Nx < 1 || p->Nx > 128 || p->NstepsinMemory < 3 ||
       p->NstepsinMemory > 640 || p->ONgroup < 1 || !(p->dt > 0) || p->pexp < KB_PEXP_LAMBDA_LO ||
-      p->pexp > KB_PEXP_NINVERSE_NLO)
+      p->pexp > KB_PEXP_LAMBDA_NLO_CLASSICAL)
     return KB_ERR_ARG;
   // the 1/N stage runs on the plane kernels (in orbit storage also as orbit
   // products: checked below, once the orbits are known)
@@ -643,7 +645,7 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
   // orbit storage gathers and scatters through the site maps in the plane
   // kernels only (Nx <= 64); past them, a transform stage needs the orbit
   // products (LambdaLO has none)
-  if (sym && !sigma_planes_supported(p->Nx) && (p->pexp == KB_PEXP_LAMBDA_NLO || p->pexp == KB_PEXP_NINVERSE_NLO) &&
+  if (sym && !sigma_planes_supported(p->Nx) && (sunset_expansion(p->pexp) || p->pexp == KB_PEXP_NINVERSE_NLO) &&
       c->V > KB_ORBIT_PRODUCT_MAX)
     return fail(KB_ERR_ARG);
   if (hipSetDevice(p->device) != hipSuccess) return fail(KB_ERR_HIP);
@@ -687,7 +689,7 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
   // orbit storage, LambdaNLO: the self-energy stage as products with the
   // orbit cosine matrix Mo[o][b] = sum_{x in orbit b} cos(2 pi k_o . x / Nx),
   // k_o the representative of orbit o (zero-padded to the product's tiles)
-  if (sym && (p->pexp == KB_PEXP_LAMBDA_NLO || p->pexp == KB_PEXP_NINVERSE_NLO) && c->V <= KB_ORBIT_PRODUCT_MAX) {
+  if (sym && (sunset_expansion(p->pexp) || p->pexp == KB_PEXP_NINVERSE_NLO) && c->V <= KB_ORBIT_PRODUCT_MAX) {
     c->Mp = (c->V + ORBIT_TILE - 1) / ORBIT_TILE * ORBIT_TILE;
     c->Kp = (c->V + ORBIT_K - 1) / ORBIT_K * ORBIT_K;
     std::vector<double> mo((size_t)c->Mp * c->Kp, 0.0), ctab(p->Nx);
@@ -1436,7 +1438,7 @@ int kb_autotune_fft(kb_ctx* c, int reps, int* best, double* ms) {
   if (!c->initialised || c->st.tmax < 3)
     return seterr(c, KB_ERR_STATE, "autotune: take a step first (the last step's self-energy is re-timed)");
   if (c->in_ensemble || c->adj) return seterr(c, KB_ERR_STATE, "autotune: not for ensemble members or while recording");
-  if (c->p.pexp != KB_PEXP_LAMBDA_NLO || !sigma_planes_supported(c->p.Nx))
+  if (!sunset_expansion(c->p.pexp) || !sigma_planes_supported(c->p.Nx))
     return seterr(c, KB_ERR_ARG, "autotune: no plane self-energy stage for this context");
   // the plane kernels' block sizes, then (orbit storage) the orbit product
   const int nv = NFFT_VARIANTS + (c->Mo ? 1 : 0);
@@ -1627,7 +1629,7 @@ int ens_enqueue(kb_ens* e) {
       if (e->lane[i] != j) continue;
       kb_ctx* c = e->m[i];
       c->stream = sj;
-      if (c->p.pexp == KB_PEXP_LAMBDA_NLO && (rc = launch_sigma(c)))
+      if (sunset_expansion(c->p.pexp) && (rc = launch_sigma(c)))
         rc = ens_err(e, rc, "member self-energy: %s", c->err.c_str());
       else if (c->p.pexp == KB_PEXP_NINVERSE_NLO && (rc = launch_ninv_stage(c)))
         rc = ens_err(e, rc, "member 1/N self-energy: %s", c->err.c_str());

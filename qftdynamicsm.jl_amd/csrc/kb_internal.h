@@ -158,8 +158,13 @@ bool plane_workspace_missing(const kb_ctx* c);
 // Hartreeprefactor of the context's expansion (Initialization.jl:245-247)
 double hartree_prefactor(const kb_ctx* c);
 // expansions with a self-energy (TwoPIScalarNLO, Problemdefinition.jl:48)
+// the lambda expansion's sunset, quantum or classical (TwoPIScalarLambdaNLO,
+// Problemdefinition.jl:37)
+inline bool sunset_expansion(int pexp) {
+  return pexp == KB_PEXP_LAMBDA_NLO || pexp == KB_PEXP_LAMBDA_NLO_CLASSICAL;
+}
 inline bool has_sigma(const kb_ctx* c) {
-  return c->p.pexp == KB_PEXP_LAMBDA_NLO || c->p.pexp == KB_PEXP_NINVERSE_NLO;
+  return sunset_expansion(c->p.pexp) || c->p.pexp == KB_PEXP_NINVERSE_NLO;
 }
 // orbits of the lattice's cubic group (axis permutations, reflections
 // n -> Nx - n): lattice site -> orbit, and orbit -> its first site

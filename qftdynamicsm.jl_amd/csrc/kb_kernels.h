@@ -112,6 +112,7 @@ struct SigmaArgs {
   double* SigFout;
   double* Sigrout;
   int rows;
+  int classical;  // the sunset without its rho^2 terms (LambdaNLOclassical)
 };
 
 // The resummation sweeps of the 1/N expansion (kb_ninverse.hip): one
@@ -260,6 +261,7 @@ struct OrbitArgs {
   const double* Iring;
   long PNP;
   double ivol;
+  int classical;  // the sunset without its rho^2 terms (LambdaNLOclassical)
 };
 // what the two products compute: the sunset (Sigma into the ring), the 1/N
 // bubbles of row t-1 (into the bubble triangles), the 1/N chain (Sigma)

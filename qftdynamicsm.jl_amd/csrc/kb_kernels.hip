@@ -211,11 +211,13 @@ __global__ __launch_bounds__(256) void sigma_pass(SigmaArgs a) {
   double2* res = lds_fft(A, B, n, P, a.plan, a.tw, (a.flags & SIGMA_INVERSE) != 0);
   if (a.flags & SIGMA_SUNSET) {
     double2* other = res == A ? B : A;
+    // quantum sunset; the classical one drops the rho^2 terms
+    const double qF = a.classical ? 0.0 : 0.75, qr = a.classical ? 0.0 : 1.0 / 12.0;
     for (int e = threadIdx.x; e < P * n; e += blockDim.x) {
       const double2 z = res[e];
       const double Fx = z.x, rx = z.y;
       const double FF = Fx * Fx, rr = rx * rx;
-      res[e] = make_double2(Fx * (FF - 0.75 * rr), rx * (FF - (1.0 / 12.0) * rr));
+      res[e] = make_double2(Fx * (FF - qF * rr), rx * (FF - qr * rr));
     }
     __syncthreads();
     res = lds_fft(res, other, n, P, a.plan, a.tw, true);

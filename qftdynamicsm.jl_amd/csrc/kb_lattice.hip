@@ -74,7 +74,7 @@ int check_params(const kb_params* p, int nrows) {
   if (p->Nx < 1 || p->Nx > NX_MAX) return fail(KB_ERR_ARG, "sunset: Nx = %d outside 1 .. %d", p->Nx, NX_MAX);
   if (p->sdim < 1 || p->sdim > 3) return fail(KB_ERR_ARG, "sunset: sdim = %d", p->sdim);
   if (p->ONgroup < 1) return fail(KB_ERR_ARG, "sunset: ONgroup = %d", p->ONgroup);
-  if (p->pexp != KB_PEXP_LAMBDA_NLO)
+  if (!sunset_expansion(p->pexp))
     return fail(KB_ERR_ARG, "sunset: pexp = %d has no sunset self-energy", p->pexp);
   if (nrows < 1 || nrows > MAX_ROWS) return fail(KB_ERR_ARG, "sunset: nrows = %d outside 1 .. %d", nrows, MAX_ROWS);
   if (p->device < 0 || p->device >= MAX_DEVICES) return fail(KB_ERR_ARG, "sunset: device %d", p->device);
@@ -161,6 +161,7 @@ int run_sunset(const kb_params* p, int nrows, const double* F, const double* rho
   a.SigFout = SigF;
   a.Sigrout = Sigr;
   a.rows = nrows;
+  a.classical = p->pexp == KB_PEXP_LAMBDA_NLO_CLASSICAL;
   e = launch_sigma_planes_plain(a, st);
   if (e != hipSuccess) return fail(KB_ERR_HIP, "sunset: plane kernel launch failed: %s", hipGetErrorString(e));
   return KB_OK;

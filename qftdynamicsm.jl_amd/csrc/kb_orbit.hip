@@ -202,9 +202,11 @@ __global__ __launch_bounds__(OTHREADS) void orbit_gemm(OrbitArgs a) {
       const bool live = o < a.V && q < L;
       const size_t gi = (size_t)o * a.ldg + col;
       if constexpr (MODE == OM_SUNSET) {
+        // quantum sunset; the classical one drops the rho^2 terms
+        const double qF = a.classical ? 0.0 : 0.75, qr = a.classical ? 0.0 : 1.0 / 12.0;
         const double Fx = comp ? other : mine, rx = comp ? mine : other;
         const double FF = Fx * Fx, rr = rx * rx;
-        const double g = comp ? rx * (FF - (1.0 / 12.0) * rr) : Fx * (FF - 0.75 * rr);
+        const double g = comp ? rx * (FF - qr * rr) : Fx * (FF - qF * rr);
         if (live) a.G[gi] = g;
       } else if constexpr (MODE == OM_BUBBLE) {
         // F2kr2 and Fr integrands: Fx^2 - rx^2 / 4, Fx rx

@@ -203,26 +203,30 @@ __device__ __forceinline__ double2* fft_lines(double2* a, double2* b, const FFTP
     return stages_rt(a, b, pl, nlines, ls, es, tw, inv);
 }
 
-__device__ __forceinline__ double2 sunset(double2 z) {
+// (Fx (Fx^2 - qF rx^2), rx (Fx^2 - qr rx^2)): qF = 3/4, qr = 1/12 for the
+// quantum sunset (Evolution.jl:25-31), 0 for the classical one (:107-120)
+__device__ __forceinline__ double2 sunset(double2 z, int classical) {
+  const double qF = classical ? 0.0 : 0.75, qr = classical ? 0.0 : 1.0 / 12.0;
   const double Fx = z.x, rx = z.y;
   const double FF = Fx * Fx, rr = rx * rx;
-  return make_double2(Fx * (FF - 0.75 * rr), rx * (FF - (1.0 / 12.0) * rr));
+  return make_double2(Fx * (FF - qF * rr), rx * (FF - qr * rr));
 }
 
 // transpose of the sunset's Jacobian at (Fx, rx) applied to
 // (gF, gr) = (preF, preR) * (C SigF_bar, C Sigr_bar)
-__device__ __forceinline__ double2 sunset_adj(double2 cs, double2 x, double preF, double preR) {
+__device__ __forceinline__ double2 sunset_adj(double2 cs, double2 x, double preF, double preR, int classical) {
+  const double a1 = classical ? 0.0 : 0.75, a2 = classical ? 0.0 : -1.5, a3 = classical ? 0.0 : 0.25;
   const double gF = preF * cs.x, gr = preR * cs.y;
   const double Fx = x.x, rx = x.y;
   const double FF = Fx * Fx, rr = rx * rx;
-  return make_double2(gF * (3.0 * FF - 0.75 * rr) + gr * (2.0 * rx * Fx), gF * (-1.5 * Fx * rx) + gr * (FF - 0.25 * rr));
+  return make_double2(gF * (3.0 * FF - a1 * rr) + gr * (2.0 * rx * Fx), gF * (a2 * Fx * rx) + gr * (FF - a3 * rr));
 }
 
 // the pointwise step of each mode; the sunset and bubble save (Fx, rx) in X when it is set
 template <int MODE>
 __device__ __forceinline__ double2 pointwise(const SigmaArgs& a, double2 z, size_t xi) {
   if constexpr (MODE == SG_SUNSET_ADJ) {
-    return sunset_adj(z, a.X[xi], a.preF, a.preR);
+    return sunset_adj(z, a.X[xi], a.preF, a.preR, a.classical);
   } else if constexpr (MODE == SG_BUBBLE) {
     if (a.X) a.X[xi] = z;
     return make_double2(z.x * z.x - 0.25 * (z.y * z.y), z.x * z.y);
@@ -232,7 +236,7 @@ __device__ __forceinline__ double2 pointwise(const SigmaArgs& a, double2 z, size
     return make_double2(x.x * IFx - 0.25 * (x.y * Irx), x.x * Irx + x.y * IFx);
   } else {
     if (a.X) a.X[xi] = z;
-    return sunset(z);
+    return sunset(z, a.classical);
   }
 }
 

@@ -77,7 +77,8 @@ pexpcode(::TwoPIScalarLambdaLO) = Cint(0)
 pexpcode(::TwoPIScalarLambdaNLOquantum) = Cint(1)
 pexpcode(::TwoPIScalarNinverseLO) = Cint(2)
 pexpcode(::TwoPIScalarNinverseNLO) = Cint(3)
-pexpcode(p) = throw(ArgumentError("GPUfull implements LambdaLO, LambdaNLO (quantum), NinverseLO and NinverseNLO, not $(typeof(p))"))
+pexpcode(::TwoPIScalarLambdaNLOclassical) = Cint(4)
+pexpcode(p) = throw(ArgumentError("GPUfull implements LambdaLO, LambdaNLO (quantum and classical), NinverseLO and NinverseNLO, not $(typeof(p))"))
 
 # replaces the CPUfull branch of getTwoPISimData
 getTwoPISimData(problem::QFTdynamicsProblem, ::TwoPIScalarGPUfull) = gpusimdata(problem, :kb_create)

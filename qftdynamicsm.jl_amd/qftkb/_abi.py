@@ -30,8 +30,10 @@ KB_PEXP_NINVERSE_LO = 2
 KB_PEXP_NINVERSE_NLO = 3
 # self-energy form of orbit-storage LambdaNLO contexts: orbit cosine products
 FFT_ORBIT_PRODUCT = -1
+# TwoPIScalarLambdaNLOclassical: the sunset without its rho^2 terms
+KB_PEXP_LAMBDA_NLO_CLASSICAL = 4
 PEXP_CODES = {"LambdaLO": KB_PEXP_LAMBDA_LO, "LambdaNLO": KB_PEXP_LAMBDA_NLO, "NinverseLO": KB_PEXP_NINVERSE_LO,
-              "NinverseNLO": KB_PEXP_NINVERSE_NLO}
+              "NinverseNLO": KB_PEXP_NINVERSE_NLO, "LambdaNLOclassical": KB_PEXP_LAMBDA_NLO_CLASSICAL}
 
 # every symbol include/qftkb.h declares
 EXPORTED = (
@@ -246,13 +248,14 @@ def sunset_workspace_bytes(Nx, sdim, nrows):
 
 
 def sunset(Nx, sdim, Lambda, ONgroup, nrows, F, rho, SigF, Sigr, workspace, workspace_bytes, stream=None,
-           device=0):
+           device=0, classical=False):
     """calcSelfEnergyattp! (TwoPIScalar_Evolution.jl:25-31) on nrows device
     lattices: F, rho, SigF, Sigr and workspace are device addresses (ints),
     rows of Nx^sdim doubles.  Enqueued on `stream` (a hipStream_t address,
     None = the null stream); does not wait."""
     L = load_library()
-    p = make_params(Nx, sdim, 3, 0.1, Lambda, ONgroup, 1.0, 1.0, device=device)
+    p = make_params(Nx, sdim, 3, 0.1, Lambda, ONgroup, 1.0, 1.0, device=device,
+                    pexp=KB_PEXP_LAMBDA_NLO_CLASSICAL if classical else KB_PEXP_LAMBDA_NLO)
     _free_chk(L.kb_sunset(ctypes.byref(p), int(nrows), F, rho, SigF, Sigr, workspace, int(workspace_bytes),
                           stream))
 

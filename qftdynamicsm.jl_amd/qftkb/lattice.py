@@ -41,7 +41,8 @@ def levels(kL2: np.ndarray, decimals: int = 12):
 
 
 def hartreeprefactor(Lambda: float, ONgroup: int, pexp: str = "LambdaNLO") -> float:
-    """Initialization.jl:245-247."""
+    """Initialization.jl:245-247 (every lambda expansion, LambdaNLOclassical
+    included, and NinverseNLO: lambda (N+2)/(6N))."""
     if pexp == "NinverseLO":
         return Lambda * (1 / 6)
     return Lambda * (ONgroup + 2) / (6 * ONgroup)

@@ -108,6 +108,9 @@ def reverse(kb, T, wgt, initmass, record=None, stop=None):
     for k, v in seed_from_n(kb, T, wgt).items():
         Fb[k] += v
     lam_bar = 0.0
+    # rho^2 terms of the sunset: quantum (Evolution.jl:25-31), or none for the
+    # classical sunset (pexp 4, :107-120)
+    qF, qr = (0.0, 0.0) if kb.pexp == 4 else (0.75, 1.0 / 12.0)
     kb.tmax = T
     for t in range(T, 2, -1):
         tmin = kb.tmin_at[t]
@@ -119,8 +122,8 @@ def reverse(kb, T, wgt, initmass, record=None, stop=None):
         for z in range(tmin, t):
             Fx[z] = cosT(F(t - 1, z), shape)
             rx[z] = cosT(r(t - 1, z), shape)
-            SF[z] = pF * cosT(Fx[z] * (Fx[z] ** 2 - 0.75 * rx[z] ** 2), shape) if kb.pexp else np.zeros(V)
-            Sr[z] = pr * cosT(rx[z] * (Fx[z] ** 2 - rx[z] ** 2 / 12.0), shape) if kb.pexp else np.zeros(V)
+            SF[z] = pF * cosT(Fx[z] * (Fx[z] ** 2 - qF * rx[z] ** 2), shape) if kb.pexp else np.zeros(V)
+            Sr[z] = pr * cosT(rx[z] * (Fx[z] ** 2 - qr * rx[z] ** 2), shape) if kb.pexp else np.zeros(V)
         M2 = kb.m2_at[t]
         Om = (kb.kL2 + M2) * dt ** 2
         w = {z: (0.5 if z == tmin else 1.0) for z in range(tmin, t)}
@@ -179,8 +182,8 @@ def reverse(kb, T, wgt, initmass, record=None, stop=None):
             for z in range(tmin, t):
                 gF = pF * cosT(SFb[z], shape)
                 gr = pr * cosT(Srb[z], shape)
-                Fxb = gF * (3 * Fx[z] ** 2 - 0.75 * rx[z] ** 2) + gr * 2 * rx[z] * Fx[z]
-                rxb = gF * (-1.5 * Fx[z] * rx[z]) + gr * (Fx[z] ** 2 - 0.25 * rx[z] ** 2)
+                Fxb = gF * (3 * Fx[z] ** 2 - qF * rx[z] ** 2) + gr * 2 * rx[z] * Fx[z]
+                rxb = gF * (-2 * qF * Fx[z] * rx[z]) + gr * (Fx[z] ** 2 - 3 * qr * rx[z] ** 2)
                 addF(t - 1, z, cosT(Fxb, shape))
                 if z != t - 1:
                     addr(t - 1, z, cosT(rxb, shape))

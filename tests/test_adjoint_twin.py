@@ -38,6 +38,7 @@ CASES = [
     pytest.param((6, 2, 6, 0.08, 1.7, 4, 0.9, 0.81, 1), 15, id="6^2-O(4)-sliding"),
     pytest.param((5, 1, 4, 0.1, 2.0, 2, 1.0, 1.0, 1), 11, id="5-1d-minimal-window"),
     pytest.param((4, 3, 5, 0.1, 1.0, 1, 1.0, 1.0, 0), 14, id="4^3-LO"),
+    pytest.param((4, 3, 6, 0.1, 1.6, 1, 1.0, 1.0, 4), 13, id="4^3-classical"),
 ]
 
 

@@ -72,6 +72,8 @@ CASES = [
     pytest.param((5, 1, 4, 0.1, 2.0, 2, 1.0, 1.0, 1), 11, id="5-1d"),
     pytest.param((5, 3, 7, 0.1, 1.2, 1, 1.0, 1.0, 1), 25, id="5^3-ragged-blocks"),
     pytest.param((4, 3, 5, 0.1, 1.0, 1, 1.0, 1.0, 0), 14, id="4^3-LO"),
+    pytest.param((4, 3, 6, 0.1, 1.6, 1, 1.0, 1.0, 4), 13, id="4^3-classical"),
+    pytest.param((6, 3, 5, 0.1, 1.4, 2, 1.0, 1.0, 4), 12, id="6^3-classical-O(2)"),
 ]
 
 
@@ -250,6 +252,45 @@ def test_differentiation_workload_vs_oracle_fd_fixture():
     if fdb and all(v is not None for v in fdb):
         edges = np.linspace(0.0, np.sqrt(12.0), 17)
         b = np.minimum(np.searchsorted(edges, np.sqrt(k2), side="right") - 1, 15)
+        ad = np.array([g["n0"][b == i].sum() for i in range(16)])
+        assert np.max(np.abs(ad - np.array(fdb))) <= FD_TOL * np.max(np.abs(fdb))
+    c.close()
+
+
+@pytest.mark.parametrize("C", [0, 25, -1], ids=["taped", "checkpoints-25", "packed-tape"])
+def test_largest_lattice_gradient_vs_oracle_fd_fixture(C):
+    """The 48^3 lattice with the box state and coupling of the 4000-step
+    N_mem = 300 differentiation (n0 = 4, Q = 1, lambda = 0.5, dt = 0.05) at
+    a window the oracle can run: N_mem = 20, 100 steps.  The gradient, taped,
+    checkpointed or on the orbit-packed tape, against the oracle's central
+    finite differences with Richardson extrapolation, R(h) = (4 D(h) - D(2h))/3
+    at h = 5e-4 (h = 1e-3 for the radial bins), committed in
+    tests/golden/adjoint_48c_n20_fd.json (tests/golden/make_adjoint_fd_48.py),
+    to 1e-6.  (D(1e-5) alone is limited to about 1e-6 by the rounding of L.)"""
+    import json
+    import os
+    from qftkb import KBContext, lattice
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "adjoint_48c_n20_fd.json")))
+    print(f"\n[case] 48^3 N_mem=20 100 steps, recording mode {C}", flush=True)
+    c = KBContext(gold["Nx"], gold["sdim"], gold["N_mem"], gold["dt"], gold["Lambda"], gold["ONgroup"],
+                  gold["Mass"], gold["Mass"] ** 2)
+    k2 = c.kL2()
+    n0 = lattice.n_box(lattice.kL2_table(gold["Nx"], gold["sdim"]), 4.0, 1.0)
+    wgt = np.exp(-k2)
+    T = gold["T_final"]
+    c.init_particles(n0, gold["Mass"])
+    c.adj_record(C, T)
+    for t in range(3, T + 1):
+        c.step(t)
+    L = float((wgt * c.measure(T)["n"]).sum())
+    assert L == pytest.approx(0.5 * (gold["L_plus_1e-05"] + gold["L_minus_1e-05"]), rel=1e-9)
+    g = c.adj_gradient(wgt)
+    assert gold["richardson_spread"] <= 2e-7  # R(5e-4) and R(1e-3) agree: the fixture is good to ~1e-7
+    fd = gold["dL_dlambda_richardson"]
+    assert abs(g["lambda"] - fd) <= FD_TOL * abs(fd), (g["lambda"], fd)
+    fdb = gold.get("dL_dn0_bins_fd")
+    if fdb and all(v is not None for v in fdb):
+        b = lattice.radial_bins(k2, gold["sdim"], 16)
         ad = np.array([g["n0"][b == i].sum() for i in range(16)])
         assert np.max(np.abs(ad - np.array(fdb))) <= FD_TOL * np.max(np.abs(fdb))
     c.close()

@@ -36,15 +36,16 @@ def test_dft_matches_numpy(Nx, sdim):
     assert np.max(np.abs(o.dft(z, +1) - ref_b)) <= 1e-13 * np.max(np.abs(ref_b))
 
 
+@pytest.mark.parametrize("pexp", [1, 4], ids=["quantum", "classical"])
 @pytest.mark.parametrize("Nx,sdim", [(4, 3), (3, 3), (6, 2), (5, 2), (8, 1), (2, 3)])
-def test_sunset_fft_equals_direct_convolution(Nx, sdim):
+def test_sunset_fft_equals_direct_convolution(Nx, sdim, pexp):
     """Evolution.jl:25-31 (FFT) against the momentum-space double
     convolution on random, non-symmetric two-time data (the role of the
     reference's fast-vs-simple test, Tests/..._LambdaNLO_checkfastimplementation.jl)."""
     N = 5
     rng = np.random.default_rng(7 + Nx)
-    o_fft = Oracle(Nx, sdim, N, 0.1, 1.7, 3, 1.0, 1.0, nthreads=2)
-    o_dir = Oracle(Nx, sdim, N, 0.1, 1.7, 3, 1.0, 1.0, nthreads=2)
+    o_fft = Oracle(Nx, sdim, N, 0.1, 1.7, 3, 1.0, 1.0, pexp=pexp, nthreads=2)
+    o_dir = Oracle(Nx, sdim, N, 0.1, 1.7, 3, 1.0, 1.0, pexp=pexp, nthreads=2)
     for o in (o_fft, o_dir):
         o.init_particles(np.zeros(o.V), 1.0)
         o.expand()
@@ -84,12 +85,13 @@ def test_sunset_prefactors_against_numpy_rfft():
     assert np.max(np.abs(o.sigma("r", 1) - Sr)) <= 1e-12 * np.max(np.abs(Sr))
 
 
-@pytest.mark.parametrize("Nx,sdim,Nmem,steps", [(4, 3, 5, 22), (6, 2, 40, 20), (5, 3, 4, 15), (8, 1, 7, 30),
-                                                (1, 3, 3, 25)])
-def test_oracle_matches_numpy_twin(Nx, sdim, Nmem, steps):
+@pytest.mark.parametrize("Nx,sdim,Nmem,steps,pexp", [(4, 3, 5, 22, 1), (6, 2, 40, 20, 1), (5, 3, 4, 15, 1),
+                                                     (8, 1, 7, 30, 1), (1, 3, 3, 25, 1),
+                                                     (5, 3, 6, 16, 4), (6, 2, 30, 20, 4)])
+def test_oracle_matches_numpy_twin(Nx, sdim, Nmem, steps, pexp):
     args = (Nx, sdim, Nmem, 0.1, 1.3, 2, 1.0, 1.0)
-    o = Oracle(*args, nthreads=2)
-    p = NumpyKB(*args)
+    o = Oracle(*args, pexp=pexp, nthreads=2)
+    p = NumpyKB(*args, pexp=pexp)
     np.testing.assert_allclose(o.kL2(), kL2_table(Nx, sdim), rtol=4e-16, atol=4e-16)
     n = thermal(p.kL2)
     o.init_particles(n, 1.0)
@@ -194,16 +196,18 @@ def energy_converges(d):
 
 
 ECONS_CASES = [(4, 3, 1), (6, 2, 4)]  # Nx, sdim, O(N)
+NLO_EXPANSIONS = pytest.mark.parametrize("pexp", [1, 4], ids=["quantum", "classical"])
 
 
+@NLO_EXPANSIONS
 @pytest.mark.parametrize("Nx,sdim,ON", ECONS_CASES)
-def test_energy_conservation_nlo(Nx, sdim, ON):
+def test_energy_conservation_nlo(Nx, sdim, ON, pexp):
     lam = 10.0
-    fac = lambda N, dt: Oracle(Nx, sdim, N, dt, lam, ON, 1.0, 1.0, nthreads=2)  # noqa: E731
+    fac = lambda N, dt: Oracle(Nx, sdim, N, dt, lam, ON, 1.0, 1.0, pexp=pexp, nthreads=2)  # noqa: E731
     d = [energy_drift(fac, dt) for dt in ECONS_DTS]
     assert energy_converges(d), f"energy drift {d}"
     # the independent numpy restatement conserves it the same way
-    twin = lambda N, dt: NumpyKB(Nx, sdim, N, dt, lam, ON, 1.0, 1.0)  # noqa: E731
+    twin = lambda N, dt: NumpyKB(Nx, sdim, N, dt, lam, ON, 1.0, 1.0, pexp=pexp)  # noqa: E731
     dn = [energy_drift(twin, dt) for dt in ECONS_DTS]
     assert np.allclose(dn, d, rtol=1e-8), (dn, d)
 
@@ -239,13 +243,14 @@ def _damping_run(o, steps):
     return rho_envelope(o, steps + 2)
 
 
+@NLO_EXPANSIONS
 @pytest.mark.parametrize("Nx,sdim,ON", ECONS_CASES)
-def test_spectral_function_is_damped(Nx, sdim, ON):
+def test_spectral_function_is_damped(Nx, sdim, ON, pexp):
     """lambda = 10, full memory, t up to 15: the envelope of rho(t, 2) falls
     over each quarter of the run (correct code: 0.63 -> 0.43 on 4^3, 0.69 ->
     0.54 for O(4) on 6^2)."""
     steps = 150
-    env = _damping_run(Oracle(Nx, sdim, steps + 2, 0.1, 10.0, ON, 1.0, 1.0, nthreads=2), steps)
+    env = _damping_run(Oracle(Nx, sdim, steps + 2, 0.1, 10.0, ON, 1.0, 1.0, pexp=pexp, nthreads=2), steps)
     assert spectral_damped(env), env
 
 
