@@ -70,12 +70,44 @@ constexpr Fac factorize(int n) {
   return F;
 }
 
+// i z (sign +1) or -i z (sign -1)
+__device__ __forceinline__ double2 imul(double2 z, double sign) { return make_double2(-sign * z.y, sign * z.x); }
+
 template <int R>
 __device__ __forceinline__ void small_dft(double2 (&v)[R], const double2* __restrict__ tw, int n, bool inv) {
   if constexpr (R == 2) {
     const double2 a = v[0], b = v[1];
     v[0] = cadd(a, b);
     v[1] = csub(a, b);
+  } else if constexpr (R == 3) {
+    // w = exp(-+2 pi i / 3): X1,2 = x0 - (x1 + x2)/2 -+ i (sqrt 3 / 2)(x1 - x2), sign flipped for the inverse
+    constexpr double S3 = 0.86602540378443864676;  // sin(2 pi / 3)
+    const double sg = inv ? 1.0 : -1.0;
+    const double2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+    const double2 m = make_double2(v[0].x - 0.5 * t1.x, v[0].y - 0.5 * t1.y);
+    const double2 r = imul(make_double2(S3 * t2.x, S3 * t2.y), sg);
+    v[0] = cadd(v[0], t1);
+    v[1] = cadd(m, r);
+    v[2] = csub(m, r);
+  } else if constexpr (R == 5) {
+    // w = exp(-+2 pi i / 5) with c1 = cos(2 pi/5), c2 = cos(4 pi/5), s1 = sin(2 pi/5),
+    // s2 = sin(4 pi/5), t1 = x1 + x4, t2 = x2 + x3, t3 = x1 - x4, t4 = x2 - x3:
+    //   X1,4 = x0 + c1 t1 + c2 t2 -+ i (s1 t3 + s2 t4)
+    //   X2,3 = x0 + c2 t1 + c1 t2 -+ i (s2 t3 - s1 t4)   (i terms flipped for the inverse)
+    constexpr double C1 = 0.30901699437494742410, C2 = -0.80901699437494742410;
+    constexpr double S1 = 0.95105651629515357212, S2 = 0.58778525229247312917;
+    const double sg = inv ? 1.0 : -1.0;
+    const double2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    const double2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    const double2 a1 = make_double2(v[0].x + C1 * t1.x + C2 * t2.x, v[0].y + C1 * t1.y + C2 * t2.y);
+    const double2 a2 = make_double2(v[0].x + C2 * t1.x + C1 * t2.x, v[0].y + C2 * t1.y + C1 * t2.y);
+    const double2 b1 = imul(make_double2(S1 * t3.x + S2 * t4.x, S1 * t3.y + S2 * t4.y), sg);
+    const double2 b2 = imul(make_double2(S2 * t3.x - S1 * t4.x, S2 * t3.y - S1 * t4.y), sg);
+    v[0] = cadd(v[0], cadd(t1, t2));
+    v[1] = cadd(a1, b1);
+    v[4] = csub(a1, b1);
+    v[2] = cadd(a2, b2);
+    v[3] = csub(a2, b2);
   } else if constexpr (R == 4) {
     const double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
     const double2 b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
@@ -115,7 +147,8 @@ __device__ __forceinline__ void stage(const double2* __restrict__ src, double2* 
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const double2 x = s[(j + r * m) * es];
-      v[r] = r == 0 ? x : cmul(x, twid(tw, r * kk * step, inv));  // r*kk*step < n
+      // the first stage (Ns = 1) has unit twiddles
+      v[r] = (r == 0 || Ns == 1) ? x : cmul(x, twid(tw, r * kk * step, inv));  // r*kk*step < n
     }
     small_dft<R>(v, tw, n, inv);
     const int base = (j / Ns) * Ns * R + kk;
