@@ -209,7 +209,7 @@ struct MemintVariant {
   const void* fn;
   const void* partial;
 };
-constexpr int NMEMINT_VARIANTS = 9;
+constexpr int NMEMINT_VARIANTS = 13;
 extern const MemintVariant kMemintVariants[NMEMINT_VARIANTS];
 constexpr int MEMINT_ROW_CHOICES[3] = {32, 64, 128};  // item tables kept per context
 __global__ void advance_kernel(StepState* const* states, const int* Ns, int nruns);

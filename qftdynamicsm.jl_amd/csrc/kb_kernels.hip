@@ -337,14 +337,23 @@ __device__ __forceinline__ void slot_weights(int x, int smin, int N, int L, doub
 #ifndef KB_NT_LOADS
 #define KB_NT_LOADS 1
 #endif
+// Loads go through global-address-space pointers: the pointers come from the
+// argument struct, so plain dereferences compile to flat loads, whose waits
+// also drain the LDS queue and every load in flight (s_waitcnt vmcnt(0)
+// lgkmcnt(0)); global loads let a batch wait only for its own lines.
 typedef double kb_d2v __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) const kb_d2v kb_gd2v;
 __device__ __forceinline__ double2 ld_stream(const double* p) {
 #if KB_NT_LOADS
-  const kb_d2v v = __builtin_nontemporal_load(reinterpret_cast<const kb_d2v*>(p));
-  return make_double2(v.x, v.y);
+  const kb_d2v v = __builtin_nontemporal_load((kb_gd2v*)(p));
 #else
-  return *reinterpret_cast<const double2*>(p);
+  const kb_d2v v = *(kb_gd2v*)(p);
 #endif
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ double2 ld_global(const void* p) {
+  const kb_d2v v = *(kb_gd2v*)(p);
+  return make_double2(v.x, v.y);
 }
 
 template <int B>
@@ -364,7 +373,7 @@ __device__ __forceinline__ void load_batch(RowBatch<B>& b, const double* __restr
     const long pb = cbase + (long)(rr - c0) * CB;
 #pragma unroll
     for (int jc = 0; jc < CB; ++jc) b.e[h][jc] = ld_stream(Hb + (size_t)(pb + jc) * MB * 2);
-    b.fs[h] = Sb[(size_t)rr * MB];
+    b.fs[h] = ld_global(Sb + (size_t)rr * MB);
   }
 }
 
@@ -444,7 +453,7 @@ __device__ __forceinline__ void memint_sweep(const MemintArgs& a, const StepStat
       for (int jc = 0; jc < CB; ++jc) {
         const int c = c0 + jc;
         const bool ok = c < Rmax && c != dead;
-        const double2 fs = Sb[(size_t)min(c, N - 1) * MB];
+        const double2 fs = ld_global(Sb + (size_t)min(c, N - 1) * MB);
         double w, m;
         slot_weights(c, smin, N, L, w, m);
         cs.u[jc] = ok ? m * w * fs.y : 0.0;
@@ -457,7 +466,7 @@ __device__ __forceinline__ void memint_sweep(const MemintArgs& a, const StepStat
       // rows crossing the diagonal of the block: columns c0 .. r only
       for (; r < r1 && r < c0 + CB; ++r) {
         const bool okr = r != dead;
-        const double2 fs = Sb[(size_t)r * MB];
+        const double2 fs = ld_global(Sb + (size_t)r * MB);
         double w, m;
         slot_weights(r, smin, N, L, w, m);
         const double ur = okr ? m * w * fs.y : 0.0;
@@ -469,7 +478,7 @@ __device__ __forceinline__ void memint_sweep(const MemintArgs& a, const StepStat
         for (int jc = 0; jc < CB; ++jc) {
           const int c = c0 + jc;
           if (c > r) break;
-          const double2 e = *reinterpret_cast<const double2*>(Hb + (size_t)(pb + jc) * MB * 2);
+          const double2 e = ld_global(Hb + (size_t)(pb + jc) * MB * 2);
           if (c == r)
             rowF -= ur * e.x;
           else
@@ -541,9 +550,9 @@ __device__ __forceinline__ void memint_leapfrog(const MemintArgs& a, const StepS
 #pragma unroll
     for (int u = 0; u < LU; ++u) {
       const int x = min(x0 + u, Rmax - 1);
-      e1[u] = *reinterpret_cast<const double2*>(Hw + ((size_t)pair_off(ss.s_tm1, x, N) * MB + lane) * 2);
-      e2[u] = *reinterpret_cast<const double2*>(Hw + ((size_t)pair_off(ss.s_tm2, x, N) * MB + lane) * 2);
-      fsx[u] = Sb[(size_t)x * MB];
+      e1[u] = ld_global(Hw + ((size_t)pair_off(ss.s_tm1, x, N) * MB + lane) * 2);
+      e2[u] = ld_global(Hw + ((size_t)pair_off(ss.s_tm2, x, N) * MB + lane) * 2);
+      fsx[u] = ld_global(Sb + (size_t)x * MB);
     }
 #pragma unroll
     for (int u = 0; u < LU; ++u) {
@@ -683,6 +692,8 @@ const MemintVariant kMemintVariants[NMEMINT_VARIANTS] = {
     KB_MEMINT_SHAPE(64, 2, 256), KB_MEMINT_SHAPE(32, 4, 256),  KB_MEMINT_SHAPE(128, 4, 256),
     KB_MEMINT_SHAPE(64, 4, 256), KB_MEMINT_SHAPE(128, 2, 256), KB_MEMINT_SHAPE(32, 2, 256),
     KB_MEMINT_SHAPE(64, 4, 512), KB_MEMINT_SHAPE(128, 2, 512), KB_MEMINT_SHAPE(32, 4, 512),
+    KB_MEMINT_SHAPE(128, 1, 512), KB_MEMINT_SHAPE(64, 1, 512),  KB_MEMINT_SHAPE(128, 1, 256),
+    KB_MEMINT_SHAPE(64, 1, 256),
 };
 #undef KB_MEMINT_SHAPE
 

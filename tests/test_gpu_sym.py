@@ -300,7 +300,7 @@ def test_orbit_storage_every_tile_shape():
     for t in range(3, 31):
         ref.step(t)
     tmin, tmax = ref.window()
-    assert len(shapes) == 9
+    assert len(shapes) == 13
     for v in range(len(shapes)):
         print(f"\n[case] orbit storage, tile {shapes[v]}", flush=True)
         g = KBContext(*args, symmetric=True, tiles=None)
