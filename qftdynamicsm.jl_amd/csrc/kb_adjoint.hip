@@ -22,7 +22,7 @@
 //      integrals, reduced per slot in LDS from one read of H;
 //   4. the self-energy stage run backwards (kb_sigma.hip, ADJ) adds the
 //      sunset adjoint into row t-1 of Hbar;
-//   5. every AK reverse steps, adj_apply_kernel adds the memory integrals'
+//   5. every AK (8) reverse steps, adj_apply_kernel adds the memory integrals'
 //      entry adjoints of those steps to Hbar in one read-modify-write
 //      (adj_row_kernel adds the deferred terms of the row it consumes);
 //   6. adj_slide_kernel: row t of Hbar is cleared and the column t-N that
@@ -45,8 +45,10 @@ using namespace qftkb_host;
 namespace qftkb {
 
 // deferred reverse steps whose memory-integral entry updates are applied
-// together
-constexpr int AK = 4;
+// together: the adjoint window's read-modify-write costs 32 B per entry per
+// AK steps, and the row kernel reads the rings of up to AK-1 deferred steps
+// per slot of the row it consumes
+constexpr int AK = 8;
 
 // Per-step arguments of the reverse kernels.
 struct AdjArgs {
@@ -229,7 +231,7 @@ __global__ void adj_seed_kernel(const double* __restrict__ H, double* __restrict
 // in registers, row sums in LDS.
 // ---------------------------------------------------------------------------
 constexpr int AM_THREADS = 256;
-constexpr int AM_B = 4;  // reduce: rows per batch (two batches in flight)
+constexpr int AM_B = 1;  // reduce: rows per batch (two batches in flight)
 
 struct SlotCoef {
   double u, v, ms, rf, rr, w, m;
@@ -238,7 +240,7 @@ struct SlotCoef {
 __device__ __forceinline__ SlotCoef slot_coef(const AdjArgs& a, const double2* Ss, const double2* Rs, int x) {
   const StepState& st = a.st;
   const int N = a.N;
-  const int rel = (x - st.s_min + N) % N;
+  const int rel = window_pos(x, st.s_min, N);
   const int tx = st.tmin + rel;
   SlotCoef k;
   k.w = rel == 0 ? 0.5 : 1.0;
@@ -291,12 +293,18 @@ __device__ __forceinline__ PCoef pend_coef(const AdjArgs& a, int j, int mb, int 
 // self-energy terms are left to the memory-integral kernels through
 // R_bar_F(t).
 // ---------------------------------------------------------------------------
-constexpr int ROW_THREADS = 1024;
+// threads per workgroup: 1024, or 512 where the coefficients of four or more
+// deferred steps need more than 128 registers per lane
+template <int NP>
+constexpr int row_threads() {
+  return NP > 3 ? 512 : 1024;
+}
 
 // NP: number of deferred steps after t (a compile-time count, so only
 // their rings are read)
 template <int NP>
-__global__ __launch_bounds__(ROW_THREADS) void adj_row_kernel(AdjArgs a) {
+__global__ __launch_bounds__(row_threads<NP>()) void adj_row_kernel(AdjArgs a) {
+  constexpr int ROW_THREADS = row_threads<NP>();
   __shared__ double red[ROW_THREADS / 64];
   __shared__ double fl[2][MB];  // f_bar at t-1 and t-2
   const StepState& st = a.st;
@@ -443,7 +451,7 @@ __device__ __forceinline__ void red_pair(double2 e, bool n, double rfr, double r
 }
 
 __device__ __forceinline__ void slot_wm(const StepState& st, int N, int x, double& w, double& m) {
-  const int rel = (x - st.s_min + N) % N;
+  const int rel = window_pos(x, st.s_min, N);
   w = rel == 0 ? 0.5 : 1.0;
   m = st.tmin + rel <= st.t - 2 ? 1.0 : 0.0;
 }
@@ -593,9 +601,9 @@ __global__ __launch_bounds__(AM_THREADS, 2) void adj_reduce_kernel(AdjArgs a) {
 // columns), keep those columns' coefficients in registers and stream the
 // rows with AP_D rows in flight, one read-modify-write per adjoint line.
 constexpr int AP_THREADS = 256;
-constexpr int AP_ROWS = 64;
-constexpr int AP_C = 2;  // columns per worker
-constexpr int AP_D = 4;  // rows per batch (two batches in flight)
+constexpr int AP_ROWS = 256 / AK;   // 64 KB of staged row coefficients
+constexpr int AP_C = 2;             // columns per worker
+constexpr int AP_D = AK > 4 ? 2 : 4;  // rows per batch (two batches in flight)
 
 __device__ __forceinline__ size_t ap_lds(int row, int j, int q, int lane) {  // q: 0 v, 1 ms, 2 rf, 3 rr
   return (((size_t)row * AK + j) * 4 + q) * MB + lane;
@@ -627,7 +635,7 @@ __device__ __forceinline__ void ap_rows(const AdjArgs& a, double2 (&e)[AP_D][AP_
   for (int d = 0; d < AP_D; ++d) {
     const int rr = r + d;
     if (rr < r1) {
-      const int tx = st.tmin + (rr - st.s_min + N) % N;
+      const int tx = st.tmin + window_pos(rr, st.s_min, N);
       const unsigned neg = rr >= st.s_min ? cs.phi : 0u;
       const int row = rr - R0;
 #pragma unroll
@@ -680,7 +688,7 @@ __global__ __launch_bounds__(AP_THREADS, 2) void adj_apply_kernel(AdjArgs a) {
     for (int i = worker; i < (R1 - R0) * AK; i += nworkers) {
       const int row = i / AK, j = i - row * AK;
       const int r = R0 + row;
-      const PCoef k = pend_coef(a, j, mb, lane, r, st.tmin + (r - smin + N) % N);
+      const PCoef k = pend_coef(a, j, mb, lane, r, st.tmin + window_pos(r, smin, N));
       rco[ap_lds(row, j, 0, lane)] = k.v;
       rco[ap_lds(row, j, 1, lane)] = k.ms;
       rco[ap_lds(row, j, 2, lane)] = k.rf;
@@ -705,7 +713,7 @@ __global__ __launch_bounds__(AP_THREADS, 2) void adj_apply_kernel(AdjArgs a) {
         for (int k = 0; k < AP_C; ++k) {
           const int c = c0 + k0 + k;
           const int cc = min(c, Rv - 1);
-          const int txc = st.tmin + (cc - smin + N) % N;
+          const int txc = st.tmin + window_pos(cc, smin, N);
 #pragma unroll
           for (int j = 0; j < AK; ++j) {
             const PCoef q = pend_coef(a, j, mb, lane, cc, txc);
@@ -925,6 +933,15 @@ int launch_sigma_adjoint(kb_ctx* c) {
   return KB_OK;
 }
 
+// adj_row_kernel with its compile-time count of deferred steps (0 .. AK-1)
+template <int NP>
+void launch_row(int np, dim3 grid, hipStream_t st, const AdjArgs& a) {
+  if constexpr (NP + 1 < AK) {
+    if (np > NP) return launch_row<NP + 1>(np, grid, st, a);
+  }
+  hipLaunchKernelGGL(adj_row_kernel<NP>, grid, dim3(row_threads<NP>()), 0, st, a);
+}
+
 // deferred steps k = k0, k0+1, .. k0+n-1 into the argument block
 void set_pending(kb_ctx* c, AdjArgs& a, int k0, int n) {
   AdjState* s = c->adj;
@@ -958,13 +975,7 @@ int reverse_step(kb_ctx* c, int t, int T, int npend, bool apply) {
                        pair_off(a.st.s_tm1, a.st.s_tm1, N), H / (double)c->Vl, c->p.Mass2, s->m2r + (t & 1) * 2);
     LAUNCHCHK(c);
     set_pending(c, a, t + 1, nlo ? npend : 0);
-    static_assert(AK == 4, "adj_row_kernel instances cover 0 .. AK-1 deferred steps");
-    switch (a.npend) {
-      case 0: hipLaunchKernelGGL(adj_row_kernel<0>, dim3(c->NB), dim3(ROW_THREADS), 0, c->stream, a); break;
-      case 1: hipLaunchKernelGGL(adj_row_kernel<1>, dim3(c->NB), dim3(ROW_THREADS), 0, c->stream, a); break;
-      case 2: hipLaunchKernelGGL(adj_row_kernel<2>, dim3(c->NB), dim3(ROW_THREADS), 0, c->stream, a); break;
-      default: hipLaunchKernelGGL(adj_row_kernel<3>, dim3(c->NB), dim3(ROW_THREADS), 0, c->stream, a); break;
-    }
+    launch_row<0>(a.npend, dim3(c->NB), c->stream, a);
     LAUNCHCHK(c);
   }
   if (nlo) {

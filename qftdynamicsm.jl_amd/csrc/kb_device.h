@@ -52,6 +52,13 @@ __host__ __device__ inline long pair_off(int a, int b, int N) {
   return colblock_offset(J, N) + (long)(hi - J * CB) * CB + (lo - J * CB);
 }
 
+// position in the window of ring slot x when the window's first time sits in
+// slot smin (both in [0, N)): x - smin wrapped once, with no integer division
+__host__ __device__ inline int window_pos(int x, int smin, int N) {
+  const int rel = x - smin;
+  return rel < 0 ? rel + N : rel;
+}
+
 // double offset of (block, pair, lane), component F; component r is +1
 __host__ __device__ inline size_t hoff(int mb, long pair, int lane, long NP) {
   return ((((size_t)mb * (size_t)NP + (size_t)pair) * MB) + lane) * 2;

@@ -326,13 +326,6 @@ __global__ void advance_kernel(StepState* const* states, const int* Ns, int nrun
 // autotuner): ROWS slot rows per work item (one chunk of a column block),
 // B rows per batch, two batches in flight.
 
-// x and smin are ring slots in [0, N): the position of x in the window is
-// x - smin, wrapped once (no integer division in the row loop)
-__device__ __forceinline__ int window_pos(int x, int smin, int N) {
-  const int rel = x - smin;
-  return rel < 0 ? rel + N : rel;
-}
-
 __device__ __forceinline__ void slot_weights(int x, int smin, int N, int L, double& w, double& m) {
   const int rel = window_pos(x, smin, N);
   w = rel == 0 ? 0.5 : 1.0;
