@@ -9,11 +9,16 @@
 // and both transforms come out as its real and imaginary parts.
 //
 //   3D: plane_fwd  gathers the z-planes iz and -iz of F, rho, symmetrises,
-//                  2D-transforms plane iz in LDS and writes plane iz and its
-//                  point reflection as plane -iz (the 2D transform of the
-//                  even lattice's plane -iz is the reflection of plane iz's);
+//                  2D-transforms plane iz in LDS and writes it;
 //       zpass      z transform, sunset products, inverse z transform;
 //       plane_inv  inverse 2D transform, scale, write SigF/Sigr.
+//   Every partial transform W of an even lattice is even under the point
+//   reflection of all three indices, W(-x, -y, -z) = W(x, y, z), so the
+//   workspace holds only the half space z in [0, nz/2] of each row: plane_fwd
+//   runs the planes of that half; zpass runs the y rows y in [0, ny/2], reads
+//   the other half of its z lines from the reflected row -y and writes both
+//   rows' half spaces; plane_inv runs the planes of the half and writes
+//   plane -z of the output as the reflection of plane z.
 //   2D / 1D: plane_fwd does everything for its row in one workgroup.
 //
 // The same passes run the adjoint of the stage (SG_SUNSET_ADJ): the input is
@@ -394,32 +399,35 @@ __global__ __launch_bounds__(TPB) void sigma_plane_fwd(SigmaArgs a) {
     }
     return;
   }
+  const size_t wrow = (size_t)q * nplane * (nz / 2 + 1);  // half-space workspace row
   for (int e = threadIdx.x; e < nplane; e += blockDim.x) {
     const int iy = e / n, ix = e - iy * n;
-    a.W[rowoff + ix + n * (iy + ny * iz)] = res[iy * ls + ix];
-    if (izn != iz) {
-      const int jx = (n - ix) % n, jy = (ny - iy) % ny;
-      a.W[rowoff + ix + n * (iy + ny * izn)] = res[jy * ls + jx];
-    }
+    a.W[wrow + ix + n * (iy + ny * iz)] = res[iy * ls + ix];
   }
 }
 
-// blockIdx.x: y row iy, blockIdx.y: row q.  Lines along z for every x.
+// blockIdx.x: y row iy in [0, ny/2], blockIdx.y: row q.  Lines along z for
+// every x; the workgroup owns the workspace rows iy and -iy.
 template <int NT, int MODE, int TPB>
 __global__ __launch_bounds__(TPB) void sigma_zpass(SigmaArgs a) {
   extern __shared__ double2 lds[];
   const int n = NT > 0 ? NT : a.n;
   const int ny = a.ny, nz = a.nz;
   const int ls = nz + 1;
+  const int nzh = nz / 2 + 1;
   double2* A = lds;
   double2* B = lds + n * ls;
   const int q = blockIdx.y;
   if (q >= stage_rows<MODE>(a)) return;
   const int iy = blockIdx.x;
-  const size_t rowoff = (size_t)q * a.V;
+  const int iyn = (ny - iy) % ny;
+  const size_t rowoff = (size_t)q * a.V;               // site-indexed arrays (X)
+  const size_t wrow = (size_t)q * n * ny * nzh;        // half-space workspace row
   for (int e = threadIdx.x; e < n * nz; e += blockDim.x) {
     const int iz = e / n, ix = e - iz * n;
-    A[ix * ls + iz] = a.W[rowoff + ix + n * (iy + ny * iz)];
+    // z >= nzh: W(x, y, z) = W(-x, -y, -z), from the half space of row -y
+    A[ix * ls + iz] = iz < nzh ? a.W[wrow + ix + n * (iy + ny * iz)]
+                               : a.W[wrow + (n - ix) % n + n * (iyn + ny * (nz - iz))];
   }
   __syncthreads();
   double2* res = fft_lines<NT>(A, B, a.plan, n, ls, 1, a.tw, false);
@@ -430,13 +438,15 @@ __global__ __launch_bounds__(TPB) void sigma_zpass(SigmaArgs a) {
   }
   __syncthreads();
   res = fft_lines<NT>(res, oth, a.plan, n, ls, 1, a.tw, true);
-  for (int e = threadIdx.x; e < n * nz; e += blockDim.x) {
+  for (int e = threadIdx.x; e < n * nzh; e += blockDim.x) {
     const int iz = e / n, ix = e - iz * n;
-    a.W[rowoff + ix + n * (iy + ny * iz)] = res[ix * ls + iz];
+    a.W[wrow + ix + n * (iy + ny * iz)] = res[ix * ls + iz];
+    if (iyn != iy) a.W[wrow + ix + n * (iyn + ny * iz)] = res[((n - ix) % n) * ls + (nz - iz) % nz];
   }
 }
 
-// blockIdx.x: plane iz, blockIdx.y: row q.  Inverse 2D transform + scatter.
+// blockIdx.x: plane iz in [0, nz/2], blockIdx.y: row q.  Inverse 2D
+// transform + scatter of planes iz and -iz.
 template <int NT, int MODE, int TPB>
 __global__ __launch_bounds__(TPB) void sigma_plane_inv(SigmaArgs a) {
   extern __shared__ double2 lds[];
@@ -447,13 +457,16 @@ __global__ __launch_bounds__(TPB) void sigma_plane_inv(SigmaArgs a) {
   double2* B = lds + ny * ls;
   const int q = blockIdx.y;
   if (q >= stage_rows<MODE>(a)) return;
+  const int nz = a.nz;
   const int iz = blockIdx.x;
+  const int izn = (nz - iz) % nz;
   const int slot = row_slot<MODE>(a, q);
   const size_t rowoff = (size_t)q * a.V;
   const int nplane = n * ny;
+  const size_t wrow = (size_t)q * nplane * (nz / 2 + 1);  // half-space workspace row
   for (int e = threadIdx.x; e < nplane; e += blockDim.x) {
     const int iy = e / n, ix = e - iy * n;
-    A[iy * ls + ix] = a.W[rowoff + ix + n * (iy + ny * iz)];
+    A[iy * ls + ix] = a.W[wrow + ix + n * (iy + ny * iz)];
   }
   __syncthreads();
   double2* res = fft_lines<NT>(A, B, a.plan, n, 1, ls, a.tw, true);
@@ -462,6 +475,8 @@ __global__ __launch_bounds__(TPB) void sigma_plane_inv(SigmaArgs a) {
   for (int e = threadIdx.x; e < nplane; e += blockDim.x) {
     const int iy = e / n, ix = e - iy * n;
     emit<MODE>(a, ix + n * (iy + ny * iz), slot, rowoff, res[iy * ls + ix]);
+    // plane -iz is the point reflection of plane iz
+    if (izn != iz) emit<MODE>(a, ix + n * (iy + ny * izn), slot, rowoff, res[((ny - iy) % ny) * ls + (n - ix) % n]);
   }
 }
 
@@ -480,10 +495,10 @@ static hipError_t launch_planes_t(const SigmaArgs& a, int L, hipStream_t st) {  
   hipLaunchKernelGGL((sigma_plane_fwd<NT, MODE, TPB>), dim3(a.nz / 2 + 1, L), dim3(TPB), lds_plane, st, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((sigma_zpass<NT, MODE, TPB>), dim3(a.ny, L), dim3(TPB), lds_z, st, a);
+  hipLaunchKernelGGL((sigma_zpass<NT, MODE, TPB>), dim3(a.ny / 2 + 1, L), dim3(TPB), lds_z, st, a);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((sigma_plane_inv<NT, MODE, TPB>), dim3(a.nz, L), dim3(TPB), lds_plane, st, a);
+  hipLaunchKernelGGL((sigma_plane_inv<NT, MODE, TPB>), dim3(a.nz / 2 + 1, L), dim3(TPB), lds_plane, st, a);
   return hipGetLastError();
 }
 
