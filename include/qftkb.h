@@ -308,7 +308,8 @@ int kb_adj_get_entry(kb_ctx *ctx, int which, int i, int j, double *out);
  * outputs may not overlap the inputs or each other.
  *
  * The workspace is caller-owned device memory of kb_sunset_workspace_bytes
- * (4 KiB of twiddles, plus nrows * V complex values in 3D), 256-byte
+ * (4 KiB of twiddles, plus nrows * Nx^2 (Nx/2 + 1) complex values in 3D: the
+ * half space of each row's transforms), 256-byte
  * aligned.  Kernels are enqueued on `stream` (a hipStream_t; NULL = the
  * device's null stream) and the call returns without waiting.  Errors are
  * reported by the return code; kb_last_error(NULL) gives the calling
@@ -323,7 +324,7 @@ int kb_sunset(const kb_params *p, int nrows, const double *F, const double *rho,
  * It makes no hipMalloc/hipFree or other allocating call.  The workspace is
  * a 64 MiB device array inside the library, one per device, held until the
  * process ends: KB_ERR_ARG when kb_sunset_workspace_bytes of the request
- * exceeds 64 MiB (more than 15 rows of 64^3, 37 of 48^3, 127 of 32^3; 1D and
+ * exceeds 64 MiB (more than 31 rows of 64^3, 72 of 48^3, 240 of 32^3; 1D and
  * 2D lattices need only the 4 KiB of twiddles).
  * Every call on one device shares that array, so calls that overlap in time
  * on one device (other streams, other threads) race; the results equal

@@ -523,8 +523,11 @@ int ensure_plane_workspace(kb_ctx* c) {
   if (!has_sigma(c)) return KB_OK;
   int rc;
   const size_t rows = (size_t)c->N * c->Vl * sizeof(double2);
-  if ((c->p.sdim == 3 || !sigma_planes_supported(c->p.Nx)) && !c->W &&
-      (rc = dmalloc(c, (void**)&c->W, rows)))
+  // the plane passes keep the half space z in [0, Nx/2] of each row; the
+  // pencil passes (Nx > 64) whole rows
+  const int n = c->p.Nx;
+  const size_t wrows = sigma_planes_supported(n) ? (size_t)c->N * n * n * (n / 2 + 1) * sizeof(double2) : rows;
+  if ((c->p.sdim == 3 || !sigma_planes_supported(n)) && !c->W && (rc = dmalloc(c, (void**)&c->W, wrows)))
     return rc;
   if (c->p.pexp == KB_PEXP_NINVERSE_NLO && !c->Xn && (rc = dmalloc(c, (void**)&c->Xn, rows))) return rc;
   return KB_OK;

@@ -7,7 +7,8 @@
 //
 // Device memory: the caller's input and output rows, plus a workspace that
 // holds the twiddle table (the first WS_TWIDDLE_BYTES) and, for 3D lattices,
-// one complex lattice per row for the transforms between the plane passes.
+// half a complex lattice per row (the planes z in [0, Nx/2]) for the
+// transforms between the plane passes.
 // kb_sunset takes the workspace from the caller; kb_sunset_noalloc uses the
 // file-scope device array `scratch` below (SCRATCH_BYTES on every device the
 // library runs on, held until the process ends), so it allocates nothing.
@@ -63,10 +64,11 @@ __global__ void put_twiddles(Twiddles t, double2* dst, int n) {
   if (k < n) dst[k] = t.w[k];
 }
 
-// rows of V complex values between the plane passes (3D only)
+// rows of complex values between the plane passes (3D only): the half space
+// z in [0, Nx/2] of each row
 size_t workspace_bytes(const kb_params* p, int nrows) {
-  const size_t V = (size_t)p->Nx * (p->sdim >= 2 ? p->Nx : 1) * (p->sdim >= 3 ? p->Nx : 1);
-  return WS_TWIDDLE_BYTES + (p->sdim == 3 ? (size_t)nrows * V * sizeof(double2) : 0);
+  const size_t half = (size_t)p->Nx * p->Nx * (p->Nx / 2 + 1);
+  return WS_TWIDDLE_BYTES + (p->sdim == 3 ? (size_t)nrows * half * sizeof(double2) : 0);
 }
 
 int check_params(const kb_params* p, int nrows) {

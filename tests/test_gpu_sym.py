@@ -350,14 +350,16 @@ def test_parameter_dict_run_with_orbit_numerics(init):
 
 
 def test_plane_workspace_only_where_a_plane_stage_runs():
-    """The plane kernels' workspace (one complex lattice per slot, plus the
-    1/N transforms) is held only by contexts that run a plane stage: not by
-    LambdaLO, not by orbit-storage contexts on the orbit products, until a
-    plane form is selected."""
+    """The plane kernels' workspace (half a complex lattice per slot: the
+    planes z in [0, Nx/2], plus the 1/N transforms, a whole lattice per slot)
+    is held only by contexts that run a plane stage: not by LambdaLO, not by
+    orbit-storage contexts on the orbit products, until a plane form is
+    selected."""
     from qftkb import KBContext
     Nx, N = 16, 12
     Vl = Nx ** 3
-    W = N * Vl * 16
+    W = N * Nx * Nx * (Nx // 2 + 1) * 16
+    Xn = N * Vl * 16
     lo_full = KBContext(Nx, 3, N, 0.1, 1.0, 1, 1.0, 1.0, pexp=0)
     nlo_full = KBContext(Nx, 3, N, 0.1, 1.0, 1, 1.0, 1.0)
     assert nlo_full.device_bytes() - lo_full.device_bytes() == W
@@ -371,7 +373,7 @@ def test_plane_workspace_only_where_a_plane_stage_runs():
     ninv = KBContext(Nx, 3, N, 0.05, 2.0, 4, 1.0, 1.0, pexp=3, symmetric=True)
     c0 = ninv.device_bytes()
     ninv.set_fft_tiles(0)
-    assert ninv.device_bytes() - c0 == 2 * W  # W and the kept transforms
+    assert ninv.device_bytes() - c0 == W + Xn  # W and the kept transforms
     # the plane form still runs after the late allocation: a step against the
     # orbit-product form of the same state
     k2 = sym.kL2()

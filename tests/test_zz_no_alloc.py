@@ -68,9 +68,9 @@ def test_noalloc_two_dimensional_and_limit():
     from qftkb._abi import sunset_noalloc
     F, R, (sf, sr, nf, nr) = both(24, 2, 9, 0.8, 1, 13)
     assert rel(nf, sf) <= 1e-10 and rel(nr, sr) <= 1e-10
-    # 16 rows of 64^3 need 64 MiB + 4 KiB of scratch: refused before any launch
-    a, b, c, d = (DeviceArray(16 * 64 ** 3) for _ in range(4))
+    # 32 rows of 64^3 need 4 KiB + 32 * 64^2 * 33 * 16 B = 66 MiB of scratch: refused before any launch
+    a, b, c, d = (DeviceArray(32 * 64 ** 3) for _ in range(4))
     with pytest.raises(KBError) as e:
-        sunset_noalloc(64, 3, 1.0, 1, 16, a.ptr, b.ptr, c.ptr, d.ptr)
+        sunset_noalloc(64, 3, 1.0, 1, 32, a.ptr, b.ptr, c.ptr, d.ptr)
     assert "limit" in str(e.value)
     synchronize()
