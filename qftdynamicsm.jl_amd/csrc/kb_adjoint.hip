@@ -227,7 +227,7 @@ __global__ void adj_seed_kernel(const double* __restrict__ H, double* __restrict
 // Row t enters with R_bar_F(t) = dt^3 F_bar(t,t) (the diagonal's memory
 // terms) and u = v = ms = 0; its own entries are cleared afterwards.
 // The slot triangle is walked like the forward kernel: 8-lane workers take
-// 64-row chunks of 4-column blocks, two row batches in flight, column sums
+// chunks of 4-column blocks (the forward's work items), two row batches in flight, column sums
 // in registers, row sums in LDS.
 // ---------------------------------------------------------------------------
 constexpr int AM_THREADS = 256;

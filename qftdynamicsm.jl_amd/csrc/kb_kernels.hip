@@ -315,8 +315,8 @@ __global__ void advance_kernel(StepState* const* states, const int* Ns, int nrun
 // ms = m Sr (m = 0 at z = t-1):
 //   R_F(a) += -u(b) F - v(b) r,  R_F(b) += -u(a) F,  R_r(b) += -ms(a) r,
 // and the diagonal (a,a) feeds R_F(a) += -u(a) F.  The kernel walks the
-// slot triangle once: every 8-lane worker streams chunks of 64 rows of one
-// 4-column block (contiguous in HBM) with two 4-row batches in flight,
+// slot triangle once: every 8-lane worker streams chunks of rows of one
+// 4-column block (contiguous in HBM) with two row batches in flight,
 // keeps the 4 column sums in registers and adds row and column sums into
 // per-slot LDS accumulators with ds_add_f64.
 // In slot space, pair (hi, lo) has time(hi) < time(lo) exactly when
