@@ -74,6 +74,11 @@ CASES = [
     (11, 2, 3, 12),
     (11, 3, 3, 8),
     (13, 1, 3, 15),
+    # the half-space workspace at the smallest 3D lattices (Nx = 2: every row and plane is its
+    # own reflection; Nx = 3: one reflected pair) and at two radix-3 stages
+    (2, 3, 4, 12),
+    (3, 3, 5, 12),
+    (9, 3, 6, 10),
 ]
 
 
