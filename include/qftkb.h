@@ -273,6 +273,19 @@ int kb_autotune_fft(kb_ctx *ctx, int reps, int *best, double *ms);
  *                         tape 38x.
  * Fails with KB_ERR_OOM if the tape or checkpoints do not fit. */
 int kb_adj_record(kb_ctx *ctx, int checkpoint_every, int max_steps);
+/* kb_adj_record with C = checkpoint_every > 0 for windows that do not fit
+ * in device memory C at a time: up to host_windows windows are kept in
+ * pinned host memory (each as large as the device history), the initial
+ * window as its three rows on the device, and the device tape holds one
+ * C-step segment.  The reverse sweep restores the nearest kept window below
+ * the segment it needs and recomputes forward from it, keeping new windows
+ * on the way by the binomial checkpointing schedule (Griewank's revolve
+ * over segments): the least recomputation for host_windows snapshots.
+ * host_windows = 0 recomputes every segment from the initial window.  Same
+ * gradient as the other modes; kb_adj_info counts the recomputed steps.
+ * No counterpart in the reference (it has no derivative code).  KB_ERR_OOM
+ * when the pinned host memory cannot be allocated. */
+int kb_adj_record_host(kb_ctx *ctx, int checkpoint_every, int max_steps, int host_windows);
 /* Reverse sweep from T = tmax down to the initial rows.  wgt[V] are the
  * loss weights.  Outputs (any may be NULL): dL/dLambda; dL/dn(p,0) per site
  * (needs kb_init_particles; n enters F(1,1) = F(2,1) = (n+1/2)/w and

@@ -151,6 +151,27 @@ __global__ void tape_unpack_kernel(double* __restrict__ H, const double* __restr
   st2(H + hoff(p / MB, pair_off(s, x, N), p % MB, NP), ld2(tape + ((size_t)x * norb + site_orb[p]) * 2));
 }
 
+// the three slot pairs of the initial rows (1,1), (2,1), (2,2) of every
+// site, saved to or restored from buf[3][V][2]: with them the initial window
+// [1,2] is rebuilt on a cleared history (host-snapshot checkpointing)
+__global__ void adj_init_pairs_kernel(double* __restrict__ H, double* __restrict__ buf, long NP, int V, int N,
+                                      int save) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= V) return;
+  const int mb = p / MB, lane = p % MB;
+  const int s1 = 0, s2 = 1 % N;
+  const long pairs[3] = {pair_off(s1, s1, N), pair_off(s2, s1, N), pair_off(s2, s2, N)};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double* h = H + hoff(mb, pairs[k], lane, NP);
+    double* b = buf + ((size_t)k * V + p) * 2;
+    if (save)
+      st2(b, ld2(h));
+    else
+      st2(h, ld2(b));
+  }
+}
+
 // clear the adjoint of the slot pairs (s, x) and, with a tape column, put
 // the primal column back
 __global__ void adj_slide_kernel(double* __restrict__ H, double* __restrict__ Hbar, const double* __restrict__ tape,
@@ -816,6 +837,18 @@ struct AdjState {
   int* orb_rep = nullptr;   // orbit -> first site
   int* site_orb = nullptr;  // site -> orbit
   std::vector<double*> ckpt;
+  // host-snapshot checkpointing (hwin >= 0): up to hwin windows at segment
+  // boundaries in pinned host memory (hsnap[i] holds boundary hseg[i], -1:
+  // free), the initial window as its three rows on the device (initbuf);
+  // plan: the boundaries the recording run keeps
+  int hwin = -1;
+  std::vector<double*> hsnap;
+  std::vector<int> hseg;
+  std::vector<int> plan;
+  std::vector<int> rvF, rvG;  // best first snapshot (segments) per (n, free slots)
+  int rv_w = 1;
+  double* initbuf = nullptr;
+  size_t host_bytes = 0;
   size_t hbytes = 0;
   size_t bytes = 0;
   long long recomputed = 0;
@@ -837,8 +870,11 @@ int amalloc(kb_ctx* c, AdjState* s, void** p, size_t bytes) {
 void free_state(AdjState* s) {
   if (!s) return;
   for (void* p : {(void*)s->Hbar, (void*)s->Sbar, (void*)s->X, (void*)s->P, (void*)s->LP, (void*)s->m2r,
-                  (void*)s->Mb, (void*)s->out, (void*)s->tape, (void*)s->orb_rep, (void*)s->site_orb})
+                  (void*)s->Mb, (void*)s->out, (void*)s->tape, (void*)s->orb_rep, (void*)s->site_orb,
+                  (void*)s->initbuf})
     if (p) (void)hipFree(p);
+  for (double* p : s->hsnap)
+    if (p) (void)hipHostFree(p);
   for (int j = 0; j < AK; ++j) {
     if (s->Sring[j]) (void)hipFree(s->Sring[j]);
     if (s->Rring[j]) (void)hipFree(s->Rring[j]);
@@ -1011,15 +1047,143 @@ int reverse_step(kb_ctx* c, int t, int T, int npend, bool apply) {
   return KB_OK;
 }
 
+// ---- host-snapshot checkpointing --------------------------------------------
+// Segments of C steps; boundary b is the window after step 2 + b C.  The
+// reverse sweep takes the segments from the last to the first and needs the
+// window at each one's start.  F(n, f): the least segments advanced to
+// reverse n segments whose start window is held, with f free snapshot slots:
+//   F(1, f) = 0,  F(n, 0) = n (n - 1) / 2,
+//   F(n, f) = min over m of  m + F(n - m, f - 1) + F(m, f)
+// (advance m segments and keep that window, reverse the n - m segments
+// above it with one slot less, free it, reverse the m below).  This is the
+// binomial checkpointing problem (Griewank's revolve), whose optimum is
+// r n - binomial(f + 2 + r - 1, f + 2) for binomial(f + r, f + 1) < n <=
+// binomial(f + 1 + r, f + 1).  G(n, f) is the same for the recording run,
+// which advances over every boundary anyway and tapes the last segment:
+//   G(1, f) = 0,  G(n, f) = min(F(n - 1, f), min over m of G(n - m, f - 1) + F(m, f)).
+// The tables keep the best m (0: keep no window).
+void revolve_tables(AdjState* s, int nseg, int slots) {
+  const size_t w = (size_t)slots + 1;
+  s->rv_w = (int)w;
+  s->rvF.assign((size_t)(nseg + 1) * w, 0);
+  s->rvG.assign((size_t)(nseg + 1) * w, 0);
+  std::vector<long long> F((size_t)(nseg + 1) * w, 0), G((size_t)(nseg + 1) * w, 0);
+  for (int n = 2; n <= nseg; ++n)
+    for (int f = 0; f <= slots; ++f) {
+      long long best = (long long)n * (n - 1) / 2;
+      int bm = 0;
+      for (int m = 1; f > 0 && m < n; ++m) {
+        const long long c = m + F[(size_t)(n - m) * w + f - 1] + F[(size_t)m * w + f];
+        if (c < best) best = c, bm = m;
+      }
+      F[(size_t)n * w + f] = best;
+      s->rvF[(size_t)n * w + f] = bm;
+      long long gbest = F[(size_t)(n - 1) * w + f];
+      int gm = 0;
+      for (int m = 1; f > 0 && m < n; ++m) {
+        const long long c = G[(size_t)(n - m) * w + f - 1] + F[(size_t)m * w + f];
+        if (c < gbest) gbest = c, gm = m;
+      }
+      G[(size_t)n * w + f] = gbest;
+      s->rvG[(size_t)n * w + f] = gm;
+    }
+}
+
+// boundaries the recording run keeps: the right spine of G over all segments
+std::vector<int> host_plan(const AdjState* s, int nseg, int slots) {
+  std::vector<int> plan;
+  int cur = 0, n = nseg;
+  for (int f = slots; f > 0 && n >= 2; --f) {
+    const int m = s->rvG[(size_t)n * s->rv_w + f];
+    if (m == 0) break;
+    cur += m;
+    n -= m;
+    plan.push_back(cur);
+  }
+  return plan;
+}
+
+int free_slot(const AdjState* s) {
+  for (size_t i = 0; i < s->hseg.size(); ++i)
+    if (s->hseg[i] < 0) return (int)i;
+  return -1;
+}
+
+// copy the window (boundary b) into a free host slot
+int store_snapshot(kb_ctx* c, int b) {
+  AdjState* s = c->adj;
+  const int i = free_slot(s);
+  if (i < 0) return seterr(c, KB_ERR_STATE, "adjoint: no free host snapshot slot");
+  HIPCHK(c, hipMemcpyAsync(s->hsnap[i], c->H, s->hbytes, hipMemcpyDeviceToHost, c->stream));
+  s->hseg[i] = b;
+  return KB_OK;
+}
+
+// run forward steps from .. to (no tape); the window of step from - 1 is in H
+int advance_steps(kb_ctx* c, int from, int to) {
+  int rc;
+  for (int t = from; t <= to; ++t) {
+    advance_state(c->st, c->N);
+    if ((rc = enqueue_step(c))) return rc;
+    c->adj->recomputed += 1;
+  }
+  return KB_OK;
+}
+
+// put the window of boundary k into H from the nearest snapshot below it,
+// keeping new snapshots on the way by the binomial schedule
+int host_window(kb_ctx* c, int k) {
+  AdjState* s = c->adj;
+  const int C = s->C;
+  int j = 0, slot = -1;
+  for (size_t i = 0; i < s->hseg.size(); ++i) {
+    if (s->hseg[i] > k) s->hseg[i] = -1;  // its segments are reversed
+    if (s->hseg[i] >= 0 && s->hseg[i] >= j) {
+      j = s->hseg[i];
+      slot = (int)i;
+    }
+  }
+  if (slot >= 0) {
+    HIPCHK(c, hipMemcpyAsync(c->H, s->hsnap[slot], s->hbytes, hipMemcpyHostToDevice, c->stream));
+  } else {
+    // the initial window: the three initial rows on a cleared history
+    HIPCHK(c, hipMemsetAsync(c->H, 0, s->hbytes, c->stream));
+    hipLaunchKernelGGL(adj_init_pairs_kernel, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->H, s->initbuf,
+                       c->NP, c->V, c->N, 0);
+    LAUNCHCHK(c);
+  }
+  c->st = state_of_step(2 + j * C, c->N);
+  int rc;
+  if ((rc = upload_state(c))) return rc;
+  int nfree = 0;
+  for (int b : s->hseg) nfree += b < 0 ? 1 : 0;
+  for (int cur = j; cur < k;) {
+    const int n = k - cur + 1;  // segments cur .. k still to be reversed
+    const int bm = n >= 2 ? s->rvF[(size_t)n * s->rv_w + nfree] : 0;
+    const int m = bm > 0 ? cur + bm : k;
+    if ((rc = advance_steps(c, 3 + cur * C, 2 + m * C))) return rc;
+    cur = m;
+    if (cur < k) {
+      if ((rc = store_snapshot(c, cur))) return rc;
+      --nfree;
+    }
+  }
+  return KB_OK;
+}
+
 // restore the checkpoint before step `first` and recompute steps
 // first .. last, keeping their dropped columns on the tape
 int recompute_segment(kb_ctx* c, int first, int last) {
   AdjState* s = c->adj;
   const int k = (first - 3) / s->C;
-  HIPCHK(c, hipMemcpyAsync(c->H, s->ckpt[k], s->hbytes, hipMemcpyDeviceToDevice, c->stream));
-  c->st = state_of_step(first - 1, c->N);
   int rc;
-  if ((rc = upload_state(c))) return rc;
+  if (s->hwin >= 0) {
+    if ((rc = host_window(c, k))) return rc;
+  } else {
+    HIPCHK(c, hipMemcpyAsync(c->H, s->ckpt[k], s->hbytes, hipMemcpyDeviceToDevice, c->stream));
+    c->st = state_of_step(first - 1, c->N);
+    if ((rc = upload_state(c))) return rc;
+  }
   for (int t = first; t <= last; ++t) {
     advance_state(c->st, c->N);
     if ((rc = save_column(c, t))) return rc;
@@ -1043,9 +1207,12 @@ int adj_after_step(kb_ctx* c) {
   AdjState* s = c->adj;
   const int t = c->st.t;
   if (s->C > 0 && (t - 2) % s->C == 0) {
-    const size_t k = (size_t)((t - 2) / s->C);
-    if (k < s->ckpt.size())
+    const int k = (t - 2) / s->C;
+    if (s->hwin >= 0) {
+      if (std::find(s->plan.begin(), s->plan.end(), k) != s->plan.end()) return store_snapshot(c, k);
+    } else if ((size_t)k < s->ckpt.size()) {
       HIPCHK(c, hipMemcpyAsync(s->ckpt[k], c->H, s->hbytes, hipMemcpyDeviceToDevice, c->stream));
+    }
   }
   return KB_OK;
 }
@@ -1061,9 +1228,10 @@ void adj_free(kb_ctx* c) {
 }  // namespace qftkb_host
 #pragma GCC visibility pop
 
-extern "C" {
+namespace {
 
-int kb_adj_record(kb_ctx* c, int checkpoint_every, int max_steps) {
+// host_windows < 0: checkpoints (if any) in device memory
+int adj_record(kb_ctx* c, int checkpoint_every, int max_steps, int host_windows) {
   if (!c || checkpoint_every < -1 || max_steps < 2) return KB_ERR_ARG;
   if (!c->initialised || c->st.tmax != 2)
     return seterr(c, KB_ERR_STATE, "adjoint recording starts right after initialisation (window [1,2])");
@@ -1153,7 +1321,26 @@ int kb_adj_record(kb_ctx* c, int checkpoint_every, int max_steps) {
       (rc = amalloc(c, s, (void**)&s->Mb, 8)) || (rc = amalloc(c, s, (void**)&s->out, 4 * (size_t)c->V * 8)) ||
       (rc = amalloc(c, s, (void**)&s->tape, (size_t)s->tape_cols * s->col_doubles * 8)))
     return fail(rc);
-  if (s->C > 0) {
+  if (s->C > 0 && host_windows >= 0) {
+    // windows in pinned host memory, the initial one as its three rows
+    s->hwin = host_windows;
+    for (int i = 0; i < host_windows; ++i) {
+      double* p = nullptr;
+      if (hipHostMalloc((void**)&p, s->hbytes, hipHostMallocDefault) != hipSuccess)
+        return fail(seterr(c, KB_ERR_OOM, "adjoint: %d host windows of %zu bytes do not fit (pinned host memory)",
+                           host_windows, s->hbytes));
+      s->hsnap.push_back(p);
+      s->hseg.push_back(-1);
+      s->host_bytes += s->hbytes;
+    }
+    const int nseg = (max_steps - 2 + s->C - 1) / s->C;
+    revolve_tables(s, nseg, host_windows);
+    s->plan = host_plan(s, nseg, host_windows);
+    if ((rc = amalloc(c, s, (void**)&s->initbuf, 6 * (size_t)c->V * sizeof(double)))) return fail(rc);
+    hipLaunchKernelGGL(adj_init_pairs_kernel, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->H, s->initbuf,
+                       c->NP, c->V, c->N, 1);
+    if (hipGetLastError() != hipSuccess) return fail(seterr(c, KB_ERR_HIP, "adjoint: initial rows not saved"));
+  } else if (s->C > 0) {
     const int nck = (max_steps - 2) / s->C + 1;  // windows at t = 2, 2 + C, ...
     for (int k = 0; k < nck; ++k) {
       double* p = nullptr;
@@ -1175,6 +1362,19 @@ int kb_adj_record(kb_ctx* c, int checkpoint_every, int max_steps) {
   c->adj = s;
   c->bytes += s->bytes;
   return KB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kb_adj_record(kb_ctx* c, int checkpoint_every, int max_steps) {
+  return adj_record(c, checkpoint_every, max_steps, -1);
+}
+
+int kb_adj_record_host(kb_ctx* c, int checkpoint_every, int max_steps, int host_windows) {
+  if (!c || checkpoint_every < 1 || host_windows < 0) return KB_ERR_ARG;
+  return adj_record(c, checkpoint_every, max_steps, host_windows);
 }
 
 int kb_adj_stop(kb_ctx* c) {
@@ -1276,6 +1476,7 @@ int kb_adj_gradient(kb_ctx* c, const double* wgt, double* dL_dlambda, double* dL
   if (dL_dF0 && (rc = download(c, dL_dF0, s->out, 3 * (size_t)c->V))) return rc;
   if (dL_dn0 && (rc = download(c, dL_dn0, s->out + 3 * (size_t)c->V, c->V))) return rc;
   // the initial window is checkpoint 0 again; a new forward run may be recorded
+  for (int& b : s->hseg) b = -1;
   return KB_OK;
 }
 

@@ -44,7 +44,7 @@ EXPORTED = (
     "kb_fill_free_window", "kb_window_bytes", "kb_save_window", "kb_load_window", "kb_synchronize", "kb_set_stream", "kb_enable_timing", "kb_timing",
     "kb_reset_timing", "kb_use_graph", "kb_ens_create", "kb_ens_create_sym", "kb_ens_set_tiles", "kb_ens_get_tiles", "kb_ens_autotune", "kb_ens_destroy", "kb_ens_last_error", "kb_ens_member",
     "kb_ens_step", "kb_ens_use_graph", "kb_ens_synchronize", "kb_ens_enable_timing", "kb_ens_timing",
-    "kb_adj_record", "kb_adj_gradient", "kb_adj_info", "kb_adj_stop", "kb_adj_debug_stop", "kb_adj_get_entry",
+    "kb_adj_record", "kb_adj_record_host", "kb_adj_gradient", "kb_adj_info", "kb_adj_stop", "kb_adj_debug_stop", "kb_adj_get_entry",
     "kb_tile_shape", "kb_set_tiles", "kb_get_tiles", "kb_autotune",
     "kb_fft_shape", "kb_set_fft_tiles", "kb_get_fft_tiles", "kb_autotune_fft",
     "kb_create_sym", "kb_storage_info", "kb_sunset_workspace_bytes", "kb_sunset", "kb_sunset_noalloc",
@@ -131,6 +131,7 @@ def load_library(path: str = LIB_PATH):
     L.kb_get_fft_tiles.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     L.kb_autotune_fft.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp]
     L.kb_adj_record.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.kb_adj_record_host.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.kb_adj_gradient.argtypes = [vp, _dp, _dp, _dp, _dp]
     L.kb_adj_info.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
     L.kb_adj_stop.argtypes = [vp]
@@ -506,15 +507,21 @@ class KBContext:
         return {"variant": b, "threads": None if b == FFT_ORBIT_PRODUCT else shapes[b], "ms": ms.tolist()}
 
     # ---- adjoint ------------------------------------------------------
-    def adj_record(self, checkpoint_every=0, max_steps=None):
+    def adj_record(self, checkpoint_every=0, max_steps=None, host_windows=None):
         """Record the coming kb_step calls for the reverse sweep (call right
         after initialisation).  checkpoint_every = 0 keeps a tape of every
         dropped column; C > 0 keeps the window every C steps and recomputes;
         -1 keeps the tape packed to one value per cubic orbit (the state must
-        be cubic-symmetric)."""
+        be cubic-symmetric).  With host_windows (C > 0), at most that many
+        windows are kept, in pinned host memory, under the binomial
+        checkpointing schedule (kb_adj_record_host)."""
         if max_steps is None:
             raise ValueError("max_steps is required")
-        self._chk(self._L.kb_adj_record(self._h, int(checkpoint_every), int(max_steps)))
+        if host_windows is None:
+            self._chk(self._L.kb_adj_record(self._h, int(checkpoint_every), int(max_steps)))
+        else:
+            self._chk(self._L.kb_adj_record_host(self._h, int(checkpoint_every), int(max_steps),
+                                                 int(host_windows)))
 
     def adj_gradient(self, wgt, n0=True, F0=False):
         """Reverse sweep for L = sum wgt * n(tmax).  Returns a dict with
