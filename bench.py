@@ -440,6 +440,91 @@ def large_adjoint_bench(fd=True):
     return out
 
 
+def large_adjoint_host_bench(C=150, host_windows=2, fd=True):
+    """The same 48^3 differentiation for a state without the cubic symmetry
+    (the box occupation modulated along x: n(p) = box(p) (1 + 0.2 sin(2 pi
+    i_x / Nx + 0.3))), so the orbit-packed tape does not apply: the window
+    (80.7 GB) is kept at segment boundaries in pinned host memory (at most
+    host_windows of them) under the binomial checkpointing schedule, and
+    the device tape holds one C-step segment (kb_adj_record_host).  Forward
+    steps/s, the gradient's cost ratio, the recomputed steps, and dL/dlambda
+    against GPU central differences (h and 2h, Richardson)."""
+    from qftkb import KBContext, lattice
+
+    a = LARGE_ADJ_WORKLOAD
+    Nx, N, dt, lam, ON, M = a["Nx"], a["NstepsinMemory"], a["dt"], a["Lambda"], a["ONgroup"], a["Mass"]
+    T = 2 + a["steps"]
+
+    def make(lmb):
+        return KBContext(Nx, a["sdim"], N, dt, lmb, ON, M, M * M)
+
+    def forward(c, graph, label):
+        c.use_graph(graph)
+        t0 = time.perf_counter()
+        for t in range(3, T + 1):
+            c.step(t)
+            if (t - 2) % 500 == 0:
+                c.synchronize()
+                print(f"[bench] {label}: step {t}/{T}, {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+                      flush=True)
+        c.synchronize()
+        return time.perf_counter() - t0
+
+    c = make(lam)
+    k2 = c.kL2()
+    ix = np.arange(Nx ** 3) % Nx
+    n0 = lattice.n_box(k2, a["n0"], a["Q"]) * (1.0 + 0.2 * np.sin(2 * np.pi * ix / Nx + 0.3))
+    wgt = np.exp(-k2)
+    c.init_particles(n0, M)
+    t_fwd = forward(c, True, "forward")
+    L = float((wgt * c.measure(T)["n"]).sum())
+    c.close()
+    out = {"workload": f"{Nx}^3 lattice, N_mem={N}, dt={dt}, {a['steps']} steps, box n0={a['n0']} Q={a['Q']} "
+                       f"modulated by 1 + 0.2 sin(2 pi i_x / {Nx} + 0.3) (no cubic symmetry), lambda={lam}, "
+                       f"L = sum_p exp(-|p|^2) n(p, t_f)",
+           "forward_s": t_fwd, "forward_steps_per_s": a["steps"] / t_fwd, "loss": L}
+    g = make(lam)
+    g.init_particles(n0, M)
+    g.adj_record(C, T, host_windows=host_windows)
+    t_rec = forward(g, False, "recording forward")
+    t0 = time.perf_counter()
+    gr = g.adj_gradient(wgt)
+    t_rev = time.perf_counter() - t0
+    info = g.adj_info()
+    g.close()
+    J = (N + 3) // 4  # column blocks of 4 slots; the history holds their slot pairs per 8-site block
+    window_gb = (Nx ** 3 + 7) // 8 * (4 * (N * J - 4 * J * (J - 1) // 2)) * 8 * 16 / 1e9
+    print(f"[bench] host-checkpointed gradient: record {t_rec:.1f} s, reverse with recomputation {t_rev:.1f} s, "
+          f"{info['recomputed_steps']} steps recomputed", file=sys.stderr, flush=True)
+    out["host_checkpoints"] = {"checkpoint_every": C, "host_windows": host_windows,
+                               "host_gb": host_windows * window_gb, "record_forward_s": t_rec,
+                               "reverse_s": t_rev, "gradient_s": t_rec + t_rev, "ratio": (t_rec + t_rev) / t_fwd,
+                               "adjoint_device_gb": info["device_bytes"] / 1e9,
+                               "recomputed_steps": info["recomputed_steps"]}
+    out["ratio"] = out["host_checkpoints"]["ratio"]
+    out["dL_dlambda"] = gr["lambda"]
+    bins = radial_bins(k2, a["sdim"], a["nbins"])
+    out["dL_dn0_bins"] = [float(gr["n0"][bins == b].sum()) for b in range(a["nbins"])]
+    if fd:
+        h = a["h"]
+        D = {}
+        for hh in (h, 2 * h):
+            Ls = []
+            for sgn in (1, -1):
+                f = make(lam + sgn * hh)
+                f.init_particles(n0, M)
+                forward(f, True, f"finite difference {'+' if sgn > 0 else '-'}{hh:g}")
+                Ls.append(float((wgt * f.measure(T)["n"]).sum()))
+                f.close()
+            D[hh] = (Ls[0] - Ls[1]) / (2 * hh)
+        fdv = (4 * D[h] - D[2 * h]) / 3
+        out["fd_central"] = {f"{hh:g}": v for hh, v in D.items()}
+        out["fd_dL_dlambda"] = fdv
+        out["fd_method"] = f"GPU forward runs, central differences at h={h:g} and {2 * h:g}, Richardson"
+        out["fd_rel_err_lambda"] = abs(gr["lambda"] - fdv) / abs(fdv)
+    return out
+
+
 AUTOTUNE_WORKLOADS = [(32, 3, 400), (24, 3, 200), (48, 3, 300), (16, 3, 202), (40, 3, 300),
                       (16, 3, 300), (24, 3, 300), (32, 3, 300)]
 
@@ -493,6 +578,13 @@ def main():
     ap.add_argument("--no-fd", action="store_true", help="with --adjoint: skip the finite differences")
     ap.add_argument("--adjoint-large", action="store_true",
                     help="only the 48^3 N_mem=300 4000-step differentiation (packed tape; about 6 minutes)")
+    ap.add_argument("--adjoint-large-host", action="store_true",
+                    help="the 48^3 differentiation for a state without cubic symmetry: window snapshots in "
+                         "pinned host memory under the binomial checkpointing schedule (about 10 minutes)")
+    ap.add_argument("--host-windows", type=int, default=2,
+                    help="with --adjoint-large-host: window snapshots kept in pinned host memory")
+    ap.add_argument("--checkpoint-every", type=int, default=150,
+                    help="with --adjoint-large-host: steps per segment (the device tape holds one)")
     ap.add_argument("--autotune", metavar="OUT_JSON",
                     help="pick the memory-integral tile shape for the standard workloads and write the choices")
     ap.add_argument("--ninverse", action="store_true",
@@ -538,6 +630,10 @@ def main():
         return
     if args.adjoint_large:
         print(json.dumps({"adjoint_large": large_adjoint_bench(fd=not args.no_fd)}), flush=True)
+        return
+    if args.adjoint_large_host:
+        r = large_adjoint_host_bench(C=args.checkpoint_every, host_windows=args.host_windows, fd=not args.no_fd)
+        print(json.dumps({"adjoint_large_host": r}), flush=True)
         return
     if args.autotune:
         print(json.dumps({"autotune": autotune_bench(args.autotune)}), flush=True)

@@ -33,8 +33,11 @@
 // is reversed.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
+#include <thread>
 
 #include "kb_internal.h"
 
@@ -838,17 +841,22 @@ struct AdjState {
   int* site_orb = nullptr;  // site -> orbit
   std::vector<double*> ckpt;
   // host-snapshot checkpointing (hwin >= 0): up to hwin windows at segment
-  // boundaries in pinned host memory (hsnap[i] holds boundary hseg[i], -1:
-  // free), the initial window as its three rows on the device (initbuf);
-  // plan: the boundaries the recording run keeps
+  // boundaries in host memory (hsnap[i] holds boundary hseg[i], -1: free),
+  // copied through two pinned staging buffers (pinned memory is scarce: a
+  // window is larger than what the driver pins), the initial window as its
+  // three rows on the device (initbuf); plan: the boundaries the recording
+  // run keeps
   int hwin = -1;
-  std::vector<double*> hsnap;
+  std::vector<char*> hsnap;
+  char* stage[2] = {};
+  hipEvent_t stage_ev[2] = {};
   std::vector<int> hseg;
   std::vector<int> plan;
   std::vector<int> rvF, rvG;  // best first snapshot (segments) per (n, free slots)
   int rv_w = 1;
   double* initbuf = nullptr;
   size_t host_bytes = 0;
+  bool progress = false;  // QFTKB_PROGRESS set: one stderr line per recomputed segment
   size_t hbytes = 0;
   size_t bytes = 0;
   long long recomputed = 0;
@@ -873,8 +881,11 @@ void free_state(AdjState* s) {
                   (void*)s->Mb, (void*)s->out, (void*)s->tape, (void*)s->orb_rep, (void*)s->site_orb,
                   (void*)s->initbuf})
     if (p) (void)hipFree(p);
-  for (double* p : s->hsnap)
-    if (p) (void)hipHostFree(p);
+  for (char* p : s->hsnap) std::free(p);
+  for (int k = 0; k < 2; ++k) {
+    if (s->stage[k]) (void)hipHostFree(s->stage[k]);
+    if (s->stage_ev[k]) (void)hipEventDestroy(s->stage_ev[k]);
+  }
   for (int j = 0; j < AK; ++j) {
     if (s->Sring[j]) (void)hipFree(s->Sring[j]);
     if (s->Rring[j]) (void)hipFree(s->Rring[j]);
@@ -1109,12 +1120,66 @@ int free_slot(const AdjState* s) {
   return -1;
 }
 
+// staged copies between the device window and a host snapshot: chunks of
+// STAGE_BYTES through two pinned buffers, the host side copied by several
+// threads while the next chunk crosses PCIe
+constexpr size_t STAGE_BYTES = size_t(1) << 30;
+constexpr int COPY_THREADS = 8;
+
+void parallel_copy(char* dst, const char* src, size_t n) {
+  std::vector<std::thread> th;
+  const size_t part = (n + COPY_THREADS - 1) / COPY_THREADS;
+  for (int k = 0; k < COPY_THREADS; ++k) {
+    const size_t o = (size_t)k * part;
+    if (o >= n) break;
+    const size_t len = std::min(part, n - o);
+    th.emplace_back([=] { std::memcpy(dst + o, src + o, len); });
+  }
+  for (auto& t : th) t.join();
+}
+
+int snapshot_out(kb_ctx* c, char* snap) {  // device window -> host
+  AdjState* s = c->adj;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const char* H = reinterpret_cast<const char*>(c->H);
+  const size_t nch = (s->hbytes + STAGE_BYTES - 1) / STAGE_BYTES;
+  auto len = [&](size_t i) { return std::min(STAGE_BYTES, s->hbytes - i * STAGE_BYTES); };
+  for (size_t i = 0; i <= nch; ++i) {
+    if (i < nch) {
+      HIPCHK(c, hipMemcpyAsync(s->stage[i % 2], H + i * STAGE_BYTES, len(i), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipEventRecord(s->stage_ev[i % 2], c->stream));
+    }
+    if (i > 0) {  // chunk i-1 landed: copy it out while chunk i crosses
+      HIPCHK(c, hipEventSynchronize(s->stage_ev[(i - 1) % 2]));
+      parallel_copy(snap + (i - 1) * STAGE_BYTES, s->stage[(i - 1) % 2], len(i - 1));
+    }
+  }
+  return KB_OK;
+}
+
+int snapshot_in(kb_ctx* c, const char* snap) {  // host -> device window
+  AdjState* s = c->adj;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  char* H = reinterpret_cast<char*>(c->H);
+  const size_t nch = (s->hbytes + STAGE_BYTES - 1) / STAGE_BYTES;
+  for (size_t i = 0; i < nch; ++i) {
+    const size_t n = std::min(STAGE_BYTES, s->hbytes - i * STAGE_BYTES);
+    if (i >= 2) HIPCHK(c, hipEventSynchronize(s->stage_ev[i % 2]));  // its last upload is done
+    parallel_copy(s->stage[i % 2], snap + i * STAGE_BYTES, n);
+    HIPCHK(c, hipMemcpyAsync(H + i * STAGE_BYTES, s->stage[i % 2], n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipEventRecord(s->stage_ev[i % 2], c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return KB_OK;
+}
+
 // copy the window (boundary b) into a free host slot
 int store_snapshot(kb_ctx* c, int b) {
   AdjState* s = c->adj;
   const int i = free_slot(s);
   if (i < 0) return seterr(c, KB_ERR_STATE, "adjoint: no free host snapshot slot");
-  HIPCHK(c, hipMemcpyAsync(s->hsnap[i], c->H, s->hbytes, hipMemcpyDeviceToHost, c->stream));
+  int rc;
+  if ((rc = snapshot_out(c, s->hsnap[i]))) return rc;
   s->hseg[i] = b;
   return KB_OK;
 }
@@ -1144,7 +1209,8 @@ int host_window(kb_ctx* c, int k) {
     }
   }
   if (slot >= 0) {
-    HIPCHK(c, hipMemcpyAsync(c->H, s->hsnap[slot], s->hbytes, hipMemcpyHostToDevice, c->stream));
+    int rc;
+    if ((rc = snapshot_in(c, s->hsnap[slot]))) return rc;
   } else {
     // the initial window: the three initial rows on a cleared history
     HIPCHK(c, hipMemsetAsync(c->H, 0, s->hbytes, c->stream));
@@ -1258,6 +1324,7 @@ int adj_record(kb_ctx* c, int checkpoint_every, int max_steps, int host_windows)
   if (!s) return KB_ERR_OOM;
   s->C = checkpoint_every < 0 ? 0 : checkpoint_every;
   s->packed = checkpoint_every < 0;
+  s->progress = std::getenv("QFTKB_PROGRESS") != nullptr;
   s->max_steps = max_steps;
   const size_t hcount = (size_t)c->NB * (size_t)c->NP * MB * 2;
   s->hbytes = hcount * sizeof(double);
@@ -1325,14 +1392,18 @@ int adj_record(kb_ctx* c, int checkpoint_every, int max_steps, int host_windows)
     // windows in pinned host memory, the initial one as its three rows
     s->hwin = host_windows;
     for (int i = 0; i < host_windows; ++i) {
-      double* p = nullptr;
-      if (hipHostMalloc((void**)&p, s->hbytes, hipHostMallocDefault) != hipSuccess)
-        return fail(seterr(c, KB_ERR_OOM, "adjoint: %d host windows of %zu bytes do not fit (pinned host memory)",
+      char* p = static_cast<char*>(std::malloc(s->hbytes));
+      if (!p)
+        return fail(seterr(c, KB_ERR_OOM, "adjoint: %d host windows of %zu bytes do not fit in host memory",
                            host_windows, s->hbytes));
       s->hsnap.push_back(p);
       s->hseg.push_back(-1);
       s->host_bytes += s->hbytes;
     }
+    for (int k = 0; k < 2 && host_windows > 0; ++k)
+      if (hipHostMalloc((void**)&s->stage[k], STAGE_BYTES, hipHostMallocDefault) != hipSuccess ||
+          hipEventCreateWithFlags(&s->stage_ev[k], hipEventDisableTiming) != hipSuccess)
+        return fail(seterr(c, KB_ERR_OOM, "adjoint: pinned staging buffers"));
     const int nseg = (max_steps - 2 + s->C - 1) / s->C;
     revolve_tables(s, nseg, host_windows);
     s->plan = host_plan(s, nseg, host_windows);
@@ -1442,6 +1513,12 @@ int kb_adj_gradient(kb_ctx* c, const double* wgt, double* dL_dlambda, double* dL
       const int first = seg_first(t, s->C);
       if ((rc = recompute_segment(c, first, t))) return rc;
       have_from = first;
+      if (s->progress) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        std::fprintf(stderr, "[qftkb] adjoint: reversing steps %d .. %d, %lld steps recomputed so far\n", t, first,
+                     s->recomputed);
+        std::fflush(stderr);
+      }
     }
     const bool apply = npend + 1 == AK || t == 3 || t == stop;
     if ((rc = reverse_step(c, t, T, npend, apply))) return rc;
