@@ -39,6 +39,8 @@
 #include <new>
 #include <thread>
 
+#include <unistd.h>
+
 #include "kb_internal.h"
 
 using namespace qftkb;
@@ -1391,6 +1393,12 @@ int adj_record(kb_ctx* c, int checkpoint_every, int max_steps, int host_windows)
   if (s->C > 0 && host_windows >= 0) {
     // windows in pinned host memory, the initial one as its three rows
     s->hwin = host_windows;
+    // the windows are touched only when first written: refuse up front what
+    // the machine's free physical memory cannot hold
+    const double avail = (double)sysconf(_SC_AVPHYS_PAGES) * (double)sysconf(_SC_PAGESIZE);
+    if ((double)host_windows * (double)s->hbytes > 0.9 * avail)
+      return fail(seterr(c, KB_ERR_OOM, "adjoint: %d host windows of %zu bytes exceed 90%% of the %.0f free bytes",
+                         host_windows, s->hbytes, avail));
     for (int i = 0; i < host_windows; ++i) {
       char* p = static_cast<char*>(std::malloc(s->hbytes));
       if (!p)
