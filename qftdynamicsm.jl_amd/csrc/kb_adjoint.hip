@@ -620,162 +620,178 @@ __global__ __launch_bounds__(AM_THREADS, 2) void adj_reduce_kernel(AdjArgs a) {
 }
 
 // ---- apply: the deferred entry updates of npend steps ----------------------
-// One workgroup per site block walks the triangle in chunks of AP_ROWS
-// absolute slot rows.  The row coefficients of the chunk (v, ms, rf, rr per
-// deferred step; u = w ms with w = 1/2 at that step's tmin) are staged in
-// LDS once.  8-lane workers then take half column blocks (2 of the 4
-// columns), keep those columns' coefficients in registers and stream the
-// rows with AP_D rows in flight, one read-modify-write per adjoint line.
+// The triangle of every site block is cut into chunks of AP_ROWS slot rows,
+// one workgroup per (site block, chunk).  The workgroup stages its rows'
+// coefficients (u = w ms, rf, ms, v, rr per deferred step and lane) in LDS.
+// Each wave then takes panels of 8 columns (two column blocks, one column per
+// 8-lane worker) and walks the chunk's rows with all its workers on the same
+// row, so the staged row coefficients are read as broadcasts.  A worker keeps
+// its column's coefficients in registers, sums the AK steps' terms of an
+// entry and adds them in one read-modify-write of the adjoint line, with
+// two batches of AP_D rows in flight:
+//   F_bar(a,b) -= sum_j u_j(a) rf_j(b) + rf_j(a) u_j(b)     (half of it at a = b)
+//   rho_bar    -= sum_j v_j(b) rf_j(a) + ms_j(a) rr_j(b)   (a later than b)
+// The stored pair (row, column) has the row later in time unless the column
+// lies before s_min and the row does not (the ring has wrapped); then the
+// roles swap and the stored rho changes sign.
+// Workgroups are dealt to the 8 XCDs in turn; the chunks of one site block
+// are consecutive workgroups of one XCD, so the coefficient rings that every
+// chunk reads for its columns come from HBM about once and then from that
+// XCD's L2.
 constexpr int AP_THREADS = 256;
-constexpr int AP_ROWS = 256 / AK;   // 64 KB of staged row coefficients
-constexpr int AP_C = 2;             // columns per worker
-constexpr int AP_D = AK > 4 ? 2 : 4;  // rows per batch (two batches in flight)
+constexpr int AP_ROWS = 28;  // rows per chunk: 70 KB of staged coefficients
+constexpr int AP_D = 4;      // rows per batch
+constexpr int AP_Q = 5;      // staged values per (row, step, lane)
 
-__device__ __forceinline__ size_t ap_lds(int row, int j, int q, int lane) {  // q: 0 v, 1 ms, 2 rf, 3 rr
-  return (((size_t)row * AK + j) * 4 + q) * MB + lane;
-}
+// staged coefficients of (row, step j): (u, rf) of the 8 lanes, (ms, v) of
+// the 8 lanes, rr of the 8 lanes
+__device__ __forceinline__ int ap_base(int row, int j) { return (row * AK + j) * AP_Q * MB; }
 
-struct ApCols {
-  double v[AK][AP_C], ms[AK][AP_C], rf[AK][AP_C], rr[AK][AP_C];
-  unsigned half;  // bit j*AP_C + k: w = 1/2 for column k in step j's window
-  unsigned phi;   // bit k: column k lies before s_min
+__host__ __device__ inline int apply_chunks(int Rv) { return (Rv + AP_ROWS - 1) / AP_ROWS; }
+
+struct ApCol {
+  double cf[AK], uc[AK], cv[AK], crr[AK], cms[AK];
 };
 
-__device__ __forceinline__ void ap_load(double2 (&e)[AP_D][AP_C], const double* Hbs, long cbase, int c0, int k0,
+// The adjoint lines are read and written through a buffer resource over the
+// site block's region: a lane whose entry is not updated (above the
+// diagonal, or a row past the chunk in the last batch) gets an offset beyond
+// the resource's size, and the hardware then returns zeros for its load and
+// drops its store, so masked entries cost no memory traffic and the loop
+// needs no branch.  Dword 3 of the resource: 32-bit data format, the value
+// for the gfx9 family.
+constexpr unsigned AP_OFF_NONE = 0x80000000u;
+typedef unsigned kb_u4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t ap_rsrc(double* base, long NP) {
+  return __builtin_amdgcn_make_buffer_rsrc(base, 0, (int)(NP * MB * 16), 0x00020000);
+}
+
+// byte offset of this worker's line in row rr, or AP_OFF_NONE
+__device__ __forceinline__ unsigned ap_off(unsigned col_off, int c, int c0, int rr, int rlast) {
+  return (rr <= rlast && c <= rr) ? col_off + (unsigned)(rr - c0) * (CB * MB * 16) : AP_OFF_NONE;
+}
+
+__device__ __forceinline__ void ap_load(double2 (&e)[AP_D], __amdgpu_buffer_rsrc_t rs, unsigned col_off, int c, int c0,
                                         int r, int rlast) {
 #pragma unroll
   for (int d = 0; d < AP_D; ++d) {
-    const int rr = min(r + d, rlast);
-#pragma unroll
-    for (int k = 0; k < AP_C; ++k)
-      e[d][k] = ld2_nt(Hbs + (size_t)(cbase + (long)(rr - c0) * CB + k0 + k) * MB * 2);
+    const kb_u4v v = __builtin_amdgcn_raw_buffer_load_b128(rs, ap_off(col_off, c, c0, r + d, rlast), 0, 2);
+    e[d] = __builtin_bit_cast(double2, v);
   }
 }
 
-__device__ __forceinline__ void ap_rows(const AdjArgs& a, double2 (&e)[AP_D][AP_C], double* Hbs, const double* rco,
-                                        long cbase, int c0, int k0, int r, int r1, int R0, const ApCols& cs,
-                                        int lane) {
-  const StepState& st = a.st;
-  const int N = a.N;
+__device__ __forceinline__ void ap_rows(double2 (&e)[AP_D], __amdgpu_buffer_rsrc_t rs, unsigned col_off,
+                                        const double* rco, const ApCol& k, int c, int c0, int r, int rlast, int R0,
+                                        bool cwrap, int smin, int lane) {
 #pragma unroll
   for (int d = 0; d < AP_D; ++d) {
-    const int rr = r + d;
-    if (rr < r1) {
-      const int tx = st.tmin + window_pos(rr, st.s_min, N);
-      const unsigned neg = rr >= st.s_min ? cs.phi : 0u;
-      const int row = rr - R0;
+    const int rc = min(r + d, rlast);
+    const double* rb = rco + ap_base(rc - R0, 0);
+    double ax = 0.0, yp = 0.0, yn = 0.0;
 #pragma unroll
-      for (int j = 0; j < AK; ++j) {
-        if (j < a.npend) {
-          const double rv = rco[ap_lds(row, j, 0, lane)];
-          const double rms = rco[ap_lds(row, j, 1, lane)];
-          const double rrf = rco[ap_lds(row, j, 2, lane)];
-          const double rrr = rco[ap_lds(row, j, 3, lane)];
-          const double ru = (tx == a.ptmin[j] ? 0.5 : 1.0) * rms;
-#pragma unroll
-          for (int k = 0; k < AP_C; ++k) {
-            const int c = c0 + k0 + k;
-            if (c < rr) {
-              const double nf = ((neg >> k) & 1u) ? 1.0 : 0.0, pf = 1.0 - nf;
-              const double uc = ((cs.half >> (j * AP_C + k)) & 1u ? 0.5 : 1.0) * cs.ms[j][k];
-              e[d][k].x -= ru * cs.rf[j][k] + uc * rrf;
-              e[d][k].y += nf * (rv * cs.rf[j][k] + cs.ms[j][k] * rrr) - pf * (cs.v[j][k] * rrf + rms * cs.rr[j][k]);
-            } else if (c == rr) {
-              e[d][k].x -= ru * rrf;
-            }
-          }
-        }
-      }
-      const long pb = cbase + (long)(rr - c0) * CB + k0;
-#pragma unroll
-      for (int k = 0; k < AP_C; ++k)
-        if (c0 + k0 + k <= rr) st2_nt(Hbs + (size_t)(pb + k) * MB * 2, e[d][k]);
+    for (int j = 0; j < AK; ++j) {
+      const double2 A = *reinterpret_cast<const double2*>(rb + j * AP_Q * MB + 2 * lane);           // u, rf
+      const double2 B = *reinterpret_cast<const double2*>(rb + j * AP_Q * MB + 2 * MB + 2 * lane);  // ms, v
+      const double R = rb[j * AP_Q * MB + 4 * MB + lane];                                           // rr
+      ax = fma(A.x, k.cf[j], ax);
+      ax = fma(k.uc[j], A.y, ax);
+      yp = fma(k.cv[j], A.y, yp);
+      yp = fma(B.x, k.crr[j], yp);
+      yn = fma(B.y, k.cf[j], yn);
+      yn = fma(k.cms[j], R, yn);
     }
+    // c == rc: half of the F term, no rho term; c > rc: the store is dropped
+    const bool nf = cwrap && rc >= smin;
+    e[d].x -= (c < rc ? 1.0 : 0.5) * ax;
+    e[d].y += c < rc ? (nf ? yn : -yp) : 0.0;
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(kb_u4v, e[d]), rs,
+                                           ap_off(col_off, c, c0, r + d, rlast), 0, 2);
   }
 }
 
 __global__ __launch_bounds__(AP_THREADS, 2) void adj_apply_kernel(AdjArgs a) {
-  extern __shared__ double rco[];  // AP_ROWS x AK x 4 x MB
-  __shared__ int next_item;
+  // AP_ROWS x AK x AP_Q x MB staged values, then the panel counter; no
+  // static LDS, so the staged pairs start 16-byte aligned
+  extern __shared__ double rco[];
+  int& next_item = *reinterpret_cast<int*>(rco + AP_ROWS * AK * AP_Q * MB);
   const StepState& st = a.st;
   const int N = a.N;
-  const int t = st.t;
-  const int Rv = t > N ? N : t;
+  const int Rv = st.t > N ? N : st.t;
   const int smin = st.s_min;
-  const int mb = blockIdx.x;
+  const int nchunk = apply_chunks(Rv);
+  const int g = blockIdx.x;
+  const int q = g >> 3;
+  const int mbg = q / nchunk;
+  const int mb = mbg * 8 + (g & 7);
+  if (mb >= a.NB) return;
+  const int chunk = nchunk - 1 - (q - mbg * nchunk);  // largest chunk first
+  const int R0 = chunk * AP_ROWS;
+  const int R1 = min(R0 + AP_ROWS, Rv);
   const int tid = threadIdx.x;
   const int lane = tid & (MB - 1);
-  const int worker = tid >> 3;
-  const int nworkers = AP_THREADS >> 3;
-  double* Hbs = a.Hbar + hoff(mb, 0, lane, a.NP);
-  for (int R0 = 0; R0 < Rv; R0 += AP_ROWS) {
-    const int R1 = min(R0 + AP_ROWS, Rv);
-    // stage the chunk's row coefficients
-    for (int i = worker; i < (R1 - R0) * AK; i += nworkers) {
-      const int row = i / AK, j = i - row * AK;
-      const int r = R0 + row;
-      const PCoef k = pend_coef(a, j, mb, lane, r, st.tmin + window_pos(r, smin, N));
-      rco[ap_lds(row, j, 0, lane)] = k.v;
-      rco[ap_lds(row, j, 1, lane)] = k.ms;
-      rco[ap_lds(row, j, 2, lane)] = k.rf;
-      rco[ap_lds(row, j, 3, lane)] = k.rr;
-    }
-    if (tid == 0) next_item = nworkers;
-    __syncthreads();
-    const int nJ = (R1 - 1) / CB + 1;  // column blocks with rows in the chunk
-    const int nitem = nJ * (CB / AP_C);
-    int item = worker;
-    while (item < nitem) {
-      const int J = item / (CB / AP_C);
-      const int k0 = (item - J * (CB / AP_C)) * AP_C;
-      const int c0 = J * CB;
-      const int r0 = max(R0, c0 + k0);  // rows from the diagonal of the first column on
-      if (c0 + k0 < Rv && r0 < R1) {
-        const long cbase = colblock_offset(J, N);
-        ApCols cs;
-        cs.half = 0;
-        cs.phi = 0;
+  for (int i = tid >> 3; i < (R1 - R0) * AK; i += AP_THREADS >> 3) {
+    const int row = i / AK, j = i - row * AK;
+    const int r = R0 + row;
+    const PCoef k = pend_coef(a, j, mb, lane, r, st.tmin + window_pos(r, smin, N));
+    double* b = rco + ap_base(row, j);
+    b[2 * lane] = k.u;
+    b[2 * lane + 1] = k.rf;
+    b[2 * MB + 2 * lane] = k.ms;
+    b[2 * MB + 2 * lane + 1] = k.v;
+    b[4 * MB + lane] = k.rr;
+  }
+  if (tid == 0) next_item = AP_THREADS / 64;
+  __syncthreads();
+  const int w = (tid & 63) >> 3;  // this worker's column in the panel
+  const __amdgpu_buffer_rsrc_t rs = ap_rsrc(a.Hbar + hoff(mb, 0, 0, a.NP), a.NP);
+  const int npan = (R1 + 7) / 8;
+  const int rlast = R1 - 1;
+  int p = tid >> 6;
+  while (p < npan) {
+    const int c = 8 * p + w;
+    const int J = c / CB;
+    const int c0 = J * CB;
+    if (c0 < R1) {  // the column block has rows in this chunk
+      const int cc = min(c, Rv - 1);
+      const int txc = st.tmin + window_pos(cc, smin, N);
+      const bool ok = c < Rv;
+      ApCol k;
 #pragma unroll
-        for (int k = 0; k < AP_C; ++k) {
-          const int c = c0 + k0 + k;
-          const int cc = min(c, Rv - 1);
-          const int txc = st.tmin + window_pos(cc, smin, N);
-#pragma unroll
-          for (int j = 0; j < AK; ++j) {
-            const PCoef q = pend_coef(a, j, mb, lane, cc, txc);
-            const bool ok = c < Rv;
-            if (txc == a.ptmin[j]) cs.half |= 1u << (j * AP_C + k);
-            cs.v[j][k] = ok ? q.v : 0.0;
-            cs.ms[j][k] = ok ? q.ms : 0.0;
-            cs.rf[j][k] = ok ? q.rf : 0.0;
-            cs.rr[j][k] = ok ? q.rr : 0.0;
-          }
-          cs.phi |= (c < smin ? 1u : 0u) << k;
-        }
-        const int rlast = R1 - 1;
-        double2 X[AP_D][AP_C], Y[AP_D][AP_C];
-        int r = r0;
-        ap_load(X, Hbs, cbase, c0, k0, r, rlast);
-        while (true) {
-          ap_load(Y, Hbs, cbase, c0, k0, r + AP_D, rlast);
-          ap_rows(a, X, Hbs, rco, cbase, c0, k0, r, R1, R0, cs, lane);
-          r += AP_D;
-          if (r >= R1) break;
-          ap_load(X, Hbs, cbase, c0, k0, r + AP_D, rlast);
-          ap_rows(a, Y, Hbs, rco, cbase, c0, k0, r, R1, R0, cs, lane);
-          r += AP_D;
-          if (r >= R1) break;
-        }
+      for (int j = 0; j < AK; ++j) {
+        const PCoef q = pend_coef(a, j, mb, lane, cc, txc);
+        k.cf[j] = ok ? q.rf : 0.0;
+        k.uc[j] = ok ? q.u : 0.0;
+        k.cv[j] = ok ? q.v : 0.0;
+        k.crr[j] = ok ? q.rr : 0.0;
+        k.cms[j] = ok ? q.ms : 0.0;
       }
-      int nx = 0;
-      if (lane == 0) nx = atomicAdd(&next_item, 1);
-      item = __shfl(nx, tid & ~(MB - 1) & 63);
+      const unsigned col_off = (unsigned)((colblock_offset(J, N) + (c - c0)) * MB + lane) * 16u;
+      double2 X[AP_D], Y[AP_D];
+      int r = max(R0, 8 * p);
+      ap_load(X, rs, col_off, c, c0, r, rlast);
+      // the scheduling barriers keep each batch's loads ahead of the other
+      // batch's arithmetic
+      while (true) {
+        ap_load(Y, rs, col_off, c, c0, r + AP_D, rlast);
+        __builtin_amdgcn_sched_barrier(0);
+        ap_rows(X, rs, col_off, rco, k, c, c0, r, rlast, R0, c < smin, smin, lane);
+        r += AP_D;
+        if (r >= R1) break;
+        ap_load(X, rs, col_off, c, c0, r + AP_D, rlast);
+        __builtin_amdgcn_sched_barrier(0);
+        ap_rows(Y, rs, col_off, rco, k, c, c0, r, rlast, R0, c < smin, smin, lane);
+        r += AP_D;
+        if (r >= R1) break;
+      }
     }
-    __syncthreads();
+    int nx = 0;
+    if ((tid & 63) == 0) nx = atomicAdd(&next_item, 1);
+    p = __shfl(nx, 0);
   }
 }
 
-size_t apply_lds() { return sizeof(double) * (size_t)AP_ROWS * AK * 4 * MB; }
+size_t apply_lds() { return sizeof(double) * (size_t)AP_ROWS * AK * AP_Q * MB + 16; }
 
 // the Hartree adjoint of step 3 and the initial-row gradients:
 // F(1,1) = F(2,1) = (n+1/2)/w, F(2,2) = (n+1/2)(w dt^2 + 1/w)
@@ -1039,7 +1055,9 @@ int reverse_step(kb_ctx* c, int t, int T, int npend, bool apply) {
   if (nlo && apply) {
     StageTimer tm(c, 3);
     set_pending(c, a, t, npend + 1);
-    hipLaunchKernelGGL(adj_apply_kernel, dim3(c->NB), dim3(AP_THREADS), apply_lds(), c->stream, a);
+    const int Rv = t > N ? N : t;
+    hipLaunchKernelGGL(adj_apply_kernel, dim3((unsigned)((c->NB + 7) / 8 * 8 * apply_chunks(Rv))), dim3(AP_THREADS),
+                       apply_lds(), c->stream, a);
     LAUNCHCHK(c);
   }
   {
