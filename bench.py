@@ -619,7 +619,7 @@ def main():
                           "streaming_read_GBs": read_bandwidth(0, 16 << 30)}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not supported "
+        print(json.dumps({"error": "bench.py measures one GPU; --gpus N > 1 is not supported "
                                    "(multi-GPU sharding is design-only)"}))
         sys.exit(2)
     if args.ensemble:
