@@ -444,7 +444,7 @@ def large_adjoint_host_bench(C=150, host_windows=2, fd=True):
     """The same 48^3 differentiation for a state without the cubic symmetry
     (the box occupation modulated along x: n(p) = box(p) (1 + 0.2 sin(2 pi
     i_x / Nx + 0.3))), so the orbit-packed tape does not apply: the window
-    (80.7 GB) is kept at segment boundaries in pinned host memory (at most
+    (80.7 GB) is kept at segment boundaries in host memory (at most
     host_windows of them) under the binomial checkpointing schedule, and
     the device tape holds one C-step segment (kb_adj_record_host).  Forward
     steps/s, the gradient's cost ratio, the recomputed steps, and dL/dlambda
