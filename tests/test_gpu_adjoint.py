@@ -100,6 +100,27 @@ def test_adjoint_matches_twin_and_oracle_fd(args, T):
     c.close()
 
 
+@pytest.mark.parametrize("args,T", [
+    pytest.param((4, 3, 61, 0.1, 1.3, 1, 1.0, 1.0, 1), 80, id="4^3-N61-three-chunks"),
+    pytest.param((4, 3, 30, 0.1, 1.1, 1, 1.0, 1.0, 1), 75, id="4^3-N30-two-chunks-wrapped"),
+    pytest.param((5, 2, 33, 0.08, 1.6, 2, 0.9, 0.81, 1), 70, id="5^2-N33-ragged-blocks"),
+])
+def test_adjoint_matches_twin_across_apply_chunks(args, T):
+    """Windows longer than one chunk of the deferred-update kernel (28 slot
+    rows), not a multiple of its 8-column panels, sliding so that the ring
+    wraps: the gradient against the numpy twin."""
+    print(f"\n[case] adjoint chunks {args} T={T}", flush=True)
+    n0, m, wgt = setup(args)
+    c, L, g = gpu_gradient(args, n0, m, T, wgt)
+    Lt, lt, dnt, dFt = gradient(args, n0, m, T, wgt)
+    assert L == pytest.approx(Lt, rel=1e-12)
+    assert g["lambda"] == pytest.approx(lt, rel=TWIN_TOL, abs=TWIN_TOL * abs(Lt))
+    assert rel(g["n0"], dnt) <= TWIN_TOL
+    for k in range(3):
+        assert rel(g["F0"][k], dFt[k]) <= TWIN_TOL, k
+    c.close()
+
+
 def test_adjoint_entries_after_partial_sweep():
     """Stop the sweep after a few reverse steps and compare every adjoint
     entry of the window with the twin's (the Hartree adjoint of the last
