@@ -137,14 +137,19 @@ __device__ __forceinline__ void small_dft(double2 (&v)[R], const double2* __rest
 }
 
 // One Stockham stage on `nlines` lines; element (line, j) at line*ls + j*es.
-template <int R>
+// NL > 0: nlines == NL, known at compile time.
+template <int R, int NL = 0>
 __device__ __forceinline__ void stage(const double2* __restrict__ src, double2* __restrict__ dst, int n, int Ns,
                                       int nlines, int ls, int es, const double2* __restrict__ tw, bool inv) {
   const int m = n / R;
   const int step = n / (Ns * R);
-  for (int idx = threadIdx.x; idx < nlines * m; idx += blockDim.x) {
-    const int line = idx / m;
-    const int j = idx - line * m;
+  const int nl = NL > 0 ? NL : nlines;
+  // neighbouring lanes take the same butterfly of neighbouring lines: their
+  // stride-R writes (and reads) then fall on different LDS banks, and they
+  // share the twiddle
+  for (int idx = threadIdx.x; idx < nl * m; idx += blockDim.x) {
+    const int j = idx / nl;
+    const int line = idx - j * nl;
     const int kk = j % Ns;
     const double2* s = src + line * ls;
     double2* d = dst + line * ls;
@@ -168,8 +173,8 @@ __device__ void stage_direct(const double2* __restrict__ src, double2* __restric
   const int m = n / R;
   const int step = n / (Ns * R);
   for (int idx = threadIdx.x; idx < nlines * n; idx += blockDim.x) {
-    const int line = idx / n;
-    const int rem = idx - line * n;
+    const int rem = idx / nlines;
+    const int line = idx - rem * nlines;
     const int j = rem / R;
     const int q = rem - j * R;
     const int kk = j % Ns;
@@ -184,11 +189,11 @@ __device__ void stage_direct(const double2* __restrict__ src, double2* __restric
   }
 }
 
-template <int R>
+template <int R, int NL = 0>
 __device__ __forceinline__ void stage_any(const double2* src, double2* dst, int n, int Ns, int nlines, int ls, int es,
                                           const double2* tw, bool inv) {
   if constexpr (R == 2 || R == 3 || R == 4 || R == 5 || R == 7)
-    stage<R>(src, dst, n, Ns, nlines, ls, es, tw, inv);
+    stage<R, NL>(src, dst, n, Ns, nlines, ls, es, tw, inv);
   else
     stage_direct(src, dst, n, Ns, R, nlines, ls, es, tw, inv);
 }
@@ -202,7 +207,10 @@ __device__ __forceinline__ double2* stages_ct(double2* a, double2* b, int nlines
     return a;
   } else {
     constexpr int R = F.f[S];
-    stage_any<R>(a, b, N, NS, nlines, ls, es, tw, inv);
+    if (nlines == N)  // the 3D planes and pencils: as many lines as points
+      stage_any<R, N>(a, b, N, NS, nlines, ls, es, tw, inv);
+    else
+      stage_any<R>(a, b, N, NS, nlines, ls, es, tw, inv);
     __syncthreads();
     return stages_ct<N, S + 1, NS * R>(b, a, nlines, ls, es, tw, inv);
   }
