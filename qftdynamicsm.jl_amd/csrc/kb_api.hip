@@ -401,8 +401,8 @@ size_t memint_lds(int N) { return sizeof(double) * 2 * (size_t)N * MB; }
 int launch_hartree_step(kb_ctx* c) {
   const double H = hartree_prefactor(c);
   const double hfac = H * (1.0 / (double)c->Vl);
-  hipLaunchKernelGGL(hartree_step_kernel, dim3(1), dim3(1024), 0, c->stream, c->H, c->NP, c->V, c->N, c->st_dev,
-                     hfac, c->p.Mass2, c->wgt_dev, c->m2);
+  hipLaunchKernelGGL(hartree_step_kernel, dim3(hartree_groups(c->V)), dim3(HARTREE_THREADS), 0, c->stream, c->H, c->NP,
+                     c->V, c->N, c->st_dev, hfac, c->p.Mass2, c->wgt_dev, c->m2, c->hpart);
   LAUNCHCHK(c);
   return KB_OK;
 }
@@ -485,7 +485,7 @@ void drop_graph(kb_ctx* c);
 
 void free_all(kb_ctx* c) {
   drop_graph(c);
-  for (void* ptr : {(void*)c->H, (void*)c->S, (void*)c->W, (void*)c->kL2, (void*)c->m2, (void*)c->partial,
+  for (void* ptr : {(void*)c->H, (void*)c->S, (void*)c->W, (void*)c->kL2, (void*)c->m2, (void*)c->hpart, (void*)c->partial,
                     (void*)c->modes, (void*)c->vbuf, (void*)c->tw, (void*)c->flag, (void*)c->items_r[0],
                     (void*)c->items_r[1], (void*)c->items_r[2], (void*)c->smap_dev, (void*)c->rmap_dev,
                     (void*)c->wgt_dev, (void*)c->PF, (void*)c->PR, (void*)c->IR, (void*)c->Iring, (void*)c->Xn,
@@ -733,6 +733,7 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
   if (c->fft_tile != KB_FFT_ORBIT_PRODUCT && (rc = ensure_plane_workspace(c))) return fail(rc);
   if ((rc = dmalloc(c, (void**)&c->kL2, (size_t)c->Vpad * sizeof(double)))) return fail(rc);
   if ((rc = dmalloc(c, (void**)&c->m2, 8 * sizeof(double)))) return fail(rc);
+  if ((rc = dmalloc(c, (void**)&c->hpart, hartree_part_doubles(c->V) * sizeof(double)))) return fail(rc);
   if ((rc = dmalloc(c, (void**)&c->partial, (size_t)6 * meas_grid(c) * sizeof(double)))) return fail(rc);
   if ((rc = dmalloc(c, (void**)&c->modes, (size_t)6 * c->V * sizeof(double)))) return fail(rc);
   // staging: 4 lattices, or the 4 site-1 history vectors of kb_measure_history
@@ -813,6 +814,8 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
   if (hipMemsetAsync(c->S, 0, (size_t)c->NB * c->N * MB * 2 * sizeof(double), c->stream) != hipSuccess)
     return fail(KB_ERR_HIP);
   if (hipMemsetAsync(c->m2, 0, 8 * sizeof(double), c->stream) != hipSuccess) return fail(KB_ERR_HIP);
+  if (hipMemsetAsync(c->hpart, 0, hartree_part_doubles(c->V) * sizeof(double), c->stream) != hipSuccess)
+    return fail(KB_ERR_HIP);
   if (c->PF) {
     for (double* ptr : {c->PF, c->PR, c->IR})
       if (hipMemsetAsync(ptr, 0, pcount * sizeof(double), c->stream) != hipSuccess) return fail(KB_ERR_HIP);

@@ -54,6 +54,7 @@ struct kb_ctx {
   double2* W = nullptr;    // FFT workspace
   double* kL2 = nullptr;
   double* m2 = nullptr;    // [0] current Hartree mass^2
+  double* hpart = nullptr; // Hartree partial sums and their counter (hartree_part_doubles)
   double* partial = nullptr;
   double* modes = nullptr; // 6 x V measurement outputs
   double* vbuf = nullptr;  // 4 x V staging

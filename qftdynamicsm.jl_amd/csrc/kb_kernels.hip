@@ -271,26 +271,55 @@ __global__ __launch_bounds__(1024) void hartree_kernel(const double* __restrict_
   }
 }
 
-__global__ __launch_bounds__(1024) void hartree_step_kernel(const double* __restrict__ H, long NP, int V, int N,
-                                                            const StepState* __restrict__ st, double hfac,
-                                                            double Mass2, const double* __restrict__ wgt,
-                                                            double* __restrict__ out) {
-  __shared__ double red[1024];
+// One workgroup per HARTREE_SITES sites, HARTREE_LOADS loads in flight per
+// thread (the diagonal entries lie one per 128-byte line of every site
+// block, so a single workgroup is latency-bound).  Each workgroup writes its
+// partial sum; the last one to finish (an agent-scope counter: the
+// workgroups sit on different XCDs) adds the partials in workgroup order, so
+// the result does not depend on which one that is, and resets the counter.
+__global__ __launch_bounds__(HARTREE_THREADS) void hartree_step_kernel(const double* __restrict__ H, long NP, int V,
+                                                                       int N, const StepState* __restrict__ st,
+                                                                       double hfac, double Mass2,
+                                                                       const double* __restrict__ wgt,
+                                                                       double* __restrict__ out, double* part) {
+  __shared__ double red[HARTREE_THREADS / 64];
+  __shared__ int is_last;
   const long pair = pair_off(st->s_tm1, st->s_tm1, N);  // F(t-1,t-1), Evolution.jl:945
-  double acc = 0.0;
-  for (int p = threadIdx.x; p < V; p += blockDim.x)
-    acc += (wgt ? wgt[p] : 1.0) * H[hoff(p / MB, pair, p % MB, NP)];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
+  const int G = gridDim.x;
+  double v[HARTREE_LOADS];
+#pragma unroll
+  for (int u = 0; u < HARTREE_LOADS; ++u) {
+    const int p = blockIdx.x * HARTREE_SITES + u * HARTREE_THREADS + threadIdx.x;
+    v[u] = p < V ? (wgt ? wgt[p] : 1.0) * H[hoff(p / MB, pair, p % MB, NP)] : 0.0;
   }
+  double acc = 0.0;
+#pragma unroll
+  for (int u = 0; u < HARTREE_LOADS; ++u) acc += v[u];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  unsigned* count = reinterpret_cast<unsigned*>(part + G);
   if (threadIdx.x == 0) {
-    double h = red[0];
-    h *= hfac;
-    h += Mass2;
-    out[0] = h;
+    double b = 0.0;
+    for (int w = 0; w < HARTREE_THREADS / 64; ++w) b += red[w];
+    part[blockIdx.x] = b;
+    is_last = __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == (unsigned)G - 1;
+  }
+  __syncthreads();
+  if (!is_last) return;
+  // the partials in a fixed order: thread k takes k, k + T, ..., then the
+  // same tree as above
+  double h = 0.0;
+  for (int g = threadIdx.x; g < G; g += HARTREE_THREADS) h += part[g];
+  for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = h;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = 0.0;
+    for (int w = 0; w < HARTREE_THREADS / 64; ++w) tot += red[w];
+    out[0] = tot * hfac + Mass2;
+    __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
