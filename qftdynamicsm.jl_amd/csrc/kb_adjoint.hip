@@ -661,6 +661,7 @@ struct ApCol {
 // needs no branch.  Dword 3 of the resource: 32-bit data format, the value
 // for the gfx9 family.
 constexpr unsigned AP_OFF_NONE = 0x80000000u;
+constexpr int AP_NT = 2;  // cache policy of the adjoint lines: non-temporal (streamed once per call)
 typedef unsigned kb_u4v __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t ap_rsrc(double* base, long NP) {
@@ -676,7 +677,7 @@ __device__ __forceinline__ void ap_load(double2 (&e)[AP_D], __amdgpu_buffer_rsrc
                                         int r, int rlast) {
 #pragma unroll
   for (int d = 0; d < AP_D; ++d) {
-    const kb_u4v v = __builtin_amdgcn_raw_buffer_load_b128(rs, ap_off(col_off, c, c0, r + d, rlast), 0, 2);
+    const kb_u4v v = __builtin_amdgcn_raw_buffer_load_b128(rs, ap_off(col_off, c, c0, r + d, rlast), 0, AP_NT);
     e[d] = __builtin_bit_cast(double2, v);
   }
 }
@@ -706,7 +707,7 @@ __device__ __forceinline__ void ap_rows(double2 (&e)[AP_D], __amdgpu_buffer_rsrc
     e[d].x -= (c < rc ? 1.0 : 0.5) * ax;
     e[d].y += c < rc ? (nf ? yn : -yp) : 0.0;
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(kb_u4v, e[d]), rs,
-                                           ap_off(col_off, c, c0, r + d, rlast), 0, 2);
+                                           ap_off(col_off, c, c0, r + d, rlast), 0, AP_NT);
   }
 }
 
