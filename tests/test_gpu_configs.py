@@ -4,7 +4,8 @@ for states whose bare mass differs from the discretisation mass.
 * 40^3 (thermal) and 48^3 (box n0 = 4, Q = 1, lambda = 0.5, dt = 0.05): the
   plane kernels compiled for Nx = 40 and 48, in every block size, and the
   cubic-orbit storage of the same states, against the oracle at a short
-  memory window.
+  memory window; 64^3 (thermal), the largest lattice of the plane kernels,
+  in every block size.
 * The seeded 8-run ensemble {16,16,24,24,32,32,40,40}^3 with box
   occupations, graph-replayed, at N_mem = 8 against one oracle run per
   member.
@@ -78,6 +79,8 @@ LARGE = {
     # the largest single-GPU workload's lattice and state: box n0 = 4, Q = 1,
     # lambda = 0.5, dt = 0.05
     48: dict(dt=0.05, lam=0.5, occ=lambda k2: np.where(k2 <= 1.0 * (1 + 1e-12), 4.0, 0.0)),
+    # the largest lattice the plane kernels take: thermal T = 2, lambda = 1
+    64: dict(dt=0.1, lam=1.0, occ=lambda k2: thermal(k2)),
 }
 
 
@@ -89,10 +92,10 @@ def _fft_variants():
     return v
 
 
-@pytest.mark.parametrize("Nx", [40, 48])
+@pytest.mark.parametrize("Nx", [40, 48, 64])
 def test_large_lattice_every_plane_shape(Nx):
-    """The plane kernels for Nx = 40 / 48 in every block size: 10 steps
-    through a sliding N_mem = 6 window, against the oracle."""
+    """The plane kernels for Nx = 40 / 48 / 64 in every block size: 10
+    steps through a sliding N_mem = 6 window, against the oracle."""
     from qftkb import KBContext
     cfg = LARGE[Nx]
     for v in _fft_variants():
