@@ -54,7 +54,7 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("Nx,sdim,Nmem,steps", CASES + [(32, 3, 5, 8), (40, 3, 4, 7)])
+@pytest.mark.parametrize("Nx,sdim,Nmem,steps", CASES + [(32, 3, 5, 8), (40, 3, 4, 7), (48, 3, 4, 6), (64, 3, 3, 5)])
 def test_ninverse_nlo_trajectory(Nx, sdim, Nmem, steps):
     print(f"\n[case] 1/N NLO trajectory Nx={Nx} sdim={sdim} Nmem={Nmem} steps={steps}", flush=True)
     o, g = make_pair(Nx, sdim, Nmem)
