@@ -121,6 +121,27 @@ def test_adjoint_matches_twin_across_apply_chunks(args, T):
     c.close()
 
 
+@pytest.mark.parametrize("args,T", [
+    pytest.param((32, 3, 5, 0.1, 1.0, 1, 1.0, 1.0, 1), 12, id="32^3"),
+    pytest.param((48, 3, 4, 0.05, 0.5, 1, 1.0, 1.0, 1), 10, id="48^3"),
+    pytest.param((64, 3, 3, 0.1, 1.0, 1, 1.0, 1.0, 1), 8, id="64^3"),
+])
+def test_adjoint_matches_twin_on_large_lattices(args, T):
+    """The adjoint self-energy planes compiled for the larger lattices (up to
+    the planes' largest, 64^3), with short windows: the gradient against
+    the numpy twin."""
+    print(f"\n[case] adjoint large lattice {args} T={T}", flush=True)
+    n0, m, wgt = setup(args)
+    c, L, g = gpu_gradient(args, n0, m, T, wgt)
+    Lt, lt, dnt, dFt = gradient(args, n0, m, T, wgt)
+    assert L == pytest.approx(Lt, rel=1e-12)
+    assert g["lambda"] == pytest.approx(lt, rel=TWIN_TOL, abs=TWIN_TOL * abs(Lt))
+    assert rel(g["n0"], dnt) <= TWIN_TOL
+    for k in range(3):
+        assert rel(g["F0"][k], dFt[k]) <= TWIN_TOL, k
+    c.close()
+
+
 def test_adjoint_entries_after_partial_sweep():
     """Stop the sweep after a few reverse steps and compare every adjoint
     entry of the window with the twin's (the Hartree adjoint of the last
