@@ -191,19 +191,6 @@ __global__ void adj_slide_kernel(double* __restrict__ H, double* __restrict__ Hb
 
 __global__ void adj_state_kernel(StepState* st, int t, int N) { *st = state_of_step(t, N); }
 
-// M^2 = Mass2 + hfac sum_p F(pair) and the plain sum
-__global__ __launch_bounds__(1024) void adj_hartree_kernel(const double* __restrict__ H, long NP, int V, long pair,
-                                                           double hfac, double Mass2, double* __restrict__ out) {
-  __shared__ double red[1024 / 64];
-  double acc = 0.0;
-  for (int p = threadIdx.x; p < V; p += blockDim.x) acc += H[hoff(p / MB, pair, p % MB, NP)];
-  const double s = block_sum<1024>(acc, red);
-  if (threadIdx.x == 0) {
-    out[0] = s * hfac + Mass2;
-    out[1] = s;
-  }
-}
-
 // dL/dF of F(T-1,T-1), F(T,T-1), F(T,T) for L = sum wgt n(T); n as
 // createMeasurement (Measurement.jl:38-59) with its stability clamp
 __global__ void adj_seed_kernel(const double* __restrict__ H, double* __restrict__ Hbar,
@@ -1037,8 +1024,10 @@ int reverse_step(kb_ctx* c, int t, int T, int npend, bool apply) {
     LAUNCHCHK(c);
     if (nlo && (rc = launch_sigma(c, s->X, s->Sring[t % AK]))) return rc;
     const double H = c->p.Lambda * (double)(c->p.ONgroup + 2) / (6.0 * (double)c->p.ONgroup);
-    hipLaunchKernelGGL(adj_hartree_kernel, dim3(1), dim3(1024), 0, c->stream, c->H, c->NP, c->V,
-                       pair_off(a.st.s_tm1, a.st.s_tm1, N), H / (double)c->Vl, c->p.Mass2, s->m2r + (t & 1) * 2);
+    // M^2(t) and sum_p F(t-1,t-1;p) of the step being reversed (st_dev holds its state)
+    double* m2t = s->m2r + (t & 1) * 2;
+    hipLaunchKernelGGL(hartree_step_kernel, dim3(hartree_groups(c->V)), dim3(HARTREE_THREADS), 0, c->stream, c->H,
+                       c->NP, c->V, N, c->st_dev, H / (double)c->Vl, c->p.Mass2, nullptr, m2t, c->hpart, m2t + 1);
     LAUNCHCHK(c);
     set_pending(c, a, t + 1, nlo ? npend : 0);
     launch_row<0>(a.npend, dim3(c->NB), c->stream, a);

@@ -402,7 +402,7 @@ int launch_hartree_step(kb_ctx* c) {
   const double H = hartree_prefactor(c);
   const double hfac = H * (1.0 / (double)c->Vl);
   hipLaunchKernelGGL(hartree_step_kernel, dim3(hartree_groups(c->V)), dim3(HARTREE_THREADS), 0, c->stream, c->H, c->NP,
-                     c->V, c->N, c->st_dev, hfac, c->p.Mass2, c->wgt_dev, c->m2, c->hpart);
+                     c->V, c->N, c->st_dev, hfac, c->p.Mass2, c->wgt_dev, c->m2, c->hpart, nullptr);
   LAUNCHCHK(c);
   return KB_OK;
 }

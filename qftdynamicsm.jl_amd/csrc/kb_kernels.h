@@ -190,14 +190,15 @@ __global__ void hartree_kernel(const double* H, long NP, int V, long pair, doubl
                                const double* wgt, double* out);
 // Hartree mass of the step being taken: F(t-1,t-1) from *st; one workgroup
 // of HARTREE_THREADS per HARTREE_SITES sites, part: hartree_groups(V)
-// partial sums followed by a zeroed counter (hartree_part_doubles(V) doubles)
+// partial sums followed by a zeroed counter (hartree_part_doubles(V) doubles);
+// sum_out (may be null) receives the weighted sum of F(t-1,t-1) itself
 constexpr int HARTREE_THREADS = 256;
 constexpr int HARTREE_LOADS = 8;
 constexpr int HARTREE_SITES = HARTREE_THREADS * HARTREE_LOADS;
 inline int hartree_groups(int V) { return (V + HARTREE_SITES - 1) / HARTREE_SITES; }
 inline size_t hartree_part_doubles(int V) { return (size_t)hartree_groups(V) + 2; }
 __global__ void hartree_step_kernel(const double* H, long NP, int V, int N, const StepState* st, double hfac,
-                                    double Mass2, const double* wgt, double* out, double* part);
+                                    double Mass2, const double* wgt, double* out, double* part, double* sum_out);
 // one launch over all runs: workgroups [wg_start[r], wg_start[r+1]) belong to run r
 template <int ROWS, int B, int TPB>
 __global__ void memint_step(const MemintArgs* runs, const int* wg_start, int nruns);

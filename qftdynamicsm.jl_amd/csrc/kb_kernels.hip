@@ -281,7 +281,8 @@ __global__ __launch_bounds__(HARTREE_THREADS) void hartree_step_kernel(const dou
                                                                        int N, const StepState* __restrict__ st,
                                                                        double hfac, double Mass2,
                                                                        const double* __restrict__ wgt,
-                                                                       double* __restrict__ out, double* part) {
+                                                                       double* __restrict__ out, double* part,
+                                                                       double* __restrict__ sum_out) {
   __shared__ double red[HARTREE_THREADS / 64];
   __shared__ int is_last;
   const long pair = pair_off(st->s_tm1, st->s_tm1, N);  // F(t-1,t-1), Evolution.jl:945
@@ -319,6 +320,7 @@ __global__ __launch_bounds__(HARTREE_THREADS) void hartree_step_kernel(const dou
     double tot = 0.0;
     for (int w = 0; w < HARTREE_THREADS / 64; ++w) tot += red[w];
     out[0] = tot * hfac + Mass2;
+    if (sum_out) sum_out[0] = tot;
     __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
