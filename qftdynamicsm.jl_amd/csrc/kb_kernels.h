@@ -240,7 +240,7 @@ __global__ void finite_kernel(const double* H, size_t count, int* flag);
 bool sigma_planes_supported(int Nx);
 hipError_t sigma_planes_prepare(int n);
 // variant: index into kFFTThreads (threads per workgroup of the plane kernels)
-constexpr int NFFT_VARIANTS = 3;
+constexpr int NFFT_VARIANTS = 4;
 extern const int kFFTThreads[NFFT_VARIANTS];
 hipError_t launch_sigma_planes(const SigmaArgs& a, int rows, hipStream_t st, int variant = 0);
 hipError_t launch_sigma_planes_adjoint(const SigmaArgs& a, int rows, hipStream_t st);

@@ -521,16 +521,23 @@ static hipError_t prepare_one(size_t lds) {
   return hipFuncSetAttribute((const void*)sigma_plane_inv<NT, MODE, TPB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
+// threads per workgroup of the plane passes that the autotuner does not
+// choose (the adjoint, 1/N bubble and chain, and context-free passes): the
+// sunset passes' measured best at 32^3-48^3 (170 against 200 us per stage at
+// 32^3, N_mem = 400, for 512 against 256 threads)
+constexpr int PLANE_THREADS = 512;
+
 template <int NT>
 static hipError_t prepare_t(size_t lds) {
   hipError_t e = prepare_one<NT, SG_SUNSET, 256>(lds);
   if (e == hipSuccess) e = prepare_one<NT, SG_SUNSET, 128>(lds);
   if (e == hipSuccess) e = prepare_one<NT, SG_SUNSET, 512>(lds);
-  if (e == hipSuccess) e = prepare_one<NT, SG_BUBBLE, 256>(lds);
-  if (e == hipSuccess) e = prepare_one<NT, SG_CHAIN, 256>(lds);
-  if (e == hipSuccess) e = prepare_one<NT, SG_PLAIN, 256>(lds);
+  if (e == hipSuccess) e = prepare_one<NT, SG_SUNSET, 1024>(lds);
+  if (e == hipSuccess) e = prepare_one<NT, SG_BUBBLE, PLANE_THREADS>(lds);
+  if (e == hipSuccess) e = prepare_one<NT, SG_CHAIN, PLANE_THREADS>(lds);
+  if (e == hipSuccess) e = prepare_one<NT, SG_PLAIN, PLANE_THREADS>(lds);
   if (e != hipSuccess) return e;
-  return prepare_one<NT, SG_SUNSET_ADJ, 256>(lds);
+  return prepare_one<NT, SG_SUNSET_ADJ, PLANE_THREADS>(lds);
 }
 
 // allow the dynamic LDS of the plane kernels for lattice size n (the
@@ -565,30 +572,31 @@ static hipError_t launch_planes_n(const SigmaArgs& a, int L, hipStream_t st) {  
 }
 
 // the plane kernels' block sizes the autotuner chooses from; entry 0 is the default
-const int kFFTThreads[NFFT_VARIANTS] = {256, 128, 512};
+const int kFFTThreads[NFFT_VARIANTS] = {256, 128, 512, 1024};
 
 hipError_t launch_sigma_planes(const SigmaArgs& a, int L, hipStream_t st, int variant) {
   switch (variant) {
     case 1: return launch_planes_n<SG_SUNSET, 128>(a, L, st);
     case 2: return launch_planes_n<SG_SUNSET, 512>(a, L, st);
+    case 3: return launch_planes_n<SG_SUNSET, 1024>(a, L, st);
     default: return launch_planes_n<SG_SUNSET, 256>(a, L, st);
   }
 }
 
 hipError_t launch_sigma_planes_adjoint(const SigmaArgs& a, int L, hipStream_t st) {
-  return launch_planes_n<SG_SUNSET_ADJ, 256>(a, L, st);
+  return launch_planes_n<SG_SUNSET_ADJ, PLANE_THREADS>(a, L, st);
 }
 
 hipError_t launch_sigma_planes_plain(const SigmaArgs& a, hipStream_t st) {
-  return launch_planes_n<SG_PLAIN, 256>(a, a.rows, st);
+  return launch_planes_n<SG_PLAIN, PLANE_THREADS>(a, a.rows, st);
 }
 
 hipError_t launch_bubble_planes(const SigmaArgs& a, int L, hipStream_t st) {
-  return launch_planes_n<SG_BUBBLE, 256>(a, L, st);
+  return launch_planes_n<SG_BUBBLE, PLANE_THREADS>(a, L, st);
 }
 
 hipError_t launch_chain_planes(const SigmaArgs& a, int L, hipStream_t st) {
-  return launch_planes_n<SG_CHAIN, 256>(a, L, st);
+  return launch_planes_n<SG_CHAIN, PLANE_THREADS>(a, L, st);
 }
 
 }  // namespace qftkb
