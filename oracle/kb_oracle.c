@@ -87,7 +87,16 @@ static void fft1d(const plan1d *p, cplx *x, cplx *tmp, int sign) {
   if (n <= 1) return;
   cplx *src = x, *dst = tmp;
   int Ns = 1;
-  cplx v[64], o[64];
+  /* one butterfly's inputs and outputs; a prime factor above 64 (Nx = 67,
+   * 71, ... on the pencil path) takes heap buffers */
+  int maxR = 1;
+  for (int s = 0; s < p->nf; ++s) maxR = p->fac[s] > maxR ? p->fac[s] : maxR;
+  cplx vbuf[64], obuf[64];
+  cplx *v = vbuf, *o = obuf;
+  if (maxR > 64) {
+    v = (cplx *)malloc(sizeof(cplx) * 2 * (size_t)maxR);
+    o = v + maxR;
+  }
   for (int s = 0; s < p->nf; ++s) {
     const int R = p->fac[s];
     const int m = n / R;
@@ -123,6 +132,7 @@ static void fft1d(const plan1d *p, cplx *x, cplx *tmp, int sign) {
     Ns *= R;
   }
   if (src != x) memcpy(x, src, sizeof(cplx) * (size_t)n);
+  if (v != vbuf) free(v);
 }
 
 /* ------------------------------------------------------------------ */

@@ -79,6 +79,12 @@ CASES = [
     (2, 3, 4, 12),
     (3, 3, 5, 12),
     (9, 3, 6, 10),
+    # past Nx = 64 the self-energy runs as pencil passes (5 in 3D, 3 in 2D, 1 in 1D), with a
+    # direct-DFT stage for the primes 67 and 127 (Nx <= 128)
+    (72, 2, 4, 8),
+    (67, 1, 5, 12),
+    (127, 1, 4, 8),
+    (66, 3, 3, 5),
 ]
 
 

@@ -24,7 +24,10 @@ def thermal(k2, T=2.0, m=1.0):
     return 1.0 / (np.exp(np.sqrt(k2 + m * m) / (T * m)) - 1.0)
 
 
-@pytest.mark.parametrize("Nx,sdim", [(8, 3), (6, 3), (5, 3), (12, 2), (7, 2), (30, 1), (1, 3), (10, 3)])
+# 67 and 131 are primes above 64: one butterfly of the full length (the
+# GPU's pencil passes run these sizes past the plane kernels)
+@pytest.mark.parametrize("Nx,sdim", [(8, 3), (6, 3), (5, 3), (12, 2), (7, 2), (30, 1), (1, 3), (10, 3),
+                                     (67, 1), (131, 1), (67, 2), (134, 1)])
 def test_dft_matches_numpy(Nx, sdim):
     o = Oracle(Nx, sdim, 4, 0.1, 1.0, 1, 1.0, 1.0, nthreads=2)
     rng = np.random.default_rng(Nx * 10 + sdim)
