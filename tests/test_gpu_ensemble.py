@@ -77,6 +77,9 @@ RUNS = [  # uneven members: lattice, window, coupling, O(N), dt
     dict(Nx=8, sdim=3, NstepsinMemory=6, dt=0.05, Lambda=1.1, ONgroup=4, Mass=0.8, Mass2=0.64),
     dict(Nx=5, sdim=2, NstepsinMemory=7, dt=0.1, Lambda=2.0, ONgroup=2, Mass=1.0, Mass2=1.0),
     dict(Nx=1, sdim=3, NstepsinMemory=4, dt=0.1, Lambda=1.0, ONgroup=1, Mass=1.0, Mass2=1.0),
+    # past the plane kernels: pencil passes, one of them a prime length
+    dict(Nx=72, sdim=2, NstepsinMemory=5, dt=0.1, Lambda=1.2, ONgroup=1, Mass=1.0, Mass2=1.0),
+    dict(Nx=67, sdim=1, NstepsinMemory=6, dt=0.1, Lambda=1.4, ONgroup=2, Mass=1.0, Mass2=1.0),
 ]
 
 
