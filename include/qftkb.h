@@ -62,7 +62,7 @@ typedef struct kb_ctx kb_ctx;
  * Mass2; Mass is the discretisation mass used for omega^2 at set-up and for
  * the measurement time. */
 typedef struct kb_params {
-  int Nx;             /* sites per dimension, 1 .. 128                   */
+  int Nx;             /* sites per dimension, 1 .. 2048; Nx^sdim <= 2^24 */
   int sdim;           /* 1, 2 or 3                                       */
   int NstepsinMemory; /* memory window length N_mem, 3 .. 640            */
   int ONgroup;        /* N of O(N)                                       */

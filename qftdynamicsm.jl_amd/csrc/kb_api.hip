@@ -20,6 +20,13 @@
 #endif
 // largest orbit count whose self-energy runs as orbit products (the matrix
 // takes orbits^2 doubles: 134 MB at the limit)
+// lattice limits: the pencil passes hold two buffers of Nx complex values per
+// pencil in LDS (64 KiB at Nx = 2048, one pencil per workgroup); with at most
+// 2^24 sites a launch of one workgroup per 8-site block stays below 2^32
+// work-items at 1024 threads per workgroup
+constexpr int KB_MAX_NX = 2048;
+constexpr long KB_MAX_SITES = 1L << 24;
+
 #ifndef KB_ORBIT_PRODUCT_MAX
 #define KB_ORBIT_PRODUCT_MAX 4096
 #endif
@@ -609,10 +616,15 @@ namespace {
 static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
   if (!p || !out) return KB_ERR_ARG;
   *out = nullptr;
-  if (p->sdim < 1 || p->sdim > 3 || p->Nx < 1 || p->Nx > 128 || p->NstepsinMemory < 3 ||
+  if (p->sdim < 1 || p->sdim > 3 || p->Nx < 1 || p->Nx > KB_MAX_NX || p->NstepsinMemory < 3 ||
       p->NstepsinMemory > 640 || p->ONgroup < 1 || !(p->dt > 0) || p->pexp < KB_PEXP_LAMBDA_LO ||
       p->pexp > KB_PEXP_LAMBDA_NLO_CLASSICAL)
     return KB_ERR_ARG;
+  {
+    long v = 1;
+    for (int d = 0; d < p->sdim; ++d) v *= p->Nx;
+    if (v > KB_MAX_SITES) return KB_ERR_ARG;
+  }
   // the 1/N stage runs on the plane kernels (in orbit storage also as orbit
   // products: checked below, once the orbits are known)
   if (p->pexp == KB_PEXP_NINVERSE_NLO && !sym && !sigma_planes_supported(p->Nx)) return KB_ERR_ARG;
@@ -636,8 +648,10 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
   c->NB = (c->V + MB - 1) / MB;
   c->Vpad = c->NB * MB;
   plan_fft(c->plan, p->Nx);
-  // pencils per workgroup: 16 complex elements per pencil row at least
+  // pencils per workgroup: 16 complex elements per pencil row at least, and
+  // the two LDS buffers (2 P Nx complex values) within 64 KiB
   c->fftP = p->Nx >= 64 ? 4 : (p->Nx >= 16 ? 16 : 32);
+  c->fftP = std::min(c->fftP, std::max(1, KB_MAX_NX / p->Nx));
   if (c->fftP > c->Vl / p->Nx) c->fftP = c->Vl / p->Nx;
   int rc;
   auto fail = [&](int code) {
@@ -1692,6 +1706,10 @@ int ens_upload_tables(kb_ens* e) {
     }
   }
   e->lrun[nl] = (int)ma.size();
+  // the pooled launch has one workgroup per site block of every member: keep
+  // its work-items (up to 1024 per workgroup) below 2^32
+  if ((long)total > (long)(0xffffffffL / 1024))
+    return ens_err(e, KB_ERR_ARG, "ensemble: %d site blocks in all, at most %ld", total, 0xffffffffL / 1024);
   e->total_wg = total;
   const size_t n = ma.size();
   if (!e->margs_dev && (hipMalloc((void**)&e->margs_dev, sizeof(MemintArgs) * n) != hipSuccess ||

@@ -387,3 +387,47 @@ def test_plane_workspace_only_where_a_plane_stage_runs():
     assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
     for c in (lo_full, nlo_full, sym, ninv, ref):
         c.close()
+
+
+@pytest.mark.parametrize("args,pexp,T", [
+    # the reference's own test lattices (test/runtests.jl): 1024 in 1D, 512 in 1D in the 1/N
+    # expansion (TestTwoPIScalar_CPUred(512, 1, 4, "NinverseNLO", 50)); past 128 in 2D and 3D
+    pytest.param((1024, 1, 5, 0.1, 1.0, 1, 1.0, 1.0), 1, 12, id="1024-1d-NLO"),
+    pytest.param((512, 1, 5, 0.05, 2.0, 4, 1.0, 1.0), 3, 10, id="512-1d-1/N"),
+    pytest.param((130, 2, 4, 0.1, 1.2, 2, 1.0, 0.8), 1, 8, id="130^2-NLO"),
+    pytest.param((130, 3, 4, 0.1, 1.0, 1, 1.0, 1.0), 0, 8, id="130^3-LO"),
+])
+def test_orbit_storage_large_lattices_match_oracle(args, pexp, T):
+    """Orbit storage past the plane kernels on large lattices (orbit products
+    up to 4096 orbits; LambdaLO without transforms), against the oracle."""
+    from qftkb import KBContext
+    print(f"\n[case] orbit storage, large lattice {args} pexp={pexp}", flush=True)
+    o = Oracle(*args, pexp=pexp, nthreads=8)
+    g = KBContext(*args, pexp=pexp, symmetric=True)
+    n = thermal(o.kL2(), m=args[6])
+    o.init_particles(n, args[6])
+    g.init_particles(n, args[6])
+    for t in range(3, T + 1):
+        o.evolve(t)
+        g.step(t)
+    tmin, tmax = o.window()
+    assert g.window() == (tmin, tmax)
+    for a in range(tmin, tmax + 1):
+        for b in range(tmin, a + 1):
+            for w in "Fr":
+                assert rel(g.get(w, a, b), o.get(w, a, b)) <= 1e-10, (w, a, b)
+    mo, mg = o.measure(T), g.measure(T)
+    assert abs(mg["E"] - mo["E"]) <= 1e-10 * abs(mo["E"])
+    assert rel(mg["n"], mo["n"]) <= 1e-10
+    g.close()
+    o.close()
+
+
+def test_lattice_limits_are_refused():
+    """Nx up to 2048 and at most 2^24 sites; beyond either, kb_create refuses."""
+    from qftkb import KBContext, KBError
+    for bad in [(2049, 1), (257, 3), (4097, 2)]:
+        with pytest.raises(KBError):
+            KBContext(bad[0], bad[1], 3, 0.1, 1.0, 1, 1.0, 1.0, pexp=0)
+    c = KBContext(256, 3, 3, 0.1, 1.0, 1, 1.0, 1.0, pexp=0)  # 2^24 sites, no transforms
+    c.close()
