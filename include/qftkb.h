@@ -100,11 +100,11 @@ int kb_info(const kb_ctx *ctx, long long *V, long long *device_bytes);
  * maps them to cubic-symmetric states).  The self-energy transforms run as
  * FP64 matrix products with the orbit cosine matrix (up to 4096 orbits,
  * KB_FFT_ORBIT_PRODUCT below) or as plane FFTs over the whole lattice
- * (Nx <= 64).  Every entry point keeps exchanging whole
+ * (Nx <= 64; in 1D up to 2048).  Every entry point keeps exchanging whole
  * lattices of V sites: inputs are read at each orbit's representative site
  * (the caller supplies symmetric data), outputs are expanded over the
  * orbits.  KB_ERR_ARG for a transform stage with neither form available
- * (Nx > 64 and more than 4096 orbits); no adjoint.  Ensembles take such
+ * (Nx > 64 in 2D/3D and more than 4096 orbits); no adjoint.  Ensembles take such
  * members through kb_ens_create_sym.  The reference's CPUred
  * storage (TwoPIScalar_SimDataStruct.jl:204-260) groups sites by FFTWhelper
  * levels instead; this is the exact-symmetry version of that idea. */

@@ -1315,7 +1315,7 @@ int adj_record(kb_ctx* c, int checkpoint_every, int max_steps, int host_windows)
   if (c->sym) return seterr(c, KB_ERR_STATE, "adjoint: not available with cubic-orbit storage");
   if (c->p.pexp != KB_PEXP_LAMBDA_LO && !sunset_expansion(c->p.pexp))
     return seterr(c, KB_ERR_ARG, "adjoint: lambda expansions only");
-  if (sunset_expansion(c->p.pexp) && !sigma_planes_supported(c->p.Nx))
+  if (sunset_expansion(c->p.pexp) && !sigma_planes_supported(c->p.Nx, c->p.sdim))
     return seterr(c, KB_ERR_ARG, "adjoint: Nx=%d above the plane self-energy limit", c->p.Nx);
   {
     // per function, shared by every context: the largest supported window

@@ -202,7 +202,7 @@ int launch_sigma(kb_ctx* c, double2* X, double* S_out) {
     return KB_OK;
   }
   if (plane_workspace_missing(c)) return seterr(c, KB_ERR_STATE, "self-energy: plane workspace not allocated");
-  if (sigma_planes_supported(c->p.Nx)) {
+  if (sigma_planes_supported(c->p.Nx, c->p.sdim)) {
     hipError_t e = launch_sigma_planes(a, L, c->stream, c->fft_tile);
     if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "self-energy launch failed: %s", hipGetErrorString(e));
     return KB_OK;
@@ -533,8 +533,8 @@ int ensure_plane_workspace(kb_ctx* c) {
   // the plane passes keep the half space z in [0, Nx/2] of each row; the
   // pencil passes (Nx > 64) whole rows
   const int n = c->p.Nx;
-  const size_t wrows = sigma_planes_supported(n) ? (size_t)c->N * n * n * (n / 2 + 1) * sizeof(double2) : rows;
-  if ((c->p.sdim == 3 || !sigma_planes_supported(n)) && !c->W && (rc = dmalloc(c, (void**)&c->W, wrows)))
+  const size_t wrows = sigma_planes_supported(n, c->p.sdim) ? (size_t)c->N * n * n * (n / 2 + 1) * sizeof(double2) : rows;
+  if ((c->p.sdim == 3 || !sigma_planes_supported(n, c->p.sdim)) && !c->W && (rc = dmalloc(c, (void**)&c->W, wrows)))
     return rc;
   if (c->p.pexp == KB_PEXP_NINVERSE_NLO && !c->Xn && (rc = dmalloc(c, (void**)&c->Xn, rows))) return rc;
   return KB_OK;
@@ -542,7 +542,7 @@ int ensure_plane_workspace(kb_ctx* c) {
 
 // a plane-kernel stage needs its workspace (a guard, not a path callers reach)
 bool plane_workspace_missing(const kb_ctx* c) {
-  return ((c->p.sdim == 3 || !sigma_planes_supported(c->p.Nx)) && !c->W) ||
+  return ((c->p.sdim == 3 || !sigma_planes_supported(c->p.Nx, c->p.sdim)) && !c->W) ||
          (c->p.pexp == KB_PEXP_NINVERSE_NLO && !c->Xn);
 }
 
@@ -627,7 +627,7 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
   }
   // the 1/N stage runs on the plane kernels (in orbit storage also as orbit
   // products: checked below, once the orbits are known)
-  if (p->pexp == KB_PEXP_NINVERSE_NLO && !sym && !sigma_planes_supported(p->Nx)) return KB_ERR_ARG;
+  if (p->pexp == KB_PEXP_NINVERSE_NLO && !sym && !sigma_planes_supported(p->Nx, p->sdim)) return KB_ERR_ARG;
   kb_ctx* c = new (std::nothrow) kb_ctx();
   if (!c) return KB_ERR_OOM;
   c->p = *p;
@@ -662,7 +662,7 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
   // orbit storage gathers and scatters through the site maps in the plane
   // kernels only (Nx <= 64); past them, a transform stage needs the orbit
   // products (LambdaLO has none)
-  if (sym && !sigma_planes_supported(p->Nx) && (sunset_expansion(p->pexp) || p->pexp == KB_PEXP_NINVERSE_NLO) &&
+  if (sym && !sigma_planes_supported(p->Nx, p->sdim) && (sunset_expansion(p->pexp) || p->pexp == KB_PEXP_NINVERSE_NLO) &&
       c->V > KB_ORBIT_PRODUCT_MAX)
     return fail(KB_ERR_ARG);
   if (hipSetDevice(p->device) != hipSuccess) return fail(KB_ERR_HIP);
@@ -786,7 +786,7 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
         hipMemcpy(c->wg0_dev, &zero, sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
       return fail(KB_ERR_HIP);
   }
-  if (sigma_planes_supported(p->Nx) && sigma_planes_prepare(p->Nx) != hipSuccess) {
+  if (sigma_planes_supported(p->Nx, p->sdim) && sigma_planes_prepare(p->Nx) != hipSuccess) {
     seterr(c, KB_ERR_ARG, "self-energy plane kernels: LDS attribute rejected");
     return fail(KB_ERR_ARG);
   }
@@ -1436,7 +1436,7 @@ int kb_set_fft_tiles(kb_ctx* c, int variant) {
       (variant == KB_FFT_ORBIT_PRODUCT && !c->Mo))
     return KB_ERR_ARG;
   // orbit storage past the plane kernels' sizes has only the orbit product
-  if (variant != KB_FFT_ORBIT_PRODUCT && c->sym && !sigma_planes_supported(c->p.Nx)) return KB_ERR_ARG;
+  if (variant != KB_FFT_ORBIT_PRODUCT && c->sym && !sigma_planes_supported(c->p.Nx, c->p.sdim)) return KB_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (variant != KB_FFT_ORBIT_PRODUCT) {
     int rc = ensure_plane_workspace(c);
@@ -1458,7 +1458,7 @@ int kb_autotune_fft(kb_ctx* c, int reps, int* best, double* ms) {
   if (!c->initialised || c->st.tmax < 3)
     return seterr(c, KB_ERR_STATE, "autotune: take a step first (the last step's self-energy is re-timed)");
   if (c->in_ensemble || c->adj) return seterr(c, KB_ERR_STATE, "autotune: not for ensemble members or while recording");
-  if (!sunset_expansion(c->p.pexp) || !sigma_planes_supported(c->p.Nx))
+  if (!sunset_expansion(c->p.pexp) || !sigma_planes_supported(c->p.Nx, c->p.sdim))
     return seterr(c, KB_ERR_ARG, "autotune: no plane self-energy stage for this context");
   // the plane kernels' block sizes, then (orbit storage) the orbit product
   const int nv = NFFT_VARIANTS + (c->Mo ? 1 : 0);

@@ -237,7 +237,8 @@ __global__ void fill_free_kernel(double* H, long NP, int V, int N, int tmax, con
 __global__ void finite_kernel(const double* H, size_t count, int* flag);
 
 // self-energy stage on whole planes (kb_sigma.hip), Nx <= 64
-bool sigma_planes_supported(int Nx);
+// the plane kernels take Nx <= 64, and 1D rows up to 2048 points (one line in LDS)
+bool sigma_planes_supported(int Nx, int sdim);
 hipError_t sigma_planes_prepare(int n);
 // variant: index into kFFTThreads (threads per workgroup of the plane kernels)
 constexpr int NFFT_VARIANTS = 4;

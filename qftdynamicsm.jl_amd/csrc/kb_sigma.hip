@@ -510,7 +510,10 @@ static hipError_t launch_planes_t(const SigmaArgs& a, int L, hipStream_t st) {  
   return hipGetLastError();
 }
 
-bool sigma_planes_supported(int Nx) { return Nx >= 1 && Nx <= 64; }
+// planes of up to 64 x 64 points; a 1D lattice is one line, whose two LDS
+// buffers (2 (Nx + 1) complex values, 64 KiB at Nx = 2048) fit the dynamic
+// LDS that sigma_planes_prepare allows for a 64 x 64 plane
+bool sigma_planes_supported(int Nx, int sdim) { return Nx >= 1 && Nx <= (sdim == 1 ? 2048 : 64); }
 
 template <int NT, int MODE, int TPB>
 static hipError_t prepare_one(size_t lds) {

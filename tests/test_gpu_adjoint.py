@@ -74,6 +74,8 @@ CASES = [
     pytest.param((4, 3, 5, 0.1, 1.0, 1, 1.0, 1.0, 0), 14, id="4^3-LO"),
     pytest.param((4, 3, 6, 0.1, 1.6, 1, 1.0, 1.0, 4), 13, id="4^3-classical"),
     pytest.param((6, 3, 5, 0.1, 1.4, 2, 1.0, 1.0, 4), 12, id="6^3-classical-O(2)"),
+    # a 1D lattice past 64 (the plane kernels' one-line form)
+    pytest.param((256, 1, 5, 0.1, 1.2, 2, 1.0, 0.9, 1), 12, id="256-1d"),
 ]
 
 

@@ -54,7 +54,13 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("Nx,sdim,Nmem,steps", CASES + [(32, 3, 5, 8), (40, 3, 4, 7), (48, 3, 4, 6), (64, 3, 3, 5)])
+# 1D lattices past 64 run on the plane kernels as one line: the reference's 1/N test lattice
+# (TestTwoPIScalar_CPUred(512, 1, 4, "NinverseNLO", 50), test/runtests.jl), a prime and the largest
+LARGE_1D = [(512, 1, 5, 10), (67, 1, 6, 12), (2048, 1, 4, 8)]
+
+
+@pytest.mark.parametrize("Nx,sdim,Nmem,steps", CASES + [(32, 3, 5, 8), (40, 3, 4, 7), (48, 3, 4, 6), (64, 3, 3, 5)]
+                         + LARGE_1D)
 def test_ninverse_nlo_trajectory(Nx, sdim, Nmem, steps):
     print(f"\n[case] 1/N NLO trajectory Nx={Nx} sdim={sdim} Nmem={Nmem} steps={steps}", flush=True)
     o, g = make_pair(Nx, sdim, Nmem)
@@ -256,8 +262,9 @@ def test_ninverse_orbit_product_equals_plane_kernels(Nx, Nmem, fill):
 def test_ninverse_unsupported_combinations_are_rejected():
     from qftkb import KBContext
     from qftkb._abi import KBError
-    with pytest.raises(KBError):
-        KBContext(72, 3, 4, 0.05, 1.0, 8, 1.0, 1.0, pexp=NINV_NLO)  # above the plane-kernel size
+    for big in [(72, 3), (72, 2)]:  # above the plane-kernel size in 2D and 3D
+        with pytest.raises(KBError):
+            KBContext(big[0], big[1], 4, 0.05, 1.0, 8, 1.0, 1.0, pexp=NINV_NLO)
     g = KBContext(4, 3, 5, 0.05, 1.0, 8, 1.0, 1.0, pexp=NINV_NLO)
     g.init_particles(np.zeros(64), 1.0)
     with pytest.raises(KBError):
