@@ -85,10 +85,11 @@ CASES = [
     (67, 1, 5, 12),
     (127, 1, 4, 8),
     (66, 3, 3, 5),
-    # the reference's own test lattices (TestTwoPIScalar_LambdaNLO_full(1024, 1, 1) and (512, 2, 5),
-    # test/runtests.jl), the largest pencil (2048), a prime near it, and a 3D lattice past 128
+    # the reference's own test lattices (TestTwoPIScalar_LambdaNLO_full(1024, 1, 1), (512, 2, 5) and
+    # (128, 3, 6), test/runtests.jl), the largest 1D line (2048), a prime near it, and 3D past 128
     (1024, 1, 5, 12),
     (512, 2, 4, 8),
+    (128, 3, 3, 5),
     (2048, 1, 4, 8),
     (2039, 1, 3, 5),
     (130, 3, 3, 4),
