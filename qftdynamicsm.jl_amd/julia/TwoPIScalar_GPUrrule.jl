@@ -13,8 +13,9 @@
 #   dL/dlambda, dL/dn0 from the pullback of occupationGPU, one reverse sweep
 #
 # Mass2 is held fixed (an RMass counterterm is not differentiated).  The
-# adjoint runs on full storage ("GPUfull", Nx <= 64) for the LambdaLO and
-# LambdaNLO expansions.
+# adjoint runs on full storage ("GPUfull") for the LambdaLO, LambdaNLO and
+# LambdaNLOclassical expansions; the two sunsets need the plane self-energy
+# kernels (Nx <= 64 in 2D and 3D, up to 2048 in 1D).
 ################################################################################
 using ChainRulesCore
 
