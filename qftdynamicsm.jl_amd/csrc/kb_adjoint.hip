@@ -97,12 +97,6 @@ __device__ __forceinline__ double2 ld2_nt(const double* p) {
   const kb_a2v v = __builtin_nontemporal_load(reinterpret_cast<const kb_a2v*>(p));
   return make_double2(v.x, v.y);
 }
-__device__ __forceinline__ void st2_nt(double* p, double2 v) {
-  kb_a2v w;
-  w.x = v.x;
-  w.y = v.y;
-  __builtin_nontemporal_store(w, reinterpret_cast<kb_a2v*>(p));
-}
 
 // sum of n values by a whole workgroup (fixed order: identical in every
 // workgroup that sums the same array)

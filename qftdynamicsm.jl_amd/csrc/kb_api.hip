@@ -274,10 +274,13 @@ int launch_bubble(kb_ctx* c, const StepState* st, bool keep) {
     return KB_OK;
   }
   if (plane_workspace_missing(c)) return seterr(c, KB_ERR_STATE, "bubble pass: plane workspace not allocated");
+  if (c->sym && !sigma_planes_supported(c->p.Nx, c->p.sdim))
+    return seterr(c, KB_ERR_STATE, "bubble pass: orbit storage past the plane kernels runs the orbit products");
   SigmaArgs a = ninv_sigma_args(c, st);
   a.preF = a.preR = ivol * ivol;
   a.X = keep ? c->Xn : nullptr;
-  hipError_t e = launch_bubble_planes(a, c->N - 1, c->stream);
+  hipError_t e = sigma_planes_supported(c->p.Nx, c->p.sdim) ? launch_bubble_planes(a, c->N - 1, c->stream)
+                                                            : launch_bubble_pencils(a, c->N - 1, c->stream);
   if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "bubble launch failed: %s", hipGetErrorString(e));
   return KB_OK;
 }
@@ -326,10 +329,13 @@ int launch_chain(kb_ctx* c) {
     return KB_OK;
   }
   if (plane_workspace_missing(c)) return seterr(c, KB_ERR_STATE, "chain pass: plane workspace not allocated");
+  if (c->sym && !sigma_planes_supported(c->p.Nx, c->p.sdim))
+    return seterr(c, KB_ERR_STATE, "chain pass: orbit storage past the plane kernels runs the orbit products");
   SigmaArgs a = ninv_sigma_args(c, c->st_dev);
   a.preF = a.preR = pre;
   a.X = c->Xn;
-  hipError_t e = launch_chain_planes(a, c->N - 1, c->stream);
+  hipError_t e = sigma_planes_supported(c->p.Nx, c->p.sdim) ? launch_chain_planes(a, c->N - 1, c->stream)
+                                                            : launch_chain_pencils(a, c->N - 1, c->stream);
   if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "chain launch failed: %s", hipGetErrorString(e));
   return KB_OK;
 }
@@ -625,9 +631,6 @@ static int create_ctx(const kb_params* p, bool sym, kb_ctx** out) {
     for (int d = 0; d < p->sdim; ++d) v *= p->Nx;
     if (v > KB_MAX_SITES) return KB_ERR_ARG;
   }
-  // the 1/N stage runs on the plane kernels (in orbit storage also as orbit
-  // products: checked below, once the orbits are known)
-  if (p->pexp == KB_PEXP_NINVERSE_NLO && !sym && !sigma_planes_supported(p->Nx, p->sdim)) return KB_ERR_ARG;
   kb_ctx* c = new (std::nothrow) kb_ctx();
   if (!c) return KB_ERR_OOM;
   c->p = *p;

@@ -249,6 +249,10 @@ hipError_t launch_sigma_planes_adjoint(const SigmaArgs& a, int rows, hipStream_t
 hipError_t launch_sigma_planes_plain(const SigmaArgs& a, hipStream_t st);
 hipError_t launch_bubble_planes(const SigmaArgs& a, int rows, hipStream_t st);
 hipError_t launch_chain_planes(const SigmaArgs& a, int rows, hipStream_t st);
+// the same passes on 2D and 3D lattices past the plane kernels (pencils of
+// whole rows in W)
+hipError_t launch_bubble_pencils(const SigmaArgs& a, int rows, hipStream_t st);
+hipError_t launch_chain_pencils(const SigmaArgs& a, int rows, hipStream_t st);
 
 // self-energy stage of a cubic-orbit-stored window as two FP64 matrix
 // products with the orbit cosine matrix Mo (kb_orbit.hip)

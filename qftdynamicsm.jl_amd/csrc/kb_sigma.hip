@@ -1,5 +1,7 @@
 // kb_sigma.hip -- self-energy stage of the KB step on whole lattice planes
-// (TwoPIScalar_Evolution.jl:16-31), for Nx <= 64.
+// (TwoPIScalar_Evolution.jl:16-31), for Nx <= 64 and 1D lines up to 2048;
+// past them the 1/N bubble and chain passes run as pencil passes
+// (sigma_pencil, at the end of the kernels).
 //
 // For every live t' the sunset needs Fx = real(rfft(F(t-1,t'))) and
 // rx = real(rfft(rho(t-1,t'))), the pointwise products rx (Fx^2 - rx^2/12)
@@ -489,6 +491,113 @@ __global__ __launch_bounds__(TPB) void sigma_plane_inv(SigmaArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// The 1/N bubble and chain passes on 2D and 3D lattices past the plane
+// kernels (Nx > 64), as pencil passes: a workgroup transforms P pencils of n
+// points along a.axis of row q (blockIdx.y), and whole rows of W hold the
+// partial transforms between passes.  a.flags:
+//   PEN_GATHER  first pass: the mode's input, inversion-symmetrised
+//   PEN_POINT   forward transform, pointwise step, inverse transform (the
+//               last axis)
+//   PEN_INV     the transform is the inverse one
+//   PEN_EMIT    last pass: the mode's output instead of W
+// A pass with neither PEN_POINT nor PEN_INV is a forward transform.
+// ---------------------------------------------------------------------------
+enum { PEN_GATHER = 1, PEN_POINT = 2, PEN_INV = 4, PEN_EMIT = 8 };
+
+namespace {
+
+// site of point j of pencil `pencil` along a.axis (pencils enumerate the two
+// other coordinates, the lower one fastest)
+__device__ __forceinline__ int pencil_site(const SigmaArgs& a, int pencil, int j) {
+  if (a.axis == 0) {
+    const int iy = pencil % a.ny, iz = pencil / a.ny;
+    return j + a.nx * (iy + a.ny * iz);
+  }
+  if (a.axis == 1) {
+    const int ix = pencil % a.nx, iz = pencil / a.nx;
+    return ix + a.nx * (j + a.ny * iz);
+  }
+  const int ix = pencil % a.nx, iy = pencil / a.nx;
+  return ix + a.nx * (iy + a.ny * j);
+}
+
+// the point reflection x -> -x of a site
+__device__ __forceinline__ int reflected_site(const SigmaArgs& a, int s) {
+  const int ix = s % a.nx, iy = (s / a.nx) % a.ny, iz = s / (a.nx * a.ny);
+  return (a.nx - ix) % a.nx + a.nx * ((a.ny - iy) % a.ny + a.ny * ((a.nz - iz) % a.nz));
+}
+
+// the pass input of full storage, symmetrised over sites s1 and s2 = -s1:
+// (IF, Ir)(t') from the ring (chain), or physical F(t-1,t'), rho(t-1,t')
+// from the history (bubble)
+template <int MODE>
+__device__ __forceinline__ double2 pencil_gather(const SigmaArgs& a, int s1, int s2, int slot) {
+  if constexpr (MODE == SG_CHAIN) {
+    const double* h1 = a.Iring + soff(s1 / MB, slot, s1 % MB, a.N);
+    const double* h2 = a.Iring + soff(s2 / MB, slot, s2 % MB, a.N);
+    return make_double2(0.5 * (h1[0] + h2[0]), 0.5 * (h1[1] + h2[1]));
+  } else {
+    const int s_tm1 = a.st->s_tm1;
+    const long pr = pair_off(s_tm1, slot, a.N);
+    const double sg = s_tm1 >= slot ? 1.0 : -1.0;
+    const double* h1 = a.H + hoff(s1 / MB, pr, s1 % MB, a.NP);
+    const double* h2 = a.H + hoff(s2 / MB, pr, s2 % MB, a.NP);
+    return make_double2(0.5 * (h1[0] + h2[0]), 0.5 * (h1[1] + h2[1]) * sg);
+  }
+}
+
+}  // namespace
+
+template <int MODE>
+__global__ __launch_bounds__(256) void sigma_pencil(SigmaArgs a) {
+  extern __shared__ double2 lds[];
+  const int n = a.n, P = a.P;
+  double2* A = lds;
+  double2* B = lds + (size_t)P * n;
+  const int q = blockIdx.y;
+  if (q >= stage_rows<MODE>(a)) return;
+  const int slot = row_slot<MODE>(a, q);
+  const size_t rowoff = (size_t)q * a.V;
+  const int pen0 = blockIdx.x * P;
+  // along x a pencil is contiguous: neighbouring threads take neighbouring
+  // points; along y and z neighbouring pencils are adjacent in memory
+  const bool contiguous = a.axis == 0;
+  for (int e = threadIdx.x; e < P * n; e += blockDim.x) {
+    const int k = contiguous ? e / n : e % P;
+    const int j = contiguous ? e - k * n : e / P;
+    const int pencil = pen0 + k;
+    double2 z = make_double2(0.0, 0.0);
+    if (pencil < a.npencils) {
+      const int s = pencil_site(a, pencil, j);
+      z = (a.flags & PEN_GATHER) ? pencil_gather<MODE>(a, s, reflected_site(a, s), slot) : a.W[rowoff + s];
+    }
+    A[k * n + j] = z;
+  }
+  __syncthreads();
+  double2* res = fft_lines<0>(A, B, a.plan, P, n, 1, a.tw, (a.flags & PEN_INV) != 0);
+  if (a.flags & PEN_POINT) {
+    double2* oth = res == A ? B : A;
+    for (int e = threadIdx.x; e < P * n; e += blockDim.x) {
+      const int k = e / n, j = e - k * n;
+      if (pen0 + k < a.npencils) res[e] = pointwise<MODE>(a, res[e], rowoff + pencil_site(a, pen0 + k, j));
+    }
+    __syncthreads();
+    res = fft_lines<0>(res, oth, a.plan, P, n, 1, a.tw, true);
+  }
+  for (int e = threadIdx.x; e < P * n; e += blockDim.x) {
+    const int k = contiguous ? e / n : e % P;
+    const int j = contiguous ? e - k * n : e / P;
+    const int pencil = pen0 + k;
+    if (pencil >= a.npencils) continue;
+    const int s = pencil_site(a, pencil, j);
+    if (a.flags & PEN_EMIT)
+      emit<MODE>(a, s, slot, rowoff, res[k * n + j]);
+    else
+      a.W[rowoff + s] = res[k * n + j];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host-side launch of the whole stage
 // ---------------------------------------------------------------------------
 template <int NT, int MODE, int TPB>
@@ -601,5 +710,29 @@ hipError_t launch_bubble_planes(const SigmaArgs& a, int L, hipStream_t st) {
 hipError_t launch_chain_planes(const SigmaArgs& a, int L, hipStream_t st) {
   return launch_planes_n<SG_CHAIN, PLANE_THREADS>(a, L, st);
 }
+
+// the pencil passes of one mode over 2D or 3D rows: forward along x (and y
+// in 3D), forward + pointwise + inverse along the last axis, inverse back
+template <int MODE>
+static hipError_t launch_pencils(SigmaArgs a, int L, hipStream_t st) {
+  if (a.ny == 1 || a.P < 1 || (size_t)a.P * a.n * 2 * sizeof(double2) > 65536) return hipErrorInvalidValue;
+  const size_t lds = sizeof(double2) * 2 * (size_t)a.P * a.n;
+  const dim3 grid((a.npencils + a.P - 1) / a.P, L);
+  const bool three = a.nz > 1;
+  const int passes[5][2] = {{0, PEN_GATHER}, {1, 0}, {three ? 2 : 1, PEN_POINT}, {1, PEN_INV}, {0, PEN_INV | PEN_EMIT}};
+  for (int i = 0; i < 5; ++i) {
+    if (!three && (i == 1 || i == 3)) continue;
+    a.axis = passes[i][0];
+    a.flags = passes[i][1];
+    hipLaunchKernelGGL((sigma_pencil<MODE>), grid, dim3(256), lds, st, a);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_bubble_pencils(const SigmaArgs& a, int L, hipStream_t st) { return launch_pencils<SG_BUBBLE>(a, L, st); }
+
+hipError_t launch_chain_pencils(const SigmaArgs& a, int L, hipStream_t st) { return launch_pencils<SG_CHAIN>(a, L, st); }
 
 }  // namespace qftkb
