@@ -975,7 +975,10 @@ int launch_sigma_adjoint(kb_ctx* c) {
   a.Sbar = s->Sbar;
   a.Hbar = s->Hbar;
   a.classical = c->p.pexp == KB_PEXP_LAMBDA_NLO_CLASSICAL;
-  hipError_t e = launch_sigma_planes_adjoint(a, c->N - 1, c->stream);
+  a.P = c->fftP;
+  a.npencils = c->V / c->p.Nx;
+  hipError_t e = sigma_planes_supported(c->p.Nx, c->p.sdim) ? launch_sigma_planes_adjoint(a, c->N - 1, c->stream)
+                                                            : launch_sunset_adjoint_pencils(a, c->N - 1, c->stream);
   if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "adjoint self-energy launch failed: %s", hipGetErrorString(e));
   return KB_OK;
 }
@@ -1309,8 +1312,6 @@ int adj_record(kb_ctx* c, int checkpoint_every, int max_steps, int host_windows)
   if (c->sym) return seterr(c, KB_ERR_STATE, "adjoint: not available with cubic-orbit storage");
   if (c->p.pexp != KB_PEXP_LAMBDA_LO && !sunset_expansion(c->p.pexp))
     return seterr(c, KB_ERR_ARG, "adjoint: lambda expansions only");
-  if (sunset_expansion(c->p.pexp) && !sigma_planes_supported(c->p.Nx, c->p.sdim))
-    return seterr(c, KB_ERR_ARG, "adjoint: Nx=%d above the plane self-energy limit", c->p.Nx);
   {
     // per function, shared by every context: the largest supported window
     const size_t lds = memint_lds(640);

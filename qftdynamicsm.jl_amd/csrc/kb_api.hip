@@ -207,6 +207,13 @@ int launch_sigma(kb_ctx* c, double2* X, double* S_out) {
     if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "self-energy launch failed: %s", hipGetErrorString(e));
     return KB_OK;
   }
+  if (X) {
+    // the adjoint's recompute keeps the forward transforms: the pencil passes
+    // that save them (2D and 3D past the plane kernels)
+    hipError_t e = launch_sunset_pencils(a, L, c->stream);
+    if (e != hipSuccess) return seterr(c, KB_ERR_HIP, "self-energy launch failed: %s", hipGetErrorString(e));
+    return KB_OK;
+  }
   const size_t lds = sizeof(double2) * 2 * (size_t)a.P * a.n;
   const dim3 grid((a.npencils + a.P - 1) / a.P, L);
   std::vector<std::pair<int, int>> passes;  // (axis, flags)

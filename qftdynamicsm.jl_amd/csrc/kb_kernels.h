@@ -253,6 +253,9 @@ hipError_t launch_chain_planes(const SigmaArgs& a, int rows, hipStream_t st);
 // whole rows in W)
 hipError_t launch_bubble_pencils(const SigmaArgs& a, int rows, hipStream_t st);
 hipError_t launch_chain_pencils(const SigmaArgs& a, int rows, hipStream_t st);
+// the sunset keeping its transforms in a.X, and its adjoint, as pencil passes
+hipError_t launch_sunset_pencils(const SigmaArgs& a, int rows, hipStream_t st);
+hipError_t launch_sunset_adjoint_pencils(const SigmaArgs& a, int rows, hipStream_t st);
 
 // self-energy stage of a cubic-orbit-stored window as two FP64 matrix
 // products with the orbit cosine matrix Mo (kb_orbit.hip)

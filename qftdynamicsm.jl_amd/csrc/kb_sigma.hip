@@ -491,8 +491,9 @@ __global__ __launch_bounds__(TPB) void sigma_plane_inv(SigmaArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// The 1/N bubble and chain passes on 2D and 3D lattices past the plane
-// kernels (Nx > 64), as pencil passes: a workgroup transforms P pencils of n
+// The 1/N bubble and chain passes, and the adjoint's sunset with its saved
+// transforms, on 2D and 3D lattices past the plane kernels (Nx > 64), as
+// pencil passes: a workgroup transforms P pencils of n
 // points along a.axis of row q (blockIdx.y), and whole rows of W hold the
 // partial transforms between passes.  a.flags:
 //   PEN_GATHER  first pass: the mode's input, inversion-symmetrised
@@ -528,13 +529,14 @@ __device__ __forceinline__ int reflected_site(const SigmaArgs& a, int s) {
 }
 
 // the pass input of full storage, symmetrised over sites s1 and s2 = -s1:
-// (IF, Ir)(t') from the ring (chain), or physical F(t-1,t'), rho(t-1,t')
-// from the history (bubble)
+// (IF, Ir)(t') (chain) or (SigF_bar, Sigr_bar)(t') (adjoint) from a ring, or
+// physical F(t-1,t'), rho(t-1,t') from the history (sunset, bubble)
 template <int MODE>
 __device__ __forceinline__ double2 pencil_gather(const SigmaArgs& a, int s1, int s2, int slot) {
-  if constexpr (MODE == SG_CHAIN) {
-    const double* h1 = a.Iring + soff(s1 / MB, slot, s1 % MB, a.N);
-    const double* h2 = a.Iring + soff(s2 / MB, slot, s2 % MB, a.N);
+  if constexpr (MODE == SG_CHAIN || MODE == SG_SUNSET_ADJ) {
+    const double* ring = MODE == SG_CHAIN ? a.Iring : a.Sbar;
+    const double* h1 = ring + soff(s1 / MB, slot, s1 % MB, a.N);
+    const double* h2 = ring + soff(s2 / MB, slot, s2 % MB, a.N);
     return make_double2(0.5 * (h1[0] + h2[0]), 0.5 * (h1[1] + h2[1]));
   } else {
     const int s_tm1 = a.st->s_tm1;
@@ -734,5 +736,11 @@ static hipError_t launch_pencils(SigmaArgs a, int L, hipStream_t st) {
 hipError_t launch_bubble_pencils(const SigmaArgs& a, int L, hipStream_t st) { return launch_pencils<SG_BUBBLE>(a, L, st); }
 
 hipError_t launch_chain_pencils(const SigmaArgs& a, int L, hipStream_t st) { return launch_pencils<SG_CHAIN>(a, L, st); }
+
+hipError_t launch_sunset_pencils(const SigmaArgs& a, int L, hipStream_t st) { return launch_pencils<SG_SUNSET>(a, L, st); }
+
+hipError_t launch_sunset_adjoint_pencils(const SigmaArgs& a, int L, hipStream_t st) {
+  return launch_pencils<SG_SUNSET_ADJ>(a, L, st);
+}
 
 }  // namespace qftkb

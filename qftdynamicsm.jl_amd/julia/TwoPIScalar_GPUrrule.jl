@@ -14,8 +14,7 @@
 #
 # Mass2 is held fixed (an RMass counterterm is not differentiated).  The
 # adjoint runs on full storage ("GPUfull") for the LambdaLO, LambdaNLO and
-# LambdaNLOclassical expansions; the two sunsets need the plane self-energy
-# kernels (Nx <= 64 in 2D and 3D, up to 2048 in 1D).
+# LambdaNLOclassical expansions, on every lattice the back-end takes.
 ################################################################################
 using ChainRulesCore
 

@@ -338,3 +338,24 @@ def test_largest_lattice_gradient_vs_oracle_fd_fixture(C):
         ad = np.array([g["n0"][b == i].sum() for i in range(16)])
         assert np.max(np.abs(ad - np.array(fdb))) <= FD_TOL * np.max(np.abs(fdb))
     c.close()
+
+
+@pytest.mark.parametrize("args,T,C", [
+    pytest.param((72, 2, 5, 0.1, 1.2, 2, 1.0, 0.9, 1), 12, 0, id="72^2-taped"),
+    pytest.param((67, 2, 4, 0.1, 1.5, 1, 1.0, 1.0, 4), 10, 3, id="67^2-classical-checkpointed"),
+    pytest.param((66, 3, 3, 0.1, 1.0, 1, 1.0, 1.0, 1), 7, 0, id="66^3"),
+])
+def test_adjoint_on_pencil_passes_matches_twin(args, T, C):
+    """2D and 3D lattices past the plane kernels: the recompute keeps its
+    transforms and the adjoint sunset runs as pencil passes; against the
+    reverse-mode twin at 1e-10."""
+    print(f"\n[case] adjoint on pencil passes {args} T={T} C={C}", flush=True)
+    n0, m, wgt = setup(args)
+    c, L, g = gpu_gradient(args, n0, m, T, wgt, C=C)
+    Lt, lt, dnt, dFt = gradient(args, n0, m, T, wgt)
+    assert L == pytest.approx(Lt, rel=1e-12)
+    assert g["lambda"] == pytest.approx(lt, rel=TWIN_TOL, abs=TWIN_TOL * abs(Lt))
+    assert rel(g["n0"], dnt) <= TWIN_TOL
+    for k in range(3):
+        assert rel(g["F0"][k], dFt[k]) <= TWIN_TOL, k
+    c.close()
