@@ -314,8 +314,12 @@ def test_ninverse_in_mixed_ensemble():
     runs = [dict(Nx=6, sdim=3, NstepsinMemory=8, dt=0.05, Lambda=10.0, ONgroup=8, Mass=1.0, Mass2=1.0, pexp=NINV_NLO),
             dict(Nx=4, sdim=3, NstepsinMemory=20, dt=0.05, Lambda=5.0, ONgroup=4, Mass=1.0, Mass2=1.0, pexp=NINV_NLO),
             dict(Nx=8, sdim=2, NstepsinMemory=6, dt=0.05, Lambda=10.0, ONgroup=8, Mass=1.0, Mass2=1.0, pexp=NINV_LO),
-            dict(Nx=5, sdim=3, NstepsinMemory=7, dt=0.05, Lambda=1.0, ONgroup=1, Mass=1.0, Mass2=1.0, pexp=1)]
-    print("\n[case] mixed-expansion ensemble, 4 runs", flush=True)
+            dict(Nx=5, sdim=3, NstepsinMemory=7, dt=0.05, Lambda=1.0, ONgroup=1, Mass=1.0, Mass2=1.0, pexp=1),
+            # 1/N on the pencil passes and on a 1D line past 64
+            dict(Nx=72, sdim=2, NstepsinMemory=5, dt=0.05, Lambda=10.0, ONgroup=8, Mass=1.0, Mass2=1.0, pexp=NINV_NLO),
+            dict(Nx=130, sdim=1, NstepsinMemory=6, dt=0.05, Lambda=10.0, ONgroup=8, Mass=1.0, Mass2=1.0,
+                 pexp=NINV_NLO)]
+    print(f"\n[case] mixed-expansion ensemble, {len(runs)} runs", flush=True)
     e = KBEnsemble(runs)
     e.use_graph(True)
     oracles = []
